@@ -1,0 +1,130 @@
+"""ctypes binding of libdamask_hip.so (declarations in include/damask_hip.h).
+
+The library is the only compute path: there is no CPU fallback.  Loading fails
+loudly when the shared object is missing or was not built.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libdamask_hip.so")
+
+
+class PhaseDesc(C.Structure):
+    _fields_ = [("lattice", C.c_int32), ("n_fam", C.c_int32), ("N_sl", C.c_int32 * 4),
+                ("C_11", C.c_double), ("C_12", C.c_double), ("C_44", C.c_double),
+                ("dot_gamma_0", C.c_double * 4), ("n", C.c_double * 4), ("a", C.c_double * 4),
+                ("h_0", C.c_double * 4), ("xi_0", C.c_double * 4), ("xi_inf", C.c_double * 4),
+                ("c_1", C.c_double * 4), ("c_2", C.c_double * 4), ("f_sat", C.c_double * 4),
+                ("n_h", C.c_int32), ("h_sl_sl", C.c_double * 32), ("n_nS", C.c_int32),
+                ("a_nS", C.c_double * 6), ("atol_xi", C.c_double), ("atol_gamma", C.c_double)]
+
+
+class Numerics(C.Structure):
+    _fields_ = [("N_iter_min", C.c_int32), ("N_iter_max", C.c_int32),
+                ("eps_abs_div", C.c_double), ("eps_rel_div", C.c_double),
+                ("eps_abs_P", C.c_double), ("eps_rel_P", C.c_double),
+                ("update_gamma", C.c_int32), ("divergence_correction", C.c_int32),
+                ("solver", C.c_int32), ("ngmres_m", C.c_int32), ("snes_max_it", C.c_int32),
+                ("N_cutback_max", C.c_int32),
+                ("r_cutback_min", C.c_double), ("r_cutback", C.c_double), ("r_increase", C.c_double),
+                ("eps_rel_state", C.c_double), ("N_iter_state_max", C.c_int32), ("N_iter_Lp_max", C.c_int32),
+                ("r_linesearch_Lp", C.c_double), ("eps_rel_Lp", C.c_double), ("eps_abs_Lp", C.c_double),
+                ("f_update_jacobi_Lp", C.c_int32), ("cref_rule", C.c_int32)]
+
+
+class LoadCase(C.Structure):
+    _fields_ = [("def_type", C.c_int32), ("def_values", C.c_double * 9), ("def_mask", C.c_int32 * 9),
+                ("stress_type", C.c_int32), ("stress_values", C.c_double * 9),
+                ("stress_mask", C.c_int32 * 9), ("q_R", C.c_double * 4), ("t", C.c_double),
+                ("r", C.c_double), ("N", C.c_int32), ("f_out", C.c_int32), ("estimate_rate", C.c_int32)]
+
+
+EXPORTS = ["dmk_mech_init", "dmk_mech_forward", "dmk_mech_materialpoint_forward", "dmk_mech_solution",
+           "dmk_mech_update_coords", "dmk_constitutive", "dmk_constitutive_device", "dmk_mech_get_field",
+           "dmk_mech_set_field", "dmk_mech_get_scalars", "dmk_mech_set_scalars", "dmk_grid_run_loadcase",
+           "dmk_mech_counters", "dmk_mech_destroy", "dmk_last_error", "dmk_nsl", "dmk_device_synchronize",
+           "dmk_gamma_convolution", "dmk_divergence_rms", "dmk_fft_forward", "dmk_fft_backward"]
+
+_lib = None
+
+
+def load(path: str = LIB_PATH):
+    """Load the HIP library (raises if it is not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError(f"{path} is missing: build it with `python -m damask_amd.build` "
+                           "(the HIP library is the only compute path)")
+    L = C.CDLL(path)
+    vp, dp, ip = C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int32)
+    L.dmk_mech_init.restype = C.c_int
+    L.dmk_mech_init.argtypes = [ip, dp, C.c_int32, C.POINTER(PhaseDesc), ip, dp, C.POINTER(Numerics),
+                                C.POINTER(vp)]
+    L.dmk_mech_forward.restype = C.c_int
+    L.dmk_mech_forward.argtypes = [vp, C.c_int32, C.c_int32, C.c_double, C.c_double, C.c_double,
+                                   C.POINTER(LoadCase)]
+    L.dmk_mech_materialpoint_forward.restype = C.c_int
+    L.dmk_mech_materialpoint_forward.argtypes = [vp]
+    L.dmk_mech_solution.restype = C.c_int
+    L.dmk_mech_solution.argtypes = [vp, ip, ip, ip]
+    L.dmk_mech_update_coords.restype = C.c_int
+    L.dmk_mech_update_coords.argtypes = [vp, dp, dp]
+    L.dmk_constitutive.restype = C.c_int
+    L.dmk_constitutive.argtypes = [vp, C.c_double, dp, dp, dp, dp, dp, ip]
+    L.dmk_constitutive_device.restype = C.c_int
+    L.dmk_constitutive_device.argtypes = [vp, C.c_double, C.c_void_p, C.c_void_p, ip]
+    L.dmk_mech_get_field.restype = C.c_int
+    L.dmk_mech_get_field.argtypes = [vp, C.c_int32, dp]
+    L.dmk_mech_set_field.restype = C.c_int
+    L.dmk_mech_set_field.argtypes = [vp, C.c_int32, dp]
+    L.dmk_mech_get_scalars.restype = C.c_int
+    L.dmk_mech_get_scalars.argtypes = [vp, C.c_int32, dp]
+    L.dmk_mech_set_scalars.restype = C.c_int
+    L.dmk_mech_set_scalars.argtypes = [vp, C.c_int32, dp]
+    L.dmk_grid_run_loadcase.restype = C.c_int
+    L.dmk_grid_run_loadcase.argtypes = [vp, C.POINTER(LoadCase), dp, dp, dp, dp, C.POINTER(C.c_int64)]
+    L.dmk_mech_counters.restype = C.c_int
+    L.dmk_mech_counters.argtypes = [vp, C.POINTER(C.c_int64)]
+    L.dmk_mech_destroy.restype = None
+    L.dmk_mech_destroy.argtypes = [vp]
+    L.dmk_last_error.restype = C.c_char_p
+    L.dmk_last_error.argtypes = []
+    L.dmk_nsl.restype = C.c_int
+    L.dmk_nsl.argtypes = [vp]
+    L.dmk_device_synchronize.restype = C.c_int
+    L.dmk_gamma_convolution.restype = C.c_int
+    L.dmk_gamma_convolution.argtypes = [vp, dp, dp, dp]
+    L.dmk_divergence_rms.restype = C.c_int
+    L.dmk_divergence_rms.argtypes = [vp, dp, dp]
+    L.dmk_fft_forward.restype = C.c_int
+    L.dmk_fft_forward.argtypes = [vp, C.c_int32, dp, dp]
+    L.dmk_fft_backward.restype = C.c_int
+    L.dmk_fft_backward.argtypes = [vp, C.c_int32, dp, dp]
+    _lib = L
+    return L
+
+
+class DamaskHipError(RuntimeError):
+    pass
+
+
+def check(rc: int, what: str = ""):
+    if rc != 0:
+        msg = load().dmk_last_error().decode(errors="replace")
+        raise DamaskHipError(f"{what} failed ({rc}): {msg}")
+
+
+def dptr(a: np.ndarray):
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def iptr(a: np.ndarray):
+    assert a.dtype == np.int32 and a.flags.c_contiguous
+    return a.ctypes.data_as(C.POINTER(C.c_int32))

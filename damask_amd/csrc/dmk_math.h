@@ -1,0 +1,134 @@
+// Register-resident 3x3 algebra for the per-cell kernels (row-major: T[3 i + j] = T_ij).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <float.h>
+
+#define DI __device__ __forceinline__
+
+DI void m_load(const double* __restrict__ f, int N, int e, double* T) {
+#pragma unroll
+  for (int i = 0; i < 3; i++)
+#pragma unroll
+    for (int j = 0; j < 3; j++) T[3 * i + j] = f[(size_t)(i + 3 * j) * N + e];
+}
+
+DI void m_store(double* __restrict__ f, int N, int e, const double* T) {
+#pragma unroll
+  for (int i = 0; i < 3; i++)
+#pragma unroll
+    for (int j = 0; j < 3; j++) f[(size_t)(i + 3 * j) * N + e] = T[3 * i + j];
+}
+
+DI void m_mul(const double* A, const double* B, double* C) {
+  double T[9];
+#pragma unroll
+  for (int i = 0; i < 3; i++)
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+      double s = A[3 * i] * B[j];
+      s = fma(A[3 * i + 1], B[3 + j], s);
+      s = fma(A[3 * i + 2], B[6 + j], s);
+      T[3 * i + j] = s;
+    }
+#pragma unroll
+  for (int k = 0; k < 9; k++) C[k] = T[k];
+}
+
+// C = A^T B
+DI void m_mul_tn(const double* A, const double* B, double* C) {
+  double T[9];
+#pragma unroll
+  for (int i = 0; i < 3; i++)
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+      double s = A[i] * B[j];
+      s = fma(A[3 + i], B[3 + j], s);
+      s = fma(A[6 + i], B[6 + j], s);
+      T[3 * i + j] = s;
+    }
+#pragma unroll
+  for (int k = 0; k < 9; k++) C[k] = T[k];
+}
+
+// C = A B^T
+DI void m_mul_nt(const double* A, const double* B, double* C) {
+  double T[9];
+#pragma unroll
+  for (int i = 0; i < 3; i++)
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+      double s = A[3 * i] * B[3 * j];
+      s = fma(A[3 * i + 1], B[3 * j + 1], s);
+      s = fma(A[3 * i + 2], B[3 * j + 2], s);
+      T[3 * i + j] = s;
+    }
+#pragma unroll
+  for (int k = 0; k < 9; k++) C[k] = T[k];
+}
+
+DI void m_transpose(const double* A, double* B) {
+  double T[9];
+#pragma unroll
+  for (int i = 0; i < 3; i++)
+#pragma unroll
+    for (int j = 0; j < 3; j++) T[3 * j + i] = A[3 * i + j];
+#pragma unroll
+  for (int k = 0; k < 9; k++) B[k] = T[k];
+}
+
+// math_det33 (src/math.f90:630-638)
+DI double m_det(const double* m) {
+  return m[0] * (m[4] * m[8] - m[5] * m[7]) - m[1] * (m[3] * m[8] - m[5] * m[6])
+       + m[2] * (m[3] * m[7] - m[4] * m[6]);
+}
+
+DI bool dEq0(double a) { return fabs(a) <= DBL_MIN * 10.0; }
+
+// math_invert33 (src/math.f90:471-505): Cramer; returns true on error (zeros out)
+DI bool m_invert(const double* A, double* inv) {
+  double c00 = A[4] * A[8] - A[5] * A[7];
+  double c10 = -A[3] * A[8] + A[5] * A[6];
+  double c20 = A[3] * A[7] - A[4] * A[6];
+  double det = A[0] * c00 + A[1] * c10 + A[2] * c20;
+  if (dEq0(det)) {
+#pragma unroll
+    for (int k = 0; k < 9; k++) inv[k] = 0.0;
+    return true;
+  }
+  double c01 = -A[1] * A[8] + A[2] * A[7];
+  double c11 = A[0] * A[8] - A[2] * A[6];
+  double c21 = -A[0] * A[7] + A[1] * A[6];
+  double c02 = A[1] * A[5] - A[2] * A[4];
+  double c12 = -A[0] * A[5] + A[2] * A[3];
+  double c22 = A[0] * A[4] - A[1] * A[3];
+  double r = 1.0 / det;
+  inv[0] = c00 * r; inv[1] = c01 * r; inv[2] = c02 * r;
+  inv[3] = c10 * r; inv[4] = c11 * r; inv[5] = c12 * r;
+  inv[6] = c20 * r; inv[7] = c21 * r; inv[8] = c22 * r;
+  return false;
+}
+
+DI double m_norm(const double* A) {
+  double s = 0.0;
+#pragma unroll
+  for (int k = 0; k < 9; k++) s = fma(A[k], A[k], s);
+  return sqrt(s);
+}
+
+// C : X for cubic stiffness in the lattice frame, X symmetric, through the
+// Mandel vector (src/math.f90:792-820 and src/crystal.f90:1711-1759)
+DI void hooke_sym(double C11, double C12, double C44, const double* X, double* S) {
+  const double W = 1.4142135623730951;          // sqrt(2)
+  const double IW = 0.7071067811865475;         // 1/sqrt(2)
+  double v1 = X[0], v2 = X[4], v3 = X[8];
+  double v4 = X[7] * W, v5 = X[6] * W, v6 = X[3] * W;
+  double s1 = C11 * v1 + C12 * v2 + C12 * v3;
+  double s2 = C12 * v1 + C11 * v2 + C12 * v3;
+  double s3 = C12 * v1 + C12 * v2 + C11 * v3;
+  double G = C44 * 2.0;
+  double s4 = G * v4, s5 = G * v5, s6 = G * v6;
+  S[0] = s1;      S[1] = s6 * IW; S[2] = s5 * IW;
+  S[3] = s6 * IW; S[4] = s2;      S[5] = s4 * IW;
+  S[6] = s5 * IW; S[7] = s4 * IW; S[8] = s3;
+}

@@ -1,0 +1,43 @@
+// Spectral-operator interfaces (FFT plans and kernel arguments).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+struct FftPlan {
+  int L;
+  int nstages;
+  int radix[16];
+};
+
+struct DmkFFT {
+  int n[3];
+  int h;
+  FftPlan plan[3];
+  double2* tw[3] = {nullptr, nullptr, nullptr};
+};
+
+struct GammaArgs {
+  int n[3];
+  int h;
+  double size[3];
+  double rg[3];            // geomSize / scaledGeomSize
+  double Ccoef[9][6];      // acoustic-tensor coefficients of C_ref
+  double aim[9];           // DC value, column-major
+  int do_div, do_gamma;
+};
+
+struct CoordsArgs {
+  int n[3];
+  double step[3];
+  double Favg[9];          // column-major
+  double wgt;
+};
+
+int dmk_fft_setup(DmkFFT* fft, const int cells[3]);
+void dmk_fft_free(DmkFFT* fft);
+int dmk_fft_r2c(const DmkFFT* fft, int howmany, const double* in, double2* out, hipStream_t s);
+int dmk_fft_c2r(const DmkFFT* fft, int howmany, double2* in, double* out, hipStream_t s);
+int dmk_gamma_div(double2* Z, const GammaArgs* g, double* div_partial, hipStream_t s);
+int dmk_gamma_div_blocks(const GammaArgs* g);
+int dmk_coords_freq(const double2* Fh, double2* U, const GammaArgs* g, hipStream_t s);
+int dmk_coords_nodes(const double* u, const CoordsArgs* c, double* x_n, double* x_p, hipStream_t s);

@@ -1,0 +1,297 @@
+// Field reductions and vector kernels of the solver (HBM-bound, FP64).
+//
+// Reductions use a fixed partition (2048 elements per block, tree reduction in
+// LDS, then one block per result), so results are bitwise reproducible from
+// run to run.
+#include "dmk_common.h"
+#include "dmk_linalg.h"
+
+#include <float.h>
+
+#define RB 256
+#define PER_BLOCK 2048
+
+namespace {
+
+__device__ double block_sum(double v) {
+  __shared__ double red[RB];
+  red[threadIdx.x] = v;
+  __syncthreads();
+  for (int s = RB / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  double r = red[0];
+  __syncthreads();
+  return r;
+}
+
+// column sums: blockIdx.y = column (base + col * ld), blockIdx.x = chunk
+__global__ void __launch_bounds__(RB) colsum_partial(const double* __restrict__ base, long ld, long n,
+                                                     double* __restrict__ partial, int nchunk) {
+  const double* x = base + (long)blockIdx.y * ld;
+  long s = (long)blockIdx.x * PER_BLOCK;
+  long e = min(s + PER_BLOCK, n);
+  double acc = 0.0;
+  for (long i = s + threadIdx.x; i < e; i += RB) acc += x[i];
+  double r = block_sum(acc);
+  if (threadIdx.x == 0) partial[(long)blockIdx.y * nchunk + blockIdx.x] = r;
+}
+
+__global__ void __launch_bounds__(RB) finalize_sum(const double* __restrict__ partial, int nchunk,
+                                                   double scale, double* __restrict__ out) {
+  const double* p = partial + (long)blockIdx.x * nchunk;
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < nchunk; i += RB) acc += p[i];
+  double r = block_sum(acc);
+  if (threadIdx.x == 0) out[blockIdx.x] = r * scale;
+}
+
+__global__ void __launch_bounds__(RB) dot_partial(const double* __restrict__ x, const double* __restrict__ Y, long ldY,
+                                                  int mode, long n, double* __restrict__ partial, int nchunk) {
+  const double* y = Y + (long)blockIdx.y * ldY;
+  long s = (long)blockIdx.x * PER_BLOCK;
+  long e = min(s + PER_BLOCK, n);
+  double acc = 0.0;
+  if (mode == 0) {
+    for (long i = s + threadIdx.x; i < e; i += RB) acc = fma(x[i], y[i], acc);
+  } else {
+    for (long i = s + threadIdx.x; i < e; i += RB) { double d = y[i] - x[i]; acc = fma(d, d, acc); }
+  }
+  double r = block_sum(acc);
+  if (threadIdx.x == 0) partial[(long)blockIdx.y * nchunk + blockIdx.x] = r;
+}
+
+__global__ void axpby_kernel(long n, double a, const double* __restrict__ x, double b, const double* __restrict__ y,
+                             double* __restrict__ z) {
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) z[i] = a * x[i] + b * y[i];
+}
+
+__global__ void combine_kernel(long n, const double* __restrict__ XM, const double* __restrict__ Xh, long ldX,
+                               CombineArgs c, double* __restrict__ XA) {
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double v = XM[i] * (1.0 - c.atot);
+  for (int k = 0; k < c.l; k++) v = fma(c.alpha[k], Xh[k * ldX + i], v);
+  XA[i] = v;
+}
+
+// per cell squared Frobenius norm of dPdF, summed in Fortran element order
+__global__ void dpdf_norm_kernel(const double* __restrict__ dPdF, long N, double* __restrict__ nrm) {
+  long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= N) return;
+  double s = 0.0;
+  for (int q = 0; q < 81; q++) { double v = dPdF[q * N + e]; s = fma(v, v, s); }
+  nrm[e] = s;
+}
+
+__global__ void __launch_bounds__(RB) minmax_partial(const double* __restrict__ nrm, long N, double* __restrict__ pmax,
+                                                     double* __restrict__ pmin) {
+  __shared__ double smax[RB], smin[RB];
+  long s = (long)blockIdx.x * PER_BLOCK, e = min(s + PER_BLOCK, N);
+  double mx = 0.0, mn = DBL_MAX;
+  for (long i = s + threadIdx.x; i < e; i += RB) {
+    double v = nrm[i];
+    if (v > mx) mx = v;
+    if (v < mn) mn = v;
+  }
+  smax[threadIdx.x] = mx; smin[threadIdx.x] = mn;
+  __syncthreads();
+  for (int k = RB / 2; k > 0; k >>= 1) {
+    if (threadIdx.x < k) {
+      smax[threadIdx.x] = fmax(smax[threadIdx.x], smax[threadIdx.x + k]);
+      smin[threadIdx.x] = fmin(smin[threadIdx.x], smin[threadIdx.x + k]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) { pmax[blockIdx.x] = smax[0]; pmin[blockIdx.x] = smin[0]; }
+}
+
+// global extrema from the block partials, then the first cell satisfying the
+// (banded) extremum rule; one block, writes sel[0] = imax, sel[1] = imin
+__global__ void __launch_bounds__(RB) select_extrema(const double* __restrict__ nrm, long N, const double* __restrict__ pmax,
+                                                     const double* __restrict__ pmin, int nchunk, double band,
+                                                     long* __restrict__ sel, double* __restrict__ ext) {
+  __shared__ double smax[RB], smin[RB];
+  __shared__ long simax[RB], simin[RB];
+  double mx = 0.0, mn = DBL_MAX;
+  for (int i = threadIdx.x; i < nchunk; i += RB) { mx = fmax(mx, pmax[i]); mn = fmin(mn, pmin[i]); }
+  smax[threadIdx.x] = mx; smin[threadIdx.x] = mn;
+  __syncthreads();
+  for (int k = RB / 2; k > 0; k >>= 1) {
+    if (threadIdx.x < k) {
+      smax[threadIdx.x] = fmax(smax[threadIdx.x], smax[threadIdx.x + k]);
+      smin[threadIdx.x] = fmin(smin[threadIdx.x], smin[threadIdx.x + k]);
+    }
+    __syncthreads();
+  }
+  double gmax = smax[0], gmin = smin[0];
+  __syncthreads();
+  long imax = N, imin = N;
+  for (long i = threadIdx.x; i < N; i += RB) {
+    double v = nrm[i];
+    if (imax == N && gmax > 0.0 && v * (1.0 + band) >= gmax) imax = i;
+    if (imin == N && gmin < DBL_MAX && v * (1.0 - band) <= gmin) imin = i;
+    if (imax != N && imin != N) break;
+  }
+  simax[threadIdx.x] = imax; simin[threadIdx.x] = imin;
+  __syncthreads();
+  for (int k = RB / 2; k > 0; k >>= 1) {
+    if (threadIdx.x < k) {
+      simax[threadIdx.x] = min(simax[threadIdx.x], simax[threadIdx.x + k]);
+      simin[threadIdx.x] = min(simin[threadIdx.x], simin[threadIdx.x + k]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    sel[0] = simax[0] < N ? simax[0] : -1;
+    sel[1] = simin[0] < N ? simin[0] : -1;
+    ext[0] = gmax;
+    ext[1] = gmin;
+  }
+}
+
+__global__ void gather_cminmax(const double* __restrict__ dPdF, long N, const long* __restrict__ sel,
+                               double* __restrict__ out) {
+  int q = threadIdx.x;
+  if (q >= 81) return;
+  double mx = sel[0] >= 0 ? dPdF[q * N + sel[0]] : 0.0;
+  double mn = sel[1] >= 0 ? dPdF[q * N + sel[1]] : DBL_MAX;
+  out[q] = 0.5 * (mx + mn);
+}
+
+// flag[0] = ST_OK if every cell converged, ST_FAIL_MECH otherwise; flag[1..2]
+// accumulate Newton and state iterations
+__global__ void __launch_bounds__(RB) status_kernel(const int32_t* __restrict__ st, const int32_t* __restrict__ nn,
+                                                    const int32_t* __restrict__ nf, long N, int* __restrict__ flag,
+                                                    unsigned long long* __restrict__ cnt) {
+  __shared__ int bad;
+  __shared__ unsigned long long s1[RB], s2[RB];
+  if (threadIdx.x == 0) bad = 0;
+  __syncthreads();
+  unsigned long long a = 0, b = 0;
+  int mybad = 0;
+  for (long i = threadIdx.x; i < N; i += RB) {
+    if (st[i] != ST_OK) mybad = 1;
+    a += (unsigned long long)nn[i];
+    b += (unsigned long long)nf[i];
+  }
+  if (mybad) atomicOr(&bad, 1);
+  s1[threadIdx.x] = a; s2[threadIdx.x] = b;
+  __syncthreads();
+  for (int k = RB / 2; k > 0; k >>= 1) {
+    if (threadIdx.x < k) { s1[threadIdx.x] += s1[threadIdx.x + k]; s2[threadIdx.x] += s2[threadIdx.x + k]; }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    flag[0] = bad ? ST_FAIL_MECH : ST_OK;
+    cnt[0] += s1[0];
+    cnt[1] += s2[0];
+  }
+}
+
+// forwardTensorField / calculateRate (grid_utilities.f90:245-297), component-major
+// F and Fnew may alias (element-wise update)
+__global__ void forward_field_kernel(long N, int guess, double dt_old, double dt, ForwardArgs fa,
+                                     const double* F, double* __restrict__ F_lastinc,
+                                     double* __restrict__ Fdot, double* Fnew) {
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= 9 * N) return;
+  int c = (int)(i / N);
+  if (!fa.cutback) {
+    Fdot[i] = guess ? (F[i] - F_lastinc[i]) / dt_old : fa.avRate[c];
+    F_lastinc[i] = F[i];
+  }
+  Fnew[i] = F_lastinc[i] + Fdot[i] * dt;
+}
+
+__global__ void shift_kernel(long N, ForwardArgs fa, double* __restrict__ F) {
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= 9 * N) return;
+  F[i] -= fa.shift[i / N];
+}
+
+// (9,N) cell-major <-> component-major
+__global__ void aos_to_soa(const double* __restrict__ a, long N, int w, double* __restrict__ s) {
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (long)w * N) return;
+  long e = i / w; int c = (int)(i % w);
+  s[c * N + e] = a[i];
+}
+__global__ void soa_to_aos(const double* __restrict__ s, long N, int w, double* __restrict__ a) {
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (long)w * N) return;
+  long e = i / w; int c = (int)(i % w);
+  a[i] = s[c * N + e];
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------- host wrappers
+int dmk_nchunk(long n) { return (int)((n + PER_BLOCK - 1) / PER_BLOCK); }
+
+int dmk_colsum(const double* base, long ld, long n, int ncols, double* partial, double* out, double scale,
+               hipStream_t s) {
+  int nc = dmk_nchunk(n);
+  hipLaunchKernelGGL(colsum_partial, dim3(nc, ncols), dim3(RB), 0, s, base, ld, n, partial, nc);
+  hipLaunchKernelGGL(finalize_sum, dim3(ncols), dim3(RB), 0, s, partial, nc, scale, out);
+  return hipGetLastError() == hipSuccess ? 0 : 2;
+}
+
+int dmk_mdot(const double* x, const double* Y, long ldY, int l, int mode, long n, double* partial, double* out,
+             hipStream_t s) {
+  int nc = dmk_nchunk(n);
+  hipLaunchKernelGGL(dot_partial, dim3(nc, l), dim3(RB), 0, s, x, Y, ldY, mode, n, partial, nc);
+  hipLaunchKernelGGL(finalize_sum, dim3(l), dim3(RB), 0, s, partial, nc, 1.0, out);
+  return hipGetLastError() == hipSuccess ? 0 : 2;
+}
+
+int dmk_axpby(long n, double a, const double* x, double b, const double* y, double* z, hipStream_t s) {
+  hipLaunchKernelGGL(axpby_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, a, x, b, y, z);
+  return hipGetLastError() == hipSuccess ? 0 : 2;
+}
+
+int dmk_combine(long n, const double* XM, const double* Xh, long ldX, const CombineArgs* c, double* XA,
+                hipStream_t s) {
+  hipLaunchKernelGGL(combine_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, XM, Xh, ldX, *c, XA);
+  return hipGetLastError() == hipSuccess ? 0 : 2;
+}
+
+int dmk_cminmax(const double* dPdF, long N, double band, double* nrm, double* pmax, double* pmin, long* sel,
+                double* ext, double* out, hipStream_t s) {
+  hipLaunchKernelGGL(dpdf_norm_kernel, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, s, dPdF, N, nrm);
+  int nc = dmk_nchunk(N);
+  hipLaunchKernelGGL(minmax_partial, dim3(nc), dim3(RB), 0, s, nrm, N, pmax, pmin);
+  hipLaunchKernelGGL(select_extrema, dim3(1), dim3(RB), 0, s, nrm, N, pmax, pmin, nc, band, sel, ext);
+  hipLaunchKernelGGL(gather_cminmax, dim3(1), dim3(128), 0, s, dPdF, N, sel, out);
+  return hipGetLastError() == hipSuccess ? 0 : 2;
+}
+
+int dmk_status(const int32_t* st, const int32_t* nn, const int32_t* nf, long N, int* flag, unsigned long long* cnt,
+               hipStream_t s) {
+  hipLaunchKernelGGL(status_kernel, dim3(1), dim3(RB), 0, s, st, nn, nf, N, flag, cnt);
+  return hipGetLastError() == hipSuccess ? 0 : 2;
+}
+
+int dmk_forward_field(long N, int guess, double dt_old, double dt, const ForwardArgs* fa, const double* F,
+                      double* F_lastinc, double* Fdot, double* Fnew, hipStream_t s) {
+  hipLaunchKernelGGL(forward_field_kernel, dim3((unsigned)((9 * N + 255) / 256)), dim3(256), 0, s, N, guess, dt_old,
+                     dt, *fa, F, F_lastinc, Fdot, Fnew);
+  return hipGetLastError() == hipSuccess ? 0 : 2;
+}
+
+int dmk_shift_field(long N, const ForwardArgs* fa, double* F, hipStream_t s) {
+  hipLaunchKernelGGL(shift_kernel, dim3((unsigned)((9 * N + 255) / 256)), dim3(256), 0, s, N, *fa, F);
+  return hipGetLastError() == hipSuccess ? 0 : 2;
+}
+
+int dmk_aos_to_soa(const double* a, long N, int w, double* soa, hipStream_t s) {
+  hipLaunchKernelGGL(aos_to_soa, dim3((unsigned)((w * N + 255) / 256)), dim3(256), 0, s, a, N, w, soa);
+  return hipGetLastError() == hipSuccess ? 0 : 2;
+}
+
+int dmk_soa_to_aos(const double* soa, long N, int w, double* a, hipStream_t s) {
+  hipLaunchKernelGGL(soa_to_aos, dim3((unsigned)((w * N + 255) / 256)), dim3(256), 0, s, soa, N, w, a);
+  return hipGetLastError() == hipSuccess ? 0 : 2;
+}

@@ -1,0 +1,1042 @@
+// Host side of the MI355X spectral-basic solver and its C-ABI.
+//
+// Restates src/grid/grid_mech_spectral_basic.f90 (init :103-237, solution
+// :243-277, forward :284-355, converged :422-458, form_residual :464-521),
+// src/grid/grid_utilities.f90 (maskedCompliance :74-127, constitutiveResponse
+// :133-207, calculateRate :245-264, forwardTensorField :271-297) and the
+// increment/cutback loop of src/grid/DAMASK_grid.f90:288-460.  Every field
+// lives on the GPU; the host keeps the 3x3 / 9x9 bookkeeping, the small NGMRES
+// least-squares problem and the convergence logic.
+//
+// The nonlinear solver is the reference's choice, PETSc SNES "ngmres"
+// (grid_mech_spectral_basic.f90:150), restated from the published method
+// (Oosterlee & Washio, SIAM J. Sci. Comput. 21(5) 2000; Brune, Knepley, Smith,
+// Tu, SIAM Review 57(4) 2015, sec. 4.2) with PETSc's default parameters:
+// m = 30, x^M = x - r(x) (unit basic line search), difference-based selection
+// (gamma_A = 2, epsilon_B = 0.1, delta_B = 0.9) and restart (gamma_C = 2,
+// after 2 consecutive restart conditions), max 50 iterations.
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+#include "dmk_common.h"
+#include "dmk_linalg.h"
+#include "dmk_spectral.h"
+#include "../../include/damask_hip.h"
+
+int dmk_build_phase(const dmk_phase_desc* d, DmkPhaseDev* p);
+void dmk_qu2om(const double* qu, double* om);
+
+static thread_local char g_err[512];
+
+void dmk_set_error(const char* fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof g_err, fmt, ap);
+  va_end(ap);
+}
+
+extern "C" const char* dmk_last_error(void) { return g_err; }
+
+#define CK(x)                                                                            \
+  do {                                                                                   \
+    hipError_t e_ = (x);                                                                 \
+    if (e_ != hipSuccess) {                                                              \
+      dmk_set_error("%s:%d %s: %s", __FILE__, __LINE__, #x, hipGetErrorString(e_));      \
+      return DMK_ERR_HIP;                                                                \
+    }                                                                                    \
+  } while (0)
+#define CKL(x)                                                                           \
+  do {                                                                                   \
+    int r_ = (x);                                                                        \
+    if (r_ != 0) {                                                                       \
+      dmk_set_error("%s:%d kernel launch %s failed (%d): %s", __FILE__, __LINE__, #x, r_, \
+                    hipGetErrorString(hipGetLastError()));                                \
+      return DMK_ERR_HIP;                                                                \
+    }                                                                                    \
+  } while (0)
+
+// ------------------------------------------------------------------ host 3x3 helpers
+// Fortran column-major 3x3 / 3x3x3x3 on the host side
+static inline int c2(int i, int j) { return i + 3 * j; }
+static inline int c4(int i, int j, int k, int l) { return i + 3 * j + 9 * k + 27 * l; }
+
+static void rot_vec(const double* q, const double* v, double* o) {
+  double A = q[0] * q[0] - (q[1] * q[1] + q[2] * q[2] + q[3] * q[3]);
+  double B = 2.0 * (q[1] * v[0] + q[2] * v[1] + q[3] * v[2]);
+  double C = -2.0 * q[0];
+  double c[3] = {q[2] * v[2] - q[3] * v[1], q[3] * v[0] - q[1] * v[2], q[1] * v[1] - q[2] * v[0]};
+  for (int k = 0; k < 3; k++) o[k] = A * v[k] + B * q[k + 1] + C * c[k];
+}
+
+// rotations.f90:282-301 rotate3x3 (active conjugates q)
+static void rotate33(const double* q0, const double* T, bool active, double* out) {
+  double q[4] = {q0[0], active ? -q0[1] : q0[1], active ? -q0[2] : q0[2], active ? -q0[3] : q0[3]};
+  double R[9], v[3], o[3];
+  for (int i = 0; i < 3; i++) {
+    for (int j = 0; j < 3; j++) v[j] = T[c2(i, j)];
+    rot_vec(q, v, o);
+    for (int j = 0; j < 3; j++) R[c2(i, j)] = o[j];
+  }
+  for (int j = 0; j < 3; j++) {
+    for (int i = 0; i < 3; i++) v[i] = R[c2(i, j)];
+    rot_vec(q, v, o);
+    for (int i = 0; i < 3; i++) R[c2(i, j)] = o[i];
+  }
+  std::memcpy(out, R, sizeof R);
+}
+
+// rotations.f90:310-339 rotate3x3x3x3 (passive)
+static void rotate3333(const double* q, const double* T, double* out) {
+  double R[81], v[3], o[3];
+  std::memcpy(R, T, sizeof R);
+  for (int i = 0; i < 3; i++) for (int j = 0; j < 3; j++) for (int k = 0; k < 3; k++) {
+    for (int x = 0; x < 3; x++) v[x] = R[c4(i, j, k, x)];
+    rot_vec(q, v, o);
+    for (int x = 0; x < 3; x++) R[c4(i, j, k, x)] = o[x];
+  }
+  for (int i = 0; i < 3; i++) for (int j = 0; j < 3; j++) for (int l = 0; l < 3; l++) {
+    for (int x = 0; x < 3; x++) v[x] = R[c4(i, j, x, l)];
+    rot_vec(q, v, o);
+    for (int x = 0; x < 3; x++) R[c4(i, j, x, l)] = o[x];
+  }
+  for (int i = 0; i < 3; i++) for (int k = 0; k < 3; k++) for (int l = 0; l < 3; l++) {
+    for (int x = 0; x < 3; x++) v[x] = R[c4(i, x, k, l)];
+    rot_vec(q, v, o);
+    for (int x = 0; x < 3; x++) R[c4(i, x, k, l)] = o[x];
+  }
+  for (int j = 0; j < 3; j++) for (int k = 0; k < 3; k++) for (int l = 0; l < 3; l++) {
+    for (int x = 0; x < 3; x++) v[x] = R[c4(x, j, k, l)];
+    rot_vec(q, v, o);
+    for (int x = 0; x < 3; x++) R[c4(x, j, k, l)] = o[x];
+  }
+  std::memcpy(out, R, sizeof R);
+}
+
+// dense inverse by LU with partial pivoting (column-major), returns true on failure
+static bool invert_n(int n, const double* A, double* inv) {
+  std::vector<double> a(A, A + n * n);
+  std::vector<int> piv(n);
+  for (int j = 0; j < n; j++) {
+    int p = j;
+    for (int i = j + 1; i < n; i++) if (std::fabs(a[i + n * j]) > std::fabs(a[p + n * j])) p = i;
+    piv[j] = p;
+    if (a[p + n * j] == 0.0) return true;
+    if (p != j) for (int k = 0; k < n; k++) std::swap(a[j + n * k], a[p + n * k]);
+    for (int i = j + 1; i < n; i++) a[i + n * j] /= a[j + n * j];
+    for (int k = j + 1; k < n; k++)
+      for (int i = j + 1; i < n; i++) a[i + n * k] -= a[i + n * j] * a[j + n * k];
+  }
+  for (int c = 0; c < n; c++) {
+    std::vector<double> b(n, 0.0);
+    b[c] = 1.0;
+    for (int i = 0; i < n; i++) if (piv[i] != i) std::swap(b[i], b[piv[i]]);
+    for (int j = 0; j < n; j++) for (int i = j + 1; i < n; i++) b[i] -= a[i + n * j] * b[j];
+    for (int j = n - 1; j >= 0; j--) { b[j] /= a[j + n * j]; for (int i = 0; i < j; i++) b[i] -= a[i + n * j] * b[j]; }
+    for (int i = 0; i < n; i++) inv[i + n * c] = b[i];
+  }
+  return false;
+}
+
+// Jacobi eigen-decomposition based least squares of a symmetric m x m system,
+// singular values below m * eps * max treated as zero (rcond = -1 of gelss)
+static void lsq_sym(int m, const double* H, const double* b, double* x) {
+  std::vector<double> A(H, H + m * m), V(m * m, 0.0);
+  for (int i = 0; i < m; i++) V[i + m * i] = 1.0;
+  for (int sweep = 0; sweep < 100; sweep++) {
+    double off = 0.0;
+    for (int p = 0; p < m; p++) for (int q = p + 1; q < m; q++) off += A[p + m * q] * A[p + m * q];
+    if (off < 1e-300) break;
+    for (int p = 0; p < m; p++)
+      for (int q = p + 1; q < m; q++) {
+        double apq = A[p + m * q];
+        if (apq == 0.0) continue;
+        double th = (A[q + m * q] - A[p + m * p]) / (2.0 * apq);
+        double t = (th >= 0 ? 1.0 : -1.0) / (std::fabs(th) + std::sqrt(th * th + 1.0));
+        double c = 1.0 / std::sqrt(t * t + 1.0), s = t * c;
+        for (int k = 0; k < m; k++) {
+          double akp = A[k + m * p], akq = A[k + m * q];
+          A[k + m * p] = c * akp - s * akq;
+          A[k + m * q] = s * akp + c * akq;
+        }
+        for (int k = 0; k < m; k++) {
+          double apk = A[p + m * k], aqk = A[q + m * k];
+          A[p + m * k] = c * apk - s * aqk;
+          A[q + m * k] = s * apk + c * aqk;
+        }
+        for (int k = 0; k < m; k++) {
+          double vkp = V[k + m * p], vkq = V[k + m * q];
+          V[k + m * p] = c * vkp - s * vkq;
+          V[k + m * q] = s * vkp + c * vkq;
+        }
+      }
+  }
+  double smax = 0.0;
+  for (int i = 0; i < m; i++) smax = std::fmax(smax, std::fabs(A[i + m * i]));
+  double cut = smax * 2.220446049250313e-16 * m;
+  for (int i = 0; i < m; i++) x[i] = 0.0;
+  for (int k = 0; k < m; k++) {
+    double lam = A[k + m * k];
+    if (std::fabs(lam) <= cut) continue;
+    double c = 0.0;
+    for (int i = 0; i < m; i++) c += V[i + m * k] * b[i];
+    c /= lam;
+    for (int i = 0; i < m; i++) x[i] += c * V[i + m * k];
+  }
+}
+
+// ------------------------------------------------------------------ handle
+struct dmk_handle {
+  int cells[3];
+  double size[3];
+  long N, Nk, n9;
+  int h;
+  double wgt;
+  hipStream_t st = nullptr;
+  DmkFFT fft;
+  int n_phases;
+  std::vector<DmkPhaseDev> ph;
+  DmkPhaseDev* ph_dev = nullptr;
+  int nsl_max;
+  DmkNumMech nm;
+  dmk_numerics num;
+  DmkMPDev mp;
+  std::vector<void*> allocs;
+  double *P_hom, *dPdF;
+  double *F, *F_lastinc, *Fdot, *R, *XM, *FM, *XA, *FA, *Xh, *Fh, *Fext, *Pext;
+  double2 *Z, *U;
+  double *u3, *xn_dev, *xp_dev;
+  double *partial, *small, *nrm, *pmax, *pmin, *ext;
+  long* sel;
+  int* flag;
+  unsigned long long* cnt;
+  double* hsmall = nullptr;          // pinned host mirror of small results
+  // solver state
+  double F_aimDot[9], F_aim[9], F_aim_lastinc[9], P_av[9], P_aim[9];
+  double C_volAvg[81], C_volAvgLastInc[81], C_minMaxAvg[81], C_minMaxAvgLastInc[81], S[81];
+  double err_div = 0, err_BC = 0;
+  int status = 0, totalIter = 0, nfuncs = 0;
+  int stress_mask[9];
+  double q_R[4];
+  double dt = 0;
+  GammaArgs ga;
+  std::vector<double> Q;
+  // driver state (DAMASK_grid.f90:78-100)
+  double t = 0.0, Delta_t = 1.0, Delta_t_prev = 0.0;
+  int cutBack = 0, cutBackLevel = 0, totalIncs = 0;
+  int64_t n_updates = 0, n_residuals = 0;
+};
+
+template <typename T>
+static int dalloc(dmk_handle* h, T** p, size_t n) {
+  void* v = nullptr;
+  CK(hipMalloc(&v, sizeof(T) * (n ? n : 1)));
+  CK(hipMemsetAsync(v, 0, sizeof(T) * (n ? n : 1), h->st));
+  h->allocs.push_back(v);
+  *p = (T*)v;
+  return 0;
+}
+
+enum { SM_PAV = 0, SM_CVOL = 9, SM_CMM = 90, SM_DIV = 171, SM_DOTS = 180, SM_SIZE = 256 };
+
+static void set_gamma_coef(dmk_handle* h, const double* C) {
+  // A_lm = sum_jq C_ref(l,j,m,q) a_j a_q with C_ref = C / wgt; products a_j a_q
+  // ordered (00, 11, 22, 12, 02, 01)
+  static const int pj[6] = {0, 1, 2, 1, 0, 0}, pq[6] = {0, 1, 2, 2, 2, 1};
+  for (int l = 0; l < 3; l++)
+    for (int m = 0; m < 3; m++)
+      for (int p = 0; p < 6; p++) {
+        int j = pj[p], q = pq[p];
+        double v = C[c4(l, j, m, q)] / h->wgt;
+        if (j != q) v += C[c4(l, q, m, j)] / h->wgt;
+        h->ga.Ccoef[3 * l + m][p] = v;
+      }
+}
+
+// utilities_constitutiveResponse on the device field F (component-major)
+static int constitutive(dmk_handle* h, const double* F, double dt, bool rotate) {
+  for (int p = 0; p < h->n_phases; p++)
+    CKL(dmk_launch_stress(&h->mp, F, h->ph_dev + p, h->ph[p].nsl, p, &h->nm, dt, h->st));
+  CKL(dmk_status(h->mp.status, h->mp.n_newton, h->mp.n_fpi, h->N, h->flag, h->cnt, h->st));
+  for (int p = 0; p < h->n_phases; p++)
+    CKL(dmk_launch_tangent(&h->mp, h->ph_dev + p, h->ph[p].nsl, p, dt, h->flag, h->P_hom, h->dPdF, h->st));
+  CKL(dmk_colsum(h->P_hom, h->N, h->N, 9, h->partial, h->small + SM_PAV, h->wgt, h->st));
+  CKL(dmk_colsum(h->dPdF, h->N, h->N, 81, h->partial, h->small + SM_CVOL, h->wgt, h->st));
+  CKL(dmk_cminmax(h->dPdF, h->N, h->num.cref_rule == 1 ? 1.0e-12 : 0.0, h->nrm, h->pmax, h->pmin, h->sel, h->ext,
+                  h->small + SM_CMM, h->st));
+  CK(hipMemcpyAsync(h->hsmall, h->small, sizeof(double) * SM_DIV, hipMemcpyDeviceToHost, h->st));
+  int flag = 0;
+  CK(hipMemcpyAsync(&flag, h->flag, sizeof(int), hipMemcpyDeviceToHost, h->st));
+  CK(hipStreamSynchronize(h->st));
+  h->status = flag;
+  std::memcpy(h->P_av, h->hsmall + SM_PAV, sizeof h->P_av);
+  if (rotate) rotate33(h->q_R, h->P_av, false, h->P_av);
+  std::memcpy(h->C_volAvg, h->hsmall + SM_CVOL, sizeof h->C_volAvg);
+  std::memcpy(h->C_minMaxAvg, h->hsmall + SM_CMM, sizeof h->C_minMaxAvg);
+  h->n_updates += h->N;
+  return 0;
+}
+
+// grid_utilities.f90:74-127 utilities_maskedCompliance
+static int masked_compliance(dmk_handle* h) {
+  int msv[9], sel[9], n = 0;
+  for (int a = 0; a < 3; a++)
+    for (int b = 0; b < 3; b++) msv[3 * a + b] = !h->stress_mask[c2(a, b)];
+  for (int i = 0; i < 9; i++) if (msv[i]) sel[n++] = i;
+  double T99[81] = {0};
+  if (n > 0) {
+    double Cr[81], C99[81], c[81], s[81];
+    rotate3333(h->q_R, h->C_volAvg, Cr);
+    for (int a = 0; a < 3; a++) for (int b = 0; b < 3; b++) for (int cc = 0; cc < 3; cc++) for (int d = 0; d < 3; d++)
+      C99[(3 * a + b) + 9 * (3 * cc + d)] = Cr[c4(a, b, cc, d)];
+    for (int q = 0; q < n; q++) for (int p = 0; p < n; p++) c[p + n * q] = C99[sel[p] + 9 * sel[q]];
+    bool err = invert_n(n, c, s);
+    for (int k = 0; k < n * n; k++) if (s[k] != s[k]) err = true;
+    if (err) { dmk_set_error("matrix inversion error (masked compliance)"); return DMK_ERR_CONFIG; }
+    for (int q = 0; q < n; q++) for (int p = 0; p < n; p++) T99[sel[p] + 9 * sel[q]] = s[p + n * q];
+  }
+  for (int a = 0; a < 3; a++) for (int b = 0; b < 3; b++) for (int cc = 0; cc < 3; cc++) for (int d = 0; d < 3; d++)
+    h->S[c4(a, b, cc, d)] = T99[(3 * a + b) + 9 * (3 * cc + d)];
+  return 0;
+}
+
+// form_residual: X -> r (device), returns 0 / error
+static int form_residual(dmk_handle* h, const double* X, double* r, int iter) {
+  if (h->nfuncs == 0 && iter == 0) h->totalIter = -1;
+  if (h->totalIter <= iter) h->totalIter++;
+  h->nfuncs++;
+  h->n_residuals++;
+  int rc = constitutive(h, X, h->dt, true);
+  if (rc) return rc;
+  // delta F_aim = S : (P_av - P_aim)
+  double dP[9], dFa[9], dFr[9];
+  for (int k = 0; k < 9; k++) dP[k] = h->P_av[k] - h->P_aim[k];
+  for (int j = 0; j < 3; j++)
+    for (int i = 0; i < 3; i++) {
+      double s = 0.0;
+      for (int n = 0; n < 3; n++)
+        for (int m = 0; m < 3; m++) s += h->S[c4(i, j, m, n)] * dP[c2(m, n)];
+      dFa[c2(i, j)] = s;
+    }
+  for (int k = 0; k < 9; k++) h->F_aim[k] -= dFa[k];
+  double e = 0.0;
+  for (int k = 0; k < 9; k++) if (!h->stress_mask[k]) e = std::fmax(e, std::fabs(dP[k]));
+  h->err_BC = e;
+  rotate33(h->q_R, dFa, true, dFr);
+  // one forward FFT serves the divergence RMS and the Gamma convolution
+  CKL(dmk_fft_r2c(&h->fft, 9, h->P_hom, h->Z, h->st));
+  h->ga.do_div = 1;
+  h->ga.do_gamma = 1;
+  std::memcpy(h->ga.aim, dFr, sizeof dFr);
+  CKL(dmk_gamma_div(h->Z, &h->ga, h->partial, h->st));
+  int nb = dmk_gamma_div_blocks(&h->ga);
+  CKL(dmk_colsum(h->partial, nb, nb, 1, h->partial + nb, h->small + SM_DIV, 1.0, h->st));
+  CKL(dmk_fft_c2r(&h->fft, 9, h->Z, r, h->st));
+  CK(hipMemcpyAsync(h->hsmall + SM_DIV, h->small + SM_DIV, sizeof(double), hipMemcpyDeviceToHost, h->st));
+  CK(hipStreamSynchronize(h->st));
+  double div2 = h->hsmall[SM_DIV];
+  h->err_div = std::sqrt(div2) * h->wgt;
+  if (h->cells[0] == 1) h->err_div *= 0.5;
+  return 0;
+}
+
+// grid_mech_spectral_basic.f90:422-458 converged
+static int converged(const dmk_handle* h) {
+  double pmax = 0.0;
+  for (int k = 0; k < 9; k++) pmax = std::fmax(pmax, std::fabs(h->P_av[k]));
+  double divTol = std::fmax(pmax * h->num.eps_rel_div, h->num.eps_abs_div);
+  double BCTol = std::fmax(pmax * h->num.eps_rel_P, h->num.eps_abs_P);
+  if (h->totalIter >= h->num.N_iter_min && h->err_div / divTol < 1.0 && h->err_BC / BCTol < 1.0 &&
+      h->status == ST_OK)
+    return 1;
+  if (h->totalIter >= h->num.N_iter_max) return -1;
+  return 0;
+}
+
+static int dots(dmk_handle* h, const double* x, const double* Y, int l, int mode, double* out) {
+  CKL(dmk_mdot(x, Y, h->n9, l, mode, h->n9, h->partial, h->small + SM_DOTS, h->st));
+  CK(hipMemcpyAsync(out, h->small + SM_DOTS, sizeof(double) * l, hipMemcpyDeviceToHost, h->st));
+  CK(hipStreamSynchronize(h->st));
+  return 0;
+}
+
+static int update_subspace(dmk_handle* h, int ivec, int l, const double* F, const double* X) {
+  const int M = h->num.ngmres_m;
+  CK(hipMemcpyAsync(h->Xh + (size_t)ivec * h->n9, X, sizeof(double) * h->n9, hipMemcpyDeviceToDevice, h->st));
+  CK(hipMemcpyAsync(h->Fh + (size_t)ivec * h->n9, F, sizeof(double) * h->n9, hipMemcpyDeviceToDevice, h->st));
+  int nl = l > 0 ? l : 1;
+  double d[32];
+  int rc = dots(h, F, h->Fh, nl, 0, d);
+  if (rc) return rc;
+  for (int i = 0; i < nl; i++) { h->Q[ivec + M * i] = d[i]; h->Q[i + M * ivec] = d[i]; }
+  return 0;
+}
+
+static int solve_ngmres(dmk_handle* h, int* reason_out) {
+  const long n = h->n9;
+  const int M = h->num.ngmres_m;
+  const double gammaA = 2.0, gammaC = 2.0, epsB = 0.1, deltaB = 0.9;
+  const int restart_it = 2;
+  double* X = h->F;
+  double* F = h->R;
+  int rc;
+  if ((rc = form_residual(h, X, F, 0))) return rc;
+  double d1;
+  if ((rc = dots(h, F, F, 1, 0, &d1))) return rc;
+  double fnorm = std::sqrt(d1);
+  int reason = converged(h);
+  if (reason || fnorm != fnorm) { *reason_out = reason ? reason : -1; return 0; }
+  double fminnorm = fnorm;
+  if ((rc = update_subspace(h, 0, 0, F, X))) return rc;
+  int k_restart = 1, l = 1, restart_count = 0;
+  for (int k = 1; k <= h->num.snes_max_it; k++) {
+    int ivec = k_restart % M;
+    CKL(dmk_axpby(n, 1.0, X, -1.0, F, h->XM, h->st));
+    if ((rc = form_residual(h, h->XM, h->FM, k - 1))) return rc;
+    double xi[33];
+    if ((rc = dots(h, h->FM, h->FM, 1, 0, &d1))) return rc;
+    double fMnorm = std::sqrt(d1);
+    if ((rc = dots(h, h->FM, h->Fh, l, 0, xi))) return rc;
+    double nu = fMnorm * fMnorm;
+    double beta[32], H[32 * 32], alpha[32];
+    for (int i = 0; i < l; i++) beta[i] = nu - xi[i];
+    for (int j = 0; j < l; j++)
+      for (int i = 0; i < l; i++) H[i + l * j] = h->Q[i + M * j] - xi[i] - xi[j] + nu;
+    lsq_sym(l, H, beta, alpha);
+    CombineArgs ca;
+    ca.l = l;
+    ca.atot = 0.0;
+    for (int i = 0; i < l; i++) { ca.alpha[i] = alpha[i]; ca.atot += alpha[i]; }
+    CKL(dmk_combine(n, h->XM, h->Xh, n, &ca, h->XA, h->st));
+    if ((rc = form_residual(h, h->XA, h->FA, k - 1))) return rc;
+    if ((rc = dots(h, h->FA, h->FA, 1, 0, &d1))) return rc;
+    double fAnorm = std::sqrt(d1);
+    if ((rc = dots(h, h->XA, h->XM, 1, 1, &d1))) return rc;
+    double dnorm = std::sqrt(d1);
+    double dist[32];
+    if ((rc = dots(h, h->XA, h->Xh, l, 1, dist))) return rc;
+    double dminnorm = -1.0;
+    for (int i = 0; i < l; i++) {
+      double dc = std::sqrt(dist[i]);
+      if (dc < dminnorm || dminnorm < 0.0) dminnorm = dc;
+    }
+    bool selectA = true;
+    if (fAnorm >= gammaA * fminnorm) selectA = false;
+    if (!(epsB * dnorm < dminnorm || std::sqrt(fnorm) < deltaB * std::sqrt(fminnorm))) selectA = false;
+    if (selectA) {
+      CK(hipMemcpyAsync(X, h->XA, sizeof(double) * n, hipMemcpyDeviceToDevice, h->st));
+      CK(hipMemcpyAsync(F, h->FA, sizeof(double) * n, hipMemcpyDeviceToDevice, h->st));
+      fnorm = fAnorm;
+    } else {
+      CK(hipMemcpyAsync(X, h->XM, sizeof(double) * n, hipMemcpyDeviceToDevice, h->st));
+      CK(hipMemcpyAsync(F, h->FM, sizeof(double) * n, hipMemcpyDeviceToDevice, h->st));
+      fnorm = fMnorm;
+    }
+    bool selectRestart = false;
+    if (epsB * dnorm > dminnorm && std::sqrt(fAnorm) > deltaB * std::sqrt(fminnorm) && l > 0) selectRestart = true;
+    if (std::sqrt(fAnorm) > gammaC * std::sqrt(fminnorm)) selectRestart = true;
+    restart_count = selectRestart ? restart_count + 1 : 0;
+    if (restart_count >= restart_it) {
+      restart_count = 0;
+      k_restart = 1;
+      l = 1;
+      if ((rc = update_subspace(h, 0, 0, h->FM, h->XM))) return rc;
+    } else {
+      if (l < M) l++;
+      k_restart++;
+      if (fminnorm > fnorm) fminnorm = fnorm;
+      if ((rc = update_subspace(h, ivec, l, F, X))) return rc;
+    }
+    reason = converged(h);
+    if (reason || fnorm != fnorm) { *reason_out = reason ? reason : -1; return 0; }
+  }
+  *reason_out = -1;
+  return 0;
+}
+
+static int solve_richardson(dmk_handle* h, int* reason_out) {
+  double* X = h->F;
+  double* F = h->R;
+  for (int k = 0; k <= h->num.snes_max_it; k++) {
+    int rc = form_residual(h, X, F, k);
+    if (rc) return rc;
+    int reason = converged(h);
+    if (reason) { *reason_out = reason; return 0; }
+    CKL(dmk_axpby(h->n9, 1.0, X, -1.0, F, X, h->st));
+  }
+  *reason_out = -1;
+  return 0;
+}
+
+// ------------------------------------------------------------------ C-ABI
+extern "C" int dmk_device_synchronize(void) {
+  CK(hipDeviceSynchronize());
+  return 0;
+}
+
+extern "C" int dmk_nsl(const dmk_handle* h) { return h ? h->nsl_max : 0; }
+
+extern "C" void dmk_mech_destroy(dmk_handle* h) {
+  if (!h) return;
+  if (h->st) hipStreamSynchronize(h->st);
+  for (void* p : h->allocs) hipFree(p);
+  if (h->ph_dev) hipFree(h->ph_dev);
+  if (h->hsmall) hipHostFree(h->hsmall);
+  dmk_fft_free(&h->fft);
+  if (h->st) hipStreamDestroy(h->st);
+  delete h;
+}
+
+static int init_impl(dmk_handle* h, const int32_t cells[3], const double size[3], int32_t n_phases,
+                     const dmk_phase_desc* phases, const int32_t* phase_of_cell, const double* q0,
+                     const dmk_numerics* num) {
+  for (int d = 0; d < 3; d++) {
+    if (cells[d] < 1 || size[d] <= 0.0) { dmk_set_error("invalid grid"); return DMK_ERR_ARG; }
+    h->cells[d] = cells[d];
+    h->size[d] = size[d];
+  }
+  if (!num) { dmk_set_error("numerics missing"); return DMK_ERR_ARG; }
+  h->num = *num;
+  if (h->num.ngmres_m < 1 || h->num.ngmres_m > 30) { dmk_set_error("ngmres_m must be in [1,30]"); return DMK_ERR_CONFIG; }
+  if (h->num.N_iter_max < 1 || h->num.N_iter_min < 1 || h->num.N_iter_min > h->num.N_iter_max) {
+    dmk_set_error("IO_error 301: N_iter_min / N_iter_max");
+    return DMK_ERR_CONFIG;
+  }
+  h->N = (long)cells[0] * cells[1] * cells[2];
+  h->h = cells[0] / 2 + 1;
+  h->Nk = (long)h->h * cells[1] * cells[2];
+  h->n9 = 9 * h->N;
+  h->wgt = 1.0 / (double)h->N;
+  CK(hipStreamCreate(&h->st));
+  if (dmk_fft_setup(&h->fft, cells)) { dmk_set_error("FFT setup failed"); return DMK_ERR_HIP; }
+  // phases
+  if (n_phases < 1 || n_phases > DMK_MAX_PHASES) { dmk_set_error("1..8 phases supported"); return DMK_ERR_CONFIG; }
+  h->n_phases = n_phases;
+  h->ph.resize(n_phases);
+  h->nsl_max = 0;
+  for (int p = 0; p < n_phases; p++) {
+    int rc = dmk_build_phase(&phases[p], &h->ph[p]);
+    if (rc) { dmk_set_error("phase %d: unsupported phenopowerlaw description (%d)", p, rc); return DMK_ERR_CONFIG; }
+    if (h->ph[p].nsl > h->nsl_max) h->nsl_max = h->ph[p].nsl;
+  }
+  CK(hipMalloc(&h->ph_dev, sizeof(DmkPhaseDev) * n_phases));
+  CK(hipMemcpy(h->ph_dev, h->ph.data(), sizeof(DmkPhaseDev) * n_phases, hipMemcpyHostToDevice));
+  h->nm.r_cutback_min = num->r_cutback_min;
+  h->nm.r_cutback = num->r_cutback;
+  h->nm.r_increase = num->r_increase;
+  h->nm.eps_rel_state = num->eps_rel_state;
+  h->nm.N_iter_state_max = num->N_iter_state_max;
+  h->nm.N_iter_Lp_max = num->N_iter_Lp_max;
+  h->nm.r_linesearch_Lp = num->r_linesearch_Lp;
+  h->nm.eps_rel_Lp = num->eps_rel_Lp;
+  h->nm.eps_abs_Lp = num->eps_abs_Lp;
+  h->nm.f_update_jacobi_Lp = num->f_update_jacobi_Lp;
+  if (num->f_update_jacobi_Lp != 1) { dmk_set_error("f_update_jacobi_Lp other than 1 is not on this path"); return DMK_ERR_CONFIG; }
+  const long N = h->N, n9 = h->n9;
+  const int ns = h->nsl_max;
+  // material point
+  DmkMPDev& mp = h->mp;
+  mp.N = (int)N;
+  mp.stride_sl = ns;
+  int32_t* phase_dev;
+  int rc;
+  if ((rc = dalloc(h, &phase_dev, N))) return rc;
+  mp.phase = phase_dev;
+  double** f9[] = {&mp.Fphase, &mp.F0, &mp.Fp, &mp.Fp0, &mp.Lp, &mp.Lp0, &mp.S, &mp.S0, &mp.Fe, &mp.Pph};
+  for (auto pp : f9) if ((rc = dalloc(h, pp, n9))) return rc;
+  double** fs[] = {&mp.xi, &mp.xi0, &mp.gam, &mp.gam0};
+  for (auto pp : fs) if ((rc = dalloc(h, pp, (size_t)ns * N))) return rc;
+  double** f2s[] = {&mp.st0s, &mp.last1, &mp.last2, &mp.dot};
+  for (auto pp : f2s) if ((rc = dalloc(h, pp, (size_t)2 * ns * N))) return rc;
+  if ((rc = dalloc(h, &mp.O, 4 * N))) return rc;
+  if ((rc = dalloc(h, &mp.status, N))) return rc;
+  if ((rc = dalloc(h, &mp.n_newton, N))) return rc;
+  if ((rc = dalloc(h, &mp.n_fpi, N))) return rc;
+  if ((rc = dalloc(h, &h->P_hom, n9))) return rc;
+  if ((rc = dalloc(h, &h->dPdF, 81 * N))) return rc;
+  double** sol[] = {&h->F, &h->F_lastinc, &h->Fdot, &h->R, &h->XM, &h->FM, &h->XA, &h->FA, &h->Fext, &h->Pext};
+  for (auto pp : sol) if ((rc = dalloc(h, pp, n9))) return rc;
+  if ((rc = dalloc(h, &h->Xh, (size_t)h->num.ngmres_m * n9))) return rc;
+  if ((rc = dalloc(h, &h->Fh, (size_t)h->num.ngmres_m * n9))) return rc;
+  if ((rc = dalloc(h, &h->Z, 9 * h->Nk))) return rc;
+  if ((rc = dalloc(h, &h->U, 3 * h->Nk))) return rc;
+  if ((rc = dalloc(h, &h->u3, 3 * N))) return rc;
+  long nn = (long)(cells[0] + 1) * (cells[1] + 1) * (cells[2] + 1);
+  if ((rc = dalloc(h, &h->xn_dev, 3 * nn))) return rc;
+  if ((rc = dalloc(h, &h->xp_dev, 3 * N))) return rc;
+  long np = 82L * dmk_nchunk(std::max(n9, 9 * h->Nk)) + 4096;
+  if ((rc = dalloc(h, &h->partial, np))) return rc;
+  if ((rc = dalloc(h, &h->small, SM_SIZE))) return rc;
+  if ((rc = dalloc(h, &h->nrm, N))) return rc;
+  if ((rc = dalloc(h, &h->pmax, dmk_nchunk(N)))) return rc;
+  if ((rc = dalloc(h, &h->pmin, dmk_nchunk(N)))) return rc;
+  if ((rc = dalloc(h, &h->ext, 2))) return rc;
+  if ((rc = dalloc(h, &h->sel, 2))) return rc;
+  if ((rc = dalloc(h, &h->flag, 4))) return rc;
+  if ((rc = dalloc(h, &h->cnt, 4))) return rc;
+  CK(hipHostMalloc((void**)&h->hsmall, sizeof(double) * SM_SIZE, hipHostMallocDefault));
+  h->Q.assign((size_t)h->num.ngmres_m * h->num.ngmres_m, 0.0);
+  // initial material-point state (phase_mechanical.f90:249-266, phenopowerlaw.f90:259-285)
+  {
+    std::vector<double> Fp(n9), Fe(n9), I(n9, 0.0), xi((size_t)ns * N, 0.0), O(4 * N);
+    std::vector<int32_t> ph(phase_of_cell, phase_of_cell + N);
+    for (long e = 0; e < N; e++) {
+      if (ph[e] < 0 || ph[e] >= n_phases) { dmk_set_error("phase index out of range at cell %ld", e); return DMK_ERR_ARG; }
+      double om[9];
+      dmk_qu2om(q0 + 4 * e, om);   // row-major
+      for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) {
+          Fp[(size_t)(i + 3 * j) * N + e] = om[3 * i + j];
+          Fe[(size_t)(i + 3 * j) * N + e] = om[3 * j + i];
+        }
+      for (int i = 0; i < 3; i++) I[(size_t)(4 * i) * N + e] = 1.0;
+      const DmkPhaseDev& P = h->ph[ph[e]];
+      const dmk_phase_desc& D = phases[ph[e]];
+      int a = 0;
+      for (int f = 0; f < D.n_fam; f++)
+        for (int s = 0; s < D.N_sl[f]; s++, a++) xi[(size_t)a * N + e] = D.xi_0[f];
+      (void)P;
+      for (int k = 0; k < 4; k++) O[k * N + e] = q0[4 * e + k];
+    }
+    CK(hipMemcpy((void*)mp.phase, ph.data(), sizeof(int32_t) * N, hipMemcpyHostToDevice));
+    CK(hipMemcpy(mp.Fp, Fp.data(), sizeof(double) * n9, hipMemcpyHostToDevice));
+    CK(hipMemcpy(mp.Fp0, Fp.data(), sizeof(double) * n9, hipMemcpyHostToDevice));
+    CK(hipMemcpy(mp.Fe, Fe.data(), sizeof(double) * n9, hipMemcpyHostToDevice));
+    CK(hipMemcpy(mp.Fphase, I.data(), sizeof(double) * n9, hipMemcpyHostToDevice));
+    CK(hipMemcpy(mp.F0, I.data(), sizeof(double) * n9, hipMemcpyHostToDevice));
+    CK(hipMemcpy(h->F, I.data(), sizeof(double) * n9, hipMemcpyHostToDevice));
+    CK(hipMemcpy(h->F_lastinc, I.data(), sizeof(double) * n9, hipMemcpyHostToDevice));
+    CK(hipMemcpy(mp.xi, xi.data(), sizeof(double) * ns * N, hipMemcpyHostToDevice));
+    CK(hipMemcpy(mp.xi0, xi.data(), sizeof(double) * ns * N, hipMemcpyHostToDevice));
+    CK(hipMemcpy(mp.O, O.data(), sizeof(double) * 4 * N, hipMemcpyHostToDevice));
+  }
+  // solver module defaults (grid_mech_spectral_basic.f90:69-82)
+  std::memset(h->F_aimDot, 0, sizeof h->F_aimDot);
+  std::memset(h->P_av, 0, sizeof h->P_av);
+  std::memset(h->P_aim, 0, sizeof h->P_aim);
+  std::memset(h->F_aim, 0, sizeof h->F_aim);
+  h->F_aim[0] = h->F_aim[4] = h->F_aim[8] = 1.0;
+  std::memcpy(h->F_aim_lastinc, h->F_aim, sizeof h->F_aim);
+  std::memset(h->C_volAvgLastInc, 0, sizeof h->C_volAvgLastInc);
+  std::memset(h->C_minMaxAvgLastInc, 0, sizeof h->C_minMaxAvgLastInc);
+  std::memset(h->S, 0, sizeof h->S);
+  for (int k = 0; k < 9; k++) h->stress_mask[k] = 1;
+  h->q_R[0] = 1.0; h->q_R[1] = h->q_R[2] = h->q_R[3] = 0.0;
+  // spectral set-up (spectral_utilities.f90:173-197)
+  for (int d = 0; d < 3; d++) { h->ga.n[d] = cells[d]; h->ga.size[d] = size[d]; }
+  h->ga.h = h->h;
+  double scaled[3] = {size[0], size[1], size[2]};
+  {
+    double r[3];
+    for (int d = 0; d < 3; d++) r[d] = size[d] / cells[d];
+    int dc = num->divergence_correction;
+    if (dc == 0 || dc == 1) {
+      const double* v = dc == 1 ? r : size;
+      int imin = 0, imax = 0;
+      for (int d = 1; d < 3; d++) { if (v[d] < v[imin]) imin = d; if (v[d] > v[imax]) imax = d; }
+      for (int j = 0; j < 3; j++)
+        if (j != imin && j != imax)
+          for (int d = 0; d < 3; d++) scaled[d] = dc == 1 ? size[d] / size[j] * (double)cells[j] : size[d] / size[j];
+    }
+  }
+  for (int d = 0; d < 3; d++) h->ga.rg[d] = size[d] / scaled[d];
+  // init: F = I, response at dt = 0, C_ref (grid_mech_spectral_basic.f90:213-235)
+  if ((rc = constitutive(h, h->F, 0.0, false))) return rc;
+  set_gamma_coef(h, h->C_minMaxAvg);
+  return 0;
+}
+
+extern "C" int dmk_mech_init(const int32_t cells[3], const double size[3], int32_t n_phases,
+                             const dmk_phase_desc* phases, const int32_t* phase_of_cell, const double* q0,
+                             const dmk_numerics* num, dmk_handle** out) {
+  if (!cells || !size || !phases || !phase_of_cell || !q0 || !out) {
+    dmk_set_error("null argument");
+    return DMK_ERR_ARG;
+  }
+  dmk_handle* h = new dmk_handle();
+  int rc = init_impl(h, cells, size, n_phases, phases, phase_of_cell, q0, num);
+  if (rc) { dmk_mech_destroy(h); *out = nullptr; return rc; }
+  *out = h;
+  return 0;
+}
+
+extern "C" int dmk_mech_forward(dmk_handle* h, int32_t cutBack, int32_t guess, double dt, double dt_old,
+                                double t_remaining, const dmk_loadcase* bc) {
+  if (!h || !bc) { dmk_set_error("null argument"); return DMK_ERR_ARG; }
+  ForwardArgs fa{};
+  fa.cutback = cutBack;
+  if (cutBack) {
+    std::memcpy(h->C_volAvg, h->C_volAvgLastInc, sizeof h->C_volAvg);
+    std::memcpy(h->C_minMaxAvg, h->C_minMaxAvgLastInc, sizeof h->C_minMaxAvg);
+  } else {
+    std::memcpy(h->C_volAvgLastInc, h->C_volAvg, sizeof h->C_volAvg);
+    std::memcpy(h->C_minMaxAvgLastInc, h->C_minMaxAvg, sizeof h->C_minMaxAvg);
+    for (int k = 0; k < 9; k++) {
+      int smask = bc->stress_type < 0 ? 1 : bc->stress_mask[k];
+      h->F_aimDot[k] = guess ? (smask ? 0.0 : (h->F_aim[k] - h->F_aim_lastinc[k]) / dt_old) : 0.0;
+    }
+    std::memcpy(h->F_aim_lastinc, h->F_aim, sizeof h->F_aim);
+    double V[9];
+    for (int k = 0; k < 9; k++) V[k] = bc->def_mask[k] ? 0.0 : bc->def_values[k];
+    if (bc->def_type == 0) {
+      for (int j = 0; j < 3; j++)
+        for (int i = 0; i < 3; i++) {
+          double s = 0.0;
+          for (int k = 0; k < 3; k++) s += V[c2(i, k)] * h->F_aim_lastinc[c2(k, j)];
+          h->F_aimDot[c2(i, j)] += s;
+        }
+    } else if (bc->def_type == 1) {
+      for (int k = 0; k < 9; k++) h->F_aimDot[k] += V[k];
+    } else {
+      for (int k = 0; k < 9; k++)
+        h->F_aimDot[k] += bc->def_mask[k] ? 0.0 : (bc->def_values[k] - h->F_aim_lastinc[k]) / t_remaining;
+    }
+    rotate33(bc->q_R, h->F_aimDot, true, fa.avRate);
+  }
+  for (int k = 0; k < 9; k++) h->F_aim[k] = h->F_aim_lastinc[k] + h->F_aimDot[k] * dt;
+  if (bc->stress_type == 0)
+    for (int k = 0; k < 9; k++)
+      h->P_aim[k] = h->P_aim[k] + (bc->stress_mask[k] ? 0.0 : (bc->stress_values[k] - h->P_aim[k]) / t_remaining) * dt;
+  if (bc->stress_type == 1)
+    for (int k = 0; k < 9; k++) h->P_aim[k] = h->P_aim[k] + (bc->stress_mask[k] ? 0.0 : bc->stress_values[k]) * dt;
+  // F = F_lastinc + Fdot dt, mean corrected to the rotated aim
+  CKL(dmk_forward_field(h->N, guess, dt_old, dt, &fa, h->F, h->F_lastinc, h->Fdot, h->F, h->st));
+  CKL(dmk_colsum(h->F, h->N, h->N, 9, h->partial, h->small + SM_DOTS, h->wgt, h->st));
+  CK(hipMemcpyAsync(h->hsmall + SM_DOTS, h->small + SM_DOTS, sizeof(double) * 9, hipMemcpyDeviceToHost, h->st));
+  CK(hipStreamSynchronize(h->st));
+  double aim[9];
+  rotate33(bc->q_R, h->F_aim, true, aim);
+  for (int k = 0; k < 9; k++) fa.shift[k] = h->hsmall[SM_DOTS + k] - aim[k];
+  CKL(dmk_shift_field(h->N, &fa, h->F, h->st));
+  for (int k = 0; k < 9; k++) h->stress_mask[k] = bc->stress_type < 0 ? 1 : bc->stress_mask[k];
+  std::memcpy(h->q_R, bc->q_R, sizeof h->q_R);
+  h->dt = dt;
+  return 0;
+}
+
+extern "C" int dmk_mech_materialpoint_forward(dmk_handle* h) {
+  DmkMPDev& mp = h->mp;
+  const size_t b9 = sizeof(double) * h->n9, bs = sizeof(double) * h->nsl_max * h->N;
+  CK(hipMemcpyAsync(mp.Fp0, mp.Fp, b9, hipMemcpyDeviceToDevice, h->st));
+  CK(hipMemcpyAsync(mp.F0, mp.Fphase, b9, hipMemcpyDeviceToDevice, h->st));
+  CK(hipMemcpyAsync(mp.Lp0, mp.Lp, b9, hipMemcpyDeviceToDevice, h->st));
+  CK(hipMemcpyAsync(mp.S0, mp.S, b9, hipMemcpyDeviceToDevice, h->st));
+  CK(hipMemcpyAsync(mp.xi0, mp.xi, bs, hipMemcpyDeviceToDevice, h->st));
+  CK(hipMemcpyAsync(mp.gam0, mp.gam, bs, hipMemcpyDeviceToDevice, h->st));
+  return 0;
+}
+
+extern "C" int dmk_mech_solution(dmk_handle* h, int32_t* iterations, int32_t* conv, int32_t* n_residuals) {
+  if (!h) { dmk_set_error("null handle"); return DMK_ERR_ARG; }
+  int rc = masked_compliance(h);
+  if (rc) return rc;
+  if (h->num.update_gamma) set_gamma_coef(h, h->C_minMaxAvg);
+  h->nfuncs = 0;
+  int reason = 0;
+  rc = h->num.solver == 1 ? solve_richardson(h, &reason) : solve_ngmres(h, &reason);
+  if (rc) return rc;
+  if (iterations) *iterations = h->totalIter;
+  if (conv) *conv = reason > 0;
+  if (n_residuals) *n_residuals = h->nfuncs;
+  for (int k = 0; k < 9; k++) if (h->stress_mask[k]) h->P_aim[k] = h->P_av[k];
+  return 0;
+}
+
+extern "C" int dmk_mech_update_coords(dmk_handle* h, double* x_p, double* x_n) {
+  CKL(dmk_fft_r2c(&h->fft, 9, h->F, h->Z, h->st));
+  double2 dc[9];
+  for (int c = 0; c < 9; c++)
+    CK(hipMemcpyAsync(&dc[c], h->Z + (size_t)c * h->Nk, sizeof(double2), hipMemcpyDeviceToHost, h->st));
+  CKL(dmk_coords_freq(h->Z, h->U, &h->ga, h->st));
+  CKL(dmk_fft_c2r(&h->fft, 3, h->U, h->u3, h->st));
+  CK(hipStreamSynchronize(h->st));
+  CoordsArgs ca;
+  for (int d = 0; d < 3; d++) { ca.n[d] = h->cells[d]; ca.step[d] = h->size[d] / h->cells[d]; }
+  for (int c = 0; c < 9; c++) ca.Favg[c] = dc[c].x * h->wgt;
+  ca.wgt = h->wgt;
+  CKL(dmk_coords_nodes(h->u3, &ca, h->xn_dev, h->xp_dev, h->st));
+  long nn = (long)(h->cells[0] + 1) * (h->cells[1] + 1) * (h->cells[2] + 1);
+  if (x_n) CK(hipMemcpyAsync(x_n, h->xn_dev, sizeof(double) * 3 * nn, hipMemcpyDeviceToHost, h->st));
+  if (x_p) CK(hipMemcpyAsync(x_p, h->xp_dev, sizeof(double) * 3 * h->N, hipMemcpyDeviceToHost, h->st));
+  CK(hipStreamSynchronize(h->st));
+  return 0;
+}
+
+extern "C" int dmk_constitutive(dmk_handle* h, double dt, const double* F, double* P, double* P_av,
+                                double* C_volAvg, double* C_minMaxAvg, int32_t* status) {
+  if (!h || !F) { dmk_set_error("null argument"); return DMK_ERR_ARG; }
+  CK(hipMemcpyAsync(h->Pext, F, sizeof(double) * h->n9, hipMemcpyHostToDevice, h->st));
+  CKL(dmk_aos_to_soa(h->Pext, h->N, 9, h->Fext, h->st));
+  int rc = constitutive(h, h->Fext, dt, false);
+  if (rc) return rc;
+  if (P) {
+    CKL(dmk_soa_to_aos(h->P_hom, h->N, 9, h->Pext, h->st));
+    CK(hipMemcpyAsync(P, h->Pext, sizeof(double) * h->n9, hipMemcpyDeviceToHost, h->st));
+    CK(hipStreamSynchronize(h->st));
+  }
+  if (P_av) std::memcpy(P_av, h->P_av, sizeof h->P_av);
+  if (C_volAvg) std::memcpy(C_volAvg, h->C_volAvg, sizeof h->C_volAvg);
+  if (C_minMaxAvg) std::memcpy(C_minMaxAvg, h->C_minMaxAvg, sizeof h->C_minMaxAvg);
+  if (status) *status = h->status;
+  return 0;
+}
+
+extern "C" int dmk_constitutive_device(dmk_handle* h, double dt, const double* F_dev, double* P_dev, int32_t* status) {
+  if (!h || !F_dev) { dmk_set_error("null argument"); return DMK_ERR_ARG; }
+  CKL(dmk_aos_to_soa(F_dev, h->N, 9, h->Fext, h->st));
+  int rc = constitutive(h, h->Fext, dt, false);
+  if (rc) return rc;
+  if (P_dev) {
+    CKL(dmk_soa_to_aos(h->P_hom, h->N, 9, P_dev, h->st));
+    CK(hipStreamSynchronize(h->st));
+  }
+  if (status) *status = h->status;
+  return 0;
+}
+
+static int field_dev(dmk_handle* h, int what, const double** p, int* w) {
+  const DmkMPDev& mp = h->mp;
+  *w = 9;
+  switch (what) {
+    case 0: *p = h->F; return 0;
+    case 1: *p = h->P_hom; return 0;
+    case 2: *p = mp.Fe; return 0;
+    case 3: *p = mp.Fp; return 0;
+    case 4: *p = mp.Lp; return 0;
+    case 5: *p = mp.S; return 0;
+    case 7: *p = mp.O; *w = 4; return 0;
+    case 8: *p = mp.xi; *w = h->nsl_max; return 0;
+    case 9: *p = mp.gam; *w = h->nsl_max; return 0;
+    case 10: *p = h->dPdF; *w = 81; return 0;
+    case 11: *p = h->F_lastinc; return 0;
+    default: return 1;
+  }
+}
+
+extern "C" int dmk_mech_get_field(dmk_handle* h, int32_t what, double* dst) {
+  if (!h || !dst) { dmk_set_error("null argument"); return DMK_ERR_ARG; }
+  if (what == 6) {   // F_i = I
+    for (long e = 0; e < h->N; e++)
+      for (int k = 0; k < 9; k++) dst[9 * e + k] = (k % 4 == 0) ? 1.0 : 0.0;
+    return 0;
+  }
+  const double* p;
+  int w;
+  if (field_dev(h, what, &p, &w)) { dmk_set_error("unknown field %d", what); return DMK_ERR_ARG; }
+  double* tmp;
+  CK(hipMallocAsync((void**)&tmp, sizeof(double) * w * h->N, h->st));
+  CKL(dmk_soa_to_aos(p, h->N, w, tmp, h->st));
+  CK(hipMemcpyAsync(dst, tmp, sizeof(double) * w * h->N, hipMemcpyDeviceToHost, h->st));
+  CK(hipFreeAsync(tmp, h->st));
+  CK(hipStreamSynchronize(h->st));
+  return 0;
+}
+
+extern "C" int dmk_mech_set_field(dmk_handle* h, int32_t what, const double* src) {
+  if (!h || !src) { dmk_set_error("null argument"); return DMK_ERR_ARG; }
+  const double* p;
+  int w;
+  if (field_dev(h, what, &p, &w) || what == 1 || what == 2 || what == 7 || what == 10) {
+    dmk_set_error("field %d cannot be set", what);
+    return DMK_ERR_ARG;
+  }
+  double* tmp;
+  CK(hipMallocAsync((void**)&tmp, sizeof(double) * w * h->N, h->st));
+  CK(hipMemcpyAsync(tmp, src, sizeof(double) * w * h->N, hipMemcpyHostToDevice, h->st));
+  CKL(dmk_aos_to_soa(tmp, h->N, w, (double*)p, h->st));
+  CK(hipFreeAsync(tmp, h->st));
+  // the restart also sets the increment-start copies (phase_mechanical.f90:1265-1280)
+  const DmkMPDev& mp = h->mp;
+  size_t b = sizeof(double) * w * h->N;
+  if (what == 3) CK(hipMemcpyAsync(mp.Fp0, mp.Fp, b, hipMemcpyDeviceToDevice, h->st));
+  if (what == 4) CK(hipMemcpyAsync(mp.Lp0, mp.Lp, b, hipMemcpyDeviceToDevice, h->st));
+  if (what == 5) CK(hipMemcpyAsync(mp.S0, mp.S, b, hipMemcpyDeviceToDevice, h->st));
+  if (what == 8) CK(hipMemcpyAsync(mp.xi0, mp.xi, b, hipMemcpyDeviceToDevice, h->st));
+  if (what == 9) CK(hipMemcpyAsync(mp.gam0, mp.gam, b, hipMemcpyDeviceToDevice, h->st));
+  if (what == 0) {
+    CK(hipMemcpyAsync(mp.Fphase, h->F, b, hipMemcpyDeviceToDevice, h->st));
+    CK(hipMemcpyAsync(mp.F0, h->F, b, hipMemcpyDeviceToDevice, h->st));
+  }
+  CK(hipStreamSynchronize(h->st));
+  return 0;
+}
+
+extern "C" int dmk_mech_get_scalars(dmk_handle* h, int32_t what, double* dst) {
+  if (!h || !dst) { dmk_set_error("null argument"); return DMK_ERR_ARG; }
+  switch (what) {
+    case 0: std::memcpy(dst, h->F_aim, 9 * sizeof(double)); break;
+    case 1: std::memcpy(dst, h->P_av, 9 * sizeof(double)); break;
+    case 2: std::memcpy(dst, h->P_aim, 9 * sizeof(double)); break;
+    case 3: std::memcpy(dst, h->C_volAvg, 81 * sizeof(double)); break;
+    case 4: std::memcpy(dst, h->C_minMaxAvg, 81 * sizeof(double)); break;
+    case 5: dst[0] = h->err_div; dst[1] = h->err_BC; break;
+    case 6: std::memcpy(dst, h->F_aimDot, 9 * sizeof(double)); break;
+    case 7: std::memcpy(dst, h->F_aim_lastinc, 9 * sizeof(double)); break;
+    case 8: std::memcpy(dst, h->C_volAvgLastInc, 81 * sizeof(double)); break;
+    default: dmk_set_error("unknown scalar %d", what); return DMK_ERR_ARG;
+  }
+  return 0;
+}
+
+extern "C" int dmk_mech_set_scalars(dmk_handle* h, int32_t what, const double* src) {
+  if (!h || !src) { dmk_set_error("null argument"); return DMK_ERR_ARG; }
+  switch (what) {
+    case 0: std::memcpy(h->F_aim, src, 9 * sizeof(double)); break;
+    case 2: std::memcpy(h->P_aim, src, 9 * sizeof(double)); break;
+    case 3: std::memcpy(h->C_volAvg, src, 81 * sizeof(double)); break;
+    case 4: std::memcpy(h->C_minMaxAvg, src, 81 * sizeof(double)); set_gamma_coef(h, src); break;
+    case 6: std::memcpy(h->F_aimDot, src, 9 * sizeof(double)); break;
+    case 7: std::memcpy(h->F_aim_lastinc, src, 9 * sizeof(double)); break;
+    case 8: std::memcpy(h->C_volAvgLastInc, src, 81 * sizeof(double)); break;
+    default: dmk_set_error("scalar %d cannot be set", what); return DMK_ERR_ARG;
+  }
+  return 0;
+}
+
+extern "C" int dmk_mech_counters(dmk_handle* h, int64_t out[4]) {
+  unsigned long long c[2];
+  CK(hipMemcpyAsync(c, h->cnt, sizeof c, hipMemcpyDeviceToHost, h->st));
+  CK(hipStreamSynchronize(h->st));
+  out[0] = h->n_updates;
+  out[1] = (int64_t)c[0];
+  out[2] = (int64_t)c[1];
+  out[3] = h->n_residuals;
+  return 0;
+}
+
+// math.f90:1222-1234
+static double geometric_fraction(double r, int i, int N) {
+  if (std::fabs(r - 1.0) <= 1.0e-9) return 1.0 / (double)N;
+  return (1.0 - r) * std::pow(r, i) / (r - std::pow(r, N + 1));
+}
+
+extern "C" int dmk_grid_run_loadcase(dmk_handle* h, const dmk_loadcase* lc, double* t_io, double* stats,
+                                     double* P_av_hist, double* F_aim_hist, int64_t* n_updates) {
+  if (!h || !lc) { dmk_set_error("null argument"); return DMK_ERR_ARG; }
+  double t_0 = h->t;
+  int guess = lc->estimate_rate;
+  int64_t u0 = h->n_updates;
+  for (int inc = 1; inc <= lc->N; inc++) {
+    h->totalIncs++;
+    h->Delta_t_prev = h->Delta_t;
+    h->Delta_t = lc->t * geometric_fraction(lc->r, inc, lc->N) * std::pow(2.0, -(double)h->cutBackLevel);
+    int stepFraction = 0, its = 0, nf_total = 0;
+    while (stepFraction < (1 << h->cutBackLevel)) {
+      double t_remaining = lc->t + t_0 - h->t;
+      h->t = h->t + h->Delta_t;
+      stepFraction++;
+      int rc = dmk_mech_forward(h, h->cutBack, guess, h->Delta_t, h->Delta_t_prev, t_remaining, lc);
+      if (rc) return rc;
+      if (!h->cutBack && (rc = dmk_mech_materialpoint_forward(h))) return rc;
+      int32_t it, conv, nf;
+      if ((rc = dmk_mech_solution(h, &it, &conv, &nf))) return rc;
+      its = it;
+      nf_total += nf;
+      if (conv) {
+        if ((rc = dmk_mech_update_coords(h, nullptr, nullptr))) return rc;
+        h->Delta_t_prev = h->Delta_t;
+        h->cutBack = 0;
+        guess = 1;
+      } else if (h->cutBackLevel < h->num.N_cutback_max) {
+        h->cutBack = 1;
+        stepFraction = (stepFraction - 1) * 2;
+        h->cutBackLevel++;
+        h->t = h->t - h->Delta_t;
+        h->Delta_t = h->Delta_t / 2.0;
+      } else {
+        if (t_io) *t_io = h->t;
+        if (n_updates) *n_updates = h->n_updates - u0;
+        dmk_set_error("IO_error 950: no convergence after %d cutbacks", h->cutBackLevel);
+        return 950;
+      }
+    }
+    if (stats) {
+      stats[4 * (inc - 1) + 0] = h->t;
+      stats[4 * (inc - 1) + 1] = h->cutBackLevel;
+      stats[4 * (inc - 1) + 2] = its;
+      stats[4 * (inc - 1) + 3] = nf_total;
+    }
+    h->cutBackLevel = h->cutBackLevel - 1 > 0 ? h->cutBackLevel - 1 : 0;
+    if (P_av_hist) std::memcpy(P_av_hist + 9 * (inc - 1), h->P_av, 9 * sizeof(double));
+    if (F_aim_hist) std::memcpy(F_aim_hist + 9 * (inc - 1), h->F_aim, 9 * sizeof(double));
+  }
+  if (t_io) *t_io = h->t;
+  if (n_updates) *n_updates = h->n_updates - u0;
+  return 0;
+}
+
+// ------------------------------------------------------------------ spectral utilities on host data
+// utilities_GammaConvolution (spectral_utilities.f90:436-501): field (3,3,N) -> out (3,3,N)
+extern "C" int dmk_gamma_convolution(dmk_handle* h, const double* field, const double aim[9], double* out) {
+  if (!h || !field || !aim || !out) { dmk_set_error("null argument"); return DMK_ERR_ARG; }
+  CK(hipMemcpyAsync(h->Pext, field, sizeof(double) * h->n9, hipMemcpyHostToDevice, h->st));
+  CKL(dmk_aos_to_soa(h->Pext, h->N, 9, h->Fext, h->st));
+  CKL(dmk_fft_r2c(&h->fft, 9, h->Fext, h->Z, h->st));
+  GammaArgs g = h->ga;
+  g.do_div = 0;
+  g.do_gamma = 1;
+  std::memcpy(g.aim, aim, sizeof g.aim);
+  CKL(dmk_gamma_div(h->Z, &g, h->partial, h->st));
+  CKL(dmk_fft_c2r(&h->fft, 9, h->Z, h->Fext, h->st));
+  CKL(dmk_soa_to_aos(h->Fext, h->N, 9, h->Pext, h->st));
+  CK(hipMemcpyAsync(out, h->Pext, sizeof(double) * h->n9, hipMemcpyDeviceToHost, h->st));
+  CK(hipStreamSynchronize(h->st));
+  return 0;
+}
+
+// utilities_divergenceRMS (spectral_utilities.f90:650-691)
+extern "C" int dmk_divergence_rms(dmk_handle* h, const double* field, double* out) {
+  if (!h || !field || !out) { dmk_set_error("null argument"); return DMK_ERR_ARG; }
+  CK(hipMemcpyAsync(h->Pext, field, sizeof(double) * h->n9, hipMemcpyHostToDevice, h->st));
+  CKL(dmk_aos_to_soa(h->Pext, h->N, 9, h->Fext, h->st));
+  CKL(dmk_fft_r2c(&h->fft, 9, h->Fext, h->Z, h->st));
+  GammaArgs g = h->ga;
+  g.do_div = 1;
+  g.do_gamma = 0;
+  CKL(dmk_gamma_div(h->Z, &g, h->partial, h->st));
+  int nb = dmk_gamma_div_blocks(&g);
+  CKL(dmk_colsum(h->partial, nb, nb, 1, h->partial + nb, h->small + SM_DIV, 1.0, h->st));
+  CK(hipMemcpyAsync(h->hsmall + SM_DIV, h->small + SM_DIV, sizeof(double), hipMemcpyDeviceToHost, h->st));
+  CK(hipStreamSynchronize(h->st));
+  double r = std::sqrt(h->hsmall[SM_DIV]) * h->wgt;
+  if (h->cells[0] == 1) r *= 0.5;
+  *out = r;
+  return 0;
+}
+
+// unnormalised r2c of `howmany` (<= 9) interleaved fields (howmany, N1, N2, N3) ->
+// (howmany, N1/2+1, N2, N3) complex as (re, im) pairs
+extern "C" int dmk_fft_forward(dmk_handle* h, int32_t howmany, const double* in, double* out) {
+  if (!h || !in || !out || howmany < 1 || howmany > 9) { dmk_set_error("bad argument"); return DMK_ERR_ARG; }
+  CK(hipMemcpyAsync(h->Pext, in, sizeof(double) * howmany * h->N, hipMemcpyHostToDevice, h->st));
+  CKL(dmk_aos_to_soa(h->Pext, h->N, howmany, h->Fext, h->st));
+  CKL(dmk_fft_r2c(&h->fft, howmany, h->Fext, h->Z, h->st));
+  std::vector<double> tmp(2 * howmany * h->Nk);
+  CK(hipMemcpyAsync(tmp.data(), h->Z, sizeof(double) * tmp.size(), hipMemcpyDeviceToHost, h->st));
+  CK(hipStreamSynchronize(h->st));
+  for (long f = 0; f < h->Nk; f++)
+    for (int c = 0; c < howmany; c++) {
+      out[2 * (c + howmany * f)] = tmp[2 * (c * h->Nk + f)];
+      out[2 * (c + howmany * f) + 1] = tmp[2 * (c * h->Nk + f) + 1];
+    }
+  return 0;
+}
+
+// inverse of dmk_fft_forward (c2r, unnormalised)
+extern "C" int dmk_fft_backward(dmk_handle* h, int32_t howmany, const double* in, double* out) {
+  if (!h || !in || !out || howmany < 1 || howmany > 9) { dmk_set_error("bad argument"); return DMK_ERR_ARG; }
+  std::vector<double> tmp(2 * howmany * h->Nk);
+  for (long f = 0; f < h->Nk; f++)
+    for (int c = 0; c < howmany; c++) {
+      tmp[2 * (c * h->Nk + f)] = in[2 * (c + howmany * f)];
+      tmp[2 * (c * h->Nk + f) + 1] = in[2 * (c + howmany * f) + 1];
+    }
+  CK(hipMemcpyAsync(h->Z, tmp.data(), sizeof(double) * tmp.size(), hipMemcpyHostToDevice, h->st));
+  CKL(dmk_fft_c2r(&h->fft, howmany, h->Z, h->Fext, h->st));
+  CKL(dmk_soa_to_aos(h->Fext, h->N, howmany, h->Pext, h->st));
+  CK(hipMemcpyAsync(out, h->Pext, sizeof(double) * howmany * h->N, hipMemcpyDeviceToHost, h->st));
+  CK(hipStreamSynchronize(h->st));
+  return 0;
+}

@@ -1,0 +1,540 @@
+// Spectral operators of the basic scheme on MI355X (src/grid/spectral_utilities.f90).
+//
+// * Batched 3-D r2c / c2r FFTs of component-major fields, built from 1-D
+//   mixed-radix Stockham passes staged in LDS:
+//     x pass: two real lines packed into one complex line (r2c/c2r split),
+//     y and z passes: c2c on groups of lines adjacent in x so that every
+//     global load and store is coalesced across the group.
+//   Unnormalised, forward sign -1, like FFTW's r2c/c2r (:239-261,455,498,661).
+// * gamma_div_kernel: one thread per frequency; the RMS divergence
+//   (:650-691) and the memory-efficient Gamma convolution (:436-501) read the
+//   same forward transform, which the reference computes twice.
+//   With the continuous derivative xi = i a, Gamma_hat:P_hat collapses to
+//   (A^-1 (P_hat a)) (x) a with the acoustic tensor A_lm = C_ref,ljmq a_j a_q.
+// * coords kernels: FFT integration of F for the node and cell positions
+//   (:857-968).
+#include "dmk_common.h"
+#include "dmk_spectral.h"
+
+namespace {
+
+__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
+  return make_double2(fma(a.x, b.x, -a.y * b.y), fma(a.x, b.y, a.y * b.x));
+}
+__device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ double2 cconj(double2 a) { return make_double2(a.x, -a.y); }
+
+// twiddle w^m, w = exp(sign 2 pi i / L); table holds the forward (sign -1) values
+__device__ __forceinline__ double2 twid(const double2* __restrict__ tw, int m, int sign) {
+  double2 t = tw[m];
+  return sign < 0 ? t : cconj(t);
+}
+
+template <int R>
+__device__ __forceinline__ void butterfly(double2* v, const double2* __restrict__ tw, int L, int sign) {
+  if constexpr (R == 2) {
+    double2 a = v[0], b = v[1];
+    v[0] = cadd(a, b);
+    v[1] = csub(a, b);
+  } else if constexpr (R == 4) {
+    double2 a = v[0], b = v[1], c = v[2], d = v[3];
+    double2 s0 = cadd(a, c), s1 = csub(a, c), s2 = cadd(b, d), s3 = csub(b, d);
+    // multiply s3 by -i (forward) or +i (inverse)
+    double2 r3 = sign < 0 ? make_double2(s3.y, -s3.x) : make_double2(-s3.y, s3.x);
+    v[0] = cadd(s0, s2);
+    v[2] = csub(s0, s2);
+    v[1] = cadd(s1, r3);
+    v[3] = csub(s1, r3);
+  } else {
+    double2 w[R];
+    const int st = L / R;
+#pragma unroll
+    for (int m = 0; m < R; m++) w[m] = twid(tw, m * st, sign);
+    double2 o[R];
+#pragma unroll
+    for (int p = 0; p < R; p++) {
+      double2 acc = v[0];
+#pragma unroll
+      for (int q = 1; q < R; q++) acc = cadd(acc, cmul(v[q], w[(p * q) % R]));
+      o[p] = acc;
+    }
+#pragma unroll
+    for (int p = 0; p < R; p++) v[p] = o[p];
+  }
+}
+
+// one Stockham stage of radix R over `nlines` lines of length L held in LDS
+template <int R>
+__device__ void stockham_stage(const double2* in, double2* out, int nlines, int L, int ldl, int Ns,
+                               const double2* __restrict__ tw, int sign) {
+  const int nb = L / R;
+  const int tot = nlines * nb;
+  const int tstep = L / (Ns * R);
+  for (int idx = threadIdx.x; idx < tot; idx += blockDim.x) {
+    int line = idx / nb, j = idx - line * nb;
+    const double2* src = in + line * ldl;
+    double2* dst = out + line * ldl;
+    double2 v[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) v[r] = src[j + r * nb];
+    int k = j % Ns;
+    if (Ns > 1) {
+#pragma unroll
+      for (int r = 1; r < R; r++) v[r] = cmul(v[r], twid(tw, (r * k * tstep) % L, sign));
+    }
+    butterfly<R>(v, tw, L, sign);
+    int o = (j / Ns) * Ns * R + k;
+#pragma unroll
+    for (int r = 0; r < R; r++) dst[o + r * Ns] = v[r];
+  }
+}
+
+// generic prime radix (rare sizes): O(R^2) DFT
+__device__ void stockham_stage_generic(const double2* in, double2* out, int nlines, int L, int ldl, int Ns, int R,
+                                       const double2* __restrict__ tw, int sign) {
+  const int nb = L / R;
+  const int tot = nlines * nb;
+  const int tstep = L / (Ns * R);
+  const int st = L / R;
+  for (int idx = threadIdx.x; idx < tot; idx += blockDim.x) {
+    int line = idx / nb, j = idx - line * nb;
+    const double2* src = in + line * ldl;
+    double2* dst = out + line * ldl;
+    int k = j % Ns;
+    int o = (j / Ns) * Ns * R + k;
+    for (int p = 0; p < R; p++) {
+      double2 acc = make_double2(0.0, 0.0);
+      for (int q = 0; q < R; q++) {
+        double2 x = src[j + q * nb];
+        if (Ns > 1 && q > 0) x = cmul(x, twid(tw, (q * k * tstep) % L, sign));
+        acc = cadd(acc, q == 0 ? x : cmul(x, twid(tw, ((p * q) % R) * st, sign)));
+      }
+      dst[o + p * Ns] = acc;
+    }
+  }
+}
+
+// full 1-D FFT of nlines LDS lines; returns the buffer holding the result
+__device__ double2* lds_fft(double2* b0, double2* b1, int nlines, int L, int ldl, const FftPlan& pl,
+                            const double2* __restrict__ tw, int sign) {
+  double2 *in = b0, *out = b1;
+  int Ns = 1;
+  for (int s = 0; s < pl.nstages; s++) {
+    int R = pl.radix[s];
+    switch (R) {
+      case 2: stockham_stage<2>(in, out, nlines, L, ldl, Ns, tw, sign); break;
+      case 3: stockham_stage<3>(in, out, nlines, L, ldl, Ns, tw, sign); break;
+      case 4: stockham_stage<4>(in, out, nlines, L, ldl, Ns, tw, sign); break;
+      case 5: stockham_stage<5>(in, out, nlines, L, ldl, Ns, tw, sign); break;
+      case 7: stockham_stage<7>(in, out, nlines, L, ldl, Ns, tw, sign); break;
+      default: stockham_stage_generic(in, out, nlines, L, ldl, Ns, R, tw, sign); break;
+    }
+    __syncthreads();
+    double2* t = in; in = out; out = t;
+    Ns *= R;
+  }
+  return in;
+}
+
+// x pass, forward: real lines (nl lines of length L, contiguous) -> half spectra
+// (nl lines of length h, contiguous); pairs of lines share one complex FFT.
+__global__ void __launch_bounds__(256) fft_x_r2c(const double* __restrict__ in, double2* __restrict__ out,
+                                                 int nl, FftPlan pl, const double2* __restrict__ tw, int pairs_per_block) {
+  extern __shared__ double2 lds[];
+  const int L = pl.L, h = L / 2 + 1, ldl = L + 1;
+  double2* b0 = lds;
+  double2* b1 = lds + pairs_per_block * ldl;
+  const int p0 = blockIdx.x * pairs_per_block;
+  const long l0 = 2L * p0;
+  const int nlines_here = min(2 * pairs_per_block, nl - (int)l0);
+  const int npairs = (nlines_here + 1) / 2;
+  for (int i = threadIdx.x; i < 2 * pairs_per_block * L; i += blockDim.x) {
+    int line = i / L, pos = i - line * L;
+    double v = line < nlines_here ? in[(l0 + line) * L + pos] : 0.0;
+    double2* c = &b0[(line >> 1) * ldl + pos];
+    if (line & 1) c->y = v; else c->x = v;
+  }
+  __syncthreads();
+  double2* r = lds_fft(b0, b1, npairs, L, ldl, pl, tw, -1);
+  for (int i = threadIdx.x; i < 2 * pairs_per_block * h; i += blockDim.x) {
+    int line = i / h, k = i - line * h;
+    if (line >= nlines_here) continue;
+    const double2* z = r + (line >> 1) * ldl;
+    double2 a = z[k], b = cconj(z[(L - k) % L]);
+    double2 v;
+    if (line & 1) {            // (Z[k] - conj Z[L-k]) / (2i)
+      double2 d = csub(a, b);
+      v = make_double2(0.5 * d.y, -0.5 * d.x);
+    } else {
+      double2 s = cadd(a, b);
+      v = make_double2(0.5 * s.x, 0.5 * s.y);
+    }
+    out[(l0 + line) * h + k] = v;
+  }
+}
+
+// x pass, inverse: half spectra -> real lines (imaginary parts of the DC and
+// Nyquist bins are ignored, as a c2r transform does)
+__global__ void __launch_bounds__(256) fft_x_c2r(const double2* __restrict__ in, double* __restrict__ out,
+                                                 int nl, FftPlan pl, const double2* __restrict__ tw, int pairs_per_block) {
+  extern __shared__ double2 lds[];
+  const int L = pl.L, h = L / 2 + 1, ldl = L + 1;
+  double2* b0 = lds;
+  double2* b1 = lds + pairs_per_block * ldl;
+  const int p0 = blockIdx.x * pairs_per_block;
+  const long l0 = 2L * p0;
+  const int nlines_here = min(2 * pairs_per_block, nl - (int)l0);
+  const int npairs = (nlines_here + 1) / 2;
+  // Z[k] = A[k] + i B[k] with A, B extended by Hermitian symmetry; the
+  // imaginary parts of the DC and Nyquist bins do not enter
+  for (int i = threadIdx.x; i < pairs_per_block * L; i += blockDim.x) {
+    int p = i / L, k = i - p * L;
+    double2 z = make_double2(0.0, 0.0);
+    if (p < npairs) {
+      int kk = (k < h) ? k : L - k;
+      bool cj = k >= h;
+      long la = l0 + 2 * p, lb = la + 1;
+      double2 a = in[la * h + kk];
+      if (kk == 0 || 2 * kk == L) a.y = 0.0;
+      if (cj) a.y = -a.y;
+      double2 b = make_double2(0.0, 0.0);
+      if (2 * p + 1 < nlines_here) {
+        b = in[lb * h + kk];
+        if (kk == 0 || 2 * kk == L) b.y = 0.0;
+        if (cj) b.y = -b.y;
+      }
+      z = make_double2(a.x - b.y, a.y + b.x);
+    }
+    b0[p * ldl + k] = z;
+  }
+  __syncthreads();
+  double2* r = lds_fft(b0, b1, npairs, L, ldl, pl, tw, +1);
+  for (int i = threadIdx.x; i < 2 * pairs_per_block * L; i += blockDim.x) {
+    int line = i / L, pos = i - line * L;
+    if (line >= nlines_here) continue;
+    double2 z = r[(line >> 1) * ldl + pos];
+    out[(l0 + line) * L + pos] = (line & 1) ? z.y : z.x;
+  }
+}
+
+// c2c pass along a strided axis.  A line group is `gx` lines adjacent in x
+// (unit stride) starting at x0, of length L with element stride `stride`;
+// groups are enumerated over (outer index, x chunk); outer o has base offset
+// (o / M) * comp_stride + (o % M) * outer_stride.
+__global__ void __launch_bounds__(256) fft_strided(double2* __restrict__ data, int nx, int gx, int M, long comp_stride,
+                                                   long outer_stride, long stride, FftPlan pl,
+                                                   const double2* __restrict__ tw, int sign) {
+  extern __shared__ double2 lds[];
+  const int L = pl.L, ldl = L + 1;
+  double2* b0 = lds;
+  double2* b1 = lds + gx * ldl;
+  const int nchunk = (nx + gx - 1) / gx;
+  const int o = blockIdx.x / nchunk, ch = blockIdx.x - o * nchunk;
+  const int x0 = ch * gx;
+  const int nlx = min(gx, nx - x0);
+  double2* base = data + (long)(o / M) * comp_stride + (long)(o % M) * outer_stride + x0;
+  for (int i = threadIdx.x; i < gx * L; i += blockDim.x) {
+    int j = i / gx, x = i - j * gx;
+    b0[x * ldl + j] = x < nlx ? base[(long)j * stride + x] : make_double2(0.0, 0.0);
+  }
+  __syncthreads();
+  double2* r = lds_fft(b0, b1, gx, L, ldl, pl, tw, sign);
+  for (int i = threadIdx.x; i < gx * L; i += blockDim.x) {
+    int j = i / gx, x = i - j * gx;
+    if (x < nlx) base[(long)j * stride + x] = r[x * ldl + j];
+  }
+}
+
+// ---------------------------------------------------------------- Gamma + div
+__device__ __forceinline__ double wrap_freq(int k, int n) { return (double)(k > n / 2 ? k - n : k); }
+
+__global__ void __launch_bounds__(256) gamma_div_kernel(double2* __restrict__ Z, GammaArgs g, double* __restrict__ div_partial) {
+  const long Nk = (long)g.h * g.n[1] * g.n[2];
+  const long f = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  double contrib = 0.0;
+  if (f < Nk) {
+    int kx = (int)(f % g.h);
+    long r = f / g.h;
+    int ky = (int)(r % g.n[1]);
+    int kz = (int)(r / g.n[1]);
+    const double TAU = 6.283185307179586;
+    double a[3];
+    a[0] = TAU * (double)kx / g.size[0];
+    a[1] = TAU * wrap_freq(ky, g.n[1]) / g.size[1];
+    a[2] = TAU * wrap_freq(kz, g.n[2]) / g.size[2];
+    // first-derivative frequencies: Nyquist set to zero on even grids
+    if (g.n[0] % 2 == 0 && 2 * kx == g.n[0]) a[0] = 0.0;
+    if (g.n[1] % 2 == 0 && 2 * ky == g.n[1]) a[1] = 0.0;
+    if (g.n[2] % 2 == 0 && 2 * kz == g.n[2]) a[2] = 0.0;
+    double2 P[9];
+#pragma unroll
+    for (int c = 0; c < 9; c++) P[c] = Z[c * Nk + f];
+    if (g.do_div) {
+      // |P_hat (conj(-xi) s)|^2 with conj(-xi) = i a
+      double s = 0.0;
+#pragma unroll
+      for (int l = 0; l < 3; l++) {
+        double2 u = make_double2(0.0, 0.0);
+#pragma unroll
+        for (int m = 0; m < 3; m++) {
+          double w = a[m] * g.rg[m];
+          u.x = fma(P[l + 3 * m].x, w, u.x);
+          u.y = fma(P[l + 3 * m].y, w, u.y);
+        }
+        s = fma(u.x, u.x, fma(u.y, u.y, s));
+      }
+      double wgt;
+      if (g.h == 1) wgt = 2.0;
+      else wgt = (kx >= 1 && kx <= g.h - 2) ? 2.0 : 1.0;
+      contrib = wgt * s;
+    }
+    if (g.do_gamma) {
+      if (f == 0) {
+#pragma unroll
+        for (int c = 0; c < 9; c++) Z[c * Nk] = make_double2(g.aim[c], 0.0);
+      } else {
+        double aa[6] = {a[0] * a[0], a[1] * a[1], a[2] * a[2], a[1] * a[2], a[0] * a[2], a[0] * a[1]};
+        double A[9];
+#pragma unroll
+        for (int q = 0; q < 9; q++) {
+          double s = 0.0;
+#pragma unroll
+          for (int p = 0; p < 6; p++) s = fma(g.Ccoef[q][p], aa[p], s);
+          A[q] = s;                      // A_lm, q = 3 l + m
+        }
+        double detA = A[0] * (A[4] * A[8] - A[5] * A[7]) - A[1] * (A[3] * A[8] - A[5] * A[6])
+                    + A[2] * (A[3] * A[7] - A[4] * A[6]);
+        if (fabs(detA) > 1.0e-16) {
+          double inv[9];
+          double r = 1.0 / detA;
+          inv[0] = (A[4] * A[8] - A[5] * A[7]) * r;
+          inv[1] = (A[2] * A[7] - A[1] * A[8]) * r;
+          inv[2] = (A[1] * A[5] - A[2] * A[4]) * r;
+          inv[3] = (A[5] * A[6] - A[3] * A[8]) * r;
+          inv[4] = (A[0] * A[8] - A[2] * A[6]) * r;
+          inv[5] = (A[2] * A[3] - A[0] * A[5]) * r;
+          inv[6] = (A[3] * A[7] - A[4] * A[6]) * r;
+          inv[7] = (A[1] * A[6] - A[0] * A[7]) * r;
+          inv[8] = (A[0] * A[4] - A[1] * A[3]) * r;
+          double2 v[3], w[3];
+#pragma unroll
+          for (int n = 0; n < 3; n++) {
+            double2 s = make_double2(0.0, 0.0);
+#pragma unroll
+            for (int o2 = 0; o2 < 3; o2++) {
+              s.x = fma(P[n + 3 * o2].x, a[o2], s.x);
+              s.y = fma(P[n + 3 * o2].y, a[o2], s.y);
+            }
+            v[n] = s;
+          }
+#pragma unroll
+          for (int l = 0; l < 3; l++) {
+            double2 s = make_double2(0.0, 0.0);
+#pragma unroll
+            for (int n = 0; n < 3; n++) {
+              s.x = fma(inv[3 * l + n], v[n].x, s.x);
+              s.y = fma(inv[3 * l + n], v[n].y, s.y);
+            }
+            w[l] = s;
+          }
+#pragma unroll
+          for (int l = 0; l < 3; l++)
+#pragma unroll
+            for (int m = 0; m < 3; m++) Z[(l + 3 * m) * Nk + f] = make_double2(w[l].x * a[m], w[l].y * a[m]);
+        } else {
+#pragma unroll
+          for (int c = 0; c < 9; c++) Z[c * Nk + f] = make_double2(0.0, 0.0);
+        }
+      }
+    }
+  }
+  if (g.do_div) {
+    __shared__ double red[256];
+    red[threadIdx.x] = contrib;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+      if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) div_partial[blockIdx.x] = red[0];
+  }
+}
+
+// updateCoords: per frequency u_hat = F_hat xi / (conj(-xi) . xi), xi second-derivative set
+__global__ void coords_freq_kernel(const double2* __restrict__ Fh, double2* __restrict__ U, GammaArgs g) {
+  const long Nk = (long)g.h * g.n[1] * g.n[2];
+  const long f = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= Nk) return;
+  int kx = (int)(f % g.h);
+  long r = f / g.h;
+  int ky = (int)(r % g.n[1]);
+  int kz = (int)(r / g.n[1]);
+  const double TAU = 6.283185307179586;
+  double a[3] = {TAU * (double)kx / g.size[0], TAU * wrap_freq(ky, g.n[1]) / g.size[1],
+                 TAU * wrap_freq(kz, g.n[2]) / g.size[2]};
+  if (f == 0) {
+    for (int l = 0; l < 3; l++) U[l * Nk] = make_double2(0.0, 0.0);
+    return;
+  }
+  // xi = i a: F_hat xi = i (F_hat a);  conj(-xi).xi = -|a|^2
+  double den = -(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]);
+  for (int l = 0; l < 3; l++) {
+    double2 s = make_double2(0.0, 0.0);
+    for (int m = 0; m < 3; m++) {
+      double2 p = Fh[(l + 3 * m) * Nk + f];
+      s.x = fma(p.x, a[m], s.x);
+      s.y = fma(p.y, a[m], s.y);
+    }
+    // i s / den
+    U[l * Nk + f] = make_double2(-s.y / den, s.x / den);
+  }
+}
+
+__global__ void coords_nodes_kernel(const double* __restrict__ u, CoordsArgs c, double* __restrict__ x_n,
+                                    double* __restrict__ x_p) {
+  const int N1 = c.n[0], N2 = c.n[1], N3 = c.n[2];
+  const long N = (long)N1 * N2 * N3;
+  const long nn = (long)(N1 + 1) * (N2 + 1) * (N3 + 1);
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < nn) {
+    int i = (int)(t % (N1 + 1));
+    long r = t / (N1 + 1);
+    int j = (int)(r % (N2 + 1));
+    int k = (int)(r / (N2 + 1));
+    double pos[3] = {c.step[0] * i, c.step[1] * j, c.step[2] * k};
+    double x[3];
+    for (int a = 0; a < 3; a++) x[a] = c.Favg[a] * pos[0] + c.Favg[a + 3] * pos[1] + c.Favg[a + 6] * pos[2];
+    for (int nb = 0; nb < 8; nb++) {
+      int dx = (nb == 1 || nb == 2 || nb == 5 || nb == 6), dy = (nb == 2 || nb == 3 || nb == 6 || nb == 7), dz = nb >= 4;
+      int mx = (i + dx - 1 + N1) % N1, my = (j + dy - 1 + N2) % N2, mz = (k + dz - 1 + N3) % N3;
+      long ci = mx + (long)N1 * (my + (long)N2 * mz);
+      for (int a = 0; a < 3; a++) x[a] = x[a] + u[a * N + ci] * c.wgt * 0.125;
+    }
+    for (int a = 0; a < 3; a++) x_n[3 * t + a] = x[a];
+  }
+  if (t < N) {
+    int i = (int)(t % N1);
+    long r = t / N1;
+    int j = (int)(r % N2);
+    int k = (int)(r / N2);
+    double pos[3] = {c.step[0] * (i + 1 - 0.5), c.step[1] * (j + 1 - 0.5), c.step[2] * (k + 1 - 0.5)};
+    for (int a = 0; a < 3; a++)
+      x_p[3 * t + a] = u[a * N + t] * c.wgt + (c.Favg[a] * pos[0] + c.Favg[a + 3] * pos[1] + c.Favg[a + 6] * pos[2]);
+  }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------- host
+static FftPlan make_plan(int L) {
+  FftPlan p{};
+  p.L = L;
+  int n = L;
+  while (n % 4 == 0 && n > 1) { p.radix[p.nstages++] = 4; n /= 4; }
+  while (n % 2 == 0) { p.radix[p.nstages++] = 2; n /= 2; }
+  for (int f = 3; n > 1; f += 2)
+    while (n % f == 0) { p.radix[p.nstages++] = f; n /= f; }
+  return p;
+}
+
+int dmk_fft_setup(DmkFFT* fft, const int cells[3]) {
+  for (int d = 0; d < 3; d++) {
+    fft->n[d] = cells[d];
+    fft->plan[d] = make_plan(cells[d]);
+    if (fft->plan[d].nstages > 16) return 1;
+    int L = cells[d];
+    double2* host = (double2*)malloc(sizeof(double2) * L);
+    for (int k = 0; k < L; k++) {
+      long double ang = -2.0L * 3.14159265358979323846264338327950288L * (long double)k / (long double)L;
+      host[k] = make_double2((double)cosl(ang), (double)sinl(ang));
+    }
+    if (hipMalloc(&fft->tw[d], sizeof(double2) * L) != hipSuccess) { free(host); return 2; }
+    if (hipMemcpy(fft->tw[d], host, sizeof(double2) * L, hipMemcpyHostToDevice) != hipSuccess) { free(host); return 2; }
+    free(host);
+  }
+  fft->h = cells[0] / 2 + 1;
+  return 0;
+}
+
+void dmk_fft_free(DmkFFT* fft) {
+  for (int d = 0; d < 3; d++) if (fft->tw[d]) hipFree(fft->tw[d]);
+}
+
+static int x_pairs_per_block(int L) { int p = 1024 / L; return p < 1 ? 1 : p; }
+static int gx_for(int L) { int g = 1024 / L; if (g > 32) g = 32; return g < 1 ? 1 : g; }
+
+int dmk_fft_r2c(const DmkFFT* fft, int howmany, const double* in, double2* out, hipStream_t s) {
+  const int N1 = fft->n[0], N2 = fft->n[1], N3 = fft->n[2], h = fft->h;
+  const long Nk = (long)h * N2 * N3;
+  int nl = howmany * N2 * N3;
+  int ppb = x_pairs_per_block(N1);
+  int nblk = (nl + 2 * ppb - 1) / (2 * ppb);
+  size_t lds = sizeof(double2) * 2 * ppb * (N1 + 1);
+  hipLaunchKernelGGL(fft_x_r2c, dim3(nblk), dim3(256), lds, s, in, out, nl, fft->plan[0], fft->tw[0], ppb);
+  if (N2 > 1) {
+    int gx = gx_for(N2);
+    int nch = (h + gx - 1) / gx;
+    size_t l2 = sizeof(double2) * 2 * gx * (N2 + 1);
+    hipLaunchKernelGGL(fft_strided, dim3(howmany * N3 * nch), dim3(256), l2, s, out, h, gx, N3, Nk, (long)h * N2,
+                       (long)h, fft->plan[1], fft->tw[1], -1);
+  }
+  if (N3 > 1) {
+    int gx = gx_for(N3);
+    int nch = (h + gx - 1) / gx;
+    size_t l3 = sizeof(double2) * 2 * gx * (N3 + 1);
+    hipLaunchKernelGGL(fft_strided, dim3(howmany * N2 * nch), dim3(256), l3, s, out, h, gx, N2, Nk, (long)h,
+                       (long)h * N2, fft->plan[2], fft->tw[2], -1);
+  }
+  return hipGetLastError() == hipSuccess ? 0 : 2;
+}
+
+// in is overwritten (the y and z passes run in place)
+int dmk_fft_c2r(const DmkFFT* fft, int howmany, double2* in, double* out, hipStream_t s) {
+  const int N1 = fft->n[0], N2 = fft->n[1], N3 = fft->n[2], h = fft->h;
+  const long Nk = (long)h * N2 * N3;
+  if (N3 > 1) {
+    int gx = gx_for(N3);
+    int nch = (h + gx - 1) / gx;
+    size_t l3 = sizeof(double2) * 2 * gx * (N3 + 1);
+    hipLaunchKernelGGL(fft_strided, dim3(howmany * N2 * nch), dim3(256), l3, s, in, h, gx, N2, Nk, (long)h,
+                       (long)h * N2, fft->plan[2], fft->tw[2], +1);
+  }
+  if (N2 > 1) {
+    int gx = gx_for(N2);
+    int nch = (h + gx - 1) / gx;
+    size_t l2 = sizeof(double2) * 2 * gx * (N2 + 1);
+    hipLaunchKernelGGL(fft_strided, dim3(howmany * N3 * nch), dim3(256), l2, s, in, h, gx, N3, Nk, (long)h * N2,
+                       (long)h, fft->plan[1], fft->tw[1], +1);
+  }
+  int nl = howmany * N2 * N3;
+  int ppb = x_pairs_per_block(N1);
+  int nblk = (nl + 2 * ppb - 1) / (2 * ppb);
+  size_t lds = sizeof(double2) * 2 * ppb * (N1 + 1);
+  hipLaunchKernelGGL(fft_x_c2r, dim3(nblk), dim3(256), lds, s, in, out, nl, fft->plan[0], fft->tw[0], ppb);
+  return hipGetLastError() == hipSuccess ? 0 : 2;
+}
+
+int dmk_gamma_div(double2* Z, const GammaArgs* g, double* div_partial, hipStream_t s) {
+  const long Nk = (long)g->h * g->n[1] * g->n[2];
+  int nblk = (int)((Nk + 255) / 256);
+  hipLaunchKernelGGL(gamma_div_kernel, dim3(nblk), dim3(256), 0, s, Z, *g, div_partial);
+  return hipGetLastError() == hipSuccess ? 0 : 2;
+}
+
+int dmk_gamma_div_blocks(const GammaArgs* g) {
+  const long Nk = (long)g->h * g->n[1] * g->n[2];
+  return (int)((Nk + 255) / 256);
+}
+
+int dmk_coords_freq(const double2* Fh, double2* U, const GammaArgs* g, hipStream_t s) {
+  const long Nk = (long)g->h * g->n[1] * g->n[2];
+  hipLaunchKernelGGL(coords_freq_kernel, dim3((unsigned)((Nk + 255) / 256)), dim3(256), 0, s, Fh, U, *g);
+  return hipGetLastError() == hipSuccess ? 0 : 2;
+}
+
+int dmk_coords_nodes(const double* u, const CoordsArgs* c, double* x_n, double* x_p, hipStream_t s) {
+  long nn = (long)(c->n[0] + 1) * (c->n[1] + 1) * (c->n[2] + 1);
+  hipLaunchKernelGGL(coords_nodes_kernel, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, s, u, *c, x_n, x_p);
+  return hipGetLastError() == hipSuccess ? 0 : 2;
+}

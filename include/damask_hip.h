@@ -1,0 +1,166 @@
+/*
+ * C-ABI of the MI355X grid spectral-basic mechanics library (libdamask_hip.so).
+ *
+ * Drop-in boundary for DAMASK_grid's mechanical solver plug-in: the five
+ * procedure pointers bound in src/grid/DAMASK_grid.f90:110-119,171-178
+ * (init / forward / solution / updateCoords / restartWrite of
+ * src/grid/grid_mech_spectral_basic.f90:91-96) plus the material-point entry
+ * homogenization_mechanical_response (src/homogenization.f90:294).
+ *
+ * Conventions (those of the Fortran arrays, so ISO_C_BINDING callers pass
+ * their arrays unchanged):
+ *   - a 3x3 tensor is 9 doubles, T_ij at [i + 3 j] (column-major);
+ *   - per-cell fields are (3,3,N) column-major: cell e, T_ij at [i + 3 j + 9 e];
+ *     cells run x fastest, then y, then z;
+ *   - 3x3x3x3 tensors are 81 doubles, T_ijkl at [i + 3 j + 9 k + 27 l];
+ *   - masks are int32, 1 where the load case has 'x' (component not prescribed);
+ *   - quaternions (w, x, y, z), passive, P = -1 (src/rotations.f90:57).
+ * Every function returns 0 on success, a DMK_ERR_* code otherwise; the text of
+ * the last error is available from dmk_last_error().  Non-convergence of a
+ * solution is not an error: it is reported through *converged = 0 so that the
+ * caller applies its cutback (src/grid/DAMASK_grid.f90:398-408).
+ * One handle = one RVE on one GPU; calls on one handle must not overlap.
+ */
+#ifndef DAMASK_HIP_H
+#define DAMASK_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DMK_ERR_ARG 1
+#define DMK_ERR_HIP 2
+#define DMK_ERR_CONFIG 3
+
+/* phenopowerlaw + Hooke phase as given in material.yaml
+ * (src/phase_mechanical_plastic_phenopowerlaw.f90:146-285,
+ *  src/phase_mechanical_elastic.f90:58-79).  Per-family arrays hold n_fam entries. */
+typedef struct {
+  int32_t lattice;          /* 0 cF, 1 cI */
+  int32_t n_fam;
+  int32_t N_sl[4];
+  double C_11, C_12, C_44;
+  double dot_gamma_0[4], n[4], a[4], h_0[4], xi_0[4], xi_inf[4];
+  double c_1[4], c_2[4], f_sat[4];
+  int32_t n_h;
+  double h_sl_sl[32];
+  int32_t n_nS;             /* number of non-Schmid coefficients (cI) */
+  double a_nS[6];
+  double atol_xi, atol_gamma;
+} dmk_phase_desc;
+
+/* numerics.yaml: solver:grid:mechanical, solver:grid, phase:mechanical
+ * (src/grid/grid_mech_spectral_basic.f90:130-136, src/phase_mechanical.f90:281-295) */
+typedef struct {
+  int32_t N_iter_min, N_iter_max;
+  double eps_abs_div, eps_rel_div, eps_abs_P, eps_rel_P;
+  int32_t update_gamma;
+  int32_t divergence_correction;   /* 0 none, 1 grid+size, 2 size */
+  int32_t solver;                  /* 0 NGMRES (reference default), 1 Richardson */
+  int32_t ngmres_m;
+  int32_t snes_max_it;
+  int32_t N_cutback_max;
+  double r_cutback_min, r_cutback, r_increase, eps_rel_state;
+  int32_t N_iter_state_max, N_iter_Lp_max;
+  double r_linesearch_Lp, eps_rel_Lp, eps_abs_Lp;
+  int32_t f_update_jacobi_Lp;
+  int32_t cref_rule;               /* 0 strict first-extremum rule, 1 with 1e-12 dead band */
+} dmk_numerics;
+
+/* mechanical boundary conditions of one load step (src/grid/DAMASK_grid.f90:496-600) */
+typedef struct {
+  int32_t def_type;                /* 0 L, 1 dot_F, 2 F */
+  double def_values[9];
+  int32_t def_mask[9];
+  int32_t stress_type;             /* -1 none, 0 P, 1 dot_P */
+  double stress_values[9];
+  int32_t stress_mask[9];
+  double q_R[4];                   /* rotation of the load frame */
+  double t, r;
+  int32_t N, f_out, estimate_rate;
+} dmk_loadcase;
+
+typedef struct dmk_handle dmk_handle;
+
+/* Allocate an RVE: cells/size (discretization_grid.f90:106-254), one phase
+ * index (0-based) and initial orientation per cell (material.f90:95-218,
+ * phase_mechanical.f90:249-266), then evaluate the constitutive response at
+ * dt = 0 with F = I and set C_ref (grid_mech_spectral_basic.f90:103-237). */
+int dmk_mech_init(const int32_t cells[3], const double size[3], int32_t n_phases,
+                  const dmk_phase_desc* phases, const int32_t* phase_of_cell,
+                  const double* q0_of_cell, const dmk_numerics* num, dmk_handle** h);
+
+/* grid_mech_spectral_basic_forward (grid_mech_spectral_basic.f90:284-355) */
+int dmk_mech_forward(dmk_handle* h, int32_t cutBack, int32_t guess, double dt, double dt_old,
+                     double t_remaining, const dmk_loadcase* bc);
+
+/* materialpoint_forward: state0 <- state (phase_mechanical.f90:987-1002) */
+int dmk_mech_materialpoint_forward(dmk_handle* h);
+
+/* grid_mech_spectral_basic_solution (grid_mech_spectral_basic.f90:243-277) */
+int dmk_mech_solution(dmk_handle* h, int32_t* iterations, int32_t* converged, int32_t* n_residuals);
+
+/* grid_mech_spectral_basic_updateCoords / utilities_updateCoords
+ * (spectral_utilities.f90:857-968): x_p 3N cell centres, x_n
+ * 3 (N1+1)(N2+1)(N3+1) nodes, either may be NULL */
+int dmk_mech_update_coords(dmk_handle* h, double* x_p, double* x_n);
+
+/* homogenization_mechanical_response + utilities_constitutiveResponse
+ * (homogenization.f90:294-362, grid_utilities.f90:133-207) for a given F field
+ * (host arrays, (3,3,N)).  P (3,3,N) out; P_av (9), C_volAvg (81),
+ * C_minMaxAvg (81) out; status = max cell status (0 ok). */
+int dmk_constitutive(dmk_handle* h, double dt, const double* F, double* P, double* P_av,
+                     double* C_volAvg, double* C_minMaxAvg, int32_t* status);
+
+/* same with device-resident F and P ((3,3,N) layout, for PyTorch callers) */
+int dmk_constitutive_device(dmk_handle* h, double dt, const double* F_dev, double* P_dev,
+                            int32_t* status);
+
+/* per-cell fields to host buffers: 0 F, 1 P, 2 F_e, 3 F_p, 4 L_p, 5 S, 6 F_i,
+ * 7 O (4/cell), 8 xi_sl (nsl/cell), 9 gamma_sl (nsl/cell), 10 dPdF (81/cell) */
+int dmk_mech_get_field(dmk_handle* h, int32_t what, double* dst);
+/* restart data (grid_mech_spectral_basic.f90:198-232, phase_mechanical.f90:1248-1280):
+ * 0 F, 3 F_p, 4 L_p, 5 S, 8 xi_sl, 9 gamma_sl, 11 F_lastinc */
+int dmk_mech_set_field(dmk_handle* h, int32_t what, const double* src);
+
+/* solver scalars: 0 F_aim (9), 1 P_av (9), 2 P_aim (9), 3 C_volAvg (81),
+ * 4 C_minMaxAvg (81), 5 {err_div, err_BC}, 6 F_aimDot (9), 7 F_aim_lastinc (9),
+ * 8 C_volAvgLastInc (81) */
+int dmk_mech_get_scalars(dmk_handle* h, int32_t what, double* dst);
+int dmk_mech_set_scalars(dmk_handle* h, int32_t what, const double* src);
+
+/* DAMASK_grid increment/cutback loop for one load case (DAMASK_grid.f90:288-460).
+ * stats: 4 doubles per increment (t, cutback level, iterations, residual
+ * evaluations); P_av_hist/F_aim_hist: 9 per increment (may be NULL);
+ * n_updates: material-point updates performed.  Returns 0, or 950 when the
+ * cutback limit is exhausted. */
+int dmk_grid_run_loadcase(dmk_handle* h, const dmk_loadcase* lc, double* t_io, double* stats,
+                          double* P_av_hist, double* F_aim_hist, int64_t* n_updates);
+
+/* utilities_GammaConvolution (spectral_utilities.f90:436-501) on a (3,3,N)
+ * host field with the handle's reference stiffness; aim = DC value */
+int dmk_gamma_convolution(dmk_handle* h, const double* field, const double aim[9], double* out);
+/* utilities_divergenceRMS (spectral_utilities.f90:650-691) of a (3,3,N) host field */
+int dmk_divergence_rms(dmk_handle* h, const double* field, double* out);
+/* unnormalised 3-D r2c / c2r of howmany (<= 9) interleaved fields, FFTW's
+ * conventions (spectral_utilities.f90:239-261): real (howmany, N1, N2, N3) <->
+ * complex (howmany, N1/2+1, N2, N3) as (re, im) pairs; c2r ignores the
+ * imaginary parts of the self-conjugate bins */
+int dmk_fft_forward(dmk_handle* h, int32_t howmany, const double* in, double* out);
+int dmk_fft_backward(dmk_handle* h, int32_t howmany, const double* in, double* out);
+
+/* totals since init: material-point updates, Newton and state iterations,
+ * residual evaluations */
+int dmk_mech_counters(dmk_handle* h, int64_t out[4]);
+
+void dmk_mech_destroy(dmk_handle* h);
+const char* dmk_last_error(void);
+int dmk_nsl(const dmk_handle* h);
+int dmk_device_synchronize(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

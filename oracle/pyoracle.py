@@ -1,0 +1,280 @@
+"""ctypes front end of the CPU oracle (liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by the smoke() check and by the
+CPU-baseline leg of bench.py.  Product code never imports this module.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "liboracle.so")
+
+NSL_MAX = 48
+
+
+class PhaseDesc(C.Structure):
+    _fields_ = [("lattice", C.c_int), ("n_fam", C.c_int), ("N_sl", C.c_int * 4),
+                ("C_11", C.c_double), ("C_12", C.c_double), ("C_44", C.c_double),
+                ("dot_gamma_0", C.c_double * 4), ("n", C.c_double * 4), ("a", C.c_double * 4),
+                ("h_0", C.c_double * 4), ("xi_0", C.c_double * 4), ("xi_inf", C.c_double * 4),
+                ("c_1", C.c_double * 4), ("c_2", C.c_double * 4), ("f_sat", C.c_double * 4),
+                ("n_h", C.c_int), ("h_sl_sl", C.c_double * 32), ("n_nS", C.c_int),
+                ("a_nS", C.c_double * 6), ("atol_xi", C.c_double), ("atol_gamma", C.c_double)]
+
+
+class Phase(C.Structure):
+    _fields_ = [("nsl", C.c_int), ("C66", C.c_double * 36), ("C3333", C.c_double * 81),
+                ("P_sl", C.c_double * (9 * NSL_MAX)), ("P_nS_pos", C.c_double * (9 * NSL_MAX)),
+                ("P_nS_neg", C.c_double * (9 * NSL_MAX)), ("H", C.c_double * (NSL_MAX * NSL_MAX)),
+                ("dot_gamma_0", C.c_double * NSL_MAX), ("n", C.c_double * NSL_MAX),
+                ("a", C.c_double * NSL_MAX), ("h_0", C.c_double * NSL_MAX),
+                ("xi_0", C.c_double * NSL_MAX), ("xi_inf", C.c_double * NSL_MAX),
+                ("c_1", C.c_double * NSL_MAX), ("c_2", C.c_double * NSL_MAX),
+                ("f_sat", C.c_double * NSL_MAX), ("atol", C.c_double * (2 * NSL_MAX))]
+
+
+class NumMech(C.Structure):
+    _fields_ = [("r_cutback_min", C.c_double), ("r_cutback", C.c_double), ("r_increase", C.c_double),
+                ("eps_rel_state", C.c_double), ("N_iter_state_max", C.c_int), ("N_iter_Lp_max", C.c_int),
+                ("r_linesearch_Lp", C.c_double), ("eps_rel_Lp", C.c_double), ("eps_abs_Lp", C.c_double),
+                ("f_update_jacobi_Lp", C.c_int), ("N_iter_Li_max", C.c_int),
+                ("r_linesearch_Li", C.c_double), ("eps_rel_Li", C.c_double), ("eps_abs_Li", C.c_double),
+                ("f_update_jacobi_Li", C.c_int)]
+
+
+class NumGrid(C.Structure):
+    _fields_ = [("N_iter_min", C.c_int), ("N_iter_max", C.c_int), ("eps_abs_div", C.c_double),
+                ("eps_rel_div", C.c_double), ("eps_abs_P", C.c_double), ("eps_rel_P", C.c_double),
+                ("update_gamma", C.c_int), ("divergence_correction", C.c_int), ("solver", C.c_int),
+                ("ngmres_m", C.c_int), ("snes_max_it", C.c_int), ("N_cutback_max", C.c_int)]
+
+
+class LoadCase(C.Structure):
+    _fields_ = [("def_type", C.c_int), ("def_values", C.c_double * 9), ("def_mask", C.c_int32 * 9),
+                ("stress_type", C.c_int), ("stress_values", C.c_double * 9),
+                ("stress_mask", C.c_int32 * 9), ("q_R", C.c_double * 4), ("t", C.c_double),
+                ("r", C.c_double), ("N", C.c_int), ("f_out", C.c_int), ("estimate_rate", C.c_int)]
+
+
+def build(force: bool = False) -> str:
+    if force or not os.path.exists(LIB):
+        subprocess.run(["make", "-B" if force else "-s", "-C", HERE, "liboracle.so"], check=True,
+                       stdout=subprocess.DEVNULL)
+    return LIB
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        L = C.CDLL(LIB)
+        vp, dp, ip = C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int32)
+        L.orc_mp_create.restype = vp
+        L.orc_mp_create.argtypes = [C.c_int, C.c_int, C.POINTER(Phase), ip, dp, C.POINTER(NumMech)]
+        L.orc_mp_destroy.argtypes = [vp]
+        L.orc_mp_response.restype = C.c_int
+        L.orc_mp_response.argtypes = [vp, C.c_double, dp, dp, dp]
+        L.orc_mp_forward.argtypes = [vp]
+        L.orc_mp_get.argtypes = [vp, C.c_int, dp]
+        L.orc_mp_set.argtypes = [vp, C.c_int, dp]
+        L.orc_mp_nsl.restype = C.c_int
+        L.orc_mp_nsl.argtypes = [vp]
+        L.orc_mp_counters.argtypes = [vp, C.POINTER(C.c_int64)] * 1 + [C.POINTER(C.c_int64)] * 2
+        L.orc_dPdF.argtypes = [vp, C.c_int, C.c_double, dp]
+        L.orc_integrate_stress.restype = C.c_int
+        L.orc_integrate_stress.argtypes = [vp, C.c_int, dp, dp, dp, C.c_double]
+        L.orc_phase_build.restype = C.c_int
+        L.orc_phase_build.argtypes = [C.POINTER(PhaseDesc), C.POINTER(Phase)]
+        L.orc_kinetics_sl.argtypes = [C.POINTER(Phase), dp, dp, dp, dp]
+        L.orc_hooke.argtypes = [C.POINTER(Phase), dp, dp, dp, dp, dp]
+        L.orc_default_num_mech.argtypes = [C.POINTER(NumMech)]
+        L.orc_default_num_grid.argtypes = [C.POINTER(NumGrid)]
+        L.orc_qu2om.argtypes = [dp, dp]
+        L.orc_om2qu.argtypes = [dp, dp]
+        L.orc_rotate33.argtypes = [dp, dp, C.c_int, dp]
+        L.orc_rotate3333.argtypes = [dp, dp, C.c_int, dp]
+        L.orc_rotational_part.argtypes = [dp, dp]
+        L.orc_invert.restype = C.c_int
+        L.orc_invert.argtypes = [C.c_int, dp, dp]
+        L.orc_schmid_slip.argtypes = [C.c_int, C.c_int, C.POINTER(C.c_int), C.c_int, C.c_int, dp, dp]
+        L.orc_interaction_slipslip.argtypes = [C.c_int, C.c_int, C.POINTER(C.c_int), C.c_int, dp, dp]
+        L.orc_grid_create.restype = vp
+        L.orc_grid_create.argtypes = [C.POINTER(C.c_int), dp, C.POINTER(NumGrid)]
+        L.orc_grid_destroy.argtypes = [vp]
+        L.orc_fft_r2c.argtypes = [vp, C.c_int, dp, dp]
+        L.orc_fft_c2r.argtypes = [vp, C.c_int, dp, dp]
+        L.orc_update_gamma.argtypes = [vp, dp]
+        L.orc_gamma_convolution.argtypes = [vp, dp, dp, dp]
+        L.orc_divergence_rms.restype = C.c_double
+        L.orc_divergence_rms.argtypes = [vp, dp]
+        L.orc_update_coords.argtypes = [vp, dp, dp, dp]
+        L.orc_solver_create.restype = vp
+        L.orc_solver_create.argtypes = [vp, vp, C.POINTER(NumGrid)]
+        L.orc_solver_destroy.argtypes = [vp]
+        L.orc_solver_forward.argtypes = [vp, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
+                                         C.POINTER(LoadCase)]
+        L.orc_solver_solution.restype = C.c_int
+        L.orc_solver_solution.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_int)]
+        L.orc_solver_update_coords.argtypes = [vp, dp, dp]
+        L.orc_solver_get.argtypes = [vp, C.c_int, dp]
+        L.orc_solver_set_cref_rule.argtypes = [vp, C.c_int]
+        L.orc_solver_updates.restype = C.c_int64
+        L.orc_solver_updates.argtypes = [vp]
+        L.orc_run_loadcase.restype = C.c_int
+        L.orc_run_loadcase.argtypes = [vp, C.POINTER(LoadCase), dp, dp, dp, dp, C.POINTER(C.c_int64)]
+        L.orc_set_threads.argtypes = [C.c_int]
+        _lib = L
+    return _lib
+
+
+def ptr(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def iptr(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_int32))
+
+
+def phase_desc(p) -> PhaseDesc:
+    """PhaseDesc from a damask_amd.config.Phase-like object."""
+    d = PhaseDesc()
+    d.lattice = {"cF": 0, "cI": 1}[p.lattice]
+    d.n_fam = len(p.N_sl)
+    for i, v in enumerate(p.N_sl):
+        d.N_sl[i] = v
+    d.C_11, d.C_12, d.C_44 = p.C_11, p.C_12, p.C_44
+    for name, src in (("dot_gamma_0", p.dot_gamma_0_sl), ("n", p.n_sl), ("a", p.a_sl),
+                      ("h_0", p.h_0_sl_sl), ("xi_0", p.xi_0_sl), ("xi_inf", p.xi_inf_sl),
+                      ("c_1", p.c_1), ("c_2", p.c_2), ("f_sat", p.f_sat_sl_tw)):
+        arr = getattr(d, name)
+        for i, v in enumerate(src):
+            arr[i] = v
+    d.n_h = len(p.h_sl_sl)
+    for i, v in enumerate(p.h_sl_sl):
+        d.h_sl_sl[i] = v
+    d.n_nS = len(p.a_nS)
+    for i, v in enumerate(p.a_nS):
+        d.a_nS[i] = v
+    d.atol_xi, d.atol_gamma = p.atol_xi, p.atol_gamma
+    return d
+
+
+def build_phase(p) -> Phase:
+    ph = Phase()
+    rc = lib().orc_phase_build(C.byref(phase_desc(p)), C.byref(ph))
+    if rc:
+        raise ValueError(f"orc_phase_build failed ({rc})")
+    return ph
+
+
+def num_mech(n=None) -> NumMech:
+    m = NumMech()
+    lib().orc_default_num_mech(C.byref(m))
+    if n is not None:
+        for f, _ in NumMech._fields_:
+            if hasattr(n, f):
+                setattr(m, f, getattr(n, f))
+    return m
+
+
+def num_grid(n=None) -> NumGrid:
+    g = NumGrid()
+    lib().orc_default_num_grid(C.byref(g))
+    if n is not None:
+        for f, _ in NumGrid._fields_:
+            if hasattr(n, f):
+                setattr(g, f, int(getattr(n, f)) if isinstance(getattr(g, f), int) else getattr(n, f))
+    return g
+
+
+def f33(a) -> np.ndarray:
+    """(3,3) [i,j] -> 9 doubles, Fortran column-major."""
+    return np.ascontiguousarray(np.asarray(a, float).T.reshape(9))
+
+
+def loadcase(lc) -> LoadCase:
+    L = LoadCase()
+    L.def_type = lc.def_type
+    for k, (v, m) in enumerate(zip(f33(lc.def_values), f33(lc.def_mask.astype(float)))):
+        L.def_values[k] = v
+        L.def_mask[k] = int(m)
+    L.stress_type = lc.stress_type
+    for k, (v, m) in enumerate(zip(f33(lc.stress_values), f33(lc.stress_mask.astype(float)))):
+        L.stress_values[k] = v
+        L.stress_mask[k] = int(m)
+    for k in range(4):
+        L.q_R[k] = lc.q_R[k]
+    L.t, L.r, L.N, L.f_out, L.estimate_rate = lc.t, lc.r, lc.N, min(lc.f_out, 2**31 - 1), int(lc.estimate_rate)
+    return L
+
+
+class Problem:
+    """Material point + spectral grid + solver built from parsed inputs."""
+
+    def __init__(self, grid, material, numerics=None, cref_rule: int = 1, threads: int = 0):
+        L = lib()
+        if threads:
+            L.orc_set_threads(threads)
+        maps = material.cell_maps(grid.material)
+        self.N = grid.n_cells
+        self.cells = np.asarray(grid.cells, np.int32)
+        self.size = np.asarray(grid.size, float)
+        self.phases = (Phase * len(material.phases))(*[build_phase(p) for p in material.phases])
+        self.phase_of = np.ascontiguousarray(maps["phase"], np.int32)
+        self.q0 = np.ascontiguousarray(maps["O"], float)
+        self.nm = num_mech(numerics)
+        self.ng = num_grid(numerics)
+        self.mp = L.orc_mp_create(self.N, len(material.phases), self.phases, iptr(self.phase_of),
+                                  ptr(self.q0), C.byref(self.nm))
+        self.grid = L.orc_grid_create(self.cells.ctypes.data_as(C.POINTER(C.c_int)), ptr(self.size),
+                                      C.byref(self.ng))
+        self.solver = L.orc_solver_create(self.grid, self.mp, C.byref(self.ng))
+        L.orc_solver_set_cref_rule(self.solver, cref_rule)
+        self.t = 0.0
+
+    def close(self):
+        L = lib()
+        if self.solver:
+            L.orc_solver_destroy(self.solver)
+            L.orc_grid_destroy(self.grid)
+            L.orc_mp_destroy(self.mp)
+            self.solver = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def get(self, what: int, width: int = 9) -> np.ndarray:
+        out = np.zeros(self.N * width)
+        lib().orc_mp_get(self.mp, what, ptr(out))
+        return out
+
+    def solver_get(self, what: int) -> np.ndarray:
+        size = {0: 9 * self.N, 1: 9 * self.N, 2: 9, 3: 9, 4: 81, 5: 81, 6: 9, 7: 2, 8: 81}[what]
+        out = np.zeros(size)
+        lib().orc_solver_get(self.solver, what, ptr(out))
+        return out
+
+    def run_loadcase(self, lc):
+        L = lib()
+        c = loadcase(lc)
+        stats = np.zeros(4 * lc.N)
+        P = np.zeros(9 * lc.N)
+        Fa = np.zeros(9 * lc.N)
+        t = np.zeros(1)
+        nu = C.c_int64(0)
+        rc = L.orc_run_loadcase(self.solver, C.byref(c), ptr(t), ptr(stats), ptr(P), ptr(Fa), C.byref(nu))
+        self.t = float(t[0])
+        return {"rc": rc, "stats": stats.reshape(-1, 4), "P_av": P.reshape(-1, 9),
+                "F_aim": Fa.reshape(-1, 9), "n_updates": nu.value}

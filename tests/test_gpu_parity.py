@@ -1,0 +1,150 @@
+"""GPU (libdamask_hip.so) against the CPU oracle on the same seeded inputs.
+
+Tolerances: per point 1e-6 relative, volume averages 1e-8 relative (FP64
+throughout); FFTs and the Gamma operator 1e-12 relative to the field norm.
+"""
+import copy
+
+import numpy as np
+import pytest
+
+from conftest import fcc_golden_inputs
+
+pytestmark = pytest.mark.gpu
+
+
+def _poly(cells, n_grains, seed, phase=None):
+    from damask_amd import config, workloads
+    grid, mat = workloads.polycrystal(cells, n_grains, seed, phase=phase)
+    return grid, config.parse_material(mat)
+
+
+def _pair(grid, material, numerics=None, oracle=None):
+    from damask_amd.config import Numerics
+    from damask_amd.grid import GridMechSpectralBasic
+    numerics = numerics or Numerics()
+    g = GridMechSpectralBasic(grid, material, numerics)
+    o = oracle.Problem(grid, material, numerics)
+    return g, o
+
+
+def _relmax(a, b):
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+
+
+@pytest.mark.parametrize("lattice", ["cF", "cI"])
+def test_init_state(gpu_lib, oracle, lattice):
+    from damask_amd import workloads
+    phase = workloads.FCC_PHASE if lattice == "cF" else workloads.BCC_PHASE
+    grid, mat = _poly([6, 5, 4], 7, 11, phase)
+    g, o = _pair(grid, mat, oracle=oracle)
+    # F_p = O.asMatrix at t = 0 (phase_mechanical.f90:255), after the dt = 0
+    # response, which re-inverts it (rounding only)
+    Fp_o = o.get(3).reshape(-1, 3, 3).transpose(0, 2, 1)
+    assert np.abs(g.field("F_p") - Fp_o).max() < 2e-15
+    # the dt = 0 response: C_volAvg and C_ref
+    Cv = o.solver_get(4).reshape(3, 3, 3, 3, order="F")
+    assert _relmax(g.scalar("C_volAvg"), Cv) < 1e-12
+    Cm = o.solver_get(5).reshape(3, 3, 3, 3, order="F")
+    assert _relmax(g.scalar("C_minMaxAvg"), Cm) < 1e-12
+
+
+@pytest.mark.parametrize("lattice,dt,scale", [("cF", 1.0, 2e-3), ("cF", 0.5, 6e-3), ("cI", 1.0, 3e-3)])
+def test_constitutive_response(gpu_lib, oracle, lattice, dt, scale):
+    """one homogenization_mechanical_response on a random F field, then a second on top"""
+    import ctypes as C
+    from damask_amd import workloads
+    phase = workloads.FCC_PHASE if lattice == "cF" else workloads.BCC_PHASE
+    grid, mat = _poly([8, 6, 5], 9, 3, phase)
+    g, o = _pair(grid, mat, oracle=oracle)
+    rng = np.random.default_rng(5)
+    N = grid.n_cells
+    F = np.eye(3) + scale * rng.standard_normal((N, 3, 3)) + np.diag([scale, -0.3 * scale, -0.3 * scale])
+    for step in range(2):
+        P_g, Pav_g, Cv_g, Cm_g, st_g = g.constitutive(F, dt)
+        Ff = np.ascontiguousarray(F.transpose(0, 2, 1).reshape(-1))
+        P_o = np.zeros_like(Ff)
+        dP_o = np.zeros(81 * N)
+        L = oracle.lib()
+        st_o = L.orc_mp_response(o.mp, dt, oracle.ptr(Ff), oracle.ptr(P_o), oracle.ptr(dP_o))
+        assert st_g == st_o == 0
+        P_o = P_o.reshape(-1, 3, 3).transpose(0, 2, 1)
+        assert _relmax(P_g, P_o) < 1e-8
+        rel_pt = np.abs(P_g - P_o).max(axis=(1, 2)) / np.abs(P_o).max(axis=(1, 2))
+        assert rel_pt.max() < 1e-6
+        dP_g = g.field("dPdF")
+        dP_o = dP_o.reshape(N, 81)
+        assert _relmax(dP_g, dP_o) < 1e-6
+        for name, w, what in (("xi_sl", None, 8), ("F_p", None, 3), ("L_p", None, 4)):
+            a = g.field(name)
+            b = o.get(what, mp_width := (a.shape[1] if a.ndim == 2 else 9))
+            b = b.reshape(a.shape[0], -1)
+            a2 = a.reshape(a.shape[0], -1) if a.ndim == 2 else a.transpose(0, 2, 1).reshape(a.shape[0], -1)
+            assert _relmax(a2, b) < 1e-6, name
+        F = F + scale * 0.5 * rng.standard_normal((N, 3, 3))
+
+
+@pytest.mark.parametrize("cells", [[6, 7, 8], [8, 8, 8], [10, 6, 5], [7, 9, 3], [16, 12, 20]])
+def test_fft_roundtrip_and_parity(gpu_lib, oracle, cells):
+    import ctypes as C
+    from damask_amd import _lib
+    grid, mat = _poly(cells, 4, 1)
+    g, o = _pair(grid, mat, oracle=oracle)
+    N = grid.n_cells
+    h = cells[0] // 2 + 1
+    Nk = h * cells[1] * cells[2]
+    rng = np.random.default_rng(0)
+    for howmany in (9, 3):
+        x = rng.random(howmany * N)
+        Xg = np.zeros(2 * howmany * Nk)
+        _lib.check(g.lib.dmk_fft_forward(g.h, howmany, _lib.dptr(x), _lib.dptr(Xg)))
+        Xo = np.zeros(2 * howmany * Nk)
+        oracle.lib().orc_fft_r2c(o.grid, howmany, oracle.ptr(x), oracle.ptr(Xo))
+        assert np.abs(Xg - Xo).max() < 1e-12 * np.abs(Xo).max()
+        # DC equals the sum (spectral_utilities.f90:998)
+        dc = Xg.reshape(-1, howmany, 2)[0, :, 0]
+        assert np.allclose(dc / x.reshape(-1, howmany).sum(0), 1.0, rtol=1e-12, atol=0)
+        y = np.zeros(howmany * N)
+        _lib.check(g.lib.dmk_fft_backward(g.h, howmany, _lib.dptr(Xg), _lib.dptr(y)))
+        assert np.abs(y / N - x).max() < 5e-15 * max(1.0, np.log2(N))
+
+
+@pytest.mark.parametrize("cells", [[6, 7, 8], [8, 8, 8], [5, 6, 4]])
+def test_gamma_and_div(gpu_lib, oracle, cells):
+    from damask_amd import _lib
+    grid, mat = _poly(cells, 5, 2)
+    g, o = _pair(grid, mat, oracle=oracle)
+    N = grid.n_cells
+    rng = np.random.default_rng(1)
+    P = 1e8 * rng.standard_normal(9 * N)
+    aim = rng.standard_normal(9) * 1e-3
+    out_g = np.zeros(9 * N)
+    _lib.check(g.lib.dmk_gamma_convolution(g.h, _lib.dptr(P), _lib.dptr(aim), _lib.dptr(out_g)))
+    out_o = np.zeros(9 * N)
+    oracle.lib().orc_gamma_convolution(o.grid, oracle.ptr(P), oracle.ptr(aim), oracle.ptr(out_o))
+    assert np.abs(out_g - out_o).max() < 1e-10 * np.abs(out_o).max()
+    d_g = np.zeros(1)
+    _lib.check(g.lib.dmk_divergence_rms(g.h, _lib.dptr(P), _lib.dptr(d_g)))
+    d_o = oracle.lib().orc_divergence_rms(o.grid, oracle.ptr(P))
+    assert abs(d_g[0] - d_o) < 1e-12 * d_o
+
+
+def test_solver_increments_fcc_golden_grid(gpu_lib, oracle):
+    """the golden 6x7x8 case, first 8 increments, GPU vs oracle"""
+    grid, mat, lcs = fcc_golden_inputs()
+    lc = copy.copy(lcs[0])
+    lc.N = 8
+    lc.t = 4.0
+    g, o = _pair(grid, mat, oracle=oracle)
+    rg = g.run_loadcase(lc)
+    ro = o.run_loadcase(lc)
+    assert rg["rc"] == ro["rc"] == 0
+    print("gpu iterations", rg["stats"][:, 2:].T, "oracle", ro["stats"][:, 2:].T)
+    assert np.array_equal(rg["stats"][:, 2], ro["stats"][:, 2])
+    assert _relmax(rg["P_av"], ro["P_av"]) < 1e-8
+    P_g = g.field("P")
+    P_o = o.get(1).reshape(-1, 3, 3).transpose(0, 2, 1)
+    assert _relmax(P_g, P_o) < 1e-6
+    F_g = g.field("F")
+    F_o = o.solver_get(0).reshape(-1, 3, 3).transpose(0, 2, 1)
+    assert np.abs(F_g - F_o).max() < 1e-8
