@@ -1,0 +1,155 @@
+#!/usr/bin/env python
+"""Benchmark: material-point updates/s of the spectral-basic FCC phenopowerlaw
+tension run on one MI355X.
+
+Workload: 64^3 cells, 200-grain seeded periodic Voronoi, uniform random
+orientations, FCC phenopowerlaw + Hooke, isostrain, uniaxial tension
+dot_F_xx = 1e-3/s with all other P components zero, 100 increments to 5 %
+(t = 50 s), FP64.  A step is one increment (forward, NGMRES solution,
+updateCoords).  W warm-up increments run first; the next K increments are
+timed.  One material-point update is one per-cell constitutive evaluation;
+updates/s = cells x residual evaluations / wall time of the timed increments.
+
+The CPU baseline is the oracle (a C/OpenMP restatement of the same algorithm)
+on a bounded sample of the same workload, timed on this host's cores.
+"""
+import argparse
+import copy
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_FP64_TFLOPS = 78.6      # MI355X FP64 (vector and matrix)
+PEAK_HBM_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=98, help="timed increments")
+    ap.add_argument("--warmup", type=int, default=2, help="untimed increments before the timed ones")
+    ap.add_argument("--cells", type=int, default=64)
+    ap.add_argument("--grains", type=int, default=200)
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--increments", type=int, default=100, help="increments of the full load case")
+    ap.add_argument("--strain", type=float, default=0.05)
+    ap.add_argument("--cpu-baseline", type=int, default=1)
+    ap.add_argument("--cpu-cells", type=int, default=32)
+    ap.add_argument("--cpu-increments", type=int, default=12)
+    ap.add_argument("--traffic-bytes", type=float, default=None,
+                    help="measured HBM bytes per stress-kernel launch (rocprofv3 --pmc)")
+    return ap.parse_args()
+
+
+def workload(cells, grains, seed, increments, strain):
+    from damask_amd import config, workloads
+    grid, mat = workloads.polycrystal([cells] * 3, grains, seed)
+    t = strain / 1.0e-3
+    _, lcs = config.parse_load(workloads.tension_x(t, increments))
+    return grid, config.parse_material(mat), lcs[0]
+
+
+def sub_loadcase(lc, n):
+    """first/next n increments of a constant-rate load case"""
+    s = copy.copy(lc)
+    s.t = lc.t / lc.N * n
+    s.N = n
+    return s
+
+
+def cpu_baseline(args):
+    from damask_amd import config
+    from oracle import pyoracle
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    grid, mat, lc = workload(args.cpu_cells, args.grains, args.seed, args.increments, args.strain)
+    pr = pyoracle.Problem(grid, mat, config.Numerics(), threads=threads)
+    t0 = time.perf_counter()
+    r = pr.run_loadcase(sub_loadcase(lc, args.cpu_increments))
+    dt = time.perf_counter() - t0
+    return {"value": r["n_updates"] / dt, "unit": "material-point updates/s", "cores": threads, "kind": "port",
+            "sample": f"oracle (C/OpenMP restatement), {args.cpu_cells}^3 cells, {args.grains} grains, increments "
+                      f"1-{args.cpu_increments} of the same tension ({r['n_updates']} updates in {dt:.1f} s)"}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (every box has one MI355X)",
+              file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    import torch
+    from damask_amd import config, flops
+    from damask_amd.grid import GridMechSpectralBasic
+
+    assert torch.cuda.is_available(), "bench.py needs the GPU"
+    grid, mat, lc = workload(args.cells, args.grains, args.seed, args.increments, args.strain)
+    t_init = time.perf_counter()
+    s = GridMechSpectralBasic(grid, mat, config.Numerics())
+    t_init = time.perf_counter() - t_init
+    if args.warmup:
+        r = s.run_loadcase(sub_loadcase(lc, args.warmup))
+        assert r["rc"] == 0, "warm-up did not converge"
+    c0 = s.counters()
+    s.profile(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    r = s.run_loadcase(sub_loadcase(lc, args.steps))
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    assert r["rc"] == 0, "timed increments did not converge"
+    c1 = s.counters()
+    kt = s.kernel_times()
+    s.profile(False)
+    updates = c1["updates"] - c0["updates"]
+    newton = c1["newton"] - c0["newton"]
+    state = c1["state"] - c0["state"]
+    residuals = c1["residuals"] - c0["residuals"]
+    value = updates / elapsed
+    nsl = mat.phases[0].nsl
+    stress_ms, stress_n = kt["stress"]
+    fl = flops.stress_kernel_flops(nsl, newton, state)
+    achieved = fl / (stress_ms * 1e-3) / 1e12
+    line = {
+        "metric": "material-point updates/s (64^3 FCC phenopowerlaw, FP64)",
+        "value": value,
+        "unit": "material-point updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded Voronoi polycrystal and orientations)",
+        "config": {"workload": f"{args.cells}^3 FCC phenopowerlaw uniaxial tension, {args.grains} grains, "
+                               f"{args.increments} increments to {args.strain * 100:g} % (increments "
+                               f"{args.warmup + 1}-{args.warmup + args.steps} timed)",
+                   "cells": args.cells, "grains": args.grains, "solver": "spectral_basic + NGMRES",
+                   "residual_evaluations": residuals,
+                   "residual_evaluations_per_increment": residuals / args.steps,
+                   "newton_iterations_per_update": newton / max(updates, 1),
+                   "state_iterations_per_update": state / max(updates, 1),
+                   "init_s": t_init,
+                   "kernel_ms": {k: round(v[0], 3) for k, v in kt.items()},
+                   "cells_x_increments_per_s": args.cells**3 * args.steps / elapsed},
+        "roofline": {"bound": "fp64-valu", "kernel": "stress_kernel<12>", "achieved": achieved,
+                     "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS,
+                     "traffic": args.traffic_bytes,
+                     "flops_per_launch": fl / max(stress_n, 1),
+                     "avg_launch_ms": stress_ms / max(stress_n, 1)},
+    }
+    if args.cpu_baseline:
+        line["cpu_baseline"] = cpu_baseline(args)
+    else:
+        line["cpu_baseline"] = None
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

@@ -48,7 +48,8 @@ EXPORTS = ["dmk_mech_init", "dmk_mech_forward", "dmk_mech_materialpoint_forward"
            "dmk_mech_update_coords", "dmk_constitutive", "dmk_constitutive_device", "dmk_mech_get_field",
            "dmk_mech_set_field", "dmk_mech_get_scalars", "dmk_mech_set_scalars", "dmk_grid_run_loadcase",
            "dmk_mech_counters", "dmk_mech_destroy", "dmk_last_error", "dmk_nsl", "dmk_device_synchronize",
-           "dmk_gamma_convolution", "dmk_divergence_rms", "dmk_fft_forward", "dmk_fft_backward"]
+           "dmk_gamma_convolution", "dmk_divergence_rms", "dmk_fft_forward", "dmk_fft_backward",
+           "dmk_mech_profile", "dmk_mech_kernel_time"]
 
 _lib = None
 
@@ -106,6 +107,10 @@ def load(path: str = LIB_PATH):
     L.dmk_fft_forward.argtypes = [vp, C.c_int32, dp, dp]
     L.dmk_fft_backward.restype = C.c_int
     L.dmk_fft_backward.argtypes = [vp, C.c_int32, dp, dp]
+    L.dmk_mech_profile.restype = C.c_int
+    L.dmk_mech_profile.argtypes = [vp, C.c_int32]
+    L.dmk_mech_kernel_time.restype = C.c_int
+    L.dmk_mech_kernel_time.argtypes = [vp, C.c_int32, dp, C.POINTER(C.c_int64)]
     _lib = L
     return L
 

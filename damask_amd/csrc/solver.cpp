@@ -227,7 +227,31 @@ struct dmk_handle {
   double t = 0.0, Delta_t = 1.0, Delta_t_prev = 0.0;
   int cutBack = 0, cutBackLevel = 0, totalIncs = 0;
   int64_t n_updates = 0, n_residuals = 0;
+  // optional per-kernel timing with HIP events on the solver stream
+  int profile = 0;
+  hipEvent_t ev[8][2];
+  double kt_ms[8] = {0};
+  int64_t kt_n[8] = {0};
+  bool ev_ready = false;
 };
+
+enum { KT_STRESS = 0, KT_TANGENT = 1, KT_REDUCE = 2, KT_FFT_FWD = 3, KT_GAMMA = 4, KT_FFT_INV = 5, KT_N = 6 };
+
+static void kt_begin(dmk_handle* h, int k) {
+  if (h->profile) hipEventRecord(h->ev[k][0], h->st);
+}
+static void kt_end(dmk_handle* h, int k) {
+  if (h->profile) hipEventRecord(h->ev[k][1], h->st);
+}
+// after a stream synchronisation: accumulate the recorded intervals
+static void kt_collect(dmk_handle* h, int k) {
+  if (!h->profile) return;
+  float ms = 0.0f;
+  if (hipEventElapsedTime(&ms, h->ev[k][0], h->ev[k][1]) == hipSuccess) {
+    h->kt_ms[k] += ms;
+    h->kt_n[k] += 1;
+  }
+}
 
 template <typename T>
 static int dalloc(dmk_handle* h, T** p, size_t n) {
@@ -257,19 +281,28 @@ static void set_gamma_coef(dmk_handle* h, const double* C) {
 
 // utilities_constitutiveResponse on the device field F (component-major)
 static int constitutive(dmk_handle* h, const double* F, double dt, bool rotate) {
+  kt_begin(h, KT_STRESS);
   for (int p = 0; p < h->n_phases; p++)
     CKL(dmk_launch_stress(&h->mp, F, h->ph_dev + p, h->ph[p].nsl, p, &h->nm, dt, h->st));
+  kt_end(h, KT_STRESS);
   CKL(dmk_status(h->mp.status, h->mp.n_newton, h->mp.n_fpi, h->N, h->flag, h->cnt, h->st));
+  kt_begin(h, KT_TANGENT);
   for (int p = 0; p < h->n_phases; p++)
     CKL(dmk_launch_tangent(&h->mp, h->ph_dev + p, h->ph[p].nsl, p, dt, h->flag, h->P_hom, h->dPdF, h->st));
+  kt_end(h, KT_TANGENT);
+  kt_begin(h, KT_REDUCE);
   CKL(dmk_colsum(h->P_hom, h->N, h->N, 9, h->partial, h->small + SM_PAV, h->wgt, h->st));
   CKL(dmk_colsum(h->dPdF, h->N, h->N, 81, h->partial, h->small + SM_CVOL, h->wgt, h->st));
   CKL(dmk_cminmax(h->dPdF, h->N, h->num.cref_rule == 1 ? 1.0e-12 : 0.0, h->nrm, h->pmax, h->pmin, h->sel, h->ext,
                   h->small + SM_CMM, h->st));
+  kt_end(h, KT_REDUCE);
   CK(hipMemcpyAsync(h->hsmall, h->small, sizeof(double) * SM_DIV, hipMemcpyDeviceToHost, h->st));
   int flag = 0;
   CK(hipMemcpyAsync(&flag, h->flag, sizeof(int), hipMemcpyDeviceToHost, h->st));
   CK(hipStreamSynchronize(h->st));
+  kt_collect(h, KT_STRESS);
+  kt_collect(h, KT_TANGENT);
+  kt_collect(h, KT_REDUCE);
   h->status = flag;
   std::memcpy(h->P_av, h->hsmall + SM_PAV, sizeof h->P_av);
   if (rotate) rotate33(h->q_R, h->P_av, false, h->P_av);
@@ -326,16 +359,25 @@ static int form_residual(dmk_handle* h, const double* X, double* r, int iter) {
   h->err_BC = e;
   rotate33(h->q_R, dFa, true, dFr);
   // one forward FFT serves the divergence RMS and the Gamma convolution
+  kt_begin(h, KT_FFT_FWD);
   CKL(dmk_fft_r2c(&h->fft, 9, h->P_hom, h->Z, h->st));
+  kt_end(h, KT_FFT_FWD);
   h->ga.do_div = 1;
   h->ga.do_gamma = 1;
   std::memcpy(h->ga.aim, dFr, sizeof dFr);
+  kt_begin(h, KT_GAMMA);
   CKL(dmk_gamma_div(h->Z, &h->ga, h->partial, h->st));
+  kt_end(h, KT_GAMMA);
   int nb = dmk_gamma_div_blocks(&h->ga);
   CKL(dmk_colsum(h->partial, nb, nb, 1, h->partial + nb, h->small + SM_DIV, 1.0, h->st));
+  kt_begin(h, KT_FFT_INV);
   CKL(dmk_fft_c2r(&h->fft, 9, h->Z, r, h->st));
+  kt_end(h, KT_FFT_INV);
   CK(hipMemcpyAsync(h->hsmall + SM_DIV, h->small + SM_DIV, sizeof(double), hipMemcpyDeviceToHost, h->st));
   CK(hipStreamSynchronize(h->st));
+  kt_collect(h, KT_FFT_FWD);
+  kt_collect(h, KT_GAMMA);
+  kt_collect(h, KT_FFT_INV);
   double div2 = h->hsmall[SM_DIV];
   h->err_div = std::sqrt(div2) * h->wgt;
   if (h->cells[0] == 1) h->err_div *= 0.5;
@@ -481,6 +523,8 @@ extern "C" int dmk_nsl(const dmk_handle* h) { return h ? h->nsl_max : 0; }
 extern "C" void dmk_mech_destroy(dmk_handle* h) {
   if (!h) return;
   if (h->st) hipStreamSynchronize(h->st);
+  if (h->ev_ready)
+    for (int k = 0; k < 8; k++) { hipEventDestroy(h->ev[k][0]); hipEventDestroy(h->ev[k][1]); }
   for (void* p : h->allocs) hipFree(p);
   if (h->ph_dev) hipFree(h->ph_dev);
   if (h->hsmall) hipHostFree(h->hsmall);
@@ -1038,5 +1082,30 @@ extern "C" int dmk_fft_backward(dmk_handle* h, int32_t howmany, const double* in
   CKL(dmk_soa_to_aos(h->Fext, h->N, howmany, h->Pext, h->st));
   CK(hipMemcpyAsync(out, h->Pext, sizeof(double) * howmany * h->N, hipMemcpyDeviceToHost, h->st));
   CK(hipStreamSynchronize(h->st));
+  return 0;
+}
+
+// per-kernel timing: enable (1) / disable (0); totals reset on enable
+extern "C" int dmk_mech_profile(dmk_handle* h, int32_t enable) {
+  if (!h) { dmk_set_error("null handle"); return DMK_ERR_ARG; }
+  if (enable && !h->ev_ready) {
+    for (int k = 0; k < 8; k++) {
+      CK(hipEventCreate(&h->ev[k][0]));
+      CK(hipEventCreate(&h->ev[k][1]));
+    }
+    h->ev_ready = true;
+  }
+  if (enable)
+    for (int k = 0; k < 8; k++) { h->kt_ms[k] = 0.0; h->kt_n[k] = 0; }
+  h->profile = enable ? 1 : 0;
+  return 0;
+}
+
+// kernel groups: 0 stress integration, 1 tangent/homogenize, 2 field reductions,
+// 3 forward FFT, 4 Gamma + div, 5 inverse FFT: total ms and launch count
+extern "C" int dmk_mech_kernel_time(dmk_handle* h, int32_t kernel, double* total_ms, int64_t* launches) {
+  if (!h || kernel < 0 || kernel >= KT_N) { dmk_set_error("bad argument"); return DMK_ERR_ARG; }
+  if (total_ms) *total_ms = h->kt_ms[kernel];
+  if (launches) *launches = h->kt_n[kernel];
   return 0;
 }

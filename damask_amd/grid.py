@@ -188,6 +188,20 @@ class GridMechSpectralBasic:
         _lib.check(self.lib.dmk_mech_counters(self.h, out.ctypes.data_as(C.POINTER(C.c_int64))), "dmk_mech_counters")
         return {"updates": int(out[0]), "newton": int(out[1]), "state": int(out[2]), "residuals": int(out[3])}
 
+    KERNEL_GROUPS = ("stress", "tangent", "reduce", "fft_fwd", "gamma_div", "fft_inv")
+
+    def profile(self, enable: bool = True) -> None:
+        _lib.check(self.lib.dmk_mech_profile(self.h, int(enable)), "dmk_mech_profile")
+
+    def kernel_times(self) -> dict:
+        out = {}
+        for k, name in enumerate(self.KERNEL_GROUPS):
+            ms = C.c_double()
+            n = C.c_int64()
+            _lib.check(self.lib.dmk_mech_kernel_time(self.h, k, C.byref(ms), C.byref(n)), "dmk_mech_kernel_time")
+            out[name] = (ms.value, n.value)
+        return out
+
     def run_loadcase(self, lc: LoadCase) -> dict:
         """All increments of one load case in the native driver loop."""
         bc = loadcase_struct(lc)

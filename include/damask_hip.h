@@ -151,6 +151,13 @@ int dmk_divergence_rms(dmk_handle* h, const double* field, double* out);
 int dmk_fft_forward(dmk_handle* h, int32_t howmany, const double* in, double* out);
 int dmk_fft_backward(dmk_handle* h, int32_t howmany, const double* in, double* out);
 
+/* timing of the kernel groups with HIP events on the solver stream:
+ * dmk_mech_profile(h, 1) enables (and resets), 0 disables.  Groups: 0 stress
+ * integration, 1 tangent + homogenize, 2 field reductions, 3 forward FFT,
+ * 4 Gamma convolution + div RMS, 5 inverse FFT */
+int dmk_mech_profile(dmk_handle* h, int32_t enable);
+int dmk_mech_kernel_time(dmk_handle* h, int32_t kernel, double* total_ms, int64_t* launches);
+
 /* totals since init: material-point updates, Newton and state iterations,
  * residual evaluations */
 int dmk_mech_counters(dmk_handle* h, int64_t out[4]);

@@ -49,39 +49,43 @@ def test_init_state(gpu_lib, oracle, lattice):
     assert _relmax(g.scalar("C_minMaxAvg"), Cm) < 1e-12
 
 
-@pytest.mark.parametrize("lattice,dt,scale", [("cF", 1.0, 2e-3), ("cF", 0.5, 6e-3), ("cI", 1.0, 3e-3)])
-def test_constitutive_response(gpu_lib, oracle, lattice, dt, scale):
-    """one homogenization_mechanical_response on a random F field, then a second on top"""
-    import ctypes as C
+@pytest.mark.parametrize("lattice,eps", [("cF", 5e-4), ("cF", 1.5e-3), ("cI", 8e-4)])
+def test_constitutive_response(gpu_lib, oracle, lattice, eps):
+    """homogenization_mechanical_response on a perturbed tension field, three
+    increments of 0.5 s at 1e-3/s-like rates, with materialpoint_forward between"""
     from damask_amd import workloads
     phase = workloads.FCC_PHASE if lattice == "cF" else workloads.BCC_PHASE
     grid, mat = _poly([8, 6, 5], 9, 3, phase)
     g, o = _pair(grid, mat, oracle=oracle)
     rng = np.random.default_rng(5)
     N = grid.n_cells
-    F = np.eye(3) + scale * rng.standard_normal((N, 3, 3)) + np.diag([scale, -0.3 * scale, -0.3 * scale])
-    for step in range(2):
+    L = oracle.lib()
+    dt = 0.5
+    for step in range(1, 4):
+        F = np.eye(3) + np.diag([eps, -0.3 * eps, -0.3 * eps]) * step \
+            + 0.1 * eps * rng.standard_normal((N, 3, 3))
         P_g, Pav_g, Cv_g, Cm_g, st_g = g.constitutive(F, dt)
         Ff = np.ascontiguousarray(F.transpose(0, 2, 1).reshape(-1))
         P_o = np.zeros_like(Ff)
         dP_o = np.zeros(81 * N)
-        L = oracle.lib()
         st_o = L.orc_mp_response(o.mp, dt, oracle.ptr(Ff), oracle.ptr(P_o), oracle.ptr(dP_o))
         assert st_g == st_o == 0
         P_o = P_o.reshape(-1, 3, 3).transpose(0, 2, 1)
-        assert _relmax(P_g, P_o) < 1e-8
         rel_pt = np.abs(P_g - P_o).max(axis=(1, 2)) / np.abs(P_o).max(axis=(1, 2))
+        import ctypes as C
+        nn, nf, nc = C.c_int64(), C.c_int64(), C.c_int64()
+        L.orc_mp_counters(o.mp, C.byref(nn), C.byref(nf), C.byref(nc))
+        print(f"step {step}: worst cell {rel_pt.argmax()} rel {rel_pt.max():.3e}; oracle newton {nn.value} "
+              f"fpi {nf.value} cutbacks {nc.value}; gpu totals {g.counters()}")
         assert rel_pt.max() < 1e-6
+        assert _relmax(Pav_g, P_o.mean(0)) < 1e-8
         dP_g = g.field("dPdF")
-        dP_o = dP_o.reshape(N, 81)
-        assert _relmax(dP_g, dP_o) < 1e-6
-        for name, w, what in (("xi_sl", None, 8), ("F_p", None, 3), ("L_p", None, 4)):
-            a = g.field(name)
-            b = o.get(what, mp_width := (a.shape[1] if a.ndim == 2 else 9))
-            b = b.reshape(a.shape[0], -1)
-            a2 = a.reshape(a.shape[0], -1) if a.ndim == 2 else a.transpose(0, 2, 1).reshape(a.shape[0], -1)
-            assert _relmax(a2, b) < 1e-6, name
-        F = F + scale * 0.5 * rng.standard_normal((N, 3, 3))
+        assert _relmax(dP_g, dP_o.reshape(N, 81)) < 1e-6
+        a = g.field("xi_sl")
+        b = o.get(8, a.shape[1]).reshape(N, -1)
+        assert _relmax(a, b) < 1e-8
+        g.materialpoint_forward()
+        L.orc_mp_forward(o.mp)
 
 
 @pytest.mark.parametrize("cells", [[6, 7, 8], [8, 8, 8], [10, 6, 5], [7, 9, 3], [16, 12, 20]])
