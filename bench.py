@@ -42,7 +42,23 @@ def parse():
     ap.add_argument("--cpu-increments", type=int, default=12)
     ap.add_argument("--traffic-bytes", type=float, default=None,
                     help="measured HBM bytes per stress-kernel launch (rocprofv3 --pmc)")
+    ap.add_argument("--copy-calibration", type=int, default=0,
+                    help="only run a plain device copy of this many MiB (counter calibration)")
     return ap.parse_args()
+
+
+def copy_calibration(mib):
+    """A plain copy kernel of known size: rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE on
+    it gives the factor between counter readings and bytes moved."""
+    import torch
+    n = mib * 1024 * 1024 // 8
+    x = torch.ones(n, dtype=torch.float64, device="cuda")
+    torch.cuda.synchronize()
+    for _ in range(3):
+        y = x.clone()
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes_read": n * 8, "copy_calibration_bytes_written": n * 8,
+                      "checksum": float(y[-1])}))
 
 
 def workload(cells, grains, seed, increments, strain):
@@ -81,6 +97,9 @@ def main():
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (every box has one MI355X)",
               file=sys.stderr)
         sys.exit(2)
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
     import numpy as np
     import torch
     from damask_amd import config, flops
