@@ -134,6 +134,7 @@ void orc_mp_set(orc_mp *m, int what, const double *src);
 int  orc_mp_nsl(const orc_mp *m);
 /* per-cell statistics of the last response (Newton and FPI iteration counts) */
 void orc_mp_counters(const orc_mp *m, int64_t *newton, int64_t *fpi, int64_t *cutbacks);
+void orc_mp_cell_counters(const orc_mp *m, int64_t *newton, int64_t *fpi);
 /* single-cell kernels used by tests */
 int  orc_integrate_stress(orc_mp *m, int ce, const double F[9], const double Fp0[9],
                           const double Fi0[9], double dt);

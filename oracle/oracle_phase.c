@@ -725,3 +725,8 @@ void orc_mp_set(orc_mp *m, int what, const double *src) {
   double *d = field_ptr(m, what);
   if (d) memcpy(d, src, sizeof(double) * 9 * N);
 }
+
+/* per-cell Newton and state-iteration counts of the last response */
+void orc_mp_cell_counters(const orc_mp *m, int64_t *newton, int64_t *fpi) {
+  for (int i = 0; i < m->N; i++) { newton[i] = m->newton[i]; fpi[i] = m->fpi[i]; }
+}
