@@ -157,7 +157,7 @@ def main():
                    "init_s": t_init,
                    "kernel_ms": {k: round(v[0], 3) for k, v in kt.items()},
                    "cells_x_increments_per_s": args.cells**3 * args.steps / elapsed},
-        "roofline": {"bound": "fp64-valu", "kernel": "stress_kernel<12>", "achieved": achieved,
+        "roofline": {"bound": "fp64-valu", "kernel": "stress_kernel<12, true>", "achieved": achieved,
                      "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS,
                      "traffic": args.traffic_bytes,
                      "flops_per_launch": fl / max(stress_n, 1),

@@ -1,4 +1,4 @@
-// Per-material-point constitutive update on MI355X: one thread per cell, FP64.
+// Per-material-point constitutive update on MI355X: one lane per cell, FP64.
 //
 // stress_kernel: homogenization_mechanical_response's first loop
 //   (src/homogenization.f90:309-348): restore, isostrain partition,
@@ -14,22 +14,85 @@
 // rank-N_sl updates of the 9x9 identity:
 //   dR_Lp/dL_p = I9 + dt * sum_a dgamma_a vec(P_a) vec(G^T Q_a)^T,  G = F_e^T A,
 //   Q_a = C : P_nS,a (tabulated per phase), vec row-major (math_33to9),
-// which is what is assembled in registers here.
+// which is assembled in registers and solved by Householder QR (dmk_dense.h).
+//
+// Register budget.  The 9x9 system (81 doubles) is the only large register
+// object.  Slip-system loops are rolled so the phase tables are read through
+// scalar loads inside the loop instead of being hoisted into VGPRs; the
+// per-system vectors of a lane (xi, rates, rate derivatives) live in LDS,
+// [system][lane], one wavefront per work-group.  Sub-step start values and
+// the state-rate history of the fixed-point iteration live in global scratch.
 #include "dmk_common.h"
 #include "dmk_math.h"
+#include "dmk_dense.h"
 
 namespace {
 
+constexpr int SWG = 64;   // work-group of the per-cell kernels: one wavefront
+
+// Library transcendental functions are kept out of line: inlined, their
+// polynomial constants are hoisted out of every enclosing loop and stay live
+// across the 9x9 Newton system.
+__device__ __noinline__ double pow_ool(double x, double y) { return pow(x, y); }
+__device__ __noinline__ double tanh_ool(double x) { return tanh(x); }
+
+// x^y for the power laws (phenopowerlaw n_sl, a_sl).  With QP (every
+// exponent of the phase is a positive multiple of 1/4 up to 64, checked on the
+// host) y is evaluated by binary powering of x, sqrt(x) or sqrt(sqrt(x)); y is
+// a per-phase constant, so the powering loop is uniform across the wavefront.
+// The result agrees with the library pow to a few ulp (a chain of at most 14
+// rounded products) and follows pow's rules for zero, negative (NaN unless y
+// is an integer, sign of odd powers), infinite and NaN bases.  Phases with
+// other exponents use the library pow.
+template <bool QP>
+DI double pow_law(double x, double y) {
+  if (!QP) return pow_ool(x, y);
+  int k = (int)(y * 4.0);
+  double b = x;
+  if (k & 1) {
+    b = sqrt(sqrt(x));
+  } else if (k & 2) {
+    b = sqrt(x);
+    k >>= 1;
+  } else {
+    k >>= 2;
+  }
+  double r = 1.0;
+  bool first = true;
+  while (k) {
+    if (k & 1) { r = first ? b : r * b; first = false; }
+    k >>= 1;
+    if (k) b = b * b;
+  }
+  return r;
+}
+
+// per-lane slip-system vectors, entry a of a lane at [a][lane]
 template <int NSL>
-DI void kinetics(const DmkPhaseDev* __restrict__ ph, const double* Mp, const double* xi, double* g,
-                 double* dg, bool* pos) {
+struct SlipLds {
+  double xi[NSL][SWG];    // critical resolved shear stress of the current state
+  double g[NSL][SWG];     // slip rates (signed), |rate| once the state rate is formed
+  double w[NSL][SWG];     // d rate / d tau in the stress Newton loop, xi rate in the state loop
+};
+
+// kinetics_sl (src/phase_mechanical_plastic_phenopowerlaw.f90:449-488) for all
+// systems of one lane: rates -> gs, derivatives -> ws (DERIV), and
+// L_p = sum_a rate_a P_a (src/phase_mechanical_plastic_phenopowerlaw.f90:325-338) when LP.
+template <int NSL, bool QP, bool DERIV, bool LP>
+DI void kinetics(const DmkPhaseDev* __restrict__ ph, const double* Mp, const double* xs, double* gs,
+                 double* ws, double* Lp) {
+  if (LP) {
 #pragma unroll
+    for (int k = 0; k < 9; k++) Lp[k] = 0.0;
+  }
+  const bool nonschmid = ph->nonschmid;
+#pragma unroll 1
   for (int a = 0; a < NSL; a++) {
     double tp = 0.0;
 #pragma unroll
     for (int k = 0; k < 9; k++) tp = fma(Mp[k], ph->Pp[a][k], tp);
     double tn;
-    if (ph->nonschmid) {
+    if (nonschmid) {
       tn = 0.0;
 #pragma unroll
       for (int k = 0; k < 9; k++) tn = fma(Mp[k], ph->Pm[a][k], tn);
@@ -38,11 +101,45 @@ DI void kinetics(const DmkPhaseDev* __restrict__ ph, const double* Mp, const dou
     }
     double sgn = tp > tn ? 1.0 : -1.0;
     double tmax = fmax(tp, tn);
-    double gg = sgn * ph->g0[a] * pow(tmax / xi[a], ph->n[a]);
-    g[a] = gg;
-    if (dg) dg[a] = !dEq0(gg) ? gg * ph->n[a] / tmax : 0.0;
-    if (pos) pos[a] = gg > 0.0;
+    double gg = sgn * ph->g0[a] * pow_law<QP>(tmax / xs[a * SWG], ph->n[a]);
+    gs[a * SWG] = gg;
+    if (DERIV) ws[a * SWG] = !dEq0(gg) ? gg * ph->n[a] / tmax : 0.0;
+    if (LP) {
+#pragma unroll
+      for (int k = 0; k < 9; k++) Lp[k] = fma(gg, ph->P[a][k], Lp[k]);
+    }
   }
+}
+
+// phenopowerlaw_dotState without twins (src/phase_mechanical_plastic_phenopowerlaw.f90:356-397)
+// from the slip rates in gs: gs <- |rate| (gamma rate), ws <- xi rate.
+// xi_sat = xi_inf + f_sat sqrt(sum F_tw) and hf = h_0 (1 + c_1 (sum F_tw)^c_2) at sum F_tw = 0
+// are tabulated per phase.  Returns true when a rate is NaN.
+template <int NSL, bool QP>
+DI bool dot_state(const DmkPhaseDev* __restrict__ ph, const double* xs, double* gs, double* ws) {
+  double hg[NSL];
+#pragma unroll
+  for (int r = 0; r < NSL; r++) hg[r] = 0.0;
+  bool nan = false;
+#pragma unroll 1
+  for (int a = 0; a < NSL; a++) {
+    double ga = fabs(gs[a * SWG]);
+    gs[a * SWG] = ga;
+    nan |= (ga != ga);
+#pragma unroll
+    for (int r = 0; r < NSL; r++) hg[r] = fma(ph->H[r][a], ga, hg[r]);
+  }
+#pragma unroll
+  for (int r = 0; r < NSL; r++) ws[r * SWG] = hg[r];
+#pragma unroll 1
+  for (int r = 0; r < NSL; r++) {
+    double x = 1.0 - xs[r * SWG] / ph->xi_sat[r];
+    double left = copysign(pow_law<QP>(fabs(x), ph->a[r]), x);
+    double d = ph->hf[r] * left * ws[r * SWG];
+    ws[r * SWG] = d;
+    nan |= (d != d);
+  }
+  return nan;
 }
 
 // E = (Fe^T Fe - I)/2 and S = C:E (lattice frame)
@@ -55,81 +152,35 @@ DI void hooke(const DmkPhaseDev* __restrict__ ph, const double* Fe, double* S) {
   hooke_sym(ph->C11, ph->C12, ph->C44, E, S);
 }
 
-// LU factorisation with partial pivoting (first maximum wins, as dgetrf) of a
-// row-major 9x9 in registers and solve for one right-hand side.  Rows are
-// swapped physically with selects so every index stays compile-time.
-DI bool lu9_factor(double* J, int* piv) {
-  bool sing = false;
+DI void copy9(double* dst, const double* src, const Lane& l) {
 #pragma unroll
-  for (int k = 0; k < 9; k++) {
-    int p = k;
-    double amax = fabs(J[9 * k + k]);
-#pragma unroll
-    for (int i = k + 1; i < 9; i++) {
-      double v = fabs(J[9 * i + k]);
-      if (v > amax) { amax = v; p = i; }
-    }
-    piv[k] = p;
-#pragma unroll
-    for (int i = k + 1; i < 9; i++) {
-      bool sw = (i == p);
-#pragma unroll
-      for (int c = 0; c < 9; c++) {
-        double a = J[9 * k + c], b = J[9 * i + c];
-        J[9 * k + c] = sw ? b : a;
-        J[9 * i + c] = sw ? a : b;
-      }
-    }
-    double d = J[9 * k + k];
-    if (d == 0.0) sing = true;
-    double r = 1.0 / d;
-#pragma unroll
-    for (int i = k + 1; i < 9; i++) {
-      double l = J[9 * i + k] * r;
-      J[9 * i + k] = l;
-#pragma unroll
-      for (int c = k + 1; c < 9; c++) J[9 * i + c] = fma(-l, J[9 * k + c], J[9 * i + c]);
-    }
-  }
-  return sing;
+  for (int c = 0; c < 9; c++) st(dst, l, c, ld(src, l, c));
 }
 
-DI void lu9_solve(const double* J, const int* piv, double* b) {
-#pragma unroll
-  for (int k = 0; k < 9; k++) {
-#pragma unroll
-    for (int i = k + 1; i < 9; i++) {
-      bool sw = (i == piv[k]);
-      double a = b[k], c = b[i];
-      b[k] = sw ? c : a;
-      b[i] = sw ? a : c;
-    }
-  }
-#pragma unroll
-  for (int k = 0; k < 9; k++)
-#pragma unroll
-    for (int i = k + 1; i < 9; i++) b[i] = fma(-J[9 * i + k], b[k], b[i]);
-#pragma unroll
-  for (int k = 8; k >= 0; k--) {
-    b[k] = b[k] / J[9 * k + k];
-#pragma unroll
-    for (int i = 0; i < k; i++) b[i] = fma(-J[9 * i + k], b[k], b[i]);
-  }
-}
+// sub-step scratch of a cell ([36][N]): sub-step F_0, sub-step F, F_p0, L_p0
+enum { SUB_F0 = 0, SUB_F = 9, SUB_FP0 = 18, SUB_LP0 = 27 };
 
 // integrateStress (src/phase_mechanical.f90:379-584) with F_i = I, L_i = 0.
-// On success writes P, S, L_p, F_p (det-regularised) and F_e of the cell.
-template <int NSL>
-DI int integrate_stress(const DmkPhaseDev* __restrict__ ph, const DmkNumMech& num, DmkMPDev& mp,
-                        int e, const double* F, const double* Fp0, double dt, const double* xi,
-                        int& newton) {
-  const int N = mp.N;
-  double invFp_cur[9], A[9];
-  if (m_invert(Fp0, invFp_cur)) return ST_FAIL_STRESS;
-  m_mul(F, invFp_cur, A);
-
-  double Lpg[9], Lpg_old[9], dLp[9], B[9], S[9];
-  m_load(mp.Lp, N, e, Lpg);
+// F and F_p0 of the sub-step are read from the sub-step scratch.  On success
+// writes P, S, L_p, F_p (det-regularised) and F_e of the cell and returns S.
+template <int NSL, bool QP>
+DI int integrate_stress(const DmkPhaseDev* __restrict__ ph, const DmkNumMech& num, const DmkMPDev& mp,
+                        const Lane& ln, double dt, SlipLds<NSL>& sh, int lane, int& newton, double* S) {
+  const double* Fsub_p = mp.sub + SUB_F * ln.sN;
+  const double* Fp0_p = mp.sub + SUB_FP0 * ln.sN;
+  const double* xs = &sh.xi[0][lane];
+  double* gs = &sh.g[0][lane];
+  double* ws = &sh.w[0][lane];
+  double A[9];
+  {
+    double Fp0[9], invFp0[9], F[9];
+    m_load(Fp0_p, ln, Fp0);
+    if (m_invert(Fp0, invFp0)) return ST_FAIL_STRESS;
+    m_load(Fsub_p, ln, F);
+    m_mul(F, invFp0, A);
+  }
+  double Lpg[9], Lpg_old[9], dLp[9];
+  m_load(mp.Lp, ln, Lpg);
 #pragma unroll
   for (int k = 0; k < 9; k++) { Lpg_old[k] = Lpg[k]; dLp[k] = 0.0; }
   double step = 1.0, res_old = DBL_MAX;
@@ -138,21 +189,15 @@ DI int integrate_stress(const DmkPhaseDev* __restrict__ ph, const DmkNumMech& nu
     it++;
     if (it > num.N_iter_Lp_max) return ST_FAIL_LP_MAXIT;
     newton++;
+    {
+      double B[9], Fe[9];
 #pragma unroll
-    for (int k = 0; k < 9; k++) B[k] = ((k % 4) == 0 ? 1.0 : 0.0) - dt * Lpg[k];
-    double Fe[9];
-    m_mul(A, B, Fe);
-    hooke(ph, Fe, S);
-    double g[NSL], dg[NSL];
-    bool pos[NSL];
-    kinetics<NSL>(ph, S, xi, g, dg, pos);
+      for (int k = 0; k < 9; k++) B[k] = ((k % 4) == 0 ? 1.0 : 0.0) - dt * Lpg[k];
+      m_mul(A, B, Fe);
+      hooke(ph, Fe, S);
+    }
     double Lpc[9];
-#pragma unroll
-    for (int k = 0; k < 9; k++) Lpc[k] = 0.0;
-#pragma unroll
-    for (int a = 0; a < NSL; a++)
-#pragma unroll
-      for (int k = 0; k < 9; k++) Lpc[k] = fma(g[a], ph->P[a][k], Lpc[k]);
+    kinetics<NSL, QP, true, true>(ph, S, xs, gs, ws, Lpc);
     double ng = m_norm(Lpg), nc = m_norm(Lpc);
     double atol = fmax(num.eps_rel_Lp * fmax(ng, nc), num.eps_abs_Lp);
     double res[9];
@@ -173,168 +218,192 @@ DI int integrate_stress(const DmkPhaseDev* __restrict__ ph, const DmkNumMech& nu
       for (int k = 0; k < 9; k++) Lpg[k] = fma(dLp[k], step, Lpg_old[k]);
       continue;
     }
-    // Jacobian, rebuilt every iteration (f_update_jacobi_Lp = 1)
+    // Jacobian, rebuilt every iteration (f_update_jacobi_Lp = 1):
+    // J = I9 + sum_a u_a vec(P_a) vec(Y_a)^T.  The Schmid tensors P_a are
+    // traceless, so tr(J x) = tr(x) and the solution of J x = res has
+    // x_22 = tr(res) - x_00 - x_11.  Substituting it into rows 0..7 leaves an
+    // 8x8 system (columns 0 and 4 reduced by column 8), which is what is
+    // assembled: J8 = rows/columns 0..7, j8 = column 8 of rows 0..7.
     double G[9];
-    m_mul_tn(Fe, A, G);
-    double J[81];
+    {
+      // F_e again (cheaper than keeping it live across the kinetics loop)
+      double B[9], Fe[9];
 #pragma unroll
-    for (int r = 0; r < 81; r++) J[r] = (r % 10 == 0) ? 1.0 : 0.0;
+      for (int k = 0; k < 9; k++) B[k] = ((k % 4) == 0 ? 1.0 : 0.0) - dt * Lpg[k];
+      m_mul(A, B, Fe);
+      m_mul_tn(Fe, A, G);
+    }
+    double J[64], j8[8];
 #pragma unroll
+    for (int r = 0; r < 64; r++) J[r] = (r % 9 == 0) ? 1.0 : 0.0;
+#pragma unroll
+    for (int r = 0; r < 8; r++) j8[r] = 0.0;
+    const bool nonschmid = ph->nonschmid;
+#pragma unroll 1
     for (int a = 0; a < NSL; a++) {
-      const double* Q = pos[a] ? ph->Qp[a] : ph->Qm[a];
+      double u = dt * ws[a * SWG];
+      const bool pos = gs[a * SWG] > 0.0;
       double Y[9];
-      m_mul_tn(G, Q, Y);
-      double u = dt * dg[a];
+      if (nonschmid) {
+        double Q[9];
 #pragma unroll
-      for (int r = 0; r < 9; r++) {
+        for (int k = 0; k < 9; k++) Q[k] = pos ? ph->Qp[a][k] : ph->Qm[a][k];
+        m_mul_tn(G, Q, Y);
+      } else {
+        // Schmid law: Q_neg = -Q_pos exactly, folded into the sign of u
+        m_mul_tn(G, ph->Qp[a], Y);
+        u = pos ? u : -u;
+      }
+#pragma unroll
+      for (int r = 0; r < 8; r++) {
         double pr = u * ph->P[a][r];
 #pragma unroll
-        for (int c = 0; c < 9; c++) J[9 * r + c] = fma(pr, Y[c], J[9 * r + c]);
+        for (int c = 0; c < 8; c++) J[8 * r + c] = fma(pr, Y[c], J[8 * r + c]);
+        j8[r] = fma(pr, Y[8], j8[r]);
       }
     }
-    int piv[9];
-    if (lu9_factor(J, piv)) return ST_FAIL_LP_DGESV;
-    lu9_solve(J, piv, res);
+    const double tr = res[0] + res[4] + res[8];
 #pragma unroll
-    for (int k = 0; k < 9; k++) dLp[k] = -res[k];
+    for (int r = 0; r < 8; r++) {
+      J[8 * r] -= j8[r];
+      J[8 * r + 4] -= j8[r];
+      res[r] = fma(-j8[r], tr, res[r]);
+    }
+    if (dmk_qr_solve<8>(J, res)) return ST_FAIL_LP_DGESV;
+    res[8] = tr - res[0] - res[4];
 #pragma unroll
-    for (int k = 0; k < 9; k++) Lpg[k] = fma(dLp[k], step, Lpg[k]);
+    for (int k = 0; k < 9; k++) { dLp[k] = -res[k]; Lpg[k] = fma(dLp[k], step, Lpg[k]); }
   }
 
   double invFp_new[9], Fp_new[9];
-  m_mul(invFp_cur, B, invFp_new);
+  {
+    double Fp0[9], invFp0[9], B[9];
+    m_load(Fp0_p, ln, Fp0);
+    m_invert(Fp0, invFp0);
+#pragma unroll
+    for (int k = 0; k < 9; k++) B[k] = ((k % 4) == 0 ? 1.0 : 0.0) - dt * Lpg[k];
+    m_mul(invFp0, B, invFp_new);
+  }
   if (m_invert(invFp_new, Fp_new)) return ST_FAIL_STRESS;
-  double T1[9], T2[9], P[9];
-  m_mul(F, invFp_new, T1);
-  m_mul_nt(S, invFp_new, T2);
-  m_mul(T1, T2, P);
-  m_store(mp.Pph, N, e, P);
-  m_store(mp.S, N, e, S);
-  m_store(mp.Lp, N, e, Lpg);
-  double d = pow(m_det(Fp_new), 1.0 / 3.0);
+  double T1[9], P[9];
+  {
+    double F[9], T2[9];
+    m_load(Fsub_p, ln, F);
+    m_mul(F, invFp_new, T1);
+    m_mul_nt(S, invFp_new, T2);
+    m_mul(T1, T2, P);
+  }
+  m_store(mp.Pph, ln, P);
+  m_store(mp.S, ln, S);
+  m_store(mp.Lp, ln, Lpg);
+  double d = pow_ool(m_det(Fp_new), 1.0 / 3.0);
 #pragma unroll
   for (int k = 0; k < 9; k++) Fp_new[k] = Fp_new[k] / d;
-  m_store(mp.Fp, N, e, Fp_new);
-  m_store(mp.Fe, N, e, T1);
+  m_store(mp.Fp, ln, Fp_new);
+  m_store(mp.Fe, ln, T1);
   return ST_OK;
 }
 
-// plastic_dotState -> phenopowerlaw_dotState (no twins): dot[0..NSL) xi, [NSL..2NSL) gamma
-template <int NSL>
-DI bool dot_state(const DmkPhaseDev* __restrict__ ph, const double* S, const double* xi, double* dot) {
-  double g[NSL];
-  kinetics<NSL>(ph, S, xi, g, nullptr, nullptr);
-  bool nan = false;
-#pragma unroll
-  for (int a = 0; a < NSL; a++) g[a] = fabs(g[a]);
-#pragma unroll
-  for (int r = 0; r < NSL; r++) {
-    double x = 1.0 - xi[r] / (ph->xi_inf[r] + ph->fsat[r] * 0.0);
-    double left = copysign(pow(fabs(x), ph->a[r]), x);
-    double hg = 0.0;
-#pragma unroll
-    for (int a = 0; a < NSL; a++) hg = fma(ph->H[r][a], g[a], hg);
-    double f = ph->h0[r] * (1.0 + ph->c1[r] * pow(0.0, ph->c2[r]));
-    dot[r] = f * left * hg;
-    dot[NSL + r] = g[r];
-    nan |= (dot[r] != dot[r]) || (g[r] != g[r]);
+// integrateStateFPI (src/phase_mechanical.f90:591-676).  State = (xi, gamma):
+// xi in LDS, gamma in mp.gam; its start values xi0_p / gam0_p (the converged
+// increment, or st0s after a completed sub-step); the two previous state
+// rates (dotState_last) in the ping-pong buffers mp.last1 / mp.last2.
+template <int NSL, bool QP>
+DI int integrate_state_fpi(const DmkPhaseDev* __restrict__ ph, const DmkNumMech& num, const DmkMPDev& mp,
+                           const Lane& ln, const double* xi0_p, const double* gam0_p, double dt,
+                           SlipLds<NSL>& sh, int lane, int& newton, int& fpi) {
+  double* xs = &sh.xi[0][lane];
+  double* gs = &sh.g[0][lane];
+  double* ws = &sh.w[0][lane];
+  {
+    double S[9];
+    m_load(mp.S, ln, S);
+    kinetics<NSL, QP, false, false>(ph, S, xs, gs, nullptr, nullptr);
   }
-  return nan;
-}
-
-// integrateStateFPI (src/phase_mechanical.f90:591-676); xi in registers,
-// gamma and the iteration history in per-cell SoA scratch.
-template <int NSL>
-DI int integrate_state_fpi(const DmkPhaseDev* __restrict__ ph, const DmkNumMech& num, DmkMPDev& mp,
-                           int e, const double* F, const double* Fp0, double dt, double* xi,
-                           const double* xi0s, int& newton, int& fpi) {
-  const int N = mp.N;
-  const size_t sN = (size_t)N;
-  double S[9];
-  m_load(mp.S, N, e, S);
-  double dot[2 * NSL];
-  if (dot_state<NSL>(ph, S, xi, dot)) return ST_FAIL_STATE;
-#pragma unroll
+  if (dot_state<NSL, QP>(ph, xs, gs, ws)) return ST_FAIL_STATE;
+#pragma unroll 1
   for (int r = 0; r < NSL; r++) {
-    xi[r] = fma(dot[r], dt, xi0s[r]);
-    mp.gam[r * sN + e] = fma(dot[NSL + r], dt, mp.st0s[(NSL + r) * sN + e]);
+    double dx = ws[r * SWG], dgm = gs[r * SWG];
+    xs[r * SWG] = fma(dx, dt, ld(xi0_p, ln, r));
+    st(mp.gam, ln, r, fma(dgm, dt, ld(gam0_p, ln, r)));
+    st(mp.last1, ln, r, dx);
+    st(mp.last1, ln, NSL + r, dgm);
   }
   int status = ST_FAIL_STATE;
+  bool cur = false;                   // last1 holds dotState_last(:,1) when !cur
   for (int it = 1; it <= num.N_iter_state_max; it++) {
-#pragma unroll
-    for (int r = 0; r < 2 * NSL; r++) {
-      mp.last2[r * sN + e] = it > 1 ? mp.last1[r * sN + e] : 0.0;
-      mp.last1[r * sN + e] = dot[r];
-    }
     fpi++;
-    status = integrate_stress<NSL>(ph, num, mp, e, F, Fp0, dt, xi, newton);
+    double S[9];
+    status = integrate_stress<NSL, QP>(ph, num, mp, ln, dt, sh, lane, newton, S);
     if (status != ST_OK) break;
-    m_load(mp.S, N, e, S);
-    if (dot_state<NSL>(ph, S, xi, dot)) break;
+    // the converged Newton iterate evaluated kinetics_sl at this (S, xi): rates are in gs
+    if (dot_state<NSL, QP>(ph, xs, gs, ws)) break;
+    const double* w1p = opaque(cur ? mp.last2 : mp.last1);
+    double* w2p = opaque(cur ? mp.last1 : mp.last2);
+    const double* x0 = opaque(xi0_p);
+    const double* g0 = opaque(gam0_p);
     // damper (src/phase_mechanical.f90:653-674)
+    double w1v[2 * NSL];
     double d12 = 0.0, d22 = 0.0, d01 = 0.0;
 #pragma unroll
     for (int r = 0; r < 2 * NSL; r++) {
-      double w1 = mp.last1[r * sN + e], w2 = mp.last2[r * sN + e];
-      d12 = fma(dot[r] - w1, w1 - w2, d12);
+      double w1 = ld(w1p, ln, r);
+      double w2 = it > 1 ? ld(w2p, ln, r) : 0.0;
+      double d0 = r < NSL ? ws[r * SWG] : gs[(r - NSL) * SWG];
+      w1v[r] = w1;
+      d12 = fma(d0 - w1, w1 - w2, d12);
       d22 = fma(w1 - w2, w1 - w2, d22);
-      d01 = fma(dot[r], w1, d01);
+      d01 = fma(d0, w1, d01);
     }
-    double zeta = (fmin(d01, d12) < 0.0 && d22 > 0.0) ? 0.75 + 0.25 * tanh(2.0 + 4.0 * d12 / d22) : 1.0;
+    double zeta = (fmin(d01, d12) < 0.0 && d22 > 0.0) ? 0.75 + 0.25 * tanh_ool(2.0 + 4.0 * d12 / d22) : 1.0;
     bool conv = true;
 #pragma unroll
     for (int r = 0; r < 2 * NSL; r++) {
-      double w1 = mp.last1[r * sN + e];
-      double d = dot[r] * zeta + w1 * (1.0 - zeta);
-      dot[r] = d;
-      double s0 = mp.st0s[r * sN + e];
-      double s = r < NSL ? xi[r] : mp.gam[(r - NSL) * sN + e];
+      double w1 = w1v[r];
+      double d0 = r < NSL ? ws[r * SWG] : gs[(r - NSL) * SWG];
+      double d = d0 * zeta + w1 * (1.0 - zeta);
+      st(w2p, ln, r, d);             // dotState_last(:,1) of the next iteration
+      double s0 = r < NSL ? ld(x0, ln, r) : ld(g0, ln, r - NSL);
+      double s = r < NSL ? xs[r * SWG] : ld(mp.gam, ln, r - NSL);
       double res = s - s0 - d * dt;
       s = s - res;
       double atol = r < NSL ? ph->atol_xi : ph->atol_gamma;
       conv = conv && (fabs(res) <= fmax(atol, num.eps_rel_state * fabs(s)));
-      if (r < NSL) xi[r] = s; else mp.gam[(r - NSL) * sN + e] = s;
+      if (r < NSL) xs[r * SWG] = s; else st(mp.gam, ln, r - NSL, s);
     }
+    cur = !cur;
     if (conv) { status = ST_OK; break; }
   }
   return status;
 }
 
-template <int NSL>
-__global__ void __launch_bounds__(128) stress_kernel(DmkMPDev mp, const double* __restrict__ Ftarget,
+template <int NSL, bool QP>
+__global__ void __launch_bounds__(SWG) stress_kernel(DmkMPDev mp, const double* __restrict__ Ftarget,
                                                      const DmkPhaseDev* __restrict__ ph, int phase_id,
                                                      DmkNumMech num, double dt) {
-  const int e = blockIdx.x * blockDim.x + threadIdx.x;
-  const int N = mp.N;
-  if (e >= N) return;
+  __shared__ SlipLds<NSL> sh;
+  const int lane = threadIdx.x;
+  const int e = blockIdx.x * SWG + lane;
+  if (e >= mp.N) return;
   if (mp.phase[e] != phase_id) return;
-  const size_t sN = (size_t)N;
+  const Lane ln = make_lane(mp.N, e);
+  double* xs = &sh.xi[0][lane];
+  double* sub = mp.sub;
 
-  // phase_restore (src/phase_mechanical.f90:1096-1115) + isostrain partition
-  double F[9], F0i[9];
-  m_load(Ftarget, N, e, F);
-  m_store(mp.Fphase, N, e, F);
-  m_load(mp.F0, N, e, F0i);
-  double Fp0s[9], Lp0s[9], F0s[9], Fsub[9];
-  m_load(mp.Fp0, N, e, Fp0s);
-  m_load(mp.Lp0, N, e, Lp0s);
-#pragma unroll
-  for (int k = 0; k < 9; k++) { F0s[k] = F0i[k]; Fsub[k] = 0.0; }
-  double xi[NSL], xi0s[NSL];
-#pragma unroll
-  for (int r = 0; r < NSL; r++) {
-    xi0s[r] = mp.xi0[r * sN + e];
-    xi[r] = xi0s[r];
-    double g0 = mp.gam0[r * sN + e];
-    mp.gam[r * sN + e] = g0;
-    mp.st0s[r * sN + e] = xi0s[r];
-    mp.st0s[(NSL + r) * sN + e] = g0;
-  }
-  for (int k = 0; k < 9; k++) mp.S[k * sN + e] = mp.S0[k * sN + e];
-  m_store(mp.Fp, N, e, Fp0s);
+  // isostrain partition (src/homogenization_mechanical_isostrain.f90:60-75)
+  copy9(mp.Fphase, Ftarget, ln);
+  // F_p0 of the sub-step always comes from the sub-step scratch; the other
+  // start values are read from the converged increment until a sub-step
+  // completes, then from copies in the sub-step scratch / st0s
+  copy9(sub + SUB_FP0 * ln.sN, mp.Fp0, ln);
+  const double* pF0 = mp.F0;
+  const double* pLp0 = mp.Lp0;
+  const double* pxi0 = mp.xi0;
+  const double* pgam0 = mp.gam0;
 
   int newton = 0, fpi = 0;
-  // phase_mechanical_constitutive substepping
+  // phase_mechanical_constitutive substepping; the first pass takes the
+  // cut-back branch, which is phase_restore (src/phase_mechanical.f90:1096-1115)
   double stepFrac = 0.0, step = 1.0 / num.r_cutback;
   int status = ST_ITERATING;
   bool todo = true;
@@ -344,37 +413,40 @@ __global__ void __launch_bounds__(128) stress_kernel(DmkMPDev mp, const double* 
       step = fmin(1.0 - stepFrac, num.r_increase * step);
       todo = step > 0.0;
       if (todo) {
-#pragma unroll
-        for (int k = 0; k < 9; k++) F0s[k] = Fsub[k];
-        m_load(mp.Lp, N, e, Lp0s);
-        m_load(mp.Fp, N, e, Fp0s);
-#pragma unroll
+        copy9(sub + SUB_F0 * ln.sN, sub + SUB_F * ln.sN, ln);
+        copy9(sub + SUB_LP0 * ln.sN, mp.Lp, ln);
+        copy9(sub + SUB_FP0 * ln.sN, mp.Fp, ln);
+#pragma unroll 1
         for (int r = 0; r < NSL; r++) {
-          xi0s[r] = xi[r];
-          mp.st0s[r * sN + e] = xi[r];
-          mp.st0s[(NSL + r) * sN + e] = mp.gam[r * sN + e];
+          st(mp.st0s, ln, r, xs[r * SWG]);
+          st(mp.st0s, ln, NSL + r, ld(mp.gam, ln, r));
         }
+        pF0 = sub + SUB_F0 * ln.sN;
+        pLp0 = sub + SUB_LP0 * ln.sN;
+        pxi0 = mp.st0s;
+        pgam0 = mp.st0s + NSL * ln.sN;
       }
     } else {
       step = num.r_cutback * step;
-      m_store(mp.Fp, N, e, Fp0s);
-      for (int k = 0; k < 9; k++) mp.S[k * sN + e] = mp.S0[k * sN + e];
-      if (step < 1.0) m_store(mp.Lp, N, e, Lp0s);
-#pragma unroll
+      copy9(mp.Fp, sub + SUB_FP0 * ln.sN, ln);
+      copy9(mp.S, mp.S0, ln);
+      if (step < 1.0) copy9(mp.Lp, pLp0, ln);
+#pragma unroll 1
       for (int r = 0; r < NSL; r++) {
-        xi[r] = xi0s[r];
-        mp.gam[r * sN + e] = mp.st0s[(NSL + r) * sN + e];
+        xs[r * SWG] = ld(pxi0, ln, r);
+        st(mp.gam, ln, r, ld(pgam0, ln, r));
       }
       todo = step > num.r_cutback_min;
     }
     if (todo) {
 #pragma unroll
-      for (int k = 0; k < 9; k++) Fsub[k] = fma(step, F[k] - F0i[k], F0s[k]);
-      status = integrate_state_fpi<NSL>(ph, num, mp, e, Fsub, Fp0s, step * dt, xi, xi0s, newton, fpi);
+      for (int c = 0; c < 9; c++)
+        st(sub + SUB_F * ln.sN, ln, c, fma(step, ld(Ftarget, ln, c) - ld(mp.F0, ln, c), ld(pF0, ln, c)));
+      status = integrate_state_fpi<NSL, QP>(ph, num, mp, ln, pxi0, pgam0, step * dt, sh, lane, newton, fpi);
     }
   }
-#pragma unroll
-  for (int r = 0; r < NSL; r++) mp.xi[r * sN + e] = xi[r];
+#pragma unroll 1
+  for (int r = 0; r < NSL; r++) st(mp.xi, ln, r, xs[r * SWG]);
   mp.status[e] = status;
   mp.n_newton[e] = newton;
   mp.n_fpi[e] = fpi;
@@ -443,107 +515,156 @@ DI void om2qu(const double* om, double* qu) {
   for (int k = 0; k < 4; k++) qu[k] /= n;
 }
 
-// Orientation, consistent tangent and homogenization (one constituent, v = 1).
-// Runs only when every cell of the response converged (flag[0] == ST_OK).
-template <int NSL>
-__global__ void __launch_bounds__(128) tangent_kernel(DmkMPDev mp, const DmkPhaseDev* __restrict__ ph,
+
+// row r (runtime, uniform) of a register 3x3 without dynamic register indexing
+DI void row3(const double* M, int r, double* v) {
+#pragma unroll
+  for (int j = 0; j < 3; j++) v[j] = r == 0 ? M[j] : (r == 1 ? M[3 + j] : M[6 + j]);
+}
+
+// Orientation and consistent tangent of the homogenization (one constituent,
+// v = 1) for a response in which every cell converged (flag[0] == ST_OK).
+// Also writes the squared Frobenius norm of each cell's dPdF, summed in
+// Fortran element order (the C_minMaxAvg selection works on it).
+template <int NSL, bool QP>
+__global__ void __launch_bounds__(SWG) tangent_kernel(DmkMPDev mp, const DmkPhaseDev* __restrict__ ph,
                                                       int phase_id, double dt, const int* __restrict__ flag,
-                                                      double* __restrict__ Phom, double* __restrict__ dPdF) {
-  const int e = blockIdx.x * blockDim.x + threadIdx.x;
-  const int N = mp.N;
-  if (e >= N || flag[0] != ST_OK) return;
+                                                      double* __restrict__ dPdF, double* __restrict__ nrm) {
+  __shared__ SlipLds<NSL> sh;
+  const int lane = threadIdx.x;
+  const int e = blockIdx.x * SWG + lane;
+  if (e >= mp.N || flag[0] != ST_OK) return;
   if (mp.phase[e] != phase_id) return;
-  const size_t sN = (size_t)N;
-  double Fe[9], Fp[9], S[9], F[9], Fp0[9];
-  m_load(mp.Fe, N, e, Fe);
-  m_load(mp.Fp, N, e, Fp);
-  m_load(mp.S, N, e, S);
-  m_load(mp.Fphase, N, e, F);
-  m_load(mp.Fp0, N, e, Fp0);
+  const Lane ln = make_lane(mp.N, e);
+  double* xs = &sh.xi[0][lane];
+  double* gs = &sh.g[0][lane];
+  double* ws = &sh.w[0][lane];
+  double Fe[9];
+  m_load(mp.Fe, ln, Fe);
   {
     double R[9], Rt[9], q[4];
     rotational_part(Fe, R);
     m_transpose(R, Rt);
     om2qu(Rt, q);
 #pragma unroll
-    for (int k = 0; k < 4; k++) mp.O[k * sN + e] = q[k];
+    for (int k = 0; k < 4; k++) st(mp.O, ln, k, q[k]);
   }
-  for (int k = 0; k < 9; k++) Phom[k * sN + e] = mp.Pph[k * sN + e] * 1.0;
+#pragma unroll 1
+  for (int r = 0; r < NSL; r++) xs[r * SWG] = ld(mp.xi, ln, r);
+  {
+    double S[9];
+    m_load(mp.S, ln, S);
+    kinetics<NSL, QP, true, false>(ph, S, xs, gs, ws, nullptr);
+  }
+  const bool nonschmid = ph->nonschmid;
 
-  double xi[NSL], g[NSL], dg[NSL];
-  bool pos[NSL];
-#pragma unroll
-  for (int r = 0; r < NSL; r++) xi[r] = mp.xi[r * sN + e];
-  kinetics<NSL>(ph, S, xi, g, dg, pos);
-
-  double invFp[9], invSubFp0[9], A2[9], G2[9];
-  m_invert(Fp, invFp);
-  m_invert(Fp0, invSubFp0);
-  m_mul(F, invSubFp0, A2);
-  m_mul_tn(Fe, A2, G2);
-  // K = I9 + dt sum_a dg_a vec(C:(G2 P_a)) vec(Pn_a)^T
+  double invFp[9], invSubFp0[9], A2[9];
+  {
+    double Fp[9], Fp0[9], F[9];
+    m_load(mp.Fp, ln, Fp);
+    m_load(mp.Fp0, ln, Fp0);
+    m_load(mp.Fphase, ln, F);
+    m_invert(Fp, invFp);
+    m_invert(Fp0, invSubFp0);
+    m_mul(F, invSubFp0, A2);
+  }
+  // K = I9 + dt sum_a dg_a vec(C:sym(G2 P_a)) vec(Pn_a)^T,  G2 = F_e^T A2
   double K[81];
+  {
+    double G2[9];
+    m_mul_tn(Fe, A2, G2);
 #pragma unroll
-  for (int r = 0; r < 81; r++) K[r] = (r % 10 == 0) ? 1.0 : 0.0;
+    for (int r = 0; r < 81; r++) K[r] = (r % 10 == 0) ? 1.0 : 0.0;
+#pragma unroll 1
+    for (int a = 0; a < NSL; a++) {
+      double X[9], Xs[9], W[9];
+      m_mul(G2, ph->P[a], X);
 #pragma unroll
-  for (int a = 0; a < NSL; a++) {
-    double X[9], Xs[9], W[9];
-    m_mul(G2, ph->P[a], X);
+      for (int i = 0; i < 3; i++)
 #pragma unroll
-    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) Xs[3 * i + j] = 0.5 * (X[3 * i + j] + X[3 * j + i]);
+      hooke_sym(ph->C11, ph->C12, ph->C44, Xs, W);
+      const bool pos = gs[a * SWG] > 0.0;
+      double u = dt * ws[a * SWG];
+      if (nonschmid) {
+        double Pn[9];
 #pragma unroll
-      for (int j = 0; j < 3; j++) Xs[3 * i + j] = 0.5 * (X[3 * i + j] + X[3 * j + i]);
-    hooke_sym(ph->C11, ph->C12, ph->C44, Xs, W);
-    const double* Pn = pos[a] ? ph->Pp[a] : ph->Pm[a];
-    double u = dt * dg[a];
+        for (int c = 0; c < 9; c++) Pn[c] = pos ? ph->Pp[a][c] : ph->Pm[a][c];
 #pragma unroll
-    for (int r = 0; r < 9; r++) {
-      double wr = u * W[r];
+        for (int r = 0; r < 9; r++) {
+          double wr = u * W[r];
 #pragma unroll
-      for (int c = 0; c < 9; c++) K[9 * r + c] = fma(wr, Pn[c], K[9 * r + c]);
+          for (int c = 0; c < 9; c++) K[9 * r + c] = fma(wr, Pn[c], K[9 * r + c]);
+        }
+      } else {
+        u = pos ? u : -u;            // Schmid law: P_nS,neg = -P_nS,pos exactly
+#pragma unroll
+        for (int r = 0; r < 9; r++) {
+          double wr = u * W[r];
+#pragma unroll
+          for (int c = 0; c < 9; c++) K[9 * r + c] = fma(wr, ph->Pp[a][c], K[9 * r + c]);
+        }
+      }
     }
   }
-  int piv[9];
-  bool sing = lu9_factor(K, piv);
+  double v0[9], tau[9];
+  const bool sing = dmk_qr_factor<9>(K, v0, tau);
 
   double M1[9], M2[9], M3[9], Mid[9];
-  m_mul_nt(S, invFp, M1);        // S invFp^T
-  m_mul(F, invFp, M2);           // F invFp
-  m_mul(M2, S, M3);              // F invFp S
-  m_mul(invFp, M1, Mid);         // invFp S invFp^T
+  {
+    double S[9], F[9];
+    m_load(mp.S, ln, S);
+    m_load(mp.Fphase, ln, F);
+    m_mul_nt(S, invFp, M1);        // S invFp^T
+    m_mul(F, invFp, M2);           // F invFp
+    m_mul(M2, S, M3);              // F invFp S
+    m_mul(invFp, M1, Mid);         // invFp S invFp^T
+  }
+  double n2 = 0.0;
 #pragma unroll 1
-  for (int k = 0; k < 3; k++) {
+  for (int l = 0; l < 3; l++) {
 #pragma unroll 1
-    for (int l = 0; l < 3; l++) {
+    for (int k = 0; k < 3; k++) {
       // rhs column: C : sym(Fe(k,:) (x) invFp(l,:))
+      double fk[3], pl[3], ml[3];
+      row3(Fe, k, fk);
+      row3(invFp, l, pl);
+      row3(Mid, l, ml);
       double X[9], y[9];
 #pragma unroll
       for (int n = 0; n < 3; n++)
 #pragma unroll
-        for (int m = 0; m < 3; m++) X[3 * n + m] = 0.5 * (Fe[3 * k + n] * invFp[3 * l + m] + Fe[3 * k + m] * invFp[3 * l + n]);
+        for (int m = 0; m < 3; m++) X[3 * n + m] = 0.5 * (fk[n] * pl[m] + fk[m] * pl[n]);
       hooke_sym(ph->C11, ph->C12, ph->C44, X, y);
-      if (!sing) lu9_solve(K, piv, y);
+      if (!sing) dmk_qr_apply_solve<9>(K, v0, tau, y);
       // t = sum_a dg_a P_a (Pn_a : dSdF)
       double t[9];
 #pragma unroll
       for (int q = 0; q < 9; q++) t[q] = 0.0;
-#pragma unroll
+#pragma unroll 1
       for (int a = 0; a < NSL; a++) {
-        const double* Pn = pos[a] ? ph->Pp[a] : ph->Pm[a];
+        const bool pos = gs[a * SWG] > 0.0;
         double c = 0.0;
+        if (nonschmid) {
 #pragma unroll
-        for (int q = 0; q < 9; q++) c = fma(Pn[q], y[q], c);
-        c *= dg[a];
+          for (int q = 0; q < 9; q++) c = fma(pos ? ph->Pp[a][q] : ph->Pm[a][q], y[q], c);
+        } else {
+#pragma unroll
+          for (int q = 0; q < 9; q++) c = fma(ph->Pp[a][q], y[q], c);
+          c = pos ? c : -c;
+        }
+        c *= ws[a * SWG];
 #pragma unroll
         for (int q = 0; q < 9; q++) t[q] = fma(c, ph->P[a][q], t[q]);
       }
-      double D[9];
+      // dFpinv = -dt invSubFp0 t;  F dFpinv = -dt A2 t
+      double D[9], AD[9];
       m_mul(invSubFp0, t, D);
+      m_mul(A2, t, AD);
 #pragma unroll
-      for (int q = 0; q < 9; q++) D[q] = -D[q] * dt;
-      double R1[9], R2[9], R3[9], R4[9], col[9];
-      m_mul(F, D, R1);
-      m_mul(R1, M1, R2);            // F dFpinv S invFp^T
+      for (int q = 0; q < 9; q++) { D[q] = -D[q] * dt; AD[q] = -AD[q] * dt; }
+      double R2[9], R3[9], R4[9], R1[9], col[9];
+      m_mul(AD, M1, R2);            // F dFpinv S invFp^T
       m_mul(M2, y, R3);
       m_mul_nt(R3, invFp, R4);      // F invFp dSdF invFp^T
       m_mul_nt(M3, D, R1);          // F invFp S dFpinv^T
@@ -551,41 +672,63 @@ __global__ void __launch_bounds__(128) tangent_kernel(DmkMPDev mp, const DmkPhas
       for (int i = 0; i < 3; i++)
 #pragma unroll
         for (int j = 0; j < 3; j++) {
-          double base = (i == k) ? Mid[3 * l + j] : 0.0;
+          double base = (i == k) ? ml[j] : 0.0;
           col[3 * i + j] = base + R2[3 * i + j] + R4[3 * i + j] + R1[3 * i + j];
         }
 #pragma unroll
-      for (int i = 0; i < 3; i++)
+      for (int j = 0; j < 3; j++)
 #pragma unroll
-        for (int j = 0; j < 3; j++)
-          dPdF[(size_t)(i + 3 * j + 9 * k + 27 * l) * sN + e] = col[3 * i + j];
+        for (int i = 0; i < 3; i++) {
+          double v = col[3 * i + j];
+          st(dPdF, ln, i + 3 * j + 9 * k + 27 * l, v);
+          n2 = fma(v, v, n2);
+        }
     }
   }
+  nrm[e] = n2;
 }
 
 }  // namespace
 
 // ---------------------------------------------------------------------------- launchers
-extern "C" int dmk_launch_stress(const DmkMPDev* mp, const double* Ftarget, const DmkPhaseDev* ph, int nsl,
-                                 int phase_id, const DmkNumMech* num, double dt, hipStream_t s) {
-  dim3 blk(128), grd((mp->N + 127) / 128);
-  if (nsl == 12)
-    hipLaunchKernelGGL(stress_kernel<12>, grd, blk, 0, s, *mp, Ftarget, ph, phase_id, *num, dt);
-  else if (nsl == 24)
-    hipLaunchKernelGGL(stress_kernel<24>, grd, blk, 0, s, *mp, Ftarget, ph, phase_id, *num, dt);
-  else
-    return 1;
+template <int NSL, bool QP>
+static void launch_stress(const DmkMPDev* mp, const double* Ftarget, const DmkPhaseDev* ph, int phase_id,
+                          const DmkNumMech* num, double dt, hipStream_t s) {
+  dim3 blk(SWG), grd((mp->N + SWG - 1) / SWG);
+  hipLaunchKernelGGL((stress_kernel<NSL, QP>), grd, blk, 0, s, *mp, Ftarget, ph, phase_id, *num, dt);
+}
+
+template <int NSL, bool QP>
+static void launch_tangent(const DmkMPDev* mp, const DmkPhaseDev* ph, int phase_id, double dt, const int* flag,
+                           double* dPdF, double* nrm, hipStream_t s) {
+  dim3 blk(SWG), grd((mp->N + SWG - 1) / SWG);
+  hipLaunchKernelGGL((tangent_kernel<NSL, QP>), grd, blk, 0, s, *mp, ph, phase_id, dt, flag, dPdF, nrm);
+}
+
+extern "C" int dmk_launch_stress(const DmkMPDev* mp, const double* Ftarget, const DmkPhaseDev* ph,
+                                 const DmkPhaseDev* ph_host, int phase_id, const DmkNumMech* num, double dt,
+                                 hipStream_t s) {
+  const bool qp = ph_host->quarter_pow != 0;
+  switch (ph_host->nsl) {
+    case 12: qp ? launch_stress<12, true>(mp, Ftarget, ph, phase_id, num, dt, s)
+                : launch_stress<12, false>(mp, Ftarget, ph, phase_id, num, dt, s); break;
+    case 24: qp ? launch_stress<24, true>(mp, Ftarget, ph, phase_id, num, dt, s)
+                : launch_stress<24, false>(mp, Ftarget, ph, phase_id, num, dt, s); break;
+    default: return 1;
+  }
   return hipGetLastError() == hipSuccess ? 0 : 2;
 }
 
-extern "C" int dmk_launch_tangent(const DmkMPDev* mp, const DmkPhaseDev* ph, int nsl, int phase_id, double dt,
-                                  const int* flag, double* Phom, double* dPdF, hipStream_t s) {
-  dim3 blk(128), grd((mp->N + 127) / 128);
-  if (nsl == 12)
-    hipLaunchKernelGGL(tangent_kernel<12>, grd, blk, 0, s, *mp, ph, phase_id, dt, flag, Phom, dPdF);
-  else if (nsl == 24)
-    hipLaunchKernelGGL(tangent_kernel<24>, grd, blk, 0, s, *mp, ph, phase_id, dt, flag, Phom, dPdF);
-  else
-    return 1;
+extern "C" int dmk_launch_tangent(const DmkMPDev* mp, const DmkPhaseDev* ph, const DmkPhaseDev* ph_host,
+                                  int phase_id, double dt, const int* flag, double* dPdF, double* nrm,
+                                  hipStream_t s) {
+  const bool qp = ph_host->quarter_pow != 0;
+  switch (ph_host->nsl) {
+    case 12: qp ? launch_tangent<12, true>(mp, ph, phase_id, dt, flag, dPdF, nrm, s)
+                : launch_tangent<12, false>(mp, ph, phase_id, dt, flag, dPdF, nrm, s); break;
+    case 24: qp ? launch_tangent<24, true>(mp, ph, phase_id, dt, flag, dPdF, nrm, s)
+                : launch_tangent<24, false>(mp, ph, phase_id, dt, flag, dPdF, nrm, s); break;
+    default: return 1;
+  }
   return hipGetLastError() == hipSuccess ? 0 : 2;
 }

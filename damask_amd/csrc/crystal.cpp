@@ -196,7 +196,13 @@ int dmk_build_phase(const dmk_phase_desc* d, DmkPhaseDev* p) {
       p->c1[a] = d->c_1[f];
       p->c2[a] = d->c_2[f];
       p->fsat[a] = d->f_sat[f];
+      p->xi_sat[a] = d->xi_inf[f] + d->f_sat[f] * std::sqrt(0.0);
+      p->hf[a] = d->h_0[f] * (1.0 + d->c_1[f] * std::pow(0.0, d->c_2[f]));
     }
+  p->quarter_pow = 1;
+  for (int b = 0; b < nsl; b++)
+    for (double y : {p->n[b], p->a[b]})
+      if (!(y > 0.0 && y <= 64.0 && y * 4.0 == std::rint(y * 4.0))) p->quarter_pow = 0;
   p->atol_xi = d->atol_xi;
   p->atol_gamma = d->atol_gamma;
   return 0;

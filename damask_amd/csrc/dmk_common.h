@@ -50,6 +50,7 @@ void dmk_set_error(const char* fmt, ...);
 struct DmkPhaseDev {
   int nsl;
   int nonschmid;                    // P_nS_neg differs from -P_nS_pos
+  int quarter_pow;                  // every n_sl and a_sl is a positive multiple of 1/4, <= 64
   double C11, C12, C44;             // cubic stiffness (Voigt constants), lattice frame
   double P[DMK_NSL_MAX][9];         // Schmid tensor d (x) n
   double Pp[DMK_NSL_MAX][9];        // non-Schmid projection, positive sense
@@ -59,6 +60,8 @@ struct DmkPhaseDev {
   double H[DMK_NSL_MAX][DMK_NSL_MAX];  // H[reacting][acting]
   double g0[DMK_NSL_MAX], n[DMK_NSL_MAX], a[DMK_NSL_MAX], h0[DMK_NSL_MAX];
   double xi_inf[DMK_NSL_MAX], c1[DMK_NSL_MAX], c2[DMK_NSL_MAX], fsat[DMK_NSL_MAX];
+  // without twins (sum F_tw = 0): xi_inf + f_sat sqrt(0) and h_0 (1 + c_1 0^c_2)
+  double xi_sat[DMK_NSL_MAX], hf[DMK_NSL_MAX];
   double atol_xi, atol_gamma;
 };
 
@@ -94,8 +97,8 @@ struct DmkMPDev {
   int32_t* n_newton;             // per cell Newton iterations of the last response
   int32_t* n_fpi;
   // scratch of the state integrator: [2*nsl_max][N] each
-  double* st0s;
-  double* last1;
+  double* st0s;                  // state at the start of the sub-step (after the first sub-step)
+  double* last1;                 // state-rate history, ping-pong pair
   double* last2;
-  double* dot;
+  double* sub;                   // [36][N]: sub-step F_0, F, F_p0, L_p0
 };

@@ -24,10 +24,13 @@ int dmk_mdot(const double* x, const double* Y, long ldY, int l, int mode, long n
 int dmk_axpby(long n, double a, const double* x, double b, const double* y, double* z, hipStream_t s);
 int dmk_combine(long n, const double* XM, const double* Xh, long ldX, const CombineArgs* c, double* XA,
                 hipStream_t s);
-int dmk_cminmax(const double* dPdF, long N, double band, double* nrm, double* pmax, double* pmin, long* sel,
-                double* ext, double* out, hipStream_t s);
-int dmk_status(const int32_t* st, const int32_t* nn, const int32_t* nf, long N, int* flag, unsigned long long* cnt,
-               hipStream_t s);
+// C_minMaxAvg from the per-cell norms nrm (written by the tangent kernel):
+// partial extrema, global extrema, per-chunk first qualifying cell, first overall, gather
+int dmk_cminmax(const double* dPdF, long N, double band, const double* nrm, double* pmax, double* pmin,
+                long* csel, long* sel, double* ext, double* out, hipStream_t s);
+// convergence flag and iteration counters; partial needs 3 * dmk_nchunk(N) doubles
+int dmk_status(const int32_t* st, const int32_t* nn, const int32_t* nf, long N, double* partial, int* flag,
+               unsigned long long* cnt, hipStream_t s);
 int dmk_forward_field(long N, int guess, double dt_old, double dt, const ForwardArgs* fa, const double* F,
                       double* F_lastinc, double* Fdot, double* Fnew, hipStream_t s);
 int dmk_shift_field(long N, const ForwardArgs* fa, double* F, hipStream_t s);
@@ -38,7 +41,9 @@ int dmk_soa_to_aos(const double* soa, long N, int w, double* a, hipStream_t s);
 struct DmkMPDev;
 struct DmkPhaseDev;
 struct DmkNumMech;
-extern "C" int dmk_launch_stress(const DmkMPDev* mp, const double* Ftarget, const DmkPhaseDev* ph, int nsl,
-                                 int phase_id, const DmkNumMech* num, double dt, hipStream_t s);
-extern "C" int dmk_launch_tangent(const DmkMPDev* mp, const DmkPhaseDev* ph, int nsl, int phase_id, double dt,
-                                  const int* flag, double* Phom, double* dPdF, hipStream_t s);
+extern "C" int dmk_launch_stress(const DmkMPDev* mp, const double* Ftarget, const DmkPhaseDev* ph,
+                                 const DmkPhaseDev* ph_host, int phase_id, const DmkNumMech* num, double dt,
+                                 hipStream_t s);
+extern "C" int dmk_launch_tangent(const DmkMPDev* mp, const DmkPhaseDev* ph, const DmkPhaseDev* ph_host,
+                                  int phase_id, double dt, const int* flag, double* dPdF, double* nrm,
+                                  hipStream_t s);

@@ -3,21 +3,59 @@
 
 #include <hip/hip_runtime.h>
 #include <float.h>
+#include <stdint.h>
 
 #define DI __device__ __forceinline__
 
-DI void m_load(const double* __restrict__ f, int N, int e, double* T) {
-#pragma unroll
-  for (int i = 0; i < 3; i++)
-#pragma unroll
-    for (int j = 0; j < 3; j++) T[3 * i + j] = f[(size_t)(i + 3 * j) * N + e];
+// Per-cell field access.  Fields are component-major ([c][N]); a lane reaches
+// its cell through a 32-bit byte offset (cell index * 8, so N < 2^29) added to
+// a wave-uniform component base.  That maps to global loads/stores with a
+// scalar base register: no 64-bit per-lane address has to stay live in VGPRs.
+struct Lane {
+  size_t sN;        // field stride (cells)
+  uint32_t off;     // byte offset of the lane's cell
+};
+
+DI Lane make_lane(int N, int e) { return Lane{(size_t)N, (uint32_t)e * 8u}; }
+
+// The offset is passed through an empty asm per access so the optimiser cannot
+// re-associate (base_c + off) into (base + off) + c N, a per-lane 64-bit
+// pointer shared by all components.
+DI uint32_t lane_off(const Lane& l) {
+  uint32_t o = l.off;
+  asm volatile("" : "+v"(o));
+  return o;
 }
 
-DI void m_store(double* __restrict__ f, int N, int e, const double* T) {
+DI double ld(const double* f, const Lane& l, int c) {
+  return *(const double*)((const char*)(f + (size_t)c * l.sN) + lane_off(l));
+}
+
+DI void st(double* f, const Lane& l, int c, double v) {
+  *(double*)((char*)(f + (size_t)c * l.sN) + lane_off(l)) = v;
+}
+
+// T_ij (row-major in registers) <-> component i + 3 j of the field
+DI void m_load(const double* f, const Lane& l, double* T) {
 #pragma unroll
   for (int i = 0; i < 3; i++)
 #pragma unroll
-    for (int j = 0; j < 3; j++) f[(size_t)(i + 3 * j) * N + e] = T[3 * i + j];
+    for (int j = 0; j < 3; j++) T[3 * i + j] = ld(f, l, i + 3 * j);
+}
+
+DI void m_store(double* f, const Lane& l, const double* T) {
+#pragma unroll
+  for (int i = 0; i < 3; i++)
+#pragma unroll
+    for (int j = 0; j < 3; j++) st(f, l, i + 3 * j, T[3 * i + j]);
+}
+
+// Make a value opaque to the optimiser (no code emitted): address arithmetic
+// derived from it cannot be hoisted out of the enclosing loop.
+template <class T>
+DI T* opaque(T* p) {
+  asm volatile("" : "+v"(p));
+  return p;
 }
 
 DI void m_mul(const double* A, const double* B, double* C) {

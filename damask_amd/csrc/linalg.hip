@@ -77,15 +77,6 @@ __global__ void combine_kernel(long n, const double* __restrict__ XM, const doub
   XA[i] = v;
 }
 
-// per cell squared Frobenius norm of dPdF, summed in Fortran element order
-__global__ void dpdf_norm_kernel(const double* __restrict__ dPdF, long N, double* __restrict__ nrm) {
-  long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= N) return;
-  double s = 0.0;
-  for (int q = 0; q < 81; q++) { double v = dPdF[q * N + e]; s = fma(v, v, s); }
-  nrm[e] = s;
-}
-
 __global__ void __launch_bounds__(RB) minmax_partial(const double* __restrict__ nrm, long N, double* __restrict__ pmax,
                                                      double* __restrict__ pmin) {
   __shared__ double smax[RB], smin[RB];
@@ -108,13 +99,10 @@ __global__ void __launch_bounds__(RB) minmax_partial(const double* __restrict__ 
   if (threadIdx.x == 0) { pmax[blockIdx.x] = smax[0]; pmin[blockIdx.x] = smin[0]; }
 }
 
-// global extrema from the block partials, then the first cell satisfying the
-// (banded) extremum rule; one block, writes sel[0] = imax, sel[1] = imin
-__global__ void __launch_bounds__(RB) select_extrema(const double* __restrict__ nrm, long N, const double* __restrict__ pmax,
-                                                     const double* __restrict__ pmin, int nchunk, double band,
-                                                     long* __restrict__ sel, double* __restrict__ ext) {
+// global extrema of the per-cell norms from the chunk partials (one block)
+__global__ void __launch_bounds__(RB) extrema_finalize(const double* __restrict__ pmax, const double* __restrict__ pmin,
+                                                       int nchunk, double* __restrict__ ext) {
   __shared__ double smax[RB], smin[RB];
-  __shared__ long simax[RB], simin[RB];
   double mx = 0.0, mn = DBL_MAX;
   for (int i = threadIdx.x; i < nchunk; i += RB) { mx = fmax(mx, pmax[i]); mn = fmin(mn, pmin[i]); }
   smax[threadIdx.x] = mx; smin[threadIdx.x] = mn;
@@ -126,15 +114,41 @@ __global__ void __launch_bounds__(RB) select_extrema(const double* __restrict__ 
     }
     __syncthreads();
   }
-  double gmax = smax[0], gmin = smin[0];
-  __syncthreads();
+  if (threadIdx.x == 0) { ext[0] = smax[0]; ext[1] = smin[0]; }
+}
+
+// first cell of each chunk that satisfies the (banded) extremum rule:
+// norm (1 + band) >= max, norm (1 - band) <= min; N when none
+__global__ void __launch_bounds__(RB) select_partial(const double* __restrict__ nrm, long N,
+                                                     const double* __restrict__ ext, double band,
+                                                     long* __restrict__ csel, int nchunk) {
+  __shared__ long simax[RB], simin[RB];
+  const double gmax = ext[0], gmin = ext[1];
+  long s = (long)blockIdx.x * PER_BLOCK, e = min(s + PER_BLOCK, N);
   long imax = N, imin = N;
-  for (long i = threadIdx.x; i < N; i += RB) {
+  for (long i = s + threadIdx.x; i < e; i += RB) {
     double v = nrm[i];
     if (imax == N && gmax > 0.0 && v * (1.0 + band) >= gmax) imax = i;
     if (imin == N && gmin < DBL_MAX && v * (1.0 - band) <= gmin) imin = i;
-    if (imax != N && imin != N) break;
   }
+  simax[threadIdx.x] = imax; simin[threadIdx.x] = imin;
+  __syncthreads();
+  for (int k = RB / 2; k > 0; k >>= 1) {
+    if (threadIdx.x < k) {
+      simax[threadIdx.x] = min(simax[threadIdx.x], simax[threadIdx.x + k]);
+      simin[threadIdx.x] = min(simin[threadIdx.x], simin[threadIdx.x + k]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) { csel[blockIdx.x] = simax[0]; csel[nchunk + blockIdx.x] = simin[0]; }
+}
+
+// first qualifying cell over all chunks (one block): sel[0] = imax, sel[1] = imin (-1: none)
+__global__ void __launch_bounds__(RB) select_finalize(const long* __restrict__ csel, int nchunk, long N,
+                                                      long* __restrict__ sel) {
+  __shared__ long simax[RB], simin[RB];
+  long imax = N, imin = N;
+  for (int i = threadIdx.x; i < nchunk; i += RB) { imax = min(imax, csel[i]); imin = min(imin, csel[nchunk + i]); }
   simax[threadIdx.x] = imax; simin[threadIdx.x] = imin;
   __syncthreads();
   for (int k = RB / 2; k > 0; k >>= 1) {
@@ -147,8 +161,6 @@ __global__ void __launch_bounds__(RB) select_extrema(const double* __restrict__ 
   if (threadIdx.x == 0) {
     sel[0] = simax[0] < N ? simax[0] : -1;
     sel[1] = simin[0] < N ? simin[0] : -1;
-    ext[0] = gmax;
-    ext[1] = gmin;
   }
 }
 
@@ -161,33 +173,61 @@ __global__ void gather_cminmax(const double* __restrict__ dPdF, long N, const lo
   out[q] = 0.5 * (mx + mn);
 }
 
-// flag[0] = ST_OK if every cell converged, ST_FAIL_MECH otherwise; flag[1..2]
-// accumulate Newton and state iterations
-__global__ void __launch_bounds__(RB) status_kernel(const int32_t* __restrict__ st, const int32_t* __restrict__ nn,
-                                                    const int32_t* __restrict__ nf, long N, int* __restrict__ flag,
-                                                    unsigned long long* __restrict__ cnt) {
-  __shared__ int bad;
-  __shared__ unsigned long long s1[RB], s2[RB];
-  if (threadIdx.x == 0) bad = 0;
-  __syncthreads();
-  unsigned long long a = 0, b = 0;
-  int mybad = 0;
-  for (long i = threadIdx.x; i < N; i += RB) {
-    if (st[i] != ST_OK) mybad = 1;
-    a += (unsigned long long)nn[i];
-    b += (unsigned long long)nf[i];
+// per chunk: any cell not converged, sums of Newton and state iterations
+// (integers, exact in double below 2^53)
+__global__ void __launch_bounds__(RB) status_partial(const int32_t* __restrict__ st, const int32_t* __restrict__ nn,
+                                                     const int32_t* __restrict__ nf, long N, double* __restrict__ part,
+                                                     int nchunk) {
+  __shared__ double sb[RB], s1[RB], s2[RB];
+  long s = (long)blockIdx.x * PER_BLOCK, e = min(s + PER_BLOCK, N);
+  double bad = 0.0, a = 0.0, b = 0.0;
+  for (long i = s + threadIdx.x; i < e; i += RB) {
+    if (st[i] != ST_OK) bad = 1.0;
+    a += (double)nn[i];
+    b += (double)nf[i];
   }
-  if (mybad) atomicOr(&bad, 1);
-  s1[threadIdx.x] = a; s2[threadIdx.x] = b;
+  sb[threadIdx.x] = bad; s1[threadIdx.x] = a; s2[threadIdx.x] = b;
   __syncthreads();
   for (int k = RB / 2; k > 0; k >>= 1) {
-    if (threadIdx.x < k) { s1[threadIdx.x] += s1[threadIdx.x + k]; s2[threadIdx.x] += s2[threadIdx.x + k]; }
+    if (threadIdx.x < k) {
+      sb[threadIdx.x] = fmax(sb[threadIdx.x], sb[threadIdx.x + k]);
+      s1[threadIdx.x] += s1[threadIdx.x + k];
+      s2[threadIdx.x] += s2[threadIdx.x + k];
+    }
     __syncthreads();
   }
   if (threadIdx.x == 0) {
-    flag[0] = bad ? ST_FAIL_MECH : ST_OK;
-    cnt[0] += s1[0];
-    cnt[1] += s2[0];
+    part[blockIdx.x] = sb[0];
+    part[nchunk + blockIdx.x] = s1[0];
+    part[2 * nchunk + blockIdx.x] = s2[0];
+  }
+}
+
+// flag[0] = ST_OK if every cell converged, ST_FAIL_MECH otherwise; cnt[0..1]
+// accumulate Newton and state iterations (one block)
+__global__ void __launch_bounds__(RB) status_finalize(const double* __restrict__ part, int nchunk,
+                                                      int* __restrict__ flag, unsigned long long* __restrict__ cnt) {
+  __shared__ double sb[RB], s1[RB], s2[RB];
+  double bad = 0.0, a = 0.0, b = 0.0;
+  for (int i = threadIdx.x; i < nchunk; i += RB) {
+    bad = fmax(bad, part[i]);
+    a += part[nchunk + i];
+    b += part[2 * nchunk + i];
+  }
+  sb[threadIdx.x] = bad; s1[threadIdx.x] = a; s2[threadIdx.x] = b;
+  __syncthreads();
+  for (int k = RB / 2; k > 0; k >>= 1) {
+    if (threadIdx.x < k) {
+      sb[threadIdx.x] = fmax(sb[threadIdx.x], sb[threadIdx.x + k]);
+      s1[threadIdx.x] += s1[threadIdx.x + k];
+      s2[threadIdx.x] += s2[threadIdx.x + k];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    flag[0] = sb[0] > 0.0 ? ST_FAIL_MECH : ST_OK;
+    cnt[0] += (unsigned long long)s1[0];
+    cnt[1] += (unsigned long long)s2[0];
   }
 }
 
@@ -258,19 +298,22 @@ int dmk_combine(long n, const double* XM, const double* Xh, long ldX, const Comb
   return hipGetLastError() == hipSuccess ? 0 : 2;
 }
 
-int dmk_cminmax(const double* dPdF, long N, double band, double* nrm, double* pmax, double* pmin, long* sel,
-                double* ext, double* out, hipStream_t s) {
-  hipLaunchKernelGGL(dpdf_norm_kernel, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, s, dPdF, N, nrm);
+int dmk_cminmax(const double* dPdF, long N, double band, const double* nrm, double* pmax, double* pmin,
+                long* csel, long* sel, double* ext, double* out, hipStream_t s) {
   int nc = dmk_nchunk(N);
   hipLaunchKernelGGL(minmax_partial, dim3(nc), dim3(RB), 0, s, nrm, N, pmax, pmin);
-  hipLaunchKernelGGL(select_extrema, dim3(1), dim3(RB), 0, s, nrm, N, pmax, pmin, nc, band, sel, ext);
+  hipLaunchKernelGGL(extrema_finalize, dim3(1), dim3(RB), 0, s, pmax, pmin, nc, ext);
+  hipLaunchKernelGGL(select_partial, dim3(nc), dim3(RB), 0, s, nrm, N, ext, band, csel, nc);
+  hipLaunchKernelGGL(select_finalize, dim3(1), dim3(RB), 0, s, csel, nc, N, sel);
   hipLaunchKernelGGL(gather_cminmax, dim3(1), dim3(128), 0, s, dPdF, N, sel, out);
   return hipGetLastError() == hipSuccess ? 0 : 2;
 }
 
-int dmk_status(const int32_t* st, const int32_t* nn, const int32_t* nf, long N, int* flag, unsigned long long* cnt,
-               hipStream_t s) {
-  hipLaunchKernelGGL(status_kernel, dim3(1), dim3(RB), 0, s, st, nn, nf, N, flag, cnt);
+int dmk_status(const int32_t* st, const int32_t* nn, const int32_t* nf, long N, double* partial, int* flag,
+               unsigned long long* cnt, hipStream_t s) {
+  int nc = dmk_nchunk(N);
+  hipLaunchKernelGGL(status_partial, dim3(nc), dim3(RB), 0, s, st, nn, nf, N, partial, nc);
+  hipLaunchKernelGGL(status_finalize, dim3(1), dim3(RB), 0, s, partial, nc, flag, cnt);
   return hipGetLastError() == hipSuccess ? 0 : 2;
 }
 

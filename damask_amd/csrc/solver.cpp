@@ -209,12 +209,13 @@ struct dmk_handle {
   double2 *Z, *U;
   double *u3, *xn_dev, *xp_dev;
   double *partial, *small, *nrm, *pmax, *pmin, *ext;
-  long* sel;
+  long *sel, *csel;
   int* flag;
   unsigned long long* cnt;
   double* hsmall = nullptr;          // pinned host mirror of small results
   // solver state
   double F_aimDot[9], F_aim[9], F_aim_lastinc[9], P_av[9], P_aim[9];
+  double P_av_lab[9] = {0};          // volume average of the accepted P, lab frame
   double C_volAvg[81], C_volAvgLastInc[81], C_minMaxAvg[81], C_minMaxAvgLastInc[81], S[81];
   double err_div = 0, err_BC = 0;
   int status = 0, totalIter = 0, nfuncs = 0;
@@ -235,7 +236,8 @@ struct dmk_handle {
   bool ev_ready = false;
 };
 
-enum { KT_STRESS = 0, KT_TANGENT = 1, KT_REDUCE = 2, KT_FFT_FWD = 3, KT_GAMMA = 4, KT_FFT_INV = 5, KT_N = 6 };
+enum { KT_STRESS = 0, KT_TANGENT = 1, KT_REDUCE = 2, KT_FFT_FWD = 3, KT_GAMMA = 4, KT_FFT_INV = 5, KT_CAVG = 6,
+       KT_N = 7 };
 
 static void kt_begin(dmk_handle* h, int k) {
   if (h->profile) hipEventRecord(h->ev[k][0], h->st);
@@ -279,36 +281,64 @@ static void set_gamma_coef(dmk_handle* h, const double* C) {
       }
 }
 
-// utilities_constitutiveResponse on the device field F (component-major)
-static int constitutive(dmk_handle* h, const double* F, double dt, bool rotate) {
-  kt_begin(h, KT_STRESS);
-  for (int p = 0; p < h->n_phases; p++)
-    CKL(dmk_launch_stress(&h->mp, F, h->ph_dev + p, h->ph[p].nsl, p, &h->nm, dt, h->st));
-  kt_end(h, KT_STRESS);
-  CKL(dmk_status(h->mp.status, h->mp.n_newton, h->mp.n_fpi, h->N, h->flag, h->cnt, h->st));
+// Orientation, dPdF and the stiffness averages C_volAvg / C_minMaxAvg of the
+// last response (homogenization_mechanical_response's second loop and the
+// averages of utilities_constitutiveResponse, grid_utilities.f90:163-214).
+// The spectral-basic solver reads the averages only between solves (solution
+// start, forward, restart; grid_mech_spectral_basic.f90:261-262,306-311), so
+// the solver runs this once after each solve instead of after every residual
+// evaluation: the values it reads are those of the last response either way.
+// Skipped when the last response did not converge (the reference then keeps
+// the previous dPdF).
+static int tangent_pass(dmk_handle* h, double dt) {
+  if (h->status != ST_OK) return 0;
   kt_begin(h, KT_TANGENT);
   for (int p = 0; p < h->n_phases; p++)
-    CKL(dmk_launch_tangent(&h->mp, h->ph_dev + p, h->ph[p].nsl, p, dt, h->flag, h->P_hom, h->dPdF, h->st));
+    CKL(dmk_launch_tangent(&h->mp, h->ph_dev + p, &h->ph[p], p, dt, h->flag, h->dPdF, h->nrm, h->st));
   kt_end(h, KT_TANGENT);
-  kt_begin(h, KT_REDUCE);
-  CKL(dmk_colsum(h->P_hom, h->N, h->N, 9, h->partial, h->small + SM_PAV, h->wgt, h->st));
+  kt_begin(h, KT_CAVG);
   CKL(dmk_colsum(h->dPdF, h->N, h->N, 81, h->partial, h->small + SM_CVOL, h->wgt, h->st));
-  CKL(dmk_cminmax(h->dPdF, h->N, h->num.cref_rule == 1 ? 1.0e-12 : 0.0, h->nrm, h->pmax, h->pmin, h->sel, h->ext,
-                  h->small + SM_CMM, h->st));
+  CKL(dmk_cminmax(h->dPdF, h->N, h->num.cref_rule == 1 ? 1.0e-12 : 0.0, h->nrm, h->pmax, h->pmin, h->csel, h->sel,
+                  h->ext, h->small + SM_CMM, h->st));
+  kt_end(h, KT_CAVG);
+  CK(hipMemcpyAsync(h->hsmall + SM_CVOL, h->small + SM_CVOL, sizeof(double) * 162, hipMemcpyDeviceToHost, h->st));
+  CK(hipStreamSynchronize(h->st));
+  kt_collect(h, KT_TANGENT);
+  kt_collect(h, KT_CAVG);
+  std::memcpy(h->C_volAvg, h->hsmall + SM_CVOL, sizeof h->C_volAvg);
+  std::memcpy(h->C_minMaxAvg, h->hsmall + SM_CMM, sizeof h->C_minMaxAvg);
+  return 0;
+}
+
+// utilities_constitutiveResponse on the device field F (component-major):
+// stress integration of every cell, convergence flag, P_av.  The homogenized
+// P of a response in which every cell converged is accepted by swapping it
+// with the previous one (a response with a failed cell leaves P and P_av as
+// they were, as the reference's skipped second loop does).
+static int constitutive(dmk_handle* h, const double* F, double dt, bool rotate, bool tangent) {
+  kt_begin(h, KT_STRESS);
+  for (int p = 0; p < h->n_phases; p++)
+    CKL(dmk_launch_stress(&h->mp, F, h->ph_dev + p, &h->ph[p], p, &h->nm, dt, h->st));
+  kt_end(h, KT_STRESS);
+  kt_begin(h, KT_REDUCE);
+  CKL(dmk_status(h->mp.status, h->mp.n_newton, h->mp.n_fpi, h->N, h->partial, h->flag, h->cnt, h->st));
+  CKL(dmk_colsum(h->mp.Pph, h->N, h->N, 9, h->partial, h->small + SM_PAV, h->wgt, h->st));
   kt_end(h, KT_REDUCE);
-  CK(hipMemcpyAsync(h->hsmall, h->small, sizeof(double) * SM_DIV, hipMemcpyDeviceToHost, h->st));
+  CK(hipMemcpyAsync(h->hsmall + SM_PAV, h->small + SM_PAV, sizeof(double) * 9, hipMemcpyDeviceToHost, h->st));
   int flag = 0;
   CK(hipMemcpyAsync(&flag, h->flag, sizeof(int), hipMemcpyDeviceToHost, h->st));
   CK(hipStreamSynchronize(h->st));
   kt_collect(h, KT_STRESS);
-  kt_collect(h, KT_TANGENT);
   kt_collect(h, KT_REDUCE);
   h->status = flag;
-  std::memcpy(h->P_av, h->hsmall + SM_PAV, sizeof h->P_av);
-  if (rotate) rotate33(h->q_R, h->P_av, false, h->P_av);
-  std::memcpy(h->C_volAvg, h->hsmall + SM_CVOL, sizeof h->C_volAvg);
-  std::memcpy(h->C_minMaxAvg, h->hsmall + SM_CMM, sizeof h->C_minMaxAvg);
   h->n_updates += h->N;
+  if (flag == ST_OK) {
+    std::swap(h->P_hom, h->mp.Pph);
+    std::memcpy(h->P_av_lab, h->hsmall + SM_PAV, sizeof h->P_av_lab);
+  }
+  std::memcpy(h->P_av, h->P_av_lab, sizeof h->P_av);
+  if (rotate) rotate33(h->q_R, h->P_av, false, h->P_av);
+  if (tangent) return tangent_pass(h, dt);
   return 0;
 }
 
@@ -341,7 +371,7 @@ static int form_residual(dmk_handle* h, const double* X, double* r, int iter) {
   if (h->totalIter <= iter) h->totalIter++;
   h->nfuncs++;
   h->n_residuals++;
-  int rc = constitutive(h, X, h->dt, true);
+  int rc = constitutive(h, X, h->dt, true, false);
   if (rc) return rc;
   // delta F_aim = S : (P_av - P_aim)
   double dP[9], dFa[9], dFr[9];
@@ -549,6 +579,8 @@ static int init_impl(dmk_handle* h, const int32_t cells[3], const double size[3]
     return DMK_ERR_CONFIG;
   }
   h->N = (long)cells[0] * cells[1] * cells[2];
+  // per-cell kernels address a field through a 32-bit byte offset of the cell
+  if (h->N >= (1L << 29)) { dmk_set_error("grid of %ld cells exceeds the 2^29 - 1 cells of one device", h->N); return DMK_ERR_ARG; }
   h->h = cells[0] / 2 + 1;
   h->Nk = (long)h->h * cells[1] * cells[2];
   h->n9 = 9 * h->N;
@@ -592,8 +624,9 @@ static int init_impl(dmk_handle* h, const int32_t cells[3], const double size[3]
   for (auto pp : f9) if ((rc = dalloc(h, pp, n9))) return rc;
   double** fs[] = {&mp.xi, &mp.xi0, &mp.gam, &mp.gam0};
   for (auto pp : fs) if ((rc = dalloc(h, pp, (size_t)ns * N))) return rc;
-  double** f2s[] = {&mp.st0s, &mp.last1, &mp.last2, &mp.dot};
+  double** f2s[] = {&mp.st0s, &mp.last1, &mp.last2};
   for (auto pp : f2s) if ((rc = dalloc(h, pp, (size_t)2 * ns * N))) return rc;
+  if ((rc = dalloc(h, &mp.sub, 36 * N))) return rc;
   if ((rc = dalloc(h, &mp.O, 4 * N))) return rc;
   if ((rc = dalloc(h, &mp.status, N))) return rc;
   if ((rc = dalloc(h, &mp.n_newton, N))) return rc;
@@ -618,6 +651,7 @@ static int init_impl(dmk_handle* h, const int32_t cells[3], const double size[3]
   if ((rc = dalloc(h, &h->pmin, dmk_nchunk(N)))) return rc;
   if ((rc = dalloc(h, &h->ext, 2))) return rc;
   if ((rc = dalloc(h, &h->sel, 2))) return rc;
+  if ((rc = dalloc(h, &h->csel, 2L * dmk_nchunk(N)))) return rc;
   if ((rc = dalloc(h, &h->flag, 4))) return rc;
   if ((rc = dalloc(h, &h->cnt, 4))) return rc;
   CK(hipHostMalloc((void**)&h->hsmall, sizeof(double) * SM_SIZE, hipHostMallocDefault));
@@ -687,7 +721,7 @@ static int init_impl(dmk_handle* h, const int32_t cells[3], const double size[3]
   }
   for (int d = 0; d < 3; d++) h->ga.rg[d] = size[d] / scaled[d];
   // init: F = I, response at dt = 0, C_ref (grid_mech_spectral_basic.f90:213-235)
-  if ((rc = constitutive(h, h->F, 0.0, false))) return rc;
+  if ((rc = constitutive(h, h->F, 0.0, false, true))) return rc;
   set_gamma_coef(h, h->C_minMaxAvg);
   return 0;
 }
@@ -781,6 +815,7 @@ extern "C" int dmk_mech_solution(dmk_handle* h, int32_t* iterations, int32_t* co
   int reason = 0;
   rc = h->num.solver == 1 ? solve_richardson(h, &reason) : solve_ngmres(h, &reason);
   if (rc) return rc;
+  if ((rc = tangent_pass(h, h->dt))) return rc;
   if (iterations) *iterations = h->totalIter;
   if (conv) *conv = reason > 0;
   if (n_residuals) *n_residuals = h->nfuncs;
@@ -813,7 +848,7 @@ extern "C" int dmk_constitutive(dmk_handle* h, double dt, const double* F, doubl
   if (!h || !F) { dmk_set_error("null argument"); return DMK_ERR_ARG; }
   CK(hipMemcpyAsync(h->Pext, F, sizeof(double) * h->n9, hipMemcpyHostToDevice, h->st));
   CKL(dmk_aos_to_soa(h->Pext, h->N, 9, h->Fext, h->st));
-  int rc = constitutive(h, h->Fext, dt, false);
+  int rc = constitutive(h, h->Fext, dt, false, true);
   if (rc) return rc;
   if (P) {
     CKL(dmk_soa_to_aos(h->P_hom, h->N, 9, h->Pext, h->st));
@@ -830,7 +865,7 @@ extern "C" int dmk_constitutive(dmk_handle* h, double dt, const double* F, doubl
 extern "C" int dmk_constitutive_device(dmk_handle* h, double dt, const double* F_dev, double* P_dev, int32_t* status) {
   if (!h || !F_dev) { dmk_set_error("null argument"); return DMK_ERR_ARG; }
   CKL(dmk_aos_to_soa(F_dev, h->N, 9, h->Fext, h->st));
-  int rc = constitutive(h, h->Fext, dt, false);
+  int rc = constitutive(h, h->Fext, dt, false, true);
   if (rc) return rc;
   if (P_dev) {
     CKL(dmk_soa_to_aos(h->P_hom, h->N, 9, P_dev, h->st));
@@ -1101,8 +1136,9 @@ extern "C" int dmk_mech_profile(dmk_handle* h, int32_t enable) {
   return 0;
 }
 
-// kernel groups: 0 stress integration, 1 tangent/homogenize, 2 field reductions,
-// 3 forward FFT, 4 Gamma + div, 5 inverse FFT: total ms and launch count
+// kernel groups: 0 stress integration, 1 orientation + tangent, 2 response
+// reductions (status, P_av), 3 forward FFT, 4 Gamma + div, 5 inverse FFT,
+// 6 stiffness averages: total ms and launch count
 extern "C" int dmk_mech_kernel_time(dmk_handle* h, int32_t kernel, double* total_ms, int64_t* launches) {
   if (!h || kernel < 0 || kernel >= KT_N) { dmk_set_error("bad argument"); return DMK_ERR_ARG; }
   if (total_ms) *total_ms = h->kt_ms[kernel];

@@ -188,7 +188,7 @@ class GridMechSpectralBasic:
         _lib.check(self.lib.dmk_mech_counters(self.h, out.ctypes.data_as(C.POINTER(C.c_int64))), "dmk_mech_counters")
         return {"updates": int(out[0]), "newton": int(out[1]), "state": int(out[2]), "residuals": int(out[3])}
 
-    KERNEL_GROUPS = ("stress", "tangent", "reduce", "fft_fwd", "gamma_div", "fft_inv")
+    KERNEL_GROUPS = ("stress", "tangent", "reduce", "fft_fwd", "gamma_div", "fft_inv", "c_avg")
 
     def profile(self, enable: bool = True) -> None:
         _lib.check(self.lib.dmk_mech_profile(self.h, int(enable)), "dmk_mech_profile")

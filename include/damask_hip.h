@@ -153,8 +153,9 @@ int dmk_fft_backward(dmk_handle* h, int32_t howmany, const double* in, double* o
 
 /* timing of the kernel groups with HIP events on the solver stream:
  * dmk_mech_profile(h, 1) enables (and resets), 0 disables.  Groups: 0 stress
- * integration, 1 tangent + homogenize, 2 field reductions, 3 forward FFT,
- * 4 Gamma convolution + div RMS, 5 inverse FFT */
+ * integration, 1 orientation + tangent, 2 response reductions (status, P_av),
+ * 3 forward FFT, 4 Gamma convolution + div RMS, 5 inverse FFT, 6 stiffness
+ * averages C_volAvg / C_minMaxAvg */
 int dmk_mech_profile(dmk_handle* h, int32_t enable);
 int dmk_mech_kernel_time(dmk_handle* h, int32_t kernel, double* total_ms, int64_t* launches);
 
