@@ -25,6 +25,10 @@ sys.path.insert(0, ROOT)
 
 PEAK_FP64_TFLOPS = 78.6      # MI355X FP64 (vector and matrix)
 PEAK_HBM_GBS = 8000.0
+# HBM bytes per stress-kernel launch on this workload: rocprofv3 --pmc
+# FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 (factors from a plain device copy), see
+# profiles/kernels_64cube_fcc_tension.md
+STRESS_TRAFFIC_BYTES = 2288069243.0
 
 
 def parse():
@@ -40,8 +44,8 @@ def parse():
     ap.add_argument("--cpu-baseline", type=int, default=1)
     ap.add_argument("--cpu-cells", type=int, default=32)
     ap.add_argument("--cpu-increments", type=int, default=12)
-    ap.add_argument("--traffic-bytes", type=float, default=None,
-                    help="measured HBM bytes per stress-kernel launch (rocprofv3 --pmc)")
+    ap.add_argument("--traffic-bytes", type=float, default=STRESS_TRAFFIC_BYTES,
+                    help="measured HBM bytes per stress-kernel launch (rocprofv3 --pmc); 0 for none")
     ap.add_argument("--copy-calibration", type=int, default=0,
                     help="only run a plain device copy of this many MiB (counter calibration)")
     return ap.parse_args()
@@ -157,9 +161,12 @@ def main():
                    "init_s": t_init,
                    "kernel_ms": {k: round(v[0], 3) for k, v in kt.items()},
                    "cells_x_increments_per_s": args.cells**3 * args.steps / elapsed},
-        "roofline": {"bound": "fp64-valu", "kernel": "stress_kernel<12, true>", "achieved": achieved,
+        "roofline": {"bound": "mfma", "compute": "fp64 vector ALU (MI355X fp64 vector and matrix peaks "
+                                                 "are both 78.6 TFLOP/s)",
+                     "kernel": "stress_kernel<12, true>", "achieved": achieved,
                      "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS,
-                     "traffic": args.traffic_bytes,
+                     "traffic": args.traffic_bytes or None,
+                     "algorithmic_bytes_per_launch": flops.stress_kernel_bytes(nsl) * args.cells**3,
                      "flops_per_launch": fl / max(stress_n, 1),
                      "avg_launch_ms": stress_ms / max(stress_n, 1)},
     }
