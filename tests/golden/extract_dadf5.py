@@ -68,7 +68,7 @@ def fcc():
             d[f"{k}_{inc}"] = dataset(path, base + k).reshape(N, w)
         d[f"u_p_{inc}"] = dataset(path, f"/increment_{inc}/geometry/u_p").reshape(N, 3)
         d[f"u_n_{inc}"] = dataset(path, f"/increment_{inc}/geometry/u_n").reshape(-1, 3)
-    d["cells"] = dataset(path, "/geometry/cells", "<i8")
+    d["cells"] = dataset(path, "/geometry/cells", "<i4").astype(np.int64)
     d["size"] = dataset(path, "/geometry/size")
     d["origin"] = dataset(path, "/geometry/origin")
     d["cell_to_phase_entry"] = cell_to_entries(path, "/cell_to/phase")
