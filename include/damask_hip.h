@@ -119,7 +119,11 @@ int dmk_constitutive_device(dmk_handle* h, double dt, const double* F_dev, doubl
                             int32_t* status);
 
 /* per-cell fields to host buffers: 0 F, 1 P, 2 F_e, 3 F_p, 4 L_p, 5 S, 6 F_i,
- * 7 O (4/cell), 8 xi_sl (nsl/cell), 9 gamma_sl (nsl/cell), 10 dPdF (81/cell) */
+ * 7 O (4/cell), 8 xi_sl (nsl/cell), 9 gamma_sl (nsl/cell), 10 dPdF (81/cell).
+ * O and dPdF are those of the last response of the last completed
+ * dmk_mech_solution or dmk_constitutive call: the solver evaluates them once
+ * per solve, since grid_mech_spectral_basic reads the stiffness only between
+ * solves (grid_mech_spectral_basic.f90:261-262,306-311). */
 int dmk_mech_get_field(dmk_handle* h, int32_t what, double* dst);
 /* restart data (grid_mech_spectral_basic.f90:198-232, phase_mechanical.f90:1248-1280):
  * 0 F, 3 F_p, 4 L_p, 5 S, 8 xi_sl, 9 gamma_sl, 11 F_lastinc */
