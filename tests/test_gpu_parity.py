@@ -8,7 +8,7 @@ import copy
 import numpy as np
 import pytest
 
-from conftest import fcc_golden_inputs
+from conftest import bcc_golden_inputs, fcc_golden_inputs
 
 pytestmark = pytest.mark.gpu
 
@@ -152,3 +152,67 @@ def test_solver_increments_fcc_golden_grid(gpu_lib, oracle):
     F_g = g.field("F")
     F_o = o.solver_get(0).reshape(-1, 3, 3).transpose(0, 2, 1)
     assert np.abs(F_g - F_o).max() < 1e-8
+
+
+def test_constitutive_response_general_exponents(gpu_lib, oracle):
+    """exponents that are not multiples of 1/4 (n_sl 19.3, a_sl 2.1) take the
+    library-pow kernels; one response against the oracle"""
+    from damask_amd import workloads
+    phase = copy.deepcopy(workloads.FCC_PHASE)
+    phase["mechanical"]["plastic"]["n_sl"] = [19.3]
+    phase["mechanical"]["plastic"]["a_sl"] = [2.1]
+    grid, mat = _poly([6, 6, 5], 8, 4, phase)
+    g, o = _pair(grid, mat, oracle=oracle)
+    rng = np.random.default_rng(9)
+    N = grid.n_cells
+    L = oracle.lib()
+    for step in (1, 2):
+        F = np.eye(3) + np.diag([8e-4, -2.4e-4, -2.4e-4]) * step + 8e-5 * rng.standard_normal((N, 3, 3))
+        P_g, Pav_g, _, _, st_g = g.constitutive(F, 0.5)
+        Ff = np.ascontiguousarray(F.transpose(0, 2, 1).reshape(-1))
+        P_o = np.zeros_like(Ff)
+        st_o = L.orc_mp_response(o.mp, 0.5, oracle.ptr(Ff), oracle.ptr(P_o), None)
+        assert st_g == st_o == 0
+        P_o = P_o.reshape(-1, 3, 3).transpose(0, 2, 1)
+        assert (np.abs(P_g - P_o).max(axis=(1, 2)) / np.abs(P_o).max(axis=(1, 2))).max() < 1e-6
+        assert _relmax(Pav_g, P_o.mean(0)) < 1e-8
+        g.materialpoint_forward()
+        L.orc_mp_forward(o.mp)
+
+
+def test_solver_increments_bcc_nonschmid_golden_grid(gpu_lib, oracle):
+    """the BCC non-Schmid regression case (10x16x12, N_iter_min 4), first 10
+    increments, GPU vs oracle"""
+    grid, mat, lcs, num = bcc_golden_inputs()
+    lc = copy.copy(lcs[0])
+    lc.N, lc.t = 10, 20.0 / 150 * 10
+    g, o = _pair(grid, mat, numerics=num, oracle=oracle)
+    rg = g.run_loadcase(lc)
+    ro = o.run_loadcase(lc)
+    assert rg["rc"] == ro["rc"] == 0
+    assert np.array_equal(rg["stats"][:, 2], ro["stats"][:, 2])
+    assert _relmax(rg["P_av"], ro["P_av"]) < 1e-8
+    a = g.field("xi_sl")
+    b = o.get(8, a.shape[1]).reshape(grid.n_cells, -1)
+    assert _relmax(a, b) < 1e-8
+
+
+def test_full_fcc_golden_run_against_reference_result(gpu_lib):
+    """the GPU path alone over the 40 increments of the golden case, against
+    the reference's stored result at the solver tolerances (as the oracle's
+    golden test): F per point 2e-5, xi 2e-5 relative, P_av 1e-3 of max |P_av|"""
+    import os
+    from conftest import GOLDEN
+    from damask_amd.grid import GridMechSpectralBasic
+    ref = np.load(os.path.join(GOLDEN, "fcc_6grains6x7x8.npz"))
+    grid, mat, lcs = fcc_golden_inputs()
+    g = GridMechSpectralBasic(grid, mat)
+    lc = copy.copy(lcs[0])
+    for inc_end in range(8, 41, 8):
+        lc.N, lc.t = 8, 4.0
+        assert g.run_loadcase(lc)["rc"] == 0
+        assert np.abs(g.field("F") - ref[f"F_{inc_end}"].reshape(-1, 3, 3)).max() < 2e-5
+        xr = ref[f"xi_sl_{inc_end}"]
+        assert np.abs(g.field("xi_sl") - xr).max() < 2e-5 * xr.max()
+        Pr = ref[f"P_{inc_end}"].reshape(-1, 3, 3).mean(0)
+        assert np.abs(g.field("P").mean(0) - Pr).max() < 1e-3 * np.abs(Pr).max()

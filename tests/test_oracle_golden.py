@@ -142,3 +142,36 @@ def test_full_run_matches_reference_result(oracle, fcc_ref):
         assert np.abs(xi - fcc_ref[f"xi_sl_{inc_end}"]).max() < 2e-5 * fcc_ref[f"xi_sl_{inc_end}"].max()
         Pr = fcc_ref[f"P_{inc_end}"].reshape(-1, 3, 3)
         assert np.abs(P.mean(0) - Pr.mean(0)).max() < 1e-3 * np.abs(Pr.mean(0)).max()
+
+
+def test_bcc_nonschmid_run_matches_reference_result(oracle):
+    """17 grains, cI with non-Schmid terms, 10x16x12 cells, tensionX with
+    N_iter_min 4: the reference's regression result
+    (tests/integration/resources/plastic_detect_changes/, DAMASK 3.0.0-beta-84)
+    stores xi_sl, gamma_sl and u_p at increments 75 and 150.  The reference's
+    own comparison uses rtol 2.5 %; agreement here is at the solver tolerances:
+    xi 5e-5, gamma 5e-4, u_p 5e-5 of the largest value."""
+    import copy
+    from conftest import bcc_golden_inputs
+    from damask_amd import geom
+    grid, mat, lcs, num = bcc_golden_inputs()
+    ref = np.load(os.path.join(GOLDEN, "bcc_17grains.npz"))
+    pr = oracle.Problem(grid, mat, num)
+    lc = copy.copy(lcs[0])
+    N = grid.n_cells
+    cells = grid.cells
+    X0 = geom.coordinates0_point(grid.cells, grid.size).transpose(2, 1, 0, 3).reshape(-1, 3)
+    for inc_end in (75, 150):
+        lc.N, lc.t = 75, 10.0
+        assert pr.run_loadcase(lc)["rc"] == 0
+        nsl = oracle.lib().orc_mp_nsl(pr.mp)
+        xi = pr.get(8, nsl).reshape(N, nsl)
+        gam = pr.get(9, nsl).reshape(N, nsl)
+        xr, gr = ref[f"xi_sl_{inc_end}"], ref[f"gamma_sl_{inc_end}"]
+        assert np.abs(xi - xr).max() < 5e-5 * np.abs(xr).max()
+        assert np.abs(gam - gr).max() < 5e-4 * np.abs(gr).max()
+        xp = np.zeros(3 * N)
+        xn = np.zeros(3 * (cells[0] + 1) * (cells[1] + 1) * (cells[2] + 1))
+        oracle.lib().orc_solver_update_coords(pr.solver, oracle.ptr(xp), oracle.ptr(xn))
+        up = xp.reshape(-1, 3) - X0
+        assert np.abs(up - ref[f"u_p_{inc_end}"]).max() < 5e-5 * np.abs(ref[f"u_p_{inc_end}"]).max()
