@@ -49,7 +49,7 @@ EXPORTS = ["dmk_mech_init", "dmk_mech_forward", "dmk_mech_materialpoint_forward"
            "dmk_mech_set_field", "dmk_mech_get_scalars", "dmk_mech_set_scalars", "dmk_grid_run_loadcase",
            "dmk_mech_counters", "dmk_mech_destroy", "dmk_last_error", "dmk_nsl", "dmk_device_synchronize",
            "dmk_gamma_convolution", "dmk_divergence_rms", "dmk_fft_forward", "dmk_fft_backward",
-           "dmk_mech_profile", "dmk_mech_kernel_time"]
+           "dmk_mech_profile", "dmk_mech_kernel_time", "dmk_slip_system"]
 
 _lib = None
 
@@ -111,6 +111,8 @@ def load(path: str = LIB_PATH):
     L.dmk_mech_profile.argtypes = [vp, C.c_int32]
     L.dmk_mech_kernel_time.restype = C.c_int
     L.dmk_mech_kernel_time.argtypes = [vp, C.c_int32, dp, C.POINTER(C.c_int64)]
+    L.dmk_slip_system.restype = C.c_int
+    L.dmk_slip_system.argtypes = [C.c_int32, C.c_int32, ip, C.c_int32, dp]
     _lib = L
     return L
 

@@ -229,3 +229,23 @@ void dmk_qu2om(const double* qu, double* om) {
   double dd = std::pow(det, 1.0 / 3.0);
   for (int k = 0; k < 9; k++) om[k] = t[k] / dd;
 }
+
+// Miller indices of the a-th active slip system (direction, normal) for the
+// family selection of dmk_build_phase: the labels of the xi_sl / gamma_sl
+// results (crystal.f90:1642-1676).  Returns 0, or 1 for an invalid request.
+extern "C" int dmk_slip_system(int32_t lattice, int32_t n_fam, const int32_t* N_sl, int32_t a, double out[6]) {
+  if ((lattice != 0 && lattice != 1) || n_fam < 1 || n_fam > 2 || !N_sl || !out || a < 0) return 1;
+  const int* maxs = lattice == 0 ? CF_MAX : CI_MAX;
+  int k = a;
+  for (int f = 0; f < n_fam; f++) {
+    if (N_sl[f] < 0 || N_sl[f] > maxs[f]) return 1;
+    if (k < N_sl[f]) {
+      const int off = f == 0 ? 0 : maxs[0];
+      const double* row = lattice == 0 ? CF_SLIP[off + k] : CI_SLIP[off + k];
+      for (int c = 0; c < 6; c++) out[c] = row[c];
+      return 0;
+    }
+    k -= N_sl[f];
+  }
+  return 1;
+}

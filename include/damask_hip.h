@@ -167,6 +167,11 @@ int dmk_mech_kernel_time(dmk_handle* h, int32_t kernel, double* total_ms, int64_
  * residual evaluations */
 int dmk_mech_counters(dmk_handle* h, int64_t out[4]);
 
+/* Miller indices (direction d1 d2 d3, normal n1 n2 n3) of active slip system a
+ * of a phase description (lattice, n_fam, N_sl): the system labels of the
+ * xi_sl / gamma_sl results (crystal.f90:1642-1676, 2133-2181). */
+int dmk_slip_system(int32_t lattice, int32_t n_fam, const int32_t* N_sl, int32_t a, double out[6]);
+
 void dmk_mech_destroy(dmk_handle* h);
 const char* dmk_last_error(void);
 int dmk_nsl(const dmk_handle* h);
