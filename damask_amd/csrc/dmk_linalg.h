@@ -16,7 +16,35 @@ struct ForwardArgs {
   double shift[9];    // mean correction (column-major)
 };
 
+// several dot products in one launch: out[j] = x_j . y_j (mode 0) or
+// |y_j - x_j|^2 (mode 1), each over n doubles
+#define DMK_MAX_DOTS 64
+struct DotJobs {
+  const double* x[DMK_MAX_DOTS];
+  const double* y[DMK_MAX_DOTS];
+  int mode[DMK_MAX_DOTS];
+  int n;
+};
+
+// per-residual stress-BC bookkeeping on the device (grid_mech_spectral_basic.f90:478-496):
+// P_av (rotated to the load frame), F_aim -= S:(P_av - P_aim), err_BC, the
+// aim of the Gamma convolution; report = {P_av 9, F_aim 9, err_BC, flag}
+struct AimArgs {
+  double S[81];
+  double P_aim[9];
+  double q_R[4];
+  int32_t stress_mask[9];   // 1 where the component is not prescribed
+  int32_t rotate;           // rotate P_av by q_R
+  int32_t update;           // update F_aim and the aim (0: report P_av and flag only)
+};
+enum { REP_PAV = 0, REP_FAIM = 9, REP_ERRBC = 18, REP_FLAG = 19, REP_DIV2 = 20, REP_DOTS = 24, REP_SIZE = 24 + DMK_MAX_DOTS };
+
 int dmk_nchunk(long n);
+int dmk_dots(const DotJobs* jobs, long n, double* partial, double* out, hipStream_t s);
+int dmk_residual_aim(const AimArgs* a, const double* pav_raw, double* faim, const int* flag, double* report,
+                     double* aim, hipStream_t s);
+// P_new <- P_old where the response failed (flag[0] != 0); nothing otherwise
+int dmk_accept_P(const int* flag, const double* P_old, double* P_new, long n, hipStream_t s);
 int dmk_colsum(const double* base, long ld, long n, int ncols, double* partial, double* out, double scale,
                hipStream_t s);
 int dmk_mdot(const double* x, const double* Y, long ldY, int l, int mode, long n, double* partial, double* out,

@@ -23,6 +23,7 @@ struct GammaArgs {
   double rg[3];            // geomSize / scaledGeomSize
   double Ccoef[9][6];      // acoustic-tensor coefficients of C_ref
   double aim[9];           // DC value, column-major
+  const double* aim_dev;   // when set, the DC value is read from device memory instead
   int do_div, do_gamma;
 };
 

@@ -5,6 +5,7 @@
 // run to run.
 #include "dmk_common.h"
 #include "dmk_linalg.h"
+#include "dmk_rot.h"
 
 #include <float.h>
 
@@ -60,6 +61,59 @@ __global__ void __launch_bounds__(RB) dot_partial(const double* __restrict__ x, 
   }
   double r = block_sum(acc);
   if (threadIdx.x == 0) partial[(long)blockIdx.y * nchunk + blockIdx.x] = r;
+}
+
+__global__ void __launch_bounds__(RB) dot_jobs_partial(DotJobs J, long n, double* __restrict__ partial, int nchunk) {
+  const int j = blockIdx.y;
+  const double* __restrict__ x = J.x[j];
+  const double* __restrict__ y = J.y[j];
+  long s = (long)blockIdx.x * PER_BLOCK;
+  long e = min(s + PER_BLOCK, n);
+  double acc = 0.0;
+  if (J.mode[j] == 0) {
+    for (long i = s + threadIdx.x; i < e; i += RB) acc = fma(x[i], y[i], acc);
+  } else {
+    for (long i = s + threadIdx.x; i < e; i += RB) { double d = y[i] - x[i]; acc = fma(d, d, acc); }
+  }
+  double r = block_sum(acc);
+  if (threadIdx.x == 0) partial[(long)j * nchunk + blockIdx.x] = r;
+}
+
+__global__ void accept_P_kernel(const int* __restrict__ flag, const double* __restrict__ P_old,
+                                double* __restrict__ P_new, long n) {
+  if (flag[0] == ST_OK) return;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+    P_new[i] = P_old[i];
+}
+
+// one thread: the order of operations is that of formResidual (:478-496)
+__global__ void residual_aim_kernel(AimArgs a, const double* __restrict__ pav_raw, double* __restrict__ faim,
+                                    const int* __restrict__ flag, double* __restrict__ rep, double* __restrict__ aim) {
+  if (threadIdx.x != 0) return;
+  double P[9];
+  for (int k = 0; k < 9; k++) P[k] = pav_raw[k];
+  if (a.rotate) dmk_rotate33(a.q_R, P, false, P);
+  for (int k = 0; k < 9; k++) rep[REP_PAV + k] = P[k];
+  rep[REP_FLAG] = (double)flag[0];
+  if (!a.update) return;
+  double dP[9], dFa[9];
+  for (int k = 0; k < 9; k++) dP[k] = P[k] - a.P_aim[k];
+  for (int j = 0; j < 3; j++)
+    for (int i = 0; i < 3; i++) {
+      double s = 0.0;
+      for (int n = 0; n < 3; n++)
+        for (int m = 0; m < 3; m++) s += a.S[i + 3 * j + 9 * m + 27 * n] * dP[m + 3 * n];
+      dFa[i + 3 * j] = s;
+    }
+  double e = 0.0;
+  for (int k = 0; k < 9; k++) {
+    double f = faim[k] - dFa[k];
+    faim[k] = f;
+    rep[REP_FAIM + k] = f;
+    if (!a.stress_mask[k]) e = fmax(e, fabs(dP[k]));
+  }
+  rep[REP_ERRBC] = e;
+  dmk_rotate33(a.q_R, dFa, true, aim);
 }
 
 __global__ void axpby_kernel(long n, double a, const double* __restrict__ x, double b, const double* __restrict__ y,
@@ -284,6 +338,25 @@ int dmk_mdot(const double* x, const double* Y, long ldY, int l, int mode, long n
   int nc = dmk_nchunk(n);
   hipLaunchKernelGGL(dot_partial, dim3(nc, l), dim3(RB), 0, s, x, Y, ldY, mode, n, partial, nc);
   hipLaunchKernelGGL(finalize_sum, dim3(l), dim3(RB), 0, s, partial, nc, 1.0, out);
+  return hipGetLastError() == hipSuccess ? 0 : 2;
+}
+
+int dmk_dots(const DotJobs* jobs, long n, double* partial, double* out, hipStream_t s) {
+  if (jobs->n <= 0) return 0;
+  int nc = dmk_nchunk(n);
+  hipLaunchKernelGGL(dot_jobs_partial, dim3(nc, jobs->n), dim3(RB), 0, s, *jobs, n, partial, nc);
+  hipLaunchKernelGGL(finalize_sum, dim3(jobs->n), dim3(RB), 0, s, partial, nc, 1.0, out);
+  return hipGetLastError() == hipSuccess ? 0 : 2;
+}
+
+int dmk_residual_aim(const AimArgs* a, const double* pav_raw, double* faim, const int* flag, double* report,
+                     double* aim, hipStream_t s) {
+  hipLaunchKernelGGL(residual_aim_kernel, dim3(1), dim3(64), 0, s, *a, pav_raw, faim, flag, report, aim);
+  return hipGetLastError() == hipSuccess ? 0 : 2;
+}
+
+int dmk_accept_P(const int* flag, const double* P_old, double* P_new, long n, hipStream_t s) {
+  hipLaunchKernelGGL(accept_P_kernel, dim3(512), dim3(256), 0, s, flag, P_old, P_new, n);
   return hipGetLastError() == hipSuccess ? 0 : 2;
 }
 

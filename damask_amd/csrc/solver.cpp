@@ -22,6 +22,7 @@
 #include <vector>
 
 #include "dmk_common.h"
+#include "dmk_rot.h"
 #include "dmk_linalg.h"
 #include "dmk_spectral.h"
 #include "../../include/damask_hip.h"
@@ -63,29 +64,11 @@ extern "C" const char* dmk_last_error(void) { return g_err; }
 static inline int c2(int i, int j) { return i + 3 * j; }
 static inline int c4(int i, int j, int k, int l) { return i + 3 * j + 9 * k + 27 * l; }
 
-static void rot_vec(const double* q, const double* v, double* o) {
-  double A = q[0] * q[0] - (q[1] * q[1] + q[2] * q[2] + q[3] * q[3]);
-  double B = 2.0 * (q[1] * v[0] + q[2] * v[1] + q[3] * v[2]);
-  double C = -2.0 * q[0];
-  double c[3] = {q[2] * v[2] - q[3] * v[1], q[3] * v[0] - q[1] * v[2], q[1] * v[1] - q[2] * v[0]};
-  for (int k = 0; k < 3; k++) o[k] = A * v[k] + B * q[k + 1] + C * c[k];
-}
+static void rot_vec(const double* q, const double* v, double* o) { dmk_rot_vec(q, v, o); }
 
 // rotations.f90:282-301 rotate3x3 (active conjugates q)
 static void rotate33(const double* q0, const double* T, bool active, double* out) {
-  double q[4] = {q0[0], active ? -q0[1] : q0[1], active ? -q0[2] : q0[2], active ? -q0[3] : q0[3]};
-  double R[9], v[3], o[3];
-  for (int i = 0; i < 3; i++) {
-    for (int j = 0; j < 3; j++) v[j] = T[c2(i, j)];
-    rot_vec(q, v, o);
-    for (int j = 0; j < 3; j++) R[c2(i, j)] = o[j];
-  }
-  for (int j = 0; j < 3; j++) {
-    for (int i = 0; i < 3; i++) v[i] = R[c2(i, j)];
-    rot_vec(q, v, o);
-    for (int i = 0; i < 3; i++) R[c2(i, j)] = o[i];
-  }
-  std::memcpy(out, R, sizeof R);
+  dmk_rotate33(q0, T, active, out);
 }
 
 // rotations.f90:310-339 rotate3x3x3x3 (passive)
@@ -215,7 +198,6 @@ struct dmk_handle {
   double* hsmall = nullptr;          // pinned host mirror of small results
   // solver state
   double F_aimDot[9], F_aim[9], F_aim_lastinc[9], P_av[9], P_aim[9];
-  double P_av_lab[9] = {0};          // volume average of the accepted P, lab frame
   double C_volAvg[81], C_volAvgLastInc[81], C_minMaxAvg[81], C_minMaxAvgLastInc[81], S[81];
   double err_div = 0, err_BC = 0;
   int status = 0, totalIter = 0, nfuncs = 0;
@@ -265,7 +247,9 @@ static int dalloc(dmk_handle* h, T** p, size_t n) {
   return 0;
 }
 
-enum { SM_PAV = 0, SM_CVOL = 9, SM_CMM = 90, SM_DIV = 171, SM_DOTS = 180, SM_SIZE = 256 };
+enum { SM_PAV = 0, SM_CVOL = 9, SM_CMM = 90, SM_DOTS = 180, SM_REP = 256, SM_FAIM = 352, SM_AIM = 368,
+       SM_SIZE = 384 };
+static_assert(SM_REP + REP_SIZE <= SM_FAIM, "report region");
 
 static void set_gamma_coef(dmk_handle* h, const double* C) {
   // A_lm = sum_jq C_ref(l,j,m,q) a_j a_q with C_ref = C / wgt; products a_j a_q
@@ -310,34 +294,51 @@ static int tangent_pass(dmk_handle* h, double dt) {
   return 0;
 }
 
-// utilities_constitutiveResponse on the device field F (component-major):
-// stress integration of every cell, convergence flag, P_av.  The homogenized
-// P of a response in which every cell converged is accepted by swapping it
-// with the previous one (a response with a failed cell leaves P and P_av as
-// they were, as the reference's skipped second loop does).
-static int constitutive(dmk_handle* h, const double* F, double dt, bool rotate, bool tangent) {
+// utilities_constitutiveResponse on the device field F (component-major),
+// enqueued without synchronisation: stress integration of every cell,
+// convergence flag, accepted P and its raw volume average.  The stress kernel
+// writes into the spare P buffer; when a cell failed, the previous P is copied
+// back into it (the reference's skipped second loop leaves P as it was,
+// homogenization.f90:349), and the buffers swap roles.
+static int response_enqueue(dmk_handle* h, const double* F, double dt) {
   kt_begin(h, KT_STRESS);
   for (int p = 0; p < h->n_phases; p++)
     CKL(dmk_launch_stress(&h->mp, F, h->ph_dev + p, &h->ph[p], p, &h->nm, dt, h->st));
   kt_end(h, KT_STRESS);
   kt_begin(h, KT_REDUCE);
   CKL(dmk_status(h->mp.status, h->mp.n_newton, h->mp.n_fpi, h->N, h->partial, h->flag, h->cnt, h->st));
-  CKL(dmk_colsum(h->mp.Pph, h->N, h->N, 9, h->partial, h->small + SM_PAV, h->wgt, h->st));
+  CKL(dmk_accept_P(h->flag, h->P_hom, h->mp.Pph, h->n9, h->st));
+  std::swap(h->P_hom, h->mp.Pph);
+  CKL(dmk_colsum(h->P_hom, h->N, h->N, 9, h->partial, h->small + SM_PAV, h->wgt, h->st));
   kt_end(h, KT_REDUCE);
-  CK(hipMemcpyAsync(h->hsmall + SM_PAV, h->small + SM_PAV, sizeof(double) * 9, hipMemcpyDeviceToHost, h->st));
-  int flag = 0;
-  CK(hipMemcpyAsync(&flag, h->flag, sizeof(int), hipMemcpyDeviceToHost, h->st));
+  h->n_updates += h->N;
+  return 0;
+}
+
+static void aim_args(const dmk_handle* h, bool rotate, bool update, AimArgs* a) {
+  std::memcpy(a->S, h->S, sizeof a->S);
+  std::memcpy(a->P_aim, h->P_aim, sizeof a->P_aim);
+  std::memcpy(a->q_R, h->q_R, sizeof a->q_R);
+  for (int k = 0; k < 9; k++) a->stress_mask[k] = h->stress_mask[k];
+  a->rotate = rotate;
+  a->update = update;
+}
+
+// constitutive response with the report read back (init, dmk_constitutive*)
+static int constitutive(dmk_handle* h, const double* F, double dt, bool rotate, bool tangent) {
+  int rc = response_enqueue(h, F, dt);
+  if (rc) return rc;
+  AimArgs a;
+  aim_args(h, rotate, false, &a);
+  CKL(dmk_residual_aim(&a, h->small + SM_PAV, h->small + SM_FAIM, h->flag, h->small + SM_REP,
+                       h->small + SM_AIM, h->st));
+  CK(hipMemcpyAsync(h->hsmall + SM_REP, h->small + SM_REP, sizeof(double) * REP_DIV2, hipMemcpyDeviceToHost,
+                    h->st));
   CK(hipStreamSynchronize(h->st));
   kt_collect(h, KT_STRESS);
   kt_collect(h, KT_REDUCE);
-  h->status = flag;
-  h->n_updates += h->N;
-  if (flag == ST_OK) {
-    std::swap(h->P_hom, h->mp.Pph);
-    std::memcpy(h->P_av_lab, h->hsmall + SM_PAV, sizeof h->P_av_lab);
-  }
-  std::memcpy(h->P_av, h->P_av_lab, sizeof h->P_av);
-  if (rotate) rotate33(h->q_R, h->P_av, false, h->P_av);
+  h->status = (int)h->hsmall[SM_REP + REP_FLAG];
+  std::memcpy(h->P_av, h->hsmall + SM_REP + REP_PAV, sizeof h->P_av);
   if (tangent) return tangent_pass(h, dt);
   return 0;
 }
@@ -365,53 +366,65 @@ static int masked_compliance(dmk_handle* h) {
   return 0;
 }
 
-// form_residual: X -> r (device), returns 0 / error
-static int form_residual(dmk_handle* h, const double* X, double* r, int iter) {
+// formResidual (grid_mech_spectral_basic.f90:464-521), enqueued: response,
+// stress-BC update of F_aim and the aim on the device, one forward FFT for
+// the divergence RMS and the Gamma convolution, inverse FFT into r.
+static int residual_enqueue(dmk_handle* h, const double* X, double* r, int iter) {
   if (h->nfuncs == 0 && iter == 0) h->totalIter = -1;
   if (h->totalIter <= iter) h->totalIter++;
   h->nfuncs++;
   h->n_residuals++;
-  int rc = constitutive(h, X, h->dt, true, false);
+  int rc = response_enqueue(h, X, h->dt);
   if (rc) return rc;
-  // delta F_aim = S : (P_av - P_aim)
-  double dP[9], dFa[9], dFr[9];
-  for (int k = 0; k < 9; k++) dP[k] = h->P_av[k] - h->P_aim[k];
-  for (int j = 0; j < 3; j++)
-    for (int i = 0; i < 3; i++) {
-      double s = 0.0;
-      for (int n = 0; n < 3; n++)
-        for (int m = 0; m < 3; m++) s += h->S[c4(i, j, m, n)] * dP[c2(m, n)];
-      dFa[c2(i, j)] = s;
-    }
-  for (int k = 0; k < 9; k++) h->F_aim[k] -= dFa[k];
-  double e = 0.0;
-  for (int k = 0; k < 9; k++) if (!h->stress_mask[k]) e = std::fmax(e, std::fabs(dP[k]));
-  h->err_BC = e;
-  rotate33(h->q_R, dFa, true, dFr);
-  // one forward FFT serves the divergence RMS and the Gamma convolution
+  AimArgs a;
+  aim_args(h, true, true, &a);
+  CKL(dmk_residual_aim(&a, h->small + SM_PAV, h->small + SM_FAIM, h->flag, h->small + SM_REP,
+                       h->small + SM_AIM, h->st));
   kt_begin(h, KT_FFT_FWD);
   CKL(dmk_fft_r2c(&h->fft, 9, h->P_hom, h->Z, h->st));
   kt_end(h, KT_FFT_FWD);
   h->ga.do_div = 1;
   h->ga.do_gamma = 1;
-  std::memcpy(h->ga.aim, dFr, sizeof dFr);
+  h->ga.aim_dev = h->small + SM_AIM;
   kt_begin(h, KT_GAMMA);
   CKL(dmk_gamma_div(h->Z, &h->ga, h->partial, h->st));
   kt_end(h, KT_GAMMA);
   int nb = dmk_gamma_div_blocks(&h->ga);
-  CKL(dmk_colsum(h->partial, nb, nb, 1, h->partial + nb, h->small + SM_DIV, 1.0, h->st));
+  CKL(dmk_colsum(h->partial, nb, nb, 1, h->partial + nb, h->small + SM_REP + REP_DIV2, 1.0, h->st));
   kt_begin(h, KT_FFT_INV);
   CKL(dmk_fft_c2r(&h->fft, 9, h->Z, r, h->st));
   kt_end(h, KT_FFT_INV);
-  CK(hipMemcpyAsync(h->hsmall + SM_DIV, h->small + SM_DIV, sizeof(double), hipMemcpyDeviceToHost, h->st));
+  return 0;
+}
+
+// The one synchronisation of a residual evaluation: optional dot products
+// (written after the report), report to the host, solver scalars updated.
+static int residual_finish(dmk_handle* h, const DotJobs* jobs, double* dots_out) {
+  int nd = jobs ? jobs->n : 0;
+  if (nd > DMK_MAX_DOTS) { dmk_set_error("too many dot products"); return DMK_ERR_ARG; }
+  if (nd) CKL(dmk_dots(jobs, h->n9, h->partial, h->small + SM_REP + REP_DOTS, h->st));
+  CK(hipMemcpyAsync(h->hsmall + SM_REP, h->small + SM_REP, sizeof(double) * (REP_DOTS + nd), hipMemcpyDeviceToHost,
+                    h->st));
   CK(hipStreamSynchronize(h->st));
+  kt_collect(h, KT_STRESS);
+  kt_collect(h, KT_REDUCE);
   kt_collect(h, KT_FFT_FWD);
   kt_collect(h, KT_GAMMA);
   kt_collect(h, KT_FFT_INV);
-  double div2 = h->hsmall[SM_DIV];
-  h->err_div = std::sqrt(div2) * h->wgt;
+  const double* rep = h->hsmall + SM_REP;
+  std::memcpy(h->P_av, rep + REP_PAV, sizeof h->P_av);
+  std::memcpy(h->F_aim, rep + REP_FAIM, sizeof h->F_aim);
+  h->err_BC = rep[REP_ERRBC];
+  h->status = (int)rep[REP_FLAG];
+  h->err_div = std::sqrt(rep[REP_DIV2]) * h->wgt;
   if (h->cells[0] == 1) h->err_div *= 0.5;
+  if (nd) std::memcpy(dots_out, rep + REP_DOTS, sizeof(double) * nd);
   return 0;
+}
+
+static int form_residual(dmk_handle* h, const double* X, double* r, int iter) {
+  int rc = residual_enqueue(h, X, r, iter);
+  return rc ? rc : residual_finish(h, nullptr, nullptr);
 }
 
 // grid_mech_spectral_basic.f90:422-458 converged
@@ -427,50 +440,47 @@ static int converged(const dmk_handle* h) {
   return 0;
 }
 
-static int dots(dmk_handle* h, const double* x, const double* Y, int l, int mode, double* out) {
-  CKL(dmk_mdot(x, Y, h->n9, l, mode, h->n9, h->partial, h->small + SM_DOTS, h->st));
-  CK(hipMemcpyAsync(out, h->small + SM_DOTS, sizeof(double) * l, hipMemcpyDeviceToHost, h->st));
-  CK(hipStreamSynchronize(h->st));
-  return 0;
+static void add_dot(DotJobs* J, const double* x, const double* y, int mode) {
+  J->x[J->n] = x;
+  J->y[J->n] = y;
+  J->mode[J->n] = mode;
+  J->n++;
 }
 
-static int update_subspace(dmk_handle* h, int ivec, int l, const double* F, const double* X) {
-  const int M = h->num.ngmres_m;
-  CK(hipMemcpyAsync(h->Xh + (size_t)ivec * h->n9, X, sizeof(double) * h->n9, hipMemcpyDeviceToDevice, h->st));
-  CK(hipMemcpyAsync(h->Fh + (size_t)ivec * h->n9, F, sizeof(double) * h->n9, hipMemcpyDeviceToDevice, h->st));
-  int nl = l > 0 ? l : 1;
-  double d[32];
-  int rc = dots(h, F, h->Fh, nl, 0, d);
-  if (rc) return rc;
-  for (int i = 0; i < nl; i++) { h->Q[ivec + M * i] = d[i]; h->Q[i + M * ivec] = d[i]; }
-  return 0;
-}
-
+// PETSc SNES NGMRES (snes/impls/ngmres, difference selection and restart,
+// basic line search on x^M = x - r(x)), restated; the dot products each step
+// needs are computed with its residual so that a residual evaluation costs
+// one host synchronisation.  X, F and the trial vectors swap by pointer.
 static int solve_ngmres(dmk_handle* h, int* reason_out) {
   const long n = h->n9;
   const int M = h->num.ngmres_m;
   const double gammaA = 2.0, gammaC = 2.0, epsB = 0.1, deltaB = 0.9;
   const int restart_it = 2;
-  double* X = h->F;
-  double* F = h->R;
   int rc;
-  if ((rc = form_residual(h, X, F, 0))) return rc;
-  double d1;
-  if ((rc = dots(h, F, F, 1, 0, &d1))) return rc;
-  double fnorm = std::sqrt(d1);
+  DotJobs J;
+  J.n = 0;
+  add_dot(&J, h->R, h->R, 0);
+  double d[DMK_MAX_DOTS];
+  if ((rc = residual_enqueue(h, h->F, h->R, 0)) || (rc = residual_finish(h, &J, d))) return rc;
+  double fnorm = std::sqrt(d[0]);
   int reason = converged(h);
   if (reason || fnorm != fnorm) { *reason_out = reason ? reason : -1; return 0; }
   double fminnorm = fnorm;
-  if ((rc = update_subspace(h, 0, 0, F, X))) return rc;
+  // history slot 0 <- (X, F)
+  CK(hipMemcpyAsync(h->Xh, h->F, sizeof(double) * n, hipMemcpyDeviceToDevice, h->st));
+  CK(hipMemcpyAsync(h->Fh, h->R, sizeof(double) * n, hipMemcpyDeviceToDevice, h->st));
+  h->Q[0] = d[0];
   int k_restart = 1, l = 1, restart_count = 0;
   for (int k = 1; k <= h->num.snes_max_it; k++) {
     int ivec = k_restart % M;
-    CKL(dmk_axpby(n, 1.0, X, -1.0, F, h->XM, h->st));
-    if ((rc = form_residual(h, h->XM, h->FM, k - 1))) return rc;
-    double xi[33];
-    if ((rc = dots(h, h->FM, h->FM, 1, 0, &d1))) return rc;
-    double fMnorm = std::sqrt(d1);
-    if ((rc = dots(h, h->FM, h->Fh, l, 0, xi))) return rc;
+    CKL(dmk_axpby(n, 1.0, h->F, -1.0, h->R, h->XM, h->st));
+    J.n = 0;
+    add_dot(&J, h->FM, h->FM, 0);
+    for (int i = 0; i < l; i++) add_dot(&J, h->FM, h->Fh + (size_t)i * n, 0);
+    if ((rc = residual_enqueue(h, h->XM, h->FM, k - 1)) || (rc = residual_finish(h, &J, d))) return rc;
+    double fMnorm = std::sqrt(d[0]);
+    double xi[32];
+    for (int i = 0; i < l; i++) xi[i] = d[1 + i];
     double nu = fMnorm * fMnorm;
     double beta[32], H[32 * 32], alpha[32];
     for (int i = 0; i < l; i++) beta[i] = nu - xi[i];
@@ -482,29 +492,37 @@ static int solve_ngmres(dmk_handle* h, int* reason_out) {
     ca.atot = 0.0;
     for (int i = 0; i < l; i++) { ca.alpha[i] = alpha[i]; ca.atot += alpha[i]; }
     CKL(dmk_combine(n, h->XM, h->Xh, n, &ca, h->XA, h->st));
-    if ((rc = form_residual(h, h->XA, h->FA, k - 1))) return rc;
-    if ((rc = dots(h, h->FA, h->FA, 1, 0, &d1))) return rc;
-    double fAnorm = std::sqrt(d1);
-    if ((rc = dots(h, h->XA, h->XM, 1, 1, &d1))) return rc;
-    double dnorm = std::sqrt(d1);
-    double dist[32];
-    if ((rc = dots(h, h->XA, h->Xh, l, 1, dist))) return rc;
-    double dminnorm = -1.0;
+    J.n = 0;
+    add_dot(&J, h->FA, h->FA, 0);
+    add_dot(&J, h->XA, h->XM, 1);
+    for (int i = 0; i < l; i++) add_dot(&J, h->XA, h->Xh + (size_t)i * n, 1);
+    for (int i = 0; i < l; i++) add_dot(&J, h->FA, h->Fh + (size_t)i * n, 0);
+    if ((rc = residual_enqueue(h, h->XA, h->FA, k - 1)) || (rc = residual_finish(h, &J, d))) return rc;
+    double fAnorm = std::sqrt(d[0]);
+    double dnorm = std::sqrt(d[1]);
+    double dminnorm = -1.0, eta[32];
     for (int i = 0; i < l; i++) {
-      double dc = std::sqrt(dist[i]);
+      double dc = std::sqrt(d[2 + i]);
       if (dc < dminnorm || dminnorm < 0.0) dminnorm = dc;
+      eta[i] = d[2 + l + i];
     }
     bool selectA = true;
     if (fAnorm >= gammaA * fminnorm) selectA = false;
     if (!(epsB * dnorm < dminnorm || std::sqrt(fnorm) < deltaB * std::sqrt(fminnorm))) selectA = false;
+    // the M pair before the swap (needed by a restart)
+    double *XMv = h->XM, *FMv = h->FM;
+    const double* fsel_dots = selectA ? eta : xi;
+    double fsel2 = selectA ? fAnorm * fAnorm : nu;
     if (selectA) {
-      CK(hipMemcpyAsync(X, h->XA, sizeof(double) * n, hipMemcpyDeviceToDevice, h->st));
-      CK(hipMemcpyAsync(F, h->FA, sizeof(double) * n, hipMemcpyDeviceToDevice, h->st));
+      std::swap(h->F, h->XA);
+      std::swap(h->R, h->FA);
       fnorm = fAnorm;
     } else {
-      CK(hipMemcpyAsync(X, h->XM, sizeof(double) * n, hipMemcpyDeviceToDevice, h->st));
-      CK(hipMemcpyAsync(F, h->FM, sizeof(double) * n, hipMemcpyDeviceToDevice, h->st));
+      std::swap(h->F, h->XM);
+      std::swap(h->R, h->FM);
       fnorm = fMnorm;
+      XMv = h->F;
+      FMv = h->R;
     }
     bool selectRestart = false;
     if (epsB * dnorm > dminnorm && std::sqrt(fAnorm) > deltaB * std::sqrt(fminnorm) && l > 0) selectRestart = true;
@@ -514,12 +532,22 @@ static int solve_ngmres(dmk_handle* h, int* reason_out) {
       restart_count = 0;
       k_restart = 1;
       l = 1;
-      if ((rc = update_subspace(h, 0, 0, h->FM, h->XM))) return rc;
+      CK(hipMemcpyAsync(h->Xh, XMv, sizeof(double) * n, hipMemcpyDeviceToDevice, h->st));
+      CK(hipMemcpyAsync(h->Fh, FMv, sizeof(double) * n, hipMemcpyDeviceToDevice, h->st));
+      h->Q[0] = nu;
     } else {
+      int l_old = l;
       if (l < M) l++;
       k_restart++;
       if (fminnorm > fnorm) fminnorm = fnorm;
-      if ((rc = update_subspace(h, ivec, l, F, X))) return rc;
+      // history slot ivec <- (X, F); Gram entries against the other slots
+      CK(hipMemcpyAsync(h->Xh + (size_t)ivec * n, h->F, sizeof(double) * n, hipMemcpyDeviceToDevice, h->st));
+      CK(hipMemcpyAsync(h->Fh + (size_t)ivec * n, h->R, sizeof(double) * n, hipMemcpyDeviceToDevice, h->st));
+      for (int i = 0; i < l; i++) {
+        double v = i == ivec ? fsel2 : (i < l_old ? fsel_dots[i] : 0.0);
+        h->Q[ivec + M * i] = v;
+        h->Q[i + M * ivec] = v;
+      }
     }
     reason = converged(h);
     if (reason || fnorm != fnorm) { *reason_out = reason ? reason : -1; return 0; }
@@ -529,14 +557,12 @@ static int solve_ngmres(dmk_handle* h, int* reason_out) {
 }
 
 static int solve_richardson(dmk_handle* h, int* reason_out) {
-  double* X = h->F;
-  double* F = h->R;
   for (int k = 0; k <= h->num.snes_max_it; k++) {
-    int rc = form_residual(h, X, F, k);
+    int rc = form_residual(h, h->F, h->R, k);
     if (rc) return rc;
     int reason = converged(h);
     if (reason) { *reason_out = reason; return 0; }
-    CKL(dmk_axpby(h->n9, 1.0, X, -1.0, F, X, h->st));
+    CKL(dmk_axpby(h->n9, 1.0, h->F, -1.0, h->R, h->F, h->st));
   }
   *reason_out = -1;
   return 0;
@@ -813,6 +839,8 @@ extern "C" int dmk_mech_solution(dmk_handle* h, int32_t* iterations, int32_t* co
   if (h->num.update_gamma) set_gamma_coef(h, h->C_minMaxAvg);
   h->nfuncs = 0;
   int reason = 0;
+  // F_aim is updated on the device during the solve (residual_aim_kernel)
+  CK(hipMemcpyAsync(h->small + SM_FAIM, h->F_aim, sizeof h->F_aim, hipMemcpyHostToDevice, h->st));
   rc = h->num.solver == 1 ? solve_richardson(h, &reason) : solve_ngmres(h, &reason);
   if (rc) return rc;
   if ((rc = tangent_pass(h, h->dt))) return rc;
@@ -1056,6 +1084,7 @@ extern "C" int dmk_gamma_convolution(dmk_handle* h, const double* field, const d
   GammaArgs g = h->ga;
   g.do_div = 0;
   g.do_gamma = 1;
+  g.aim_dev = nullptr;
   std::memcpy(g.aim, aim, sizeof g.aim);
   CKL(dmk_gamma_div(h->Z, &g, h->partial, h->st));
   CKL(dmk_fft_c2r(&h->fft, 9, h->Z, h->Fext, h->st));
@@ -1076,10 +1105,10 @@ extern "C" int dmk_divergence_rms(dmk_handle* h, const double* field, double* ou
   g.do_gamma = 0;
   CKL(dmk_gamma_div(h->Z, &g, h->partial, h->st));
   int nb = dmk_gamma_div_blocks(&g);
-  CKL(dmk_colsum(h->partial, nb, nb, 1, h->partial + nb, h->small + SM_DIV, 1.0, h->st));
-  CK(hipMemcpyAsync(h->hsmall + SM_DIV, h->small + SM_DIV, sizeof(double), hipMemcpyDeviceToHost, h->st));
+  CKL(dmk_colsum(h->partial, nb, nb, 1, h->partial + nb, h->small + SM_DOTS, 1.0, h->st));
+  CK(hipMemcpyAsync(h->hsmall + SM_DOTS, h->small + SM_DOTS, sizeof(double), hipMemcpyDeviceToHost, h->st));
   CK(hipStreamSynchronize(h->st));
-  double r = std::sqrt(h->hsmall[SM_DIV]) * h->wgt;
+  double r = std::sqrt(h->hsmall[SM_DOTS]) * h->wgt;
   if (h->cells[0] == 1) r *= 0.5;
   *out = r;
   return 0;

@@ -292,7 +292,7 @@ __global__ void __launch_bounds__(256) gamma_div_kernel(double2* __restrict__ Z,
     if (g.do_gamma) {
       if (f == 0) {
 #pragma unroll
-        for (int c = 0; c < 9; c++) Z[c * Nk] = make_double2(g.aim[c], 0.0);
+        for (int c = 0; c < 9; c++) Z[c * Nk] = make_double2(g.aim_dev ? g.aim_dev[c] : g.aim[c], 0.0);
       } else {
         double aa[6] = {a[0] * a[0], a[1] * a[1], a[2] * a[2], a[1] * a[2], a[0] * a[2], a[0] * a[1]};
         double A[9];
