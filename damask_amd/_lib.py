@@ -49,7 +49,9 @@ EXPORTS = ["dmk_mech_init", "dmk_mech_forward", "dmk_mech_materialpoint_forward"
            "dmk_mech_set_field", "dmk_mech_get_scalars", "dmk_mech_set_scalars", "dmk_grid_run_loadcase",
            "dmk_mech_counters", "dmk_mech_destroy", "dmk_last_error", "dmk_nsl", "dmk_device_synchronize",
            "dmk_gamma_convolution", "dmk_divergence_rms", "dmk_fft_forward", "dmk_fft_backward",
-           "dmk_mech_profile", "dmk_mech_kernel_time", "dmk_slip_system"]
+           "dmk_mech_profile", "dmk_mech_kernel_time", "dmk_slip_system", "dmk_adjoint_tape",
+           "dmk_adjoint_record", "dmk_adjoint_info", "dmk_adjoint_backward", "dmk_adjoint_release",
+           "dmk_adjoint_point_host"]
 
 _lib = None
 
@@ -113,6 +115,14 @@ def load(path: str = LIB_PATH):
     L.dmk_mech_kernel_time.argtypes = [vp, C.c_int32, dp, C.POINTER(C.c_int64)]
     L.dmk_slip_system.restype = C.c_int
     L.dmk_slip_system.argtypes = [C.c_int32, C.c_int32, ip, C.c_int32, dp]
+    for f in ("dmk_adjoint_tape", "dmk_adjoint_record", "dmk_adjoint_info", "dmk_adjoint_backward",
+              "dmk_adjoint_release", "dmk_adjoint_point_host"):
+        getattr(L, f).restype = C.c_int
+    L.dmk_adjoint_tape.argtypes = [vp, C.c_int32]
+    L.dmk_adjoint_record.argtypes = [vp]
+    L.dmk_adjoint_info.argtypes = [vp, ip, dp, dp]
+    L.dmk_adjoint_backward.argtypes = [vp, dp, C.c_double, dp, dp]
+    L.dmk_adjoint_release.argtypes = [vp]
     _lib = L
     return L
 

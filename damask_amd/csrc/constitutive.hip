@@ -409,7 +409,7 @@ __global__ void __launch_bounds__(SWG) __attribute__((amdgpu_waves_per_eu(DMK_ST
   const double* pxi0 = mp.xi0;
   const double* pgam0 = mp.gam0;
 
-  int newton = 0, fpi = 0;
+  int newton = 0, fpi = 0, nsub = 0;
   // phase_mechanical_constitutive substepping; the first pass takes the
   // cut-back branch, which is phase_restore (src/phase_mechanical.f90:1096-1115)
   double stepFrac = 0.0, step = 1.0 / num.r_cutback;
@@ -451,6 +451,7 @@ __global__ void __launch_bounds__(SWG) __attribute__((amdgpu_waves_per_eu(DMK_ST
       for (int c = 0; c < 9; c++)
         st(sub + SUB_F * ln.sN, ln, c, fma(step, ld(Ftarget, ln, c) - ld(mp.F0, ln, c), ld(pF0, ln, c)));
       status = integrate_state_fpi<NSL, QP>(ph, num, mp, ln, pxi0, pgam0, step * dt, sh, lane, newton, fpi);
+      nsub++;
     }
   }
 #pragma unroll 1
@@ -458,6 +459,7 @@ __global__ void __launch_bounds__(SWG) __attribute__((amdgpu_waves_per_eu(DMK_ST
   mp.status[e] = status;
   mp.n_newton[e] = newton;
   mp.n_fpi[e] = fpi;
+  mp.n_sub[e] = nsub;
 }
 
 // math_rotationalPart (src/math.f90:1004-1043)

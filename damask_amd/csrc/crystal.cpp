@@ -184,7 +184,9 @@ int dmk_build_phase(const dmk_phase_desc* d, DmkPhaseDev* p) {
       int t = d->lattice == 0 ? CF_INT[used[r]][used[a]] : CI_INT[used[r]][used[a]];
       if (t > d->n_h) return 5;
       p->H[r][a] = d->h_sl_sl[t - 1];
+      p->Ht[r][a] = t - 1;
     }
+  p->n_h = d->n_h;
   int a = 0;
   for (int f = 0; f < d->n_fam; f++)
     for (int s = 0; s < d->N_sl[f]; s++, a++) {

@@ -63,6 +63,10 @@ struct DmkPhaseDev {
   // without twins (sum F_tw = 0): xi_inf + f_sat sqrt(0) and h_0 (1 + c_1 0^c_2)
   double xi_sat[DMK_NSL_MAX], hf[DMK_NSL_MAX];
   double atol_xi, atol_gamma;
+  // interaction type (index into h_sl_sl) of H[reacting][acting]: parameter
+  // cotangents of the adjoint
+  int n_h;
+  int32_t Ht[DMK_NSL_MAX][DMK_NSL_MAX];
 };
 
 struct DmkNumMech {
@@ -96,6 +100,7 @@ struct DmkMPDev {
   int32_t* status;
   int32_t* n_newton;             // per cell Newton iterations of the last response
   int32_t* n_fpi;
+  int32_t* n_sub;                // per cell state integrations (sub-steps) of the last response
   // scratch of the state integrator: [2*nsl_max][N] each
   double* st0s;                  // state at the start of the sub-step (after the first sub-step)
   double* last1;                 // state-rate history, ping-pong pair

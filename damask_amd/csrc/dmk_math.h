@@ -6,6 +6,8 @@
 #include <stdint.h>
 
 #define DI __device__ __forceinline__
+// pure register algebra, also compiled for the host (adjoint point checks)
+#define DHI __host__ __device__ __forceinline__
 
 // Per-cell field access.  Fields are component-major ([c][N]); a lane reaches
 // its cell through a 32-bit byte offset (cell index * 8, so N < 2^29) added to
@@ -58,7 +60,7 @@ DI T* opaque(T* p) {
   return p;
 }
 
-DI void m_mul(const double* A, const double* B, double* C) {
+DHI void m_mul(const double* A, const double* B, double* C) {
   double T[9];
 #pragma unroll
   for (int i = 0; i < 3; i++)
@@ -74,7 +76,7 @@ DI void m_mul(const double* A, const double* B, double* C) {
 }
 
 // C = A^T B
-DI void m_mul_tn(const double* A, const double* B, double* C) {
+DHI void m_mul_tn(const double* A, const double* B, double* C) {
   double T[9];
 #pragma unroll
   for (int i = 0; i < 3; i++)
@@ -90,7 +92,7 @@ DI void m_mul_tn(const double* A, const double* B, double* C) {
 }
 
 // C = A B^T
-DI void m_mul_nt(const double* A, const double* B, double* C) {
+DHI void m_mul_nt(const double* A, const double* B, double* C) {
   double T[9];
 #pragma unroll
   for (int i = 0; i < 3; i++)
@@ -105,7 +107,7 @@ DI void m_mul_nt(const double* A, const double* B, double* C) {
   for (int k = 0; k < 9; k++) C[k] = T[k];
 }
 
-DI void m_transpose(const double* A, double* B) {
+DHI void m_transpose(const double* A, double* B) {
   double T[9];
 #pragma unroll
   for (int i = 0; i < 3; i++)
@@ -116,15 +118,15 @@ DI void m_transpose(const double* A, double* B) {
 }
 
 // math_det33 (src/math.f90:630-638)
-DI double m_det(const double* m) {
+DHI double m_det(const double* m) {
   return m[0] * (m[4] * m[8] - m[5] * m[7]) - m[1] * (m[3] * m[8] - m[5] * m[6])
        + m[2] * (m[3] * m[7] - m[4] * m[6]);
 }
 
-DI bool dEq0(double a) { return fabs(a) <= DBL_MIN * 10.0; }
+DHI bool dEq0(double a) { return fabs(a) <= DBL_MIN * 10.0; }
 
 // math_invert33 (src/math.f90:471-505): Cramer; returns true on error (zeros out)
-DI bool m_invert(const double* A, double* inv) {
+DHI bool m_invert(const double* A, double* inv) {
   double c00 = A[4] * A[8] - A[5] * A[7];
   double c10 = -A[3] * A[8] + A[5] * A[6];
   double c20 = A[3] * A[7] - A[4] * A[6];
@@ -147,7 +149,7 @@ DI bool m_invert(const double* A, double* inv) {
   return false;
 }
 
-DI double m_norm(const double* A) {
+DHI double m_norm(const double* A) {
   double s = 0.0;
 #pragma unroll
   for (int k = 0; k < 9; k++) s = fma(A[k], A[k], s);
@@ -156,7 +158,7 @@ DI double m_norm(const double* A) {
 
 // C : X for cubic stiffness in the lattice frame, X symmetric, through the
 // Mandel vector (src/math.f90:792-820 and src/crystal.f90:1711-1759)
-DI void hooke_sym(double C11, double C12, double C44, const double* X, double* S) {
+DHI void hooke_sym(double C11, double C12, double C44, const double* X, double* S) {
   const double W = 1.4142135623730951;          // sqrt(2)
   const double IW = 0.7071067811865475;         // 1/sqrt(2)
   double v1 = X[0], v2 = X[4], v3 = X[8];

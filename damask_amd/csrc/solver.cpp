@@ -22,6 +22,7 @@
 #include <vector>
 
 #include "dmk_common.h"
+#include "dmk_adjoint_k.h"
 #include "dmk_rot.h"
 #include "dmk_linalg.h"
 #include "dmk_spectral.h"
@@ -206,6 +207,7 @@ struct dmk_handle {
   double dt = 0;
   GammaArgs ga;
   std::vector<double> Q;
+  struct AdjState* adj = nullptr;    // adjoint tape and work buffers (dmk_adjoint_*)
   // driver state (DAMASK_grid.f90:78-100)
   double t = 0.0, Delta_t = 1.0, Delta_t_prev = 0.0;
   int cutBack = 0, cutBackLevel = 0, totalIncs = 0;
@@ -248,7 +250,11 @@ static int dalloc(dmk_handle* h, T** p, size_t n) {
 }
 
 enum { SM_PAV = 0, SM_CVOL = 9, SM_CMM = 90, SM_DOTS = 180, SM_REP = 256, SM_FAIM = 352, SM_AIM = 368,
-       SM_SIZE = 384 };
+       // adjoint: aim bookkeeping, report, mean, dot products, maxima, tangent average
+       SM_AFAIM = 384, SM_AAIM = 400, SM_AREP = 416, SM_APAV = 504, SM_ADOTS = 520, SM_AMAX = 584, SM_ACAV = 592,
+       SM_SIZE = 680 };
+static_assert(SM_AREP + REP_SIZE <= SM_APAV, "adjoint report region");
+static_assert(SM_ADOTS + DMK_MAX_DOTS <= SM_AMAX, "adjoint dot region");
 static_assert(SM_REP + REP_SIZE <= SM_FAIM, "report region");
 
 static void set_gamma_coef(dmk_handle* h, const double* C) {
@@ -343,16 +349,17 @@ static int constitutive(dmk_handle* h, const double* F, double dt, bool rotate, 
   return 0;
 }
 
-// grid_utilities.f90:74-127 utilities_maskedCompliance
-static int masked_compliance(dmk_handle* h) {
+// grid_utilities.f90:74-127 utilities_maskedCompliance of stiffness C for the
+// load-case frame q_R and stress mask (1 = component not prescribed)
+static int masked_compliance_of(const double* C, const int* stress_mask, const double* q_R, double* S) {
   int msv[9], sel[9], n = 0;
   for (int a = 0; a < 3; a++)
-    for (int b = 0; b < 3; b++) msv[3 * a + b] = !h->stress_mask[c2(a, b)];
+    for (int b = 0; b < 3; b++) msv[3 * a + b] = !stress_mask[c2(a, b)];
   for (int i = 0; i < 9; i++) if (msv[i]) sel[n++] = i;
   double T99[81] = {0};
   if (n > 0) {
     double Cr[81], C99[81], c[81], s[81];
-    rotate3333(h->q_R, h->C_volAvg, Cr);
+    rotate3333(q_R, C, Cr);
     for (int a = 0; a < 3; a++) for (int b = 0; b < 3; b++) for (int cc = 0; cc < 3; cc++) for (int d = 0; d < 3; d++)
       C99[(3 * a + b) + 9 * (3 * cc + d)] = Cr[c4(a, b, cc, d)];
     for (int q = 0; q < n; q++) for (int p = 0; p < n; p++) c[p + n * q] = C99[sel[p] + 9 * sel[q]];
@@ -362,8 +369,12 @@ static int masked_compliance(dmk_handle* h) {
     for (int q = 0; q < n; q++) for (int p = 0; p < n; p++) T99[sel[p] + 9 * sel[q]] = s[p + n * q];
   }
   for (int a = 0; a < 3; a++) for (int b = 0; b < 3; b++) for (int cc = 0; cc < 3; cc++) for (int d = 0; d < 3; d++)
-    h->S[c4(a, b, cc, d)] = T99[(3 * a + b) + 9 * (3 * cc + d)];
+    S[c4(a, b, cc, d)] = T99[(3 * a + b) + 9 * (3 * cc + d)];
   return 0;
+}
+
+static int masked_compliance(dmk_handle* h) {
+  return masked_compliance_of(h->C_volAvg, h->stress_mask, h->q_R, h->S);
 }
 
 // formResidual (grid_mech_spectral_basic.f90:464-521), enqueued: response,
@@ -568,6 +579,360 @@ static int solve_richardson(dmk_handle* h, int* reason_out) {
   return 0;
 }
 
+// ------------------------------------------------------------------ adjoint
+// Reverse sweep over recorded increments.  This has no counterpart in the
+// reference: it differentiates the converged spectral-basic increments with
+// respect to the phenopowerlaw parameters.  Increment n's converged field F
+// satisfies equilibrium (Gamma_0 * P = 0 off the mean, prescribed mean stress
+// components, spectral_utilities.f90:436-501 and grid_mech_spectral_basic.f90
+// :478-496) with P = P(F, state_{n-1}; theta).  Its adjoint is the same linear
+// problem with the transposed tangent A^T of the increment map: find mu,
+// compatible plus a mean on the stress-controlled components, with
+// Phi(A^T mu) = Phi(g), where Phi is the Gamma_0 convolution whose mean is
+// S : <.> (S the masked compliance of <A^T>, the basic scheme's own mean
+// rule).  It is solved by restarted GMRES.  The local reverse steps
+// (dmk_adjoint.h) give g, then carry the cotangents of F_p and xi to
+// increment n-1 and into the parameters.
+struct AdjState {
+  int cap = 0, n = 0;
+  size_t slot = 0;                 // doubles per tape entry
+  double* tape = nullptr;          // entries 0..cap: F, F_p, L_p, xi (component-major)
+  std::vector<double> dt, pav, qR;
+  std::vector<int> mask, maxsub;
+  int M = 30;                      // GMRES restart length
+  bool work_ready = false;
+  double *A = nullptr, *g = nullptr, *res = nullptr, *scr = nullptr, *Fpb[2] = {nullptr, nullptr},
+         *xib[2] = {nullptr, nullptr}, *thb = nullptr, *mu = nullptr, *y = nullptr, *b = nullptr, *r = nullptr,
+         *V = nullptr, *red = nullptr;
+  std::vector<void*> work;
+};
+
+enum { TP_F = 0, TP_FP = 9, TP_LP = 18, TP_XI = 27 };   // tape entry offsets, in fields of N
+
+static double* tape_entry(const dmk_handle* h, int k) { return h->adj->tape + (size_t)k * h->adj->slot; }
+
+static void adj_free(dmk_handle* h) {
+  if (!h->adj) return;
+  if (h->st) hipStreamSynchronize(h->st);
+  if (h->adj->tape) hipFree(h->adj->tape);
+  for (void* p : h->adj->work) hipFree(p);
+  delete h->adj;
+  h->adj = nullptr;
+}
+
+static int adj_walloc(dmk_handle* h, double** p, size_t n) {
+  void* v = nullptr;
+  CK(hipMalloc(&v, sizeof(double) * (n ? n : 1)));
+  h->adj->work.push_back(v);
+  *p = (double*)v;
+  return 0;
+}
+
+static int tape_store(dmk_handle* h, int k) {
+  double* E = tape_entry(h, k);
+  const size_t N = h->N, b9 = sizeof(double) * 9 * N;
+  CK(hipMemcpyAsync(E + TP_F * N, h->mp.Fphase, b9, hipMemcpyDeviceToDevice, h->st));
+  CK(hipMemcpyAsync(E + TP_FP * N, h->mp.Fp, b9, hipMemcpyDeviceToDevice, h->st));
+  CK(hipMemcpyAsync(E + TP_LP * N, h->mp.Lp, b9, hipMemcpyDeviceToDevice, h->st));
+  CK(hipMemcpyAsync(E + TP_XI * N, h->mp.xi, sizeof(double) * h->nsl_max * N, hipMemcpyDeviceToDevice, h->st));
+  return 0;
+}
+
+extern "C" int dmk_adjoint_tape(dmk_handle* h, int32_t max_increments) {
+  if (!h || max_increments < 1) { dmk_set_error("dmk_adjoint_tape: invalid argument"); return DMK_ERR_ARG; }
+  adj_free(h);
+  h->adj = new AdjState;
+  AdjState& a = *h->adj;
+  a.cap = max_increments;
+  a.slot = (size_t)(27 + h->nsl_max) * h->N;
+  if (hipMalloc((void**)&a.tape, sizeof(double) * a.slot * (a.cap + 1)) != hipSuccess) {
+    hipGetLastError();
+    delete h->adj;
+    h->adj = nullptr;
+    dmk_set_error("dmk_adjoint_tape: %zu bytes of device memory not available",
+                  sizeof(double) * a.slot * (size_t)(max_increments + 1));
+    return DMK_ERR_HIP;
+  }
+  return tape_store(h, 0);
+}
+
+extern "C" int dmk_adjoint_record(dmk_handle* h) {
+  if (!h || !h->adj) { dmk_set_error("dmk_adjoint_record: no tape"); return DMK_ERR_ARG; }
+  AdjState& a = *h->adj;
+  if (a.n >= a.cap) { dmk_set_error("dmk_adjoint_record: tape full (%d increments)", a.cap); return DMK_ERR_ARG; }
+  int rc = tape_store(h, a.n + 1);
+  if (rc) return rc;
+  CKL(dmk_adj_imax(h->mp.n_sub, h->N, h->small + SM_AMAX, h->st));
+  CK(hipMemcpyAsync(h->hsmall + SM_AMAX, h->small + SM_AMAX, sizeof(double), hipMemcpyDeviceToHost, h->st));
+  CK(hipStreamSynchronize(h->st));
+  a.n++;
+  a.dt.push_back(h->dt);
+  a.pav.insert(a.pav.end(), h->P_av, h->P_av + 9);
+  a.qR.insert(a.qR.end(), h->q_R, h->q_R + 4);
+  a.mask.insert(a.mask.end(), h->stress_mask, h->stress_mask + 9);
+  a.maxsub.push_back((int)h->hsmall[SM_AMAX]);
+  return 0;
+}
+
+extern "C" int dmk_adjoint_info(dmk_handle* h, int32_t* n, double* P_av, double* dt) {
+  if (!h || !h->adj) { dmk_set_error("dmk_adjoint_info: no tape"); return DMK_ERR_ARG; }
+  const AdjState& a = *h->adj;
+  if (n) *n = a.n;
+  if (P_av) std::memcpy(P_av, a.pav.data(), sizeof(double) * 9 * a.n);
+  if (dt) std::memcpy(dt, a.dt.data(), sizeof(double) * a.n);
+  return 0;
+}
+
+extern "C" int dmk_adjoint_release(dmk_handle* h) {
+  if (!h) { dmk_set_error("null handle"); return DMK_ERR_ARG; }
+  adj_free(h);
+  return 0;
+}
+
+static int adj_dots(dmk_handle* h, const DotJobs& J, double* out) {
+  CKL(dmk_dots(&J, h->n9, h->partial, h->small + SM_ADOTS, h->st));
+  CK(hipMemcpyAsync(h->hsmall + SM_ADOTS, h->small + SM_ADOTS, sizeof(double) * J.n, hipMemcpyDeviceToHost, h->st));
+  CK(hipStreamSynchronize(h->st));
+  std::memcpy(out, h->hsmall + SM_ADOTS, sizeof(double) * J.n);
+  return 0;
+}
+
+// out = Phi(y): Gamma_0 convolution with the mean rule of the increment
+static int adj_phi(dmk_handle* h, const AimArgs& aa, const double* y, double* out) {
+  CKL(dmk_colsum(y, h->N, h->N, 9, h->partial, h->small + SM_APAV, h->wgt, h->st));
+  CKL(dmk_residual_aim(&aa, h->small + SM_APAV, h->small + SM_AFAIM, h->flag, h->small + SM_AREP,
+                       h->small + SM_AAIM, h->st));
+  CKL(dmk_fft_r2c(&h->fft, 9, y, h->Z, h->st));
+  GammaArgs g = h->ga;
+  g.do_div = 0;
+  g.do_gamma = 1;
+  g.aim_dev = h->small + SM_AAIM;
+  CKL(dmk_gamma_div(h->Z, &g, h->partial, h->st));
+  CKL(dmk_fft_c2r(&h->fft, 9, h->Z, out, h->st));
+  return 0;
+}
+
+// out = Phi(A^T v)
+static int adj_op(dmk_handle* h, const AimArgs& aa, const double* v, double* out) {
+  CKL(dmk_adj_atmul(h->N, h->adj->A, v, h->adj->y, h->st));
+  return adj_phi(h, aa, h->adj->y, out);
+}
+
+// restarted GMRES (classical Gram-Schmidt, twice) for Phi(A^T mu) = b
+static int adj_gmres(dmk_handle* h, const AimArgs& aa, double rtol, int maxit, int* its_out, double* rel_out) {
+  AdjState& a = *h->adj;
+  const long n = h->n9;
+  const int M = a.M;
+  DotJobs J;
+  double d[DMK_MAX_DOTS];
+  CK(hipMemsetAsync(a.mu, 0, sizeof(double) * n, h->st));
+  J.n = 0;
+  add_dot(&J, a.b, a.b, 0);
+  int rc = adj_dots(h, J, d);
+  if (rc) return rc;
+  const double bnorm = std::sqrt(d[0]);
+  *its_out = 0;
+  *rel_out = 0.0;
+  if (!(bnorm > 0.0)) return bnorm == 0.0 ? 0 : (dmk_set_error("adjoint: non-finite right-hand side"), DMK_ERR_NOCONV);
+  CKL(dmk_axpby(n, 1.0 / bnorm, a.b, 0.0, a.b, a.V, h->st));
+  double rnorm = bnorm;
+  int its = 0;
+  std::vector<double> H((size_t)(M + 1) * M), cs(M), sn(M), gv(M + 1), yv(M);
+  while (true) {
+    std::fill(H.begin(), H.end(), 0.0);
+    std::fill(gv.begin(), gv.end(), 0.0);
+    gv[0] = rnorm;
+    int j = 0;
+    bool done = false;
+    for (; j < M && its < maxit; j++) {
+      its++;
+      double* w = a.V + (size_t)(j + 1) * n;
+      if ((rc = adj_op(h, aa, a.V + (size_t)j * n, w))) return rc;
+      double hc[DMK_GS_MAX] = {0};
+      for (int pass = 0; pass < 2; pass++) {
+        J.n = 0;
+        for (int i = 0; i <= j; i++) add_dot(&J, a.V + (size_t)i * n, w, 0);
+        if ((rc = adj_dots(h, J, d))) return rc;
+        GsArgs c;
+        c.l = j + 1;
+        for (int i = 0; i <= j; i++) { c.c[i] = d[i]; hc[i] += d[i]; }
+        CKL(dmk_adj_gs(n, w, a.V, n, &c, h->st));
+      }
+      J.n = 0;
+      add_dot(&J, w, w, 0);
+      if ((rc = adj_dots(h, J, d))) return rc;
+      double hn = std::sqrt(d[0]);
+      for (int i = 0; i <= j; i++) H[i + (size_t)(M + 1) * j] = hc[i];
+      H[j + 1 + (size_t)(M + 1) * j] = hn;
+      if (hn > 0.0) CKL(dmk_axpby(n, 1.0 / hn, w, 0.0, w, w, h->st));
+      for (int i = 0; i < j; i++) {
+        double* c0 = &H[i + (size_t)(M + 1) * j];
+        double t = cs[i] * c0[0] + sn[i] * c0[1];
+        c0[1] = -sn[i] * c0[0] + cs[i] * c0[1];
+        c0[0] = t;
+      }
+      double* cj = &H[j + (size_t)(M + 1) * j];
+      double den = std::hypot(cj[0], cj[1]);
+      cs[j] = den > 0.0 ? cj[0] / den : 1.0;
+      sn[j] = den > 0.0 ? cj[1] / den : 0.0;
+      cj[0] = den;
+      cj[1] = 0.0;
+      gv[j + 1] = -sn[j] * gv[j];
+      gv[j] = cs[j] * gv[j];
+      rnorm = std::fabs(gv[j + 1]);
+      if (rnorm != rnorm) { dmk_set_error("adjoint GMRES: non-finite residual"); return DMK_ERR_NOCONV; }
+      if (rnorm <= rtol * bnorm || hn == 0.0) { j++; done = true; break; }
+    }
+    for (int i = j - 1; i >= 0; i--) {
+      double s = gv[i];
+      for (int k = i + 1; k < j; k++) s -= H[i + (size_t)(M + 1) * k] * yv[k];
+      yv[i] = s / H[i + (size_t)(M + 1) * i];
+    }
+    GsArgs c;
+    c.l = j;
+    for (int i = 0; i < j; i++) c.c[i] = -yv[i];
+    CKL(dmk_adj_gs(n, a.mu, a.V, n, &c, h->st));
+    // true residual b - Phi(A^T mu) (restart vector, and the reported value)
+    if ((rc = adj_op(h, aa, a.mu, a.r))) return rc;
+    CKL(dmk_axpby(n, 1.0, a.b, -1.0, a.r, a.r, h->st));
+    J.n = 0;
+    add_dot(&J, a.r, a.r, 0);
+    if ((rc = adj_dots(h, J, d))) return rc;
+    rnorm = std::sqrt(d[0]);
+    if (done || rnorm <= rtol * bnorm || its >= maxit) break;
+    CKL(dmk_axpby(n, 1.0 / rnorm, a.r, 0.0, a.r, a.V, h->st));
+  }
+  *its_out = its;
+  *rel_out = rnorm / bnorm;
+  return 0;
+}
+
+extern "C" int dmk_adjoint_backward(dmk_handle* h, const double* P_av_bar, double rtol, double* theta_bar,
+                                    double* stats) {
+  if (!h || !h->adj || !P_av_bar || !theta_bar) { dmk_set_error("dmk_adjoint_backward: invalid argument"); return DMK_ERR_ARG; }
+  AdjState& a = *h->adj;
+  if (a.n < 1) { dmk_set_error("dmk_adjoint_backward: no recorded increment"); return DMK_ERR_ARG; }
+  for (int k = 0; k < a.n; k++)
+    if (a.maxsub[k] > 1) {
+      dmk_set_error("dmk_adjoint_backward: increment %d was integrated in material-point sub-steps; "
+                    "the recorded state does not define the single-step increment map", k + 1);
+      return DMK_ERR_CONFIG;
+    }
+  const size_t N = h->N, n9 = h->n9;
+  const int mmax = h->nsl_max, nslot = ADJ_TH_SYS * mmax + DMK_NH_MAX;
+  int rc;
+  if (!a.work_ready) {
+    if ((rc = adj_walloc(h, &a.A, 81 * N)) || (rc = adj_walloc(h, &a.g, n9)) || (rc = adj_walloc(h, &a.res, N)) ||
+        (rc = adj_walloc(h, &a.scr, dmk_adj_scratch_slots(mmax) * N)) || (rc = adj_walloc(h, &a.Fpb[0], n9)) ||
+        (rc = adj_walloc(h, &a.Fpb[1], n9)) || (rc = adj_walloc(h, &a.xib[0], (size_t)mmax * N)) ||
+        (rc = adj_walloc(h, &a.xib[1], (size_t)mmax * N)) || (rc = adj_walloc(h, &a.thb, (size_t)nslot * N)) ||
+        (rc = adj_walloc(h, &a.mu, n9)) || (rc = adj_walloc(h, &a.y, n9)) || (rc = adj_walloc(h, &a.b, n9)) ||
+        (rc = adj_walloc(h, &a.r, n9)) || (rc = adj_walloc(h, &a.V, (size_t)(a.M + 1) * n9)) ||
+        (rc = adj_walloc(h, &a.red, (size_t)h->n_phases * (nslot + mmax))))
+      return rc;
+    a.work_ready = true;
+  }
+  CK(hipMemsetAsync(a.Fpb[0], 0, sizeof(double) * n9, h->st));
+  CK(hipMemsetAsync(a.xib[0], 0, sizeof(double) * mmax * N, h->st));
+  CK(hipMemsetAsync(a.thb, 0, sizeof(double) * nslot * N, h->st));
+  int cur = 0;
+  for (int k = a.n; k >= 1; k--) {
+    const double* E = tape_entry(h, k);
+    const double* Ep = tape_entry(h, k - 1);
+    const double* qR = &a.qR[4 * (k - 1)];
+    AdjKArgs ka{};
+    ka.N = (int)N;
+    ka.mmax = mmax;
+    ka.dt = a.dt[k - 1];
+    ka.phase = h->mp.phase;
+    ka.F = E + TP_F * N;
+    ka.Lp = E + TP_LP * N;
+    ka.xi = E + TP_XI * N;
+    ka.Fp0 = Ep + TP_FP * N;
+    ka.xi0 = Ep + TP_XI * N;
+    // P_av = R <P> R^T (passive rotation of the raw average): <P>_bar = R^T P_av_bar R
+    double pr[9];
+    rotate33(qR, P_av_bar + 9 * (k - 1), true, pr);
+    for (int c = 0; c < 9; c++) ka.Pu[c] = h->wgt * pr[c];
+    ka.Fpb_in = a.Fpb[cur];
+    ka.xib_in = a.xib[cur];
+    ka.scr = a.scr;
+    ka.A = a.A;
+    ka.g = a.g;
+    ka.res = a.res;
+    ka.mode = 0;
+    for (int p = 0; p < h->n_phases; p++) {
+      ka.phase_id = p;
+      CKL(dmk_adj_local(&ka, h->ph_dev + p, h->st));
+    }
+    CKL(dmk_adj_max(a.res, N, h->small + SM_AMAX, h->st));
+    CKL(dmk_colsum(a.A, N, N, 81, h->partial, h->small + SM_ACAV, h->wgt, h->st));
+    CK(hipMemcpyAsync(h->hsmall + SM_AMAX, h->small + SM_AMAX, sizeof(double) * (SM_ACAV + 81 - SM_AMAX),
+                      hipMemcpyDeviceToHost, h->st));
+    CK(hipStreamSynchronize(h->st));
+    const double resmax = h->hsmall[SM_AMAX];
+    if (!(resmax < 1.0)) {
+      dmk_set_error("dmk_adjoint_backward: increment %d: recorded state inconsistent or singular (residual %g)", k,
+                    resmax);
+      return DMK_ERR_NOCONV;
+    }
+    // mean rule: masked compliance of <A^T>
+    double Ct[81], Sadj[81];
+    const double* Abar = h->hsmall + SM_ACAV;
+    for (int p = 0; p < 9; p++)
+      for (int q = 0; q < 9; q++) Ct[p + 9 * q] = Abar[q + 9 * p];
+    if ((rc = masked_compliance_of(Ct, &a.mask[9 * (k - 1)], qR, Sadj))) return rc;
+    AimArgs aa;
+    std::memcpy(aa.S, Sadj, sizeof aa.S);
+    for (int c = 0; c < 9; c++) { aa.P_aim[c] = 0.0; aa.stress_mask[c] = a.mask[9 * (k - 1) + c]; }
+    std::memcpy(aa.q_R, qR, sizeof aa.q_R);
+    aa.rotate = 1;
+    aa.update = 1;
+    CK(hipMemsetAsync(h->small + SM_AFAIM, 0, sizeof(double) * 9, h->st));
+    if ((rc = adj_phi(h, aa, a.g, a.b))) return rc;
+    int its = 0;
+    double rel = 0.0;
+    if ((rc = adj_gmres(h, aa, rtol, 600, &its, &rel))) return rc;
+    ka.mode = 1;
+    ka.mu = a.mu;
+    ka.Fpb_out = a.Fpb[1 - cur];
+    ka.xib_out = a.xib[1 - cur];
+    ka.thb = a.thb;
+    for (int p = 0; p < h->n_phases; p++) {
+      ka.phase_id = p;
+      CKL(dmk_adj_local(&ka, h->ph_dev + p, h->st));
+    }
+    cur = 1 - cur;
+    if (stats) {
+      stats[4 * (k - 1) + 0] = its;
+      stats[4 * (k - 1) + 1] = rel;
+      stats[4 * (k - 1) + 2] = resmax;
+      stats[4 * (k - 1) + 3] = a.maxsub[k - 1];
+    }
+  }
+  // xi_0 enters as the initial state (phenopowerlaw.f90:259-262)
+  double* red_xi = a.red;
+  double* red_th = a.red + (size_t)h->n_phases * mmax;
+  CKL(dmk_adj_phase_sum(a.xib[cur], N, h->mp.phase, h->n_phases, mmax, red_xi, h->st));
+  CKL(dmk_adj_phase_sum(a.thb, N, h->mp.phase, h->n_phases, nslot, red_th, h->st));
+  std::vector<double> hr((size_t)h->n_phases * (nslot + mmax));
+  CK(hipMemcpyAsync(hr.data(), a.red, sizeof(double) * hr.size(), hipMemcpyDeviceToHost, h->st));
+  CK(hipStreamSynchronize(h->st));
+  const double* hx = hr.data();
+  const double* ht = hr.data() + (size_t)h->n_phases * mmax;
+  for (int p = 0; p < h->n_phases; p++) {
+    double* o = theta_bar + (size_t)p * DMK_ADJ_NOUT;
+    for (int k = 0; k < DMK_ADJ_NOUT; k++) o[k] = 0.0;
+    const int m = h->ph[p].nsl;
+    for (int s = 0; s < m; s++) o[s] = hx[(size_t)p * mmax + s];
+    for (int slot = 0; slot < ADJ_TH_SYS; slot++)
+      for (int s = 0; s < m; s++) o[(1 + slot) * DMK_NSL_MAX + s] = ht[(size_t)p * nslot + slot * mmax + s];
+    for (int t = 0; t < DMK_NH_MAX; t++)
+      o[(ADJ_TH_SYS + 1) * DMK_NSL_MAX + t] = ht[(size_t)p * nslot + ADJ_TH_SYS * mmax + t];
+  }
+  return 0;
+}
+
 // ------------------------------------------------------------------ C-ABI
 extern "C" int dmk_device_synchronize(void) {
   CK(hipDeviceSynchronize());
@@ -581,6 +946,7 @@ extern "C" void dmk_mech_destroy(dmk_handle* h) {
   if (h->st) hipStreamSynchronize(h->st);
   if (h->ev_ready)
     for (int k = 0; k < 8; k++) { hipEventDestroy(h->ev[k][0]); hipEventDestroy(h->ev[k][1]); }
+  adj_free(h);
   for (void* p : h->allocs) hipFree(p);
   if (h->ph_dev) hipFree(h->ph_dev);
   if (h->hsmall) hipHostFree(h->hsmall);
@@ -657,6 +1023,7 @@ static int init_impl(dmk_handle* h, const int32_t cells[3], const double size[3]
   if ((rc = dalloc(h, &mp.status, N))) return rc;
   if ((rc = dalloc(h, &mp.n_newton, N))) return rc;
   if ((rc = dalloc(h, &mp.n_fpi, N))) return rc;
+  if ((rc = dalloc(h, &mp.n_sub, N))) return rc;
   if ((rc = dalloc(h, &h->P_hom, n9))) return rc;
   if ((rc = dalloc(h, &h->dPdF, 81 * N))) return rc;
   double** sol[] = {&h->F, &h->F_lastinc, &h->Fdot, &h->R, &h->XM, &h->FM, &h->XA, &h->FA, &h->Fext, &h->Pext};
@@ -1043,6 +1410,7 @@ extern "C" int dmk_grid_run_loadcase(dmk_handle* h, const dmk_loadcase* lc, doub
       nf_total += nf;
       if (conv) {
         if ((rc = dmk_mech_update_coords(h, nullptr, nullptr))) return rc;
+        if (h->adj && (rc = dmk_adjoint_record(h))) return rc;
         h->Delta_t_prev = h->Delta_t;
         h->cutBack = 0;
         guess = 1;

@@ -92,8 +92,15 @@ def loadcase_struct(lc: LoadCase) -> _lib.LoadCase:
     return L
 
 
+# per-phase parameter cotangents of dmk_adjoint_backward: xi_0, n, xi_inf,
+# h_0, a, dot_gamma_0 per slip system (24 each), h_sl-sl per interaction type (32)
+ADJ_NSYS = 24
+ADJ_NOUT = 6 * ADJ_NSYS + 32
+
+
 class GridMechSpectralBasic:
     """One RVE on the GPU behind the spectral-basic plug-in interface."""
+    taping = False
 
     def __init__(self, grid: Grid, material: Material, numerics: Numerics | None = None, cref_rule: int = 1):
         self.lib = _lib.load()
@@ -217,6 +224,38 @@ class GridMechSpectralBasic:
         return {"rc": rc, "t": float(t[0]), "stats": stats.reshape(-1, 4), "P_av": P.reshape(-1, 9),
                 "F_aim": Fa.reshape(-1, 9), "n_updates": nu.value}
 
+    # --- adjoint ------------------------------------------------------------
+    def adjoint_tape(self, max_increments: int) -> None:
+        """Record the current state and, from now on, every converged increment."""
+        _lib.check(self.lib.dmk_adjoint_tape(self.h, int(max_increments)), "dmk_adjoint_tape")
+        self.taping = True
+
+    def adjoint_record(self) -> None:
+        _lib.check(self.lib.dmk_adjoint_record(self.h), "dmk_adjoint_record")
+
+    def adjoint_info(self) -> dict:
+        n = C.c_int32()
+        _lib.check(self.lib.dmk_adjoint_info(self.h, C.byref(n), None, None), "dmk_adjoint_info")
+        P = np.zeros(9 * n.value)
+        dt = np.zeros(n.value)
+        _lib.check(self.lib.dmk_adjoint_info(self.h, C.byref(n), _lib.dptr(P), _lib.dptr(dt)), "dmk_adjoint_info")
+        return {"n": n.value, "P_av": P.reshape(-1, 9), "dt": dt}
+
+    def adjoint_backward(self, P_av_bar: np.ndarray, rtol: float = 1e-12) -> tuple[np.ndarray, np.ndarray]:
+        """Cotangents of the recorded P_av (n, 9 column-major) -> per-phase parameter
+        cotangents (n_phases, 176) and per-increment solver statistics (n, 4)."""
+        n = int(np.asarray(P_av_bar).size // 9)
+        Pb = np.ascontiguousarray(P_av_bar, float).reshape(-1)
+        tb = np.zeros((len(self.material.phases), ADJ_NOUT))
+        st = np.zeros((n, 4))
+        _lib.check(self.lib.dmk_adjoint_backward(self.h, _lib.dptr(Pb), float(rtol), _lib.dptr(tb), _lib.dptr(st)),
+                   "dmk_adjoint_backward")
+        return tb, st
+
+    def adjoint_release(self) -> None:
+        _lib.check(self.lib.dmk_adjoint_release(self.h), "dmk_adjoint_release")
+        self.taping = False
+
     def close(self):
         if getattr(self, "h", None):
             self.lib.dmk_mech_destroy(self.h)
@@ -269,6 +308,8 @@ class DamaskGrid:
                     sol = self.s.solution()
                     if sol.converged:
                         self.coords = self.s.updateCoords()
+                        if self.s.taping:
+                            self.s.adjoint_record()
                         self.Delta_t_prev = self.Delta_t
                         self.cutBack = False
                         guess = True

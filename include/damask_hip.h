@@ -172,6 +172,39 @@ int dmk_mech_counters(dmk_handle* h, int64_t out[4]);
  * xi_sl / gamma_sl results (crystal.f90:1642-1676, 2133-2181). */
 int dmk_slip_system(int32_t lattice, int32_t n_fam, const int32_t* N_sl, int32_t a, double out[6]);
 
+/* ---- Adjoint (parameter gradients of the converged increments).
+ * None of these has a counterpart in the reference: they differentiate
+ * grid_mech_spectral_basic's converged increments (implicit-function theorem
+ * through the material-point increment, src/phase_mechanical.f90:379-676, and
+ * through the spectral equilibrium, spectral_utilities.f90:436-501). */
+
+/* Start a tape: the current state becomes entry 0, then up to max_increments
+ * converged increments are recorded (dmk_grid_run_loadcase records each one
+ * itself; a host driving forward/solution calls dmk_adjoint_record after each
+ * converged increment).  Device memory: (max_increments + 1) (27 + nsl) N
+ * doubles.  A new tape replaces the old one. */
+int dmk_adjoint_tape(dmk_handle* h, int32_t max_increments);
+int dmk_adjoint_record(dmk_handle* h);
+/* recorded increments n, their P_av (9 n, load frame) and time steps (n) */
+int dmk_adjoint_info(dmk_handle* h, int32_t* n, double* P_av, double* dt);
+/* Reverse sweep.  P_av_bar: cotangent of each recorded P_av (9 n).  rtol:
+ * relative tolerance of the adjoint equilibrium solves.  theta_bar
+ * (n_phases x 176): per phase the gradient with respect to xi_0 (24, per
+ * slip system), n, xi_inf, h_0, a, dot_gamma_0 (24 each, per slip system;
+ * sum over a family for the yaml value) and h_sl-sl (32, per interaction
+ * type).  stats (4 n, optional): GMRES iterations, final relative residual,
+ * largest local consistency residual, largest sub-step count. */
+int dmk_adjoint_backward(dmk_handle* h, const double* P_av_bar, double rtol, double* theta_bar, double* stats);
+int dmk_adjoint_release(dmk_handle* h);
+/* The local reverse step of one cell on the host (no GPU): tangent A (81,
+ * dP_p/dF_q at p + 9 q) and the cotangents of F, F_p0, xi_0 and of the
+ * parameters (accumulated into thbar, layout of theta_bar without the xi_0
+ * block) for output cotangents (Pbar, Fpbar, xibar).  3x3 in field order. */
+int dmk_adjoint_point_host(const dmk_phase_desc* d, double dt, const double F[9], const double Fp0[9],
+                           const double* xi0, const double Lp[9], const double* xi, const double Pbar[9],
+                           const double Fpbar[9], const double* xibar, double Fbar[9], double Fp0bar[9],
+                           double* xi0bar, double* thbar, double A[81], double* residual);
+
 void dmk_mech_destroy(dmk_handle* h);
 const char* dmk_last_error(void);
 int dmk_nsl(const dmk_handle* h);

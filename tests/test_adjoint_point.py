@@ -1,0 +1,136 @@
+"""Local reverse step of one converged material-point increment (the
+adjoint's implicit-function-theorem derivatives, run on the host) against
+central differences of the oracle's converged single-cell increment
+(src/phase_mechanical.f90:379-676 restated in oracle/): tangent dP/dF with
+the internal variables re-equilibrated, cotangents of F, F_p0, xi_0 and of
+the phenopowerlaw parameters."""
+import copy
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from conftest import bcc_golden_inputs, fcc_golden_inputs
+
+NS = 24          # per-system slot stride of the parameter cotangents
+DT = 0.5
+
+
+def _phase(lattice):
+    if lattice == "cF":
+        _, mat, _ = fcc_golden_inputs()
+    else:
+        _, mat, _, _ = bcc_golden_inputs()
+    p = copy.deepcopy(mat.phases[0])
+    p.atol_xi, p.atol_gamma = 1e-10, 1e-16       # converge the oracle's state far below the checks
+    return p
+
+
+class _Cell:
+    """one oracle cell with tight tolerances, restarted from a fixed state"""
+
+    def __init__(self, p, q):
+        from oracle import pyoracle as po
+        self.po, self.O = po, po.lib()
+        n = po.num_mech()
+        n.eps_rel_state, n.eps_rel_Lp, n.eps_abs_Lp = 1e-14, 1e-11, 1e-16
+        n.N_iter_Lp_max = n.N_iter_state_max = 400
+        self.n = n
+        self.ph = (po.Phase * 1)(po.build_phase(p))
+        self.phase_of = np.zeros(1, np.int32)
+        self.mp = self.O.orc_mp_create(1, 1, self.ph, po.iptr(self.phase_of), po.ptr(q.copy()), C.byref(n))
+        self.m = p.nsl
+
+    def close(self):
+        self.O.orc_mp_destroy(self.mp)
+
+    def get(self, what, w=9):
+        o = np.zeros(w)
+        self.O.orc_mp_get(self.mp, what, self.po.ptr(o))
+        return o
+
+    def step(self, F, Fp0=None, st0=None):
+        po = self.po
+        if Fp0 is not None:
+            self.O.orc_mp_set(self.mp, 10, po.ptr(Fp0.copy()))
+            self.O.orc_mp_set(self.mp, 15, po.ptr(st0.copy()))
+        P = np.zeros(9)
+        assert self.O.orc_mp_response(self.mp, DT, po.ptr(po.f33(F)), po.ptr(P), None) == 0
+        return P, self.get(3), self.get(8, self.m), self.get(4)
+
+
+@pytest.mark.parametrize("lattice,tol", [("cF", 1e-6), ("cI", 5e-6)])
+def test_point_adjoint_against_oracle_differences(lattice, tol):
+    from damask_amd import _lib
+    from damask_amd.grid import phase_desc
+    p = _phase(lattice)
+    m = p.nsl
+    q = np.array([0.8, 0.3, -0.4, 0.33])
+    q /= np.linalg.norm(q)
+    F1 = np.eye(3) + np.array([[2.0e-3, 3e-4, 1e-4], [-2e-4, -0.9e-3, 2e-4], [1e-4, -1e-4, -0.8e-3]])
+    F2 = F1 + np.array([[1.6e-3, 1e-4, -2e-4], [3e-4, -0.7e-3, 1e-4], [-1e-4, 2e-4, -0.6e-3]])
+    cell = _Cell(p, q)
+    cell.step(F1)
+    cell.O.orc_mp_forward(cell.mp)
+    Fp0, st0 = cell.get(10), cell.get(15, 2 * m)
+    P2, Fp2, xi2, Lp2 = cell.step(F2, Fp0, st0)
+    xi0 = st0[:m].copy()
+    assert np.linalg.norm(Lp2) > 1e-3          # a plastic increment
+
+    rng = np.random.default_rng(1)
+    Pbar, Fpbar, xibar = rng.standard_normal(9), rng.standard_normal(9) * 1e9, rng.standard_normal(m) * 1e2
+    L = _lib.load()
+    desc = phase_desc(p)
+    Fb, Fp0b, x0b = np.zeros(9), np.zeros(9), np.zeros(m)
+    th, A, res = np.zeros(6 * NS + 32), np.zeros(81), C.c_double()
+    d = _lib.dptr
+    po = cell.po
+    _lib.check(L.dmk_adjoint_point_host(C.byref(desc), C.c_double(DT), d(po.f33(F2)), d(Fp0), d(xi0), d(Lp2), d(xi2),
+                                        d(Pbar), d(Fpbar), d(xibar), d(Fb), d(Fp0b), d(x0b), d(th), d(A), C.byref(res)),
+               "dmk_adjoint_point_host")
+    assert res.value < 1e-9                     # the recorded state solves the increment equations
+
+    def J(o):
+        return Pbar @ o[0] + Fpbar @ o[1] + xibar @ o[2]
+
+    h = 1e-7
+    Afd, Ffd, Gfd = np.zeros((9, 9)), np.zeros(9), np.zeros(9)
+    for c in range(9):
+        E = np.zeros((3, 3))
+        E[c % 3, c // 3] = h
+        op, om = cell.step(F2 + E, Fp0, st0), cell.step(F2 - E, Fp0, st0)
+        Afd[:, c] = (op[0] - om[0]) / (2 * h)
+        Ffd[c] = (J(op) - J(om)) / (2 * h)
+        e = np.zeros(9)
+        e[c] = h
+        Gfd[c] = (J(cell.step(F2, Fp0 + e, st0)) - J(cell.step(F2, Fp0 - e, st0))) / (2 * h)
+    X0fd = np.zeros(m)
+    for a in range(m):
+        e = np.zeros(2 * m)
+        e[a] = 1e-3 * xi0[a]
+        X0fd[a] = (J(cell.step(F2, Fp0, st0 + e)) - J(cell.step(F2, Fp0, st0 - e))) / (2 * e[a])
+    cell.close()
+    rel = lambda a, b: np.abs(a - b).max() / np.abs(b).max()
+    assert rel(A.reshape(9, 9, order="F"), Afd) < tol
+    assert rel(Fb, Ffd) < tol
+    assert rel(Fp0b, Gfd) < tol
+    assert rel(x0b, X0fd) < tol
+
+    fam = np.repeat(np.arange(len(p.N_sl)), p.N_sl)
+
+    def fd_param(name, idx, rel_step=1e-4):
+        out = []
+        for s in (1, -1):
+            pp = copy.deepcopy(p)
+            getattr(pp, name)[idx] *= 1 + s * rel_step
+            c2 = _Cell(pp, q)
+            out.append(J(c2.step(F2, Fp0, st0)))
+            c2.close()
+        return (out[0] - out[1]) / (2 * rel_step * getattr(p, name)[idx])
+
+    for name, slot in (("n_sl", 0), ("xi_inf_sl", 1), ("h_0_sl_sl", 2), ("a_sl", 3), ("dot_gamma_0_sl", 4)):
+        ad = th[slot * NS: slot * NS + m][fam == 0].sum()
+        assert abs(ad - fd_param(name, 0)) < 10 * tol * abs(ad), name
+    gmax = np.abs(th[5 * NS: 5 * NS + len(p.h_sl_sl)]).max()
+    for t in range(len(p.h_sl_sl)):
+        assert abs(th[5 * NS + t] - fd_param("h_sl_sl", t)) < 10 * tol * gmax, t

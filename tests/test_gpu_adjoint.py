@@ -1,0 +1,82 @@
+"""Adjoint parameter gradients of the spectral-basic run (HIP path through the
+C-ABI) against central finite differences of the same HIP forward run, and
+the torch.autograd front end against the numpy entry point."""
+import copy
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _case(cells=8, grains=4, seed=3, t=8.0, N=5, lattice="cF"):
+    from damask_amd import config, workloads
+    grid, mat = workloads.polycrystal([cells] * 3, grains, seed)
+    mat = config.parse_material(mat)
+    mat.phases[0].atol_xi, mat.phases[0].atol_gamma = 1e-8, 1e-14
+    _, lcs = config.parse_load(workloads.tension_x(t, N))
+    num = config.Numerics(eps_abs_div=1e-8, eps_rel_div=1e-11, eps_abs_P=1e-3, eps_rel_P=1e-11,
+                          N_iter_max=400, snes_max_it=400, eps_rel_state=1e-13, N_iter_state_max=200,
+                          eps_rel_Lp=1e-11, eps_abs_Lp=1e-16, N_iter_Lp_max=200)
+    return grid, mat, lcs, num
+
+
+def _loss(P, w):
+    # weighted sum over the curve of P_xx plus a small P_xy share
+    return float(np.sum(w * P[:, 0, 0]) + 0.1 * np.sum(w * P[:, 0, 1]))
+
+
+def _Pbar(n, w):
+    Pb = np.zeros((n, 3, 3))
+    Pb[:, 0, 0] = w
+    Pb[:, 0, 1] = 0.1 * w
+    return Pb
+
+
+def test_adjoint_gradient_matches_central_differences(gpu_lib):
+    from damask_amd import adjoint
+    grid, mat, lcs, num = _case()
+    s, P = adjoint.run_curve(grid, mat, lcs, num)
+    n = P.shape[0]
+    assert n == lcs[0].N
+    w = np.linspace(1.0, 2.0, n) / 1e6
+    g, st = adjoint.curve_gradient(s, 0, _Pbar(n, w))
+    s.close()
+    print("adjoint stats (its, rel, consistency, substeps):\n", st)
+    assert np.all(st[:, 1] < 1e-10), st
+    theta0 = adjoint.get_params(mat.phases[0])
+    names = adjoint.param_names(mat.phases[0])
+    fd = np.zeros_like(theta0)
+    for k in range(theta0.size):
+        vals = []
+        for sgn in (1.0, -1.0):
+            th = theta0.copy()
+            th[k] *= 1.0 + sgn * 1e-4
+            m2 = copy.deepcopy(mat)
+            m2.phases[0] = adjoint.set_params(mat.phases[0], th)
+            s2, P2 = adjoint.run_curve(grid, m2, lcs, num)
+            s2.close()
+            vals.append(_loss(P2, w))
+        fd[k] = (vals[0] - vals[1]) / (2e-4 * theta0[k])
+    for k in range(theta0.size):
+        print(f"{names[k]:12s} adjoint {g[k]: .10e}  fd {fd[k]: .10e}  rel {abs(g[k] - fd[k]) / abs(fd[k]):.2e}")
+    scale = np.abs(fd * theta0).max()
+    assert np.all(np.abs(g - fd) * np.abs(theta0) <= 1e-5 * np.abs(fd * theta0) + 1e-7 * scale)
+
+
+def test_autograd_curve_loss(gpu_lib):
+    torch = pytest.importorskip("torch")
+    from damask_amd import adjoint
+    grid, mat, lcs, num = _case(N=3)
+    theta = torch.tensor(adjoint.get_params(mat.phases[0]), dtype=torch.float64, requires_grad=True)
+    P = adjoint.stress_strain_curve(theta, grid, mat, lcs, num)
+    target = P.detach() * 1.01
+    loss = ((P[:, 0, 0] - target[:, 0, 0]) ** 2).mean()
+    loss.backward()
+    s, P2 = adjoint.run_curve(grid, mat, lcs, num)
+    assert np.allclose(P2, P.detach().numpy(), rtol=0, atol=1e-6 * np.abs(P2).max())
+    Pb = np.zeros_like(P2)
+    Pb[:, 0, 0] = 2.0 * (P2[:, 0, 0] - target[:, 0, 0].numpy()) / P2.shape[0]
+    g, _ = adjoint.curve_gradient(s, 0, Pb)
+    s.close()
+    assert np.allclose(theta.grad.numpy(), g, rtol=1e-8, atol=1e-12 * np.abs(g).max())
