@@ -46,6 +46,10 @@ def parse():
     ap.add_argument("--cpu-increments", type=int, default=12)
     ap.add_argument("--traffic-bytes", type=float, default=STRESS_TRAFFIC_BYTES,
                     help="measured HBM bytes per stress-kernel launch (rocprofv3 --pmc); 0 for none")
+    ap.add_argument("--adjoint", type=int, default=1,
+                    help="also measure adjoint / forward cost on the 32^3 differentiation workload")
+    ap.add_argument("--adjoint-rtol", type=float, default=1e-6,
+                    help="adjoint GMRES tolerance (1e-6: gradient within 2e-7 of the 1e-12 solve at 32^3)")
     ap.add_argument("--copy-calibration", type=int, default=0,
                     help="only run a plain device copy of this many MiB (counter calibration)")
     return ap.parse_args()
@@ -79,6 +83,45 @@ def sub_loadcase(lc, n):
     s.t = lc.t / lc.N * n
     s.N = n
     return s
+
+
+def adjoint_cost(args):
+    """Wall time of forward + adjoint gradient over a plain forward: 32^3 cells,
+    50 grains, uniaxial tension 60 increments to 3 %, loss = mean squared
+    deviation of the P_xx curve from a target curve (P_xx * 1.01)."""
+    import numpy as np
+    from damask_amd import adjoint, config
+    from damask_amd.grid import GridMechSpectralBasic
+    grid, mat, lc = workload(32, 50, args.seed, 60, 0.03)
+    num = config.Numerics()
+    s = GridMechSpectralBasic(grid, mat, num)
+    t0 = time.perf_counter()
+    r = s.run_loadcase(lc)
+    t_fwd = time.perf_counter() - t0
+    assert r["rc"] == 0
+    s.close()
+    s = GridMechSpectralBasic(grid, mat, num)
+    s.adjoint_tape(2 * lc.N + 16)
+    t0 = time.perf_counter()
+    r = s.run_loadcase(lc)
+    t_fwd_tape = time.perf_counter() - t0
+    assert r["rc"] == 0
+    info = s.adjoint_info()
+    P = adjoint._P33(info["P_av"])
+    n = P.shape[0]
+    Pb = np.zeros_like(P)
+    Pb[:, 0, 0] = 2.0 * (P[:, 0, 0] - 1.01 * P[:, 0, 0]) / n
+    t0 = time.perf_counter()
+    g, st = adjoint.curve_gradient(s, 0, Pb, args.adjoint_rtol)
+    t_bwd = time.perf_counter() - t0
+    s.close()
+    return {"workload": "32^3 FCC phenopowerlaw, 50 grains, tension 60 increments to 3 %, gradient of the "
+                        "P_xx-curve MSE w.r.t. (xi_0, xi_inf, h_0, n, a, h_sl-sl[7])",
+            "forward_s": t_fwd, "forward_with_tape_s": t_fwd_tape, "backward_s": t_bwd,
+            "adjoint_over_forward": (t_fwd_tape + t_bwd) / t_fwd, "increments": n,
+            "gmres_iterations_per_increment": float(st[:, 0].mean()), "gmres_rtol": args.adjoint_rtol,
+            "max_gmres_relative_residual": float(st[:, 1].max()),
+            "gradient_norm": float(np.linalg.norm(g))}
 
 
 def cpu_baseline(args):
@@ -170,6 +213,8 @@ def main():
                      "flops_per_launch": fl / max(stress_n, 1),
                      "avg_launch_ms": stress_ms / max(stress_n, 1)},
     }
+    if args.adjoint:
+        line["adjoint"] = adjoint_cost(args)
     if args.cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(args)
     else:

@@ -31,47 +31,185 @@ __device__ __forceinline__ void store_rm(double* f, long N, int e, const double*
     for (int j = 0; j < 3; j++) f[(size_t)(i + 3 * j) * N + e] = T[3 * i + j];
 }
 
-__global__ void __launch_bounds__(ADJ_WG) adj_local_kernel(AdjKArgs a, const DmkPhaseDev* __restrict__ php) {
+// scratch regions: per-system temporaries [AV_N mmax][N], then per cell the
+// matrix block (nmax^2 + nmax) and the 9 tangent right-hand sides (9 nmax)
+struct AdjScr {
+  double *av, *g, *x;
+  int nmax, blk;
+};
+__device__ __forceinline__ AdjScr adj_scr(const AdjKArgs& a) {
+  AdjScr r;
+  r.nmax = 9 + a.mmax;
+  r.blk = r.nmax * r.nmax + r.nmax;
+  r.av = a.scr;
+  r.g = a.scr + (size_t)AV_N * a.mmax * a.N;
+  r.x = r.g + (size_t)r.blk * a.N;
+  return r;
+}
+
+__device__ __forceinline__ void load_point(const AdjKArgs& a, int e, AdjPoint& pt) {
+  load_rm(a.F, a.N, e, pt.F);
+  load_rm(a.Fp0, a.N, e, pt.Fp0);
+  load_rm(a.Lp, a.N, e, pt.Lp);
+  pt.dt = a.dt;
+}
+
+// K1 (per cell): forward intermediates, per-system temporaries, consistency
+// residual, Jacobian G_y and the 9 tangent right-hand sides
+__global__ void __launch_bounds__(ADJ_WG) adj_assemble_kernel(AdjKArgs a, const DmkPhaseDev* __restrict__ php) {
   const int e = blockIdx.x * ADJ_WG + threadIdx.x;
   if (e >= a.N || a.phase[e] != a.phase_id) return;
   const DmkPhaseDev& ph = *php;
-  const long N = a.N;
+  const AdjScr sc = adj_scr(a);
   AdjPoint pt;
-  load_rm(a.F, N, e, pt.F);
-  load_rm(a.Fp0, N, e, pt.Fp0);
-  load_rm(a.Lp, N, e, pt.Lp);
-  pt.dt = a.dt;
-  AdjAcc acc{(size_t)N, (size_t)e, 9 + a.mmax, a.mmax, a.scr};
-  AdjVec xi{(double*)a.xi, (size_t)N, (size_t)e}, xi0{(double*)a.xi0, (size_t)N, (size_t)e};
-  AdjVec xib{(double*)a.xib_in, (size_t)N, (size_t)e};
+  load_point(a, e, pt);
+  AdjAcc acc{sc.av, (size_t)a.N, (size_t)e, sc.g + (size_t)e * sc.blk, sc.nmax, a.mmax};
+  AdjVec xi{(double*)a.xi, (size_t)a.N, (size_t)e}, xi0{(double*)a.xi0, (size_t)a.N, (size_t)e};
   AdjFwd f;
   if (adj_forward(ph, pt, xi, f, acc)) {
-    if (a.mode == 0) a.res[e] = HUGE_VAL;
+    a.res[e] = HUGE_VAL;
     return;
   }
-  double Pb[9], Fpb[9], Fb[9];
+  a.res[e] = adj_residual(ph, pt, xi, xi0, acc, a.eps_rel_Lp, a.eps_abs_Lp, a.eps_rel_state);
+  adj_assemble(ph, pt, xi, acc);
+  double* X = sc.x + (size_t)e * 9 * sc.nmax;
+  for (int q = 0; q < 9; q++) {
+    double x[9 + DMK_NSL_MAX];
+    adj_tangent_rhs(ph, pt, f, acc, q, x);
+    for (int i = 0; i < 9 + ph.nsl; i++) X[q * sc.nmax + i] = x[i];
+  }
+}
+
+// K2 (16 lanes per cell, 4 cells per wave): LU factorisation of G_y with
+// partial pivoting in LDS, the 9 tangent right-hand sides carried along as
+// extra columns, then back substitution.  Pivot choice, operation order and
+// rounding per entry are those of adj_lu / adj_solve (first largest pivot).
+#define COOP_CELLS 4
+#define COOP_LANES 16
+#define COOP_NC (9 + DMK_NSL_MAX + 9)   // columns: G_y, then the 9 right-hand sides
+#define COOP_LS (COOP_NC + 1)           // LDS row stride
+__global__ void __launch_bounds__(COOP_CELLS * COOP_LANES) adj_lu_kernel(AdjKArgs a) {
+  __shared__ double Gs[COOP_CELLS][(9 + DMK_NSL_MAX) * COOP_LS];
+  __shared__ int pv[COOP_CELLS][9 + DMK_NSL_MAX];
+  const int g = threadIdx.x / COOP_LANES, l = threadIdx.x % COOP_LANES;
+  const int e = blockIdx.x * COOP_CELLS + g;
+  const bool act = e < a.N && a.phase[e] == a.phase_id;
+  const int n = 9 + a.nsl, nc = n + 9;
+  const AdjScr sc = adj_scr(a);
+  double* Gg = sc.g + (size_t)(act ? e : 0) * sc.blk;
+  double* Xg = sc.x + (size_t)(act ? e : 0) * 9 * sc.nmax;
+  double* G = Gs[g];
+  if (act) {
+    for (int i = 0; i < n; i++) {
+      for (int j = l; j < n; j += COOP_LANES) G[i * COOP_LS + j] = Gg[i * sc.nmax + j];
+      if (l < 9) G[i * COOP_LS + n + l] = Xg[l * sc.nmax + i];
+    }
+  }
+  bool sing = false;
+  __syncthreads();
+  for (int j = 0; j < n; j++) {
+    double best = -1.0;
+    int bi = j;
+    if (act)
+      for (int i = j + l; i < n; i += COOP_LANES) {
+        double v = fabs(G[i * COOP_LS + j]);
+        if (v > best) { best = v; bi = i; }
+      }
+    for (int off = COOP_LANES / 2; off > 0; off >>= 1) {
+      double ob = __shfl_xor(best, off, COOP_LANES);
+      int oi = __shfl_xor(bi, off, COOP_LANES);
+      if (ob > best || (ob == best && oi < bi)) { best = ob; bi = oi; }
+    }
+    const int p = bi;
+    if (act && best == 0.0) sing = true;
+    if (l == 0) pv[g][j] = p;
+    if (act && !sing && p != j)
+      for (int k = l; k < nc; k += COOP_LANES) {
+        double t = G[j * COOP_LS + k];
+        G[j * COOP_LS + k] = G[p * COOP_LS + k];
+        G[p * COOP_LS + k] = t;
+      }
+    __syncthreads();
+    if (act && !sing) {
+      double r = 1.0 / G[j * COOP_LS + j];
+      for (int i = j + 1 + l; i < n; i += COOP_LANES) G[i * COOP_LS + j] *= r;
+    }
+    __syncthreads();
+    // trailing update; each lane owns columns k (matrix and right-hand sides)
+    if (act && !sing)
+      for (int k = j + 1 + l; k < nc; k += COOP_LANES) {
+        const double gjk = G[j * COOP_LS + k];
+        for (int i = j + 1; i < n; i++) G[i * COOP_LS + k] = fma(-G[i * COOP_LS + j], gjk, G[i * COOP_LS + k]);
+      }
+    __syncthreads();
+  }
+  // back substitution of the 9 right-hand sides (column n + d)
+  for (int j = n - 1; j >= 0; j--) {
+    if (act && !sing && l < 9) G[j * COOP_LS + n + l] /= G[j * COOP_LS + j];
+    __syncthreads();
+    if (act && !sing)
+      for (int t = l; t < 9 * j; t += COOP_LANES) {
+        const int d = t % 9, i = t / 9;
+        G[i * COOP_LS + n + d] = fma(-G[i * COOP_LS + j], G[j * COOP_LS + n + d], G[i * COOP_LS + n + d]);
+      }
+    __syncthreads();
+  }
+  if (!act) return;
+  if (sing) {
+    if (l == 0) a.res[e] = HUGE_VAL;
+    return;
+  }
+  for (int i = 0; i < n; i++) {
+    for (int j = l; j < n; j += COOP_LANES) Gg[i * sc.nmax + j] = G[i * COOP_LS + j];
+    if (l < 9) Xg[l * sc.nmax + i] = G[i * COOP_LS + n + l];
+  }
+  for (int j = l; j < n; j += COOP_LANES) Gg[sc.nmax * sc.nmax + j] = (double)pv[g][j];
+}
+
+// K3 (per cell).  mode 0: tangent A from the 9 solutions, F cotangent g for
+// the output cotangent (Pu, Fpb_in, xib_in).  mode 1: cotangents of F_p0,
+// xi_0 and the parameters for (Pu - mu, Fpb_in, xib_in).  Both reuse the LU
+// and the per-system temporaries of K1/K2 of the same increment.
+__global__ void __launch_bounds__(ADJ_WG) adj_vjp_kernel(AdjKArgs a, const DmkPhaseDev* __restrict__ php) {
+  const int e = blockIdx.x * ADJ_WG + threadIdx.x;
+  if (e >= a.N || a.phase[e] != a.phase_id) return;
+  if (!(a.res[e] < HUGE_VAL)) return;
+  const DmkPhaseDev& ph = *php;
+  const long N = a.N;
+  const AdjScr sc = adj_scr(a);
+  AdjPoint pt;
+  load_point(a, e, pt);
+  AdjAcc acc{sc.av, (size_t)N, (size_t)e, sc.g + (size_t)e * sc.blk, sc.nmax, a.mmax};
+  AdjVec xi{(double*)a.xi, (size_t)N, (size_t)e};
+  AdjVec xib{(double*)a.xib_in, (size_t)N, (size_t)e};
+  AdjFwd f;
+  adj_fwd33(ph, pt, f);
+  if (a.mode == 0) {
+    const double* X = sc.x + (size_t)e * 9 * sc.nmax;
+    double A[81];
+    for (int q = 0; q < 9; q++) {
+      double x[9];
+      for (int k = 0; k < 9; k++) x[k] = X[q * sc.nmax + k];
+      adj_tangent_col(ph, pt, f, q, x, A);
+    }
+    for (int k = 0; k < 81; k++) a.A[(size_t)k * N + e] = A[k];
+  }
+  double Pb[9], Pr[9], Fpb[9], Fb[9], Fp0ib[9], lam[9 + DMK_NSL_MAX];
   for (int k = 0; k < 9; k++) Pb[k] = a.Pu[k];
   if (a.mode == 1 && a.mu)
     for (int k = 0; k < 9; k++) Pb[k] -= a.mu[(size_t)k * N + e];
-  double Pr[9];
   fld_to_rm(Pb, Pr);
   load_rm(a.Fpb_in, N, e, Fpb);
+  adj_vjp_rhs(ph, pt, f, Pr, Fpb, xib, lam, Fb, Fp0ib);
+  adj_solve_t(acc, 9 + ph.nsl, lam);
   if (a.mode == 0) {
-    a.res[e] = adj_residual(ph, pt, xi, xi0, acc);
-    if (adj_factor(ph, pt, xi, acc)) {
-      a.res[e] = HUGE_VAL;
-      return;
-    }
-    double A[81];
-    adj_tangent(ph, pt, f, acc, A);
-    for (int k = 0; k < 81; k++) a.A[(size_t)k * N + e] = A[k];
     AdjVec none{nullptr, 0, 0};
-    adj_vjp(ph, pt, f, xi, acc, Pr, Fpb, xib, Fb, false, nullptr, none, none, 0);
+    adj_vjp_finish(ph, pt, f, xi, acc, lam, Fb, Fp0ib, false, nullptr, none, none, 0);
     store_rm(a.g, N, e, Fb);
   } else {
     double Fp0b[9];
     AdjVec xi0b{a.xib_out, (size_t)N, (size_t)e}, thb{a.thb, (size_t)N, (size_t)e};
-    adj_vjp(ph, pt, f, xi, acc, Pr, Fpb, xib, Fb, true, Fp0b, xi0b, thb, a.mmax);
+    adj_vjp_finish(ph, pt, f, xi, acc, lam, Fb, Fp0ib, true, Fp0b, xi0b, thb, a.mmax);
     store_rm(a.Fpb_out, N, e, Fp0b);
   }
 }
@@ -165,8 +303,14 @@ __global__ void adj_gs_kernel(long n, double* __restrict__ w, const double* __re
 }
 
 int dmk_adj_local(const AdjKArgs* a, const DmkPhaseDev* ph, hipStream_t s) {
+  if (a->nsl < 1 || a->nsl > a->mmax || a->mmax > DMK_NSL_MAX) return 1;
   unsigned nb = (unsigned)((a->N + ADJ_WG - 1) / ADJ_WG);
-  hipLaunchKernelGGL(adj_local_kernel, dim3(nb), dim3(ADJ_WG), 0, s, *a, ph);
+  if (a->mode == 0) {
+    hipLaunchKernelGGL(adj_assemble_kernel, dim3(nb), dim3(ADJ_WG), 0, s, *a, ph);
+    unsigned nc = (unsigned)((a->N + COOP_CELLS - 1) / COOP_CELLS);
+    hipLaunchKernelGGL(adj_lu_kernel, dim3(nc), dim3(COOP_CELLS * COOP_LANES), 0, s, *a);
+  }
+  hipLaunchKernelGGL(adj_vjp_kernel, dim3(nb), dim3(ADJ_WG), 0, s, *a, ph);
   return hipGetLastError() == hipSuccess ? 0 : 2;
 }
 
@@ -198,7 +342,8 @@ int dmk_adj_gs(long n, double* w, const double* V, long ldV, const GsArgs* c, hi
 }
 
 // The local reverse step of one cell on the host (CPU checks of the
-// derivatives; no GPU involved).  3x3 arguments in field order (i + 3 j),
+// derivatives; no GPU involved).  The consistency residual uses the default
+// tolerances (eps_rel_Lp 1e-6, eps_abs_Lp 1e-8, eps_rel_state 1e-6).  3x3 arguments in field order (i + 3 j),
 // per-system arrays of length nsl.  thbar: per-system slots
 // (slot * DMK_NSL_MAX + system) then interaction types, accumulated.
 extern "C" int dmk_adjoint_point_host(const dmk_phase_desc* d, double dt, const double F[9], const double Fp0[9],
@@ -209,8 +354,8 @@ extern "C" int dmk_adjoint_point_host(const dmk_phase_desc* d, double dt, const 
   static DmkPhaseDev ph;
   if (!d || dmk_build_phase(d, &ph)) return DMK_ERR_CONFIG;
   const int m = ph.nsl, mmax = DMK_NSL_MAX;
-  double scr[(9 + DMK_NSL_MAX) * (9 + DMK_NSL_MAX) + 9 + DMK_NSL_MAX + AV_N * DMK_NSL_MAX];
-  AdjAcc acc{1, 0, 9 + mmax, mmax, scr};
+  double gblk[(9 + DMK_NSL_MAX) * (9 + DMK_NSL_MAX) + 9 + DMK_NSL_MAX], av[AV_N * DMK_NSL_MAX];
+  AdjAcc acc{av, 1, 0, gblk, 9 + mmax, mmax};
   double xiv[DMK_NSL_MAX], xi0v[DMK_NSL_MAX], xibv[DMK_NSL_MAX];
   for (int a = 0; a < m; a++) { xiv[a] = xi[a]; xi0v[a] = xi0[a]; xibv[a] = xibar ? xibar[a] : 0.0; }
   AdjVec X{xiv, 1, 0}, X0{xi0v, 1, 0}, XB{xibv, 1, 0};
@@ -221,7 +366,7 @@ extern "C" int dmk_adjoint_point_host(const dmk_phase_desc* d, double dt, const 
   pt.dt = dt;
   AdjFwd f;
   if (adj_forward(ph, pt, X, f, acc)) return DMK_ERR_ARG;
-  if (residual) *residual = adj_residual(ph, pt, X, X0, acc);
+  if (residual) *residual = adj_residual(ph, pt, X, X0, acc, 1.0e-6, 1.0e-8, 1.0e-6);
   if (adj_factor(ph, pt, X, acc)) return DMK_ERR_ARG;
   if (A) adj_tangent(ph, pt, f, acc, A);
   double Pr[9], Fpr[9], Fb[9], Fp0b[9], x0b[DMK_NSL_MAX];

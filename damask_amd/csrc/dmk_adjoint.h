@@ -30,17 +30,19 @@
 #include "dmk_math.h"
 #include "dmk_adjoint_k.h"
 
-// Per-cell scratch and strided per-system arrays.  Device: component-major
-// (slot * N + e); host check: N = 1, e = 0.
+// Per-cell storage: per-system temporaries av ([slot][avN] at cell e) and the
+// cell's matrix block g (G_y row-major with row stride nmax, then the pivots).
+// Device: av component-major over the cells, g contiguous per cell; host
+// check: avN = 1, e = 0.
 struct AdjAcc {
-  size_t N, e;
+  double* av;
+  size_t avN, e;
+  double* g;
   int nmax, mmax;       // nmax = 9 + mmax
-  double* scr;
-  DHI double& G(int i, int j) const { return scr[((size_t)i * nmax + j) * N + e]; }
-  DHI double& piv(int i) const { return scr[((size_t)nmax * nmax + i) * N + e]; }
-  DHI double& v(int k, int a) const { return scr[((size_t)nmax * nmax + nmax + (size_t)k * mmax + a) * N + e]; }
+  DHI double& G(int i, int j) const { return g[(size_t)i * nmax + j]; }
+  DHI double& piv(int i) const { return g[(size_t)nmax * nmax + i]; }
+  DHI double& v(int k, int a) const { return av[((size_t)k * mmax + a) * avN + e]; }
 };
-
 
 // strided per-system vector (xi, xi_0 and their cotangents)
 struct AdjVec {
@@ -72,9 +74,8 @@ DHI void adj_hooke(const DmkPhaseDev& ph, const double* X, double* S) {
   hooke_sym(ph.C11, ph.C12, ph.C44, Y, S);
 }
 
-// forward intermediates and per-system temporaries at the converged y;
-// returns 1 when F_p0 is singular
-DHI int adj_forward(const DmkPhaseDev& ph, const AdjPoint& pt, const AdjVec& xi, AdjFwd& f, const AdjAcc& acc) {
+// 3x3 forward intermediates at the converged y; returns 1 when F_p0 is singular
+DHI int adj_fwd33(const DmkPhaseDev& ph, const AdjPoint& pt, AdjFwd& f) {
   if (m_invert(pt.Fp0, f.Fp0i)) return 1;
   m_mul(pt.F, f.Fp0i, f.A0);
   for (int k = 0; k < 9; k++) f.B[k] = -pt.dt * pt.Lp[k];
@@ -87,6 +88,12 @@ DHI int adj_forward(const DmkPhaseDev& ph, const AdjPoint& pt, const AdjVec& xi,
   hooke_sym(ph.C11, ph.C12, ph.C44, E, f.S);
   m_mul_tn(f.A0, f.Fe, f.W);
   m_mul(f.Fp0i, f.B, f.M);
+  return 0;
+}
+
+// per-system temporaries (rates, their derivatives, hardening factors)
+DHI void adj_fwd_sys(const DmkPhaseDev& ph, const AdjPoint& pt, const AdjVec& xi, const AdjFwd& f,
+                     const AdjAcc& acc) {
   const int m = ph.nsl;
   for (int a = 0; a < m; a++) {
     double tp = adj_dot9(f.S, ph.Pp[a]), tn = adj_dot9(f.S, ph.Pm[a]);
@@ -114,13 +121,22 @@ DHI int adj_forward(const DmkPhaseDev& ph, const AdjPoint& pt, const AdjVec& xi,
     for (int b = 0; b < m; b++) r = fma(ph.H[a][b], fabs(acc.v(AV_GD, b)), r);
     acc.v(AV_R, a) = r;
   }
+}
+
+DHI int adj_forward(const DmkPhaseDev& ph, const AdjPoint& pt, const AdjVec& xi, AdjFwd& f, const AdjAcc& acc) {
+  if (adj_fwd33(ph, pt, f)) return 1;
+  adj_fwd_sys(ph, pt, xi, f, acc);
   return 0;
 }
 
-// residual of the increment equations at the recorded y (consistency check):
-// max(|G1| / max(|L_p|, |L_p,c|), max_a |G2_a| / xi_a)
+// residual of the increment equations at the recorded y in units of the
+// reference's convergence tolerances (consistency check): max of
+// |G1| / max(eps_rel_Lp max(|L_p|, |L_p,c|), eps_abs_Lp)   (phase_mechanical.f90:440-445)
+// |G2_a| / max(atol_xi, eps_rel_state |xi_a|)              (phase.f90:703-716)
+// The state is one fixed-point update past the stress solve, so values of a
+// few units are normal.
 DHI double adj_residual(const DmkPhaseDev& ph, const AdjPoint& pt, const AdjVec& xi, const AdjVec& xi0,
-                        const AdjAcc& acc) {
+                        const AdjAcc& acc, double eps_rel_Lp, double eps_abs_Lp, double eps_rel_state) {
   const int m = ph.nsl;
   double Lc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
   for (int a = 0; a < m; a++)
@@ -131,19 +147,17 @@ DHI double adj_residual(const DmkPhaseDev& ph, const AdjPoint& pt, const AdjVec&
     nl += pt.Lp[k] * pt.Lp[k];
     nc += Lc[k] * Lc[k];
   }
-  double den = sqrt(fmax(nl, nc));
-  double res = den > 0.0 ? sqrt(r1) / den : sqrt(r1);
+  double res = sqrt(r1) / fmax(eps_rel_Lp * sqrt(fmax(nl, nc)), eps_abs_Lp);
   for (int a = 0; a < m; a++) {
     double g2 = xi(a) - xi0(a) - pt.dt * ph.hf[a] * acc.v(AV_PHI, a) * acc.v(AV_R, a);
-    res = fmax(res, fabs(g2) / fabs(xi(a)));
+    res = fmax(res, fabs(g2) / fmax(ph.atol_xi, eps_rel_state * fabs(xi(a))));
   }
   return res;
 }
 
-// Jacobian G_y, LU-factorised in place (row interchanges in piv);
-// returns 1 for an exactly zero pivot
-DHI int adj_factor(const DmkPhaseDev& ph, const AdjPoint& pt, const AdjVec& xi, const AdjAcc& acc) {
-  const int m = ph.nsl, n = 9 + m;
+// Jacobian G_y of the increment equations
+DHI void adj_assemble(const DmkPhaseDev& ph, const AdjPoint& pt, const AdjVec& xi, const AdjAcc& acc) {
+  const int m = ph.nsl;
   const double dt = pt.dt;
   // d G1 / d L_p = I - sum_b P_b kt_b T_b^T ; d G1 / d xi_b = P_b n_b gdot_b / xi_b
   for (int k = 0; k < 9; k++) {
@@ -172,6 +186,11 @@ DHI int adj_factor(const DmkPhaseDev& ph, const AdjPoint& pt, const AdjVec& xi, 
       acc.G(9 + a, 9 + b) = (a == b ? 1.0 - dt * ph.hf[a] * acc.v(AV_DPHI, a) * acc.v(AV_R, a) : 0.0)
                           - ca * ph.H[a][b] * ph.n[b] * fabs(acc.v(AV_GD, b)) / xi(b);
   }
+}
+
+// LU factorisation with partial pivoting in place (row interchanges in piv;
+// the first largest pivot, as dgetrf); returns 1 for an exactly zero pivot
+DHI int adj_lu(const AdjAcc& acc, int n) {
   for (int j = 0; j < n; j++) {
     int p = j;
     double amax = fabs(acc.G(j, j));
@@ -196,6 +215,11 @@ DHI int adj_factor(const DmkPhaseDev& ph, const AdjPoint& pt, const AdjVec& xi, 
     }
   }
   return 0;
+}
+
+DHI int adj_factor(const DmkPhaseDev& ph, const AdjPoint& pt, const AdjVec& xi, const AdjAcc& acc) {
+  adj_assemble(ph, pt, xi, acc);
+  return adj_lu(acc, 9 + ph.nsl);
 }
 
 // G_y x = b in place
@@ -230,60 +254,74 @@ DHI void adj_solve_t(const AdjAcc& acc, int n, double* x) {
   }
 }
 
-// dP/dF of the increment map with the internal variables re-equilibrated;
-// A[p + 9 q] = d P_p / d F_q with p, q in field order (i + 3 j)
-DHI void adj_tangent(const DmkPhaseDev& ph, const AdjPoint& pt, const AdjFwd& f, const AdjAcc& acc, double* A) {
-  const int m = ph.nsl, n = 9 + m;
+// dP/dF of the increment map with the internal variables re-equilibrated,
+// one direction dF = e_q0 (x) e_q1 (q = q0 + 3 q1) at a time:
+// right-hand side -dG/dF dF of G_y dy = -dG/dF dF ...
+DHI void adj_tangent_rhs(const DmkPhaseDev& ph, const AdjPoint& pt, const AdjFwd& f, const AdjAcc& acc, int q,
+                         double* x) {
+  const int m = ph.nsl, n = 9 + m, q0 = q % 3, q1 = q / 3;
   const double dt = pt.dt;
-  double SMt[9], MSMt[9];
+  double dA0[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  for (int j = 0; j < 3; j++) dA0[3 * q0 + j] = f.Fp0i[3 * q1 + j];
+  double dFe[9], dE[9], dS[9];
+  m_mul(dA0, f.B, dFe);
+  m_mul_tn(f.Fe, dFe, dE);
+  adj_hooke(ph, dE, dS);
+  for (int k = 0; k < n; k++) x[k] = 0.0;
+  for (int b = 0; b < m; b++) {
+    const double* Pa = acc.v(AV_SG, b) > 0.0 ? ph.Pp[b] : ph.Pm[b];
+    double dg = adj_dot9(Pa, dS) * acc.v(AV_KT, b);     // d gdot_b
+    if (dg == 0.0) continue;
+    for (int k = 0; k < 9; k++) x[k] = fma(dg, ph.P[b][k], x[k]);
+    double s = acc.v(AV_SG, b) * dg;                     // d |gdot_b|
+    for (int a = 0; a < m; a++) x[9 + a] = fma(dt * ph.hf[a] * acc.v(AV_PHI, a) * ph.H[a][b], s, x[9 + a]);
+  }
+}
+
+// ... and column q of A[p + 9 q] = d P_p / d F_q (field order) from dL_p = x[0..8]
+DHI void adj_tangent_col(const DmkPhaseDev& ph, const AdjPoint& pt, const AdjFwd& f, int q, const double* x,
+                         double* A) {
+  const int q0 = q % 3, q1 = q / 3;
+  const double dt = pt.dt;
+  double SMt[9], MSMt[9], X[9];
   m_mul_nt(f.S, f.M, SMt);
   m_mul(f.M, SMt, MSMt);
-  double X[9];
   m_mul(pt.F, f.M, X);
-  for (int q0 = 0; q0 < 3; q0++)
-    for (int q1 = 0; q1 < 3; q1++) {
-      // dF = e_q0 (x) e_q1
-      double dA0[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-      for (int j = 0; j < 3; j++) dA0[3 * q0 + j] = f.Fp0i[3 * q1 + j];
-      double dFe[9], dE[9], dS[9];
-      m_mul(dA0, f.B, dFe);
-      m_mul_tn(f.Fe, dFe, dE);
-      adj_hooke(ph, dE, dS);
-      double x[9 + DMK_NSL_MAX];
-      for (int k = 0; k < n; k++) x[k] = 0.0;
-      for (int b = 0; b < m; b++) {
-        const double* Pa = acc.v(AV_SG, b) > 0.0 ? ph.Pp[b] : ph.Pm[b];
-        double dt_b = adj_dot9(Pa, dS) * acc.v(AV_KT, b);     // d gdot_b
-        if (dt_b == 0.0) continue;
-        for (int k = 0; k < 9; k++) x[k] = fma(dt_b, ph.P[b][k], x[k]);
-        double s = acc.v(AV_SG, b) * dt_b;                     // d |gdot_b|
-        for (int a = 0; a < m; a++) x[9 + a] = fma(dt * ph.hf[a] * acc.v(AV_PHI, a) * ph.H[a][b], s, x[9 + a]);
-      }
-      adj_solve(acc, n, x);
-      double dB[9], dM[9], dFe2[9], dE2[9], dS2[9];
-      for (int k = 0; k < 9; k++) dB[k] = -dt * x[k];
-      m_mul(f.Fp0i, dB, dM);
-      m_mul(f.A0, dB, dFe2);
-      for (int k = 0; k < 9; k++) dFe2[k] += dFe[k];
-      m_mul_tn(f.Fe, dFe2, dE2);
-      adj_hooke(ph, dE2, dS2);
-      // dP = dF M S M^T + F dM S M^T + F M dS M^T + F M S dM^T
-      double dP[9], T1[9], T2[9];
-      for (int k = 0; k < 9; k++) dP[k] = 0.0;
-      for (int j = 0; j < 3; j++) dP[3 * q0 + j] = MSMt[3 * q1 + j];
-      m_mul(pt.F, dM, T1);
-      m_mul(T1, SMt, T2);
-      for (int k = 0; k < 9; k++) dP[k] += T2[k];
-      m_mul_nt(dS2, f.M, T1);
-      m_mul(X, T1, T2);
-      for (int k = 0; k < 9; k++) dP[k] += T2[k];
-      m_mul(X, f.S, T1);
-      m_mul_nt(T1, dM, T2);
-      for (int k = 0; k < 9; k++) dP[k] += T2[k];
-      const int q = q0 + 3 * q1;
-      for (int i = 0; i < 3; i++)
-        for (int j = 0; j < 3; j++) A[(i + 3 * j) + 9 * q] = dP[3 * i + j];
-    }
+  double dA0[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  for (int j = 0; j < 3; j++) dA0[3 * q0 + j] = f.Fp0i[3 * q1 + j];
+  double dB[9], dM[9], dFe2[9], dE2[9], dS2[9];
+  for (int k = 0; k < 9; k++) dB[k] = -dt * x[k];
+  m_mul(f.Fp0i, dB, dM);
+  m_mul(f.A0, dB, dFe2);
+  double T1[9];
+  m_mul(dA0, f.B, T1);
+  for (int k = 0; k < 9; k++) dFe2[k] += T1[k];
+  m_mul_tn(f.Fe, dFe2, dE2);
+  adj_hooke(ph, dE2, dS2);
+  // dP = dF M S M^T + F dM S M^T + F M dS M^T + F M S dM^T
+  double dP[9], T2[9];
+  for (int k = 0; k < 9; k++) dP[k] = 0.0;
+  for (int j = 0; j < 3; j++) dP[3 * q0 + j] = MSMt[3 * q1 + j];
+  m_mul(pt.F, dM, T1);
+  m_mul(T1, SMt, T2);
+  for (int k = 0; k < 9; k++) dP[k] += T2[k];
+  m_mul_nt(dS2, f.M, T1);
+  m_mul(X, T1, T2);
+  for (int k = 0; k < 9; k++) dP[k] += T2[k];
+  m_mul(X, f.S, T1);
+  m_mul_nt(T1, dM, T2);
+  for (int k = 0; k < 9; k++) dP[k] += T2[k];
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) A[(i + 3 * j) + 9 * q] = dP[3 * i + j];
+}
+
+DHI void adj_tangent(const DmkPhaseDev& ph, const AdjPoint& pt, const AdjFwd& f, const AdjAcc& acc, double* A) {
+  for (int q = 0; q < 9; q++) {
+    double x[9 + DMK_NSL_MAX];
+    adj_tangent_rhs(ph, pt, f, acc, q, x);
+    adj_solve(acc, 9 + ph.nsl, x);
+    adj_tangent_col(ph, pt, f, q, x, A);
+  }
 }
 
 // chain S_bar -> (F_bar, Fp0i_bar, B_bar) through F_e = F Fp0^-1 B, E, S = C : E
@@ -303,17 +341,16 @@ DHI void adj_back_S(const DmkPhaseDev& ph, const double* F, const AdjFwd& f, con
   }
 }
 
-// Vector-Jacobian product of the increment map.  Cotangents of the outputs:
-// Pb (P), Fpb (F_p, may be null), xib (xi, ld 0 for none).  Returns the
-// cotangents of F (Fb), and with params: F_p0 (Fp0b), xi_0 (xi0b, written)
-// and the parameters (thb, accumulated; slot * ldth + system, interaction
-// types at ADJ_TH_SYS * ldth + t).
-DHI void adj_vjp(const DmkPhaseDev& ph, const AdjPoint& pt, const AdjFwd& f, const AdjVec& xi, const AdjAcc& acc,
-                 const double* Pb, const double* Fpb, const AdjVec& xib, double* Fb, bool params, double* Fp0b,
-                 const AdjVec& xi0b, const AdjVec& thb, int ldth) {
-  const int m = ph.nsl, n = 9 + m;
+// Vector-Jacobian product of the increment map, in two halves around the
+// transposed solve.  Cotangents of the outputs: Pb (P), Fpb (F_p, may be
+// null), xib (xi, ld 0 for none).  The first half returns the right-hand side
+// (do/dy)^T o_bar in lam and the explicit parts of the F and F_p0^-1
+// cotangents in Fb, Fp0ib.
+DHI void adj_vjp_rhs(const DmkPhaseDev& ph, const AdjPoint& pt, const AdjFwd& f, const double* Pb,
+                     const double* Fpb, const AdjVec& xib, double* lam, double* Fb, double* Fp0ib) {
+  const int m = ph.nsl;
   const double dt = pt.dt;
-  double X[9], K[9], T[9], T2[9], Sb[9], Mb[9], Bb[9], Fp0ib[9];
+  double X[9], K[9], T[9], T2[9], Sb[9], Mb[9], Bb[9];
   m_mul(pt.F, f.M, X);
   // P = X S M^T
   m_mul_nt(f.S, f.M, T);
@@ -341,10 +378,20 @@ DHI void adj_vjp(const DmkPhaseDev& ph, const AdjPoint& pt, const AdjFwd& f, con
   m_mul_nt(Mb, f.B, Fp0ib);               // M = Fp0i B
   m_mul_tn(f.Fp0i, Mb, Bb);
   adj_back_S(ph, pt.F, f, Sb, Fb, Fp0ib, Bb, 1.0);
-  double lam[9 + DMK_NSL_MAX];
   for (int k = 0; k < 9; k++) lam[k] = -dt * Bb[k];
   for (int a = 0; a < m; a++) lam[9 + a] = xib.ld ? xib(a) : 0.0;
-  adj_solve_t(acc, n, lam);
+}
+
+// Second half, with lam = G_y^-T (do/dy)^T o_bar: F_bar -= G_F^T lam, and with
+// params the cotangents of F_p0 (Fp0b), xi_0 (xi0b, written) and the
+// parameters (thb, accumulated; slot * ldth + system, interaction types at
+// ADJ_TH_SYS * ldth + t).
+DHI void adj_vjp_finish(const DmkPhaseDev& ph, const AdjPoint& pt, const AdjFwd& f, const AdjVec& xi,
+                        const AdjAcc& acc, const double* lam, double* Fb, double* Fp0ib, bool params, double* Fp0b,
+                        const AdjVec& xi0b, const AdjVec& thb, int ldth) {
+  const int m = ph.nsl;
+  const double dt = pt.dt;
+  double T[9], T2[9];
   // lambda . G through tau: w_c = -kt_c (<Lambda, P_c> + dt s_c psi_c)
   double SbG[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
   for (int c = 0; c < m; c++) {
@@ -380,4 +427,13 @@ DHI void adj_vjp(const DmkPhaseDev& ph, const AdjPoint& pt, const AdjFwd& f, con
     double c = dt * l2 * ph.hf[a] * phi;
     for (int b = 0; b < m; b++) thb(ADJ_TH_SYS * ldth + ph.Ht[a][b]) += c * fabs(acc.v(AV_GD, b));
   }
+}
+
+DHI void adj_vjp(const DmkPhaseDev& ph, const AdjPoint& pt, const AdjFwd& f, const AdjVec& xi, const AdjAcc& acc,
+                 const double* Pb, const double* Fpb, const AdjVec& xib, double* Fb, bool params, double* Fp0b,
+                 const AdjVec& xi0b, const AdjVec& thb, int ldth) {
+  double lam[9 + DMK_NSL_MAX], Fp0ib[9];
+  adj_vjp_rhs(ph, pt, f, Pb, Fpb, xib, lam, Fb, Fp0ib);
+  adj_solve_t(acc, 9 + ph.nsl, lam);
+  adj_vjp_finish(ph, pt, f, xi, acc, lam, Fb, Fp0ib, params, Fp0b, xi0b, thb, ldth);
 }

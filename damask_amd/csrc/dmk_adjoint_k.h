@@ -20,22 +20,26 @@ enum { ADJ_TH_N = 0, ADJ_TH_XISAT = 1, ADJ_TH_H0 = 2, ADJ_TH_A = 3, ADJ_TH_G0 = 
 // H |gdot|, then d tau_a / d L_p (9 each)
 enum { AV_GD = 0, AV_KT, AV_TM, AV_SG, AV_PHI, AV_DPHI, AV_R, AV_T, AV_N = AV_T + 9 };
 
+// scratch doubles per cell: per-system temporaries, the matrix block (G_y and
+// its pivots) and the 9 tangent right-hand sides
 inline size_t dmk_adj_scratch_slots(int mmax) {
   size_t n = 9 + (size_t)mmax;
-  return n * n + n + (size_t)AV_N * mmax;
+  return (size_t)AV_N * mmax + n * n + n + 9 * n;
 }
 
 
 // local reverse step of one increment for the cells of one phase.
-// mode 0: consistency residual (res), LU of G_y (scr), tangent (A), F cotangent
-//         (g) for the output cotangent (Pu, Fpb_in, xib_in);
+// mode 0: consistency residual (res, HUGE_VAL for a singular G_y), LU of G_y
+//         (scr), tangent (A), F cotangent (g) for the output cotangent
+//         (Pu, Fpb_in, xib_in);
 // mode 1: with the LU of mode 0, the cotangents of F_p0 (Fpb_out), xi_0
 //         (xib_out) and the parameters (thb, accumulated) for the output
 //         cotangent (Pu - mu, Fpb_in, xib_in).
 // Fields component-major ([c][N], c = i + 3 j); per-system arrays [a][N].
 struct AdjKArgs {
-  int N, phase_id, mode, mmax;
+  int N, phase_id, mode, mmax, nsl;    // nsl: slip systems of this phase
   double dt;
+  double eps_rel_Lp, eps_abs_Lp, eps_rel_state;   // tolerances of the consistency residual
   const int32_t* phase;
   const double *F, *Fp0, *xi0, *Lp, *xi;
   double Pu[9];

@@ -257,7 +257,7 @@ static_assert(SM_AREP + REP_SIZE <= SM_APAV, "adjoint report region");
 static_assert(SM_ADOTS + DMK_MAX_DOTS <= SM_AMAX, "adjoint dot region");
 static_assert(SM_REP + REP_SIZE <= SM_FAIM, "report region");
 
-static void set_gamma_coef(dmk_handle* h, const double* C) {
+static void gamma_coef(double wgt, const double* C, GammaArgs* g) {
   // A_lm = sum_jq C_ref(l,j,m,q) a_j a_q with C_ref = C / wgt; products a_j a_q
   // ordered (00, 11, 22, 12, 02, 01)
   static const int pj[6] = {0, 1, 2, 1, 0, 0}, pq[6] = {0, 1, 2, 2, 2, 1};
@@ -265,11 +265,13 @@ static void set_gamma_coef(dmk_handle* h, const double* C) {
     for (int m = 0; m < 3; m++)
       for (int p = 0; p < 6; p++) {
         int j = pj[p], q = pq[p];
-        double v = C[c4(l, j, m, q)] / h->wgt;
-        if (j != q) v += C[c4(l, q, m, j)] / h->wgt;
-        h->ga.Ccoef[3 * l + m][p] = v;
+        double v = C[c4(l, j, m, q)] / wgt;
+        if (j != q) v += C[c4(l, q, m, j)] / wgt;
+        g->Ccoef[3 * l + m][p] = v;
       }
 }
+
+static void set_gamma_coef(dmk_handle* h, const double* C) { gamma_coef(h->wgt, C, &h->ga); }
 
 // Orientation, dPdF and the stiffness averages C_volAvg / C_minMaxAvg of the
 // last response (homogenization_mechanical_response's second loop and the
@@ -600,6 +602,7 @@ struct AdjState {
   std::vector<double> dt, pav, qR;
   std::vector<int> mask, maxsub;
   int M = 30;                      // GMRES restart length
+  GammaArgs ga;                    // Gamma_0 of the adjoint solves (reference medium sym <A>)
   bool work_ready = false;
   double *A = nullptr, *g = nullptr, *res = nullptr, *scr = nullptr, *Fpb[2] = {nullptr, nullptr},
          *xib[2] = {nullptr, nullptr}, *thb = nullptr, *mu = nullptr, *y = nullptr, *b = nullptr, *r = nullptr,
@@ -699,11 +702,12 @@ static int adj_dots(dmk_handle* h, const DotJobs& J, double* out) {
 
 // out = Phi(y): Gamma_0 convolution with the mean rule of the increment
 static int adj_phi(dmk_handle* h, const AimArgs& aa, const double* y, double* out) {
+  const GammaArgs& ga = h->adj->ga;
   CKL(dmk_colsum(y, h->N, h->N, 9, h->partial, h->small + SM_APAV, h->wgt, h->st));
   CKL(dmk_residual_aim(&aa, h->small + SM_APAV, h->small + SM_AFAIM, h->flag, h->small + SM_AREP,
                        h->small + SM_AAIM, h->st));
   CKL(dmk_fft_r2c(&h->fft, 9, y, h->Z, h->st));
-  GammaArgs g = h->ga;
+  GammaArgs g = ga;
   g.do_div = 0;
   g.do_gamma = 1;
   g.aim_dev = h->small + SM_AAIM;
@@ -718,50 +722,61 @@ static int adj_op(dmk_handle* h, const AimArgs& aa, const double* v, double* out
   return adj_phi(h, aa, h->adj->y, out);
 }
 
-// restarted GMRES (classical Gram-Schmidt, twice) for Phi(A^T mu) = b
+// Restarted GMRES for Phi(A^T mu) = b, started from the mu in place (the
+// previous increment's adjoint field: consecutive increments' adjoints differ
+// little).  Arnoldi by classical Gram-Schmidt with the norm from the same
+// reduction (|w - V h|^2 = |w|^2 - |h|^2), so one host synchronisation per
+// step; a second pass when that cancels by more than a factor 100.
 static int adj_gmres(dmk_handle* h, const AimArgs& aa, double rtol, int maxit, int* its_out, double* rel_out) {
   AdjState& a = *h->adj;
   const long n = h->n9;
   const int M = a.M;
   DotJobs J;
   double d[DMK_MAX_DOTS];
-  CK(hipMemsetAsync(a.mu, 0, sizeof(double) * n, h->st));
+  int rc;
+  // r = b - Phi(A^T mu), |b|, |r|
+  if ((rc = adj_op(h, aa, a.mu, a.r))) return rc;
+  CKL(dmk_axpby(n, 1.0, a.b, -1.0, a.r, a.r, h->st));
   J.n = 0;
   add_dot(&J, a.b, a.b, 0);
-  int rc = adj_dots(h, J, d);
-  if (rc) return rc;
+  add_dot(&J, a.r, a.r, 0);
+  if ((rc = adj_dots(h, J, d))) return rc;
   const double bnorm = std::sqrt(d[0]);
+  double rnorm = std::sqrt(d[1]);
   *its_out = 0;
   *rel_out = 0.0;
-  if (!(bnorm > 0.0)) return bnorm == 0.0 ? 0 : (dmk_set_error("adjoint: non-finite right-hand side"), DMK_ERR_NOCONV);
-  CKL(dmk_axpby(n, 1.0 / bnorm, a.b, 0.0, a.b, a.V, h->st));
-  double rnorm = bnorm;
+  if (bnorm != bnorm || rnorm != rnorm) { dmk_set_error("adjoint: non-finite right-hand side"); return DMK_ERR_NOCONV; }
+  if (bnorm == 0.0) {
+    CK(hipMemsetAsync(a.mu, 0, sizeof(double) * n, h->st));
+    return 0;
+  }
   int its = 0;
   std::vector<double> H((size_t)(M + 1) * M), cs(M), sn(M), gv(M + 1), yv(M);
-  while (true) {
+  while (rnorm > rtol * bnorm && its < maxit) {
+    CKL(dmk_axpby(n, 1.0 / rnorm, a.r, 0.0, a.r, a.V, h->st));
     std::fill(H.begin(), H.end(), 0.0);
     std::fill(gv.begin(), gv.end(), 0.0);
     gv[0] = rnorm;
     int j = 0;
-    bool done = false;
     for (; j < M && its < maxit; j++) {
       its++;
       double* w = a.V + (size_t)(j + 1) * n;
       if ((rc = adj_op(h, aa, a.V + (size_t)j * n, w))) return rc;
-      double hc[DMK_GS_MAX] = {0};
+      double hc[DMK_GS_MAX] = {0}, hn = 0.0;
       for (int pass = 0; pass < 2; pass++) {
         J.n = 0;
         for (int i = 0; i <= j; i++) add_dot(&J, a.V + (size_t)i * n, w, 0);
+        add_dot(&J, w, w, 0);
         if ((rc = adj_dots(h, J, d))) return rc;
         GsArgs c;
         c.l = j + 1;
-        for (int i = 0; i <= j; i++) { c.c[i] = d[i]; hc[i] += d[i]; }
+        double hh = 0.0;
+        for (int i = 0; i <= j; i++) { c.c[i] = d[i]; hc[i] += d[i]; hh += d[i] * d[i]; }
         CKL(dmk_adj_gs(n, w, a.V, n, &c, h->st));
+        double w2 = d[j + 1], r2 = w2 - hh;
+        hn = std::sqrt(std::fmax(r2, 0.0));
+        if (r2 > 1e-4 * w2) break;
       }
-      J.n = 0;
-      add_dot(&J, w, w, 0);
-      if ((rc = adj_dots(h, J, d))) return rc;
-      double hn = std::sqrt(d[0]);
       for (int i = 0; i <= j; i++) H[i + (size_t)(M + 1) * j] = hc[i];
       H[j + 1 + (size_t)(M + 1) * j] = hn;
       if (hn > 0.0) CKL(dmk_axpby(n, 1.0 / hn, w, 0.0, w, w, h->st));
@@ -779,9 +794,9 @@ static int adj_gmres(dmk_handle* h, const AimArgs& aa, double rtol, int maxit, i
       cj[1] = 0.0;
       gv[j + 1] = -sn[j] * gv[j];
       gv[j] = cs[j] * gv[j];
-      rnorm = std::fabs(gv[j + 1]);
-      if (rnorm != rnorm) { dmk_set_error("adjoint GMRES: non-finite residual"); return DMK_ERR_NOCONV; }
-      if (rnorm <= rtol * bnorm || hn == 0.0) { j++; done = true; break; }
+      double est = std::fabs(gv[j + 1]);
+      if (est != est) { dmk_set_error("adjoint GMRES: non-finite residual"); return DMK_ERR_NOCONV; }
+      if (est <= 0.5 * rtol * bnorm || hn == 0.0) { j++; break; }
     }
     for (int i = j - 1; i >= 0; i--) {
       double s = gv[i];
@@ -792,15 +807,13 @@ static int adj_gmres(dmk_handle* h, const AimArgs& aa, double rtol, int maxit, i
     c.l = j;
     for (int i = 0; i < j; i++) c.c[i] = -yv[i];
     CKL(dmk_adj_gs(n, a.mu, a.V, n, &c, h->st));
-    // true residual b - Phi(A^T mu) (restart vector, and the reported value)
+    // true residual (restart vector and the reported value)
     if ((rc = adj_op(h, aa, a.mu, a.r))) return rc;
     CKL(dmk_axpby(n, 1.0, a.b, -1.0, a.r, a.r, h->st));
     J.n = 0;
     add_dot(&J, a.r, a.r, 0);
     if ((rc = adj_dots(h, J, d))) return rc;
     rnorm = std::sqrt(d[0]);
-    if (done || rnorm <= rtol * bnorm || its >= maxit) break;
-    CKL(dmk_axpby(n, 1.0 / rnorm, a.r, 0.0, a.r, a.V, h->st));
   }
   *its_out = its;
   *rel_out = rnorm / bnorm;
@@ -835,6 +848,7 @@ extern "C" int dmk_adjoint_backward(dmk_handle* h, const double* P_av_bar, doubl
   CK(hipMemsetAsync(a.Fpb[0], 0, sizeof(double) * n9, h->st));
   CK(hipMemsetAsync(a.xib[0], 0, sizeof(double) * mmax * N, h->st));
   CK(hipMemsetAsync(a.thb, 0, sizeof(double) * nslot * N, h->st));
+  CK(hipMemsetAsync(a.mu, 0, sizeof(double) * n9, h->st));
   int cur = 0;
   for (int k = a.n; k >= 1; k--) {
     const double* E = tape_entry(h, k);
@@ -844,6 +858,9 @@ extern "C" int dmk_adjoint_backward(dmk_handle* h, const double* P_av_bar, doubl
     ka.N = (int)N;
     ka.mmax = mmax;
     ka.dt = a.dt[k - 1];
+    ka.eps_rel_Lp = h->nm.eps_rel_Lp;
+    ka.eps_abs_Lp = h->nm.eps_abs_Lp;
+    ka.eps_rel_state = h->nm.eps_rel_state;
     ka.phase = h->mp.phase;
     ka.F = E + TP_F * N;
     ka.Lp = E + TP_LP * N;
@@ -863,6 +880,7 @@ extern "C" int dmk_adjoint_backward(dmk_handle* h, const double* P_av_bar, doubl
     ka.mode = 0;
     for (int p = 0; p < h->n_phases; p++) {
       ka.phase_id = p;
+      ka.nsl = h->ph[p].nsl;
       CKL(dmk_adj_local(&ka, h->ph_dev + p, h->st));
     }
     CKL(dmk_adj_max(a.res, N, h->small + SM_AMAX, h->st));
@@ -871,9 +889,8 @@ extern "C" int dmk_adjoint_backward(dmk_handle* h, const double* P_av_bar, doubl
                       hipMemcpyDeviceToHost, h->st));
     CK(hipStreamSynchronize(h->st));
     const double resmax = h->hsmall[SM_AMAX];
-    if (!(resmax < 1.0)) {
-      dmk_set_error("dmk_adjoint_backward: increment %d: recorded state inconsistent or singular (residual %g)", k,
-                    resmax);
+    if (!(resmax < HUGE_VAL)) {
+      dmk_set_error("dmk_adjoint_backward: increment %d: singular increment Jacobian or non-finite state", k);
       return DMK_ERR_NOCONV;
     }
     // mean rule: masked compliance of <A^T>
@@ -882,6 +899,12 @@ extern "C" int dmk_adjoint_backward(dmk_handle* h, const double* P_av_bar, doubl
     for (int p = 0; p < 9; p++)
       for (int q = 0; q < 9; q++) Ct[p + 9 * q] = Abar[q + 9 * p];
     if ((rc = masked_compliance_of(Ct, &a.mask[9 * (k - 1)], qR, Sadj))) return rc;
+    // preconditioner: Gamma_0 of the reference medium sym <A> of this increment
+    // (any positive-definite medium gives the same solution)
+    double Cs[81];
+    for (int p = 0; p < 81; p++) Cs[p] = 0.5 * (Abar[p] + Ct[p]);
+    a.ga = h->ga;
+    gamma_coef(h->wgt, Cs, &a.ga);
     AimArgs aa;
     std::memcpy(aa.S, Sadj, sizeof aa.S);
     for (int c = 0; c < 9; c++) { aa.P_aim[c] = 0.0; aa.stress_mask[c] = a.mask[9 * (k - 1) + c]; }
@@ -900,6 +923,7 @@ extern "C" int dmk_adjoint_backward(dmk_handle* h, const double* P_av_bar, doubl
     ka.thb = a.thb;
     for (int p = 0; p < h->n_phases; p++) {
       ka.phase_id = p;
+      ka.nsl = h->ph[p].nsl;
       CKL(dmk_adj_local(&ka, h->ph_dev + p, h->st));
     }
     cur = 1 - cur;
