@@ -74,9 +74,9 @@ __global__ void __launch_bounds__(ADJ_WG) adj_assemble_kernel(AdjKArgs a, const 
   adj_assemble(ph, pt, xi, acc);
   double* X = sc.x + (size_t)e * 9 * sc.nmax;
   for (int q = 0; q < 9; q++) {
-    double x[9 + DMK_NSL_MAX];
-    adj_tangent_rhs(ph, pt, f, acc, q, x);
-    for (int i = 0; i < 9 + ph.nsl; i++) X[q * sc.nmax + i] = x[i];
+    double xl[9], w[DMK_NSL_MAX];
+    adj_tangent_rhs(ph, pt, f, acc, q, xl, X + q * sc.nmax + 9, w);
+    for (int i = 0; i < 9; i++) X[q * sc.nmax + i] = xl[i];
   }
 }
 
@@ -84,25 +84,25 @@ __global__ void __launch_bounds__(ADJ_WG) adj_assemble_kernel(AdjKArgs a, const 
 // partial pivoting in LDS, the 9 tangent right-hand sides carried along as
 // extra columns, then back substitution.  Pivot choice, operation order and
 // rounding per entry are those of adj_lu / adj_solve (first largest pivot).
+// LDS per cell: n rows of n + 9 columns (+1 pad), sized at launch.
 #define COOP_CELLS 4
 #define COOP_LANES 16
-#define COOP_NC (9 + DMK_NSL_MAX + 9)   // columns: G_y, then the 9 right-hand sides
-#define COOP_LS (COOP_NC + 1)           // LDS row stride
+__host__ __device__ constexpr int coop_ls(int n) { return n + 10; }
 __global__ void __launch_bounds__(COOP_CELLS * COOP_LANES) adj_lu_kernel(AdjKArgs a) {
-  __shared__ double Gs[COOP_CELLS][(9 + DMK_NSL_MAX) * COOP_LS];
-  __shared__ int pv[COOP_CELLS][9 + DMK_NSL_MAX];
+  extern __shared__ double lds[];
   const int g = threadIdx.x / COOP_LANES, l = threadIdx.x % COOP_LANES;
   const int e = blockIdx.x * COOP_CELLS + g;
   const bool act = e < a.N && a.phase[e] == a.phase_id;
-  const int n = 9 + a.nsl, nc = n + 9;
+  const int n = 9 + a.nsl, nc = n + 9, LS = coop_ls(n);
+  double* G = lds + (size_t)g * n * LS;
+  int* pv = (int*)(lds + (size_t)COOP_CELLS * n * LS) + g * n;
   const AdjScr sc = adj_scr(a);
   double* Gg = sc.g + (size_t)(act ? e : 0) * sc.blk;
   double* Xg = sc.x + (size_t)(act ? e : 0) * 9 * sc.nmax;
-  double* G = Gs[g];
   if (act) {
     for (int i = 0; i < n; i++) {
-      for (int j = l; j < n; j += COOP_LANES) G[i * COOP_LS + j] = Gg[i * sc.nmax + j];
-      if (l < 9) G[i * COOP_LS + n + l] = Xg[l * sc.nmax + i];
+      for (int j = l; j < n; j += COOP_LANES) G[i * LS + j] = Gg[i * sc.nmax + j];
+      if (l < 9) G[i * LS + n + l] = Xg[l * sc.nmax + i];
     }
   }
   bool sing = false;
@@ -112,7 +112,7 @@ __global__ void __launch_bounds__(COOP_CELLS * COOP_LANES) adj_lu_kernel(AdjKArg
     int bi = j;
     if (act)
       for (int i = j + l; i < n; i += COOP_LANES) {
-        double v = fabs(G[i * COOP_LS + j]);
+        double v = fabs(G[i * LS + j]);
         if (v > best) { best = v; bi = i; }
       }
     for (int off = COOP_LANES / 2; off > 0; off >>= 1) {
@@ -122,36 +122,56 @@ __global__ void __launch_bounds__(COOP_CELLS * COOP_LANES) adj_lu_kernel(AdjKArg
     }
     const int p = bi;
     if (act && best == 0.0) sing = true;
-    if (l == 0) pv[g][j] = p;
+    if (l == 0) pv[j] = p;
     if (act && !sing && p != j)
       for (int k = l; k < nc; k += COOP_LANES) {
-        double t = G[j * COOP_LS + k];
-        G[j * COOP_LS + k] = G[p * COOP_LS + k];
-        G[p * COOP_LS + k] = t;
+        double t = G[j * LS + k];
+        G[j * LS + k] = G[p * LS + k];
+        G[p * LS + k] = t;
       }
     __syncthreads();
     if (act && !sing) {
-      double r = 1.0 / G[j * COOP_LS + j];
-      for (int i = j + 1 + l; i < n; i += COOP_LANES) G[i * COOP_LS + j] *= r;
+      double r = 1.0 / G[j * LS + j];
+      for (int i = j + 1 + l; i < n; i += COOP_LANES) G[i * LS + j] *= r;
     }
     __syncthreads();
-    // trailing update; each lane owns columns k (matrix and right-hand sides)
+    // trailing update; each lane owns columns k (matrix and right-hand sides),
+    // rows four at a time so that their loads are in flight together
     if (act && !sing)
       for (int k = j + 1 + l; k < nc; k += COOP_LANES) {
-        const double gjk = G[j * COOP_LS + k];
-        for (int i = j + 1; i < n; i++) G[i * COOP_LS + k] = fma(-G[i * COOP_LS + j], gjk, G[i * COOP_LS + k]);
+        const double gjk = G[j * LS + k];
+        int i = j + 1;
+        for (; i + 4 <= n; i += 4) {
+          double l0 = G[i * LS + j], l1 = G[(i + 1) * LS + j], l2 = G[(i + 2) * LS + j], l3 = G[(i + 3) * LS + j];
+          double c0 = G[i * LS + k], c1 = G[(i + 1) * LS + k], c2 = G[(i + 2) * LS + k], c3 = G[(i + 3) * LS + k];
+          G[i * LS + k] = fma(-l0, gjk, c0);
+          G[(i + 1) * LS + k] = fma(-l1, gjk, c1);
+          G[(i + 2) * LS + k] = fma(-l2, gjk, c2);
+          G[(i + 3) * LS + k] = fma(-l3, gjk, c3);
+        }
+        for (; i < n; i++) G[i * LS + k] = fma(-G[i * LS + j], gjk, G[i * LS + k]);
       }
     __syncthreads();
   }
-  // back substitution of the 9 right-hand sides (column n + d)
+  // back substitution of the 9 right-hand sides (column n + d), each lane
+  // owning right-hand side d for rows i = l / 9 + ..., four rows at a time
   for (int j = n - 1; j >= 0; j--) {
-    if (act && !sing && l < 9) G[j * COOP_LS + n + l] /= G[j * COOP_LS + j];
+    if (act && !sing && l < 9) G[j * LS + n + l] /= G[j * LS + j];
     __syncthreads();
-    if (act && !sing)
-      for (int t = l; t < 9 * j; t += COOP_LANES) {
-        const int d = t % 9, i = t / 9;
-        G[i * COOP_LS + n + d] = fma(-G[i * COOP_LS + j], G[j * COOP_LS + n + d], G[i * COOP_LS + n + d]);
+    if (act && !sing && l < 9) {
+      const double xj = G[j * LS + n + l];
+      int i = 0;
+      for (; i + 4 <= j; i += 4) {
+        double u0 = G[i * LS + j], u1 = G[(i + 1) * LS + j], u2 = G[(i + 2) * LS + j], u3 = G[(i + 3) * LS + j];
+        double c0 = G[i * LS + n + l], c1 = G[(i + 1) * LS + n + l], c2 = G[(i + 2) * LS + n + l],
+               c3 = G[(i + 3) * LS + n + l];
+        G[i * LS + n + l] = fma(-u0, xj, c0);
+        G[(i + 1) * LS + n + l] = fma(-u1, xj, c1);
+        G[(i + 2) * LS + n + l] = fma(-u2, xj, c2);
+        G[(i + 3) * LS + n + l] = fma(-u3, xj, c3);
       }
+      for (; i < j; i++) G[i * LS + n + l] = fma(-G[i * LS + j], xj, G[i * LS + n + l]);
+    }
     __syncthreads();
   }
   if (!act) return;
@@ -160,10 +180,10 @@ __global__ void __launch_bounds__(COOP_CELLS * COOP_LANES) adj_lu_kernel(AdjKArg
     return;
   }
   for (int i = 0; i < n; i++) {
-    for (int j = l; j < n; j += COOP_LANES) Gg[i * sc.nmax + j] = G[i * COOP_LS + j];
-    if (l < 9) Xg[l * sc.nmax + i] = G[i * COOP_LS + n + l];
+    for (int j = l; j < n; j += COOP_LANES) Gg[i * sc.nmax + j] = G[i * LS + j];
+    if (l < 9) Xg[l * sc.nmax + i] = G[i * LS + n + l];
   }
-  for (int j = l; j < n; j += COOP_LANES) Gg[sc.nmax * sc.nmax + j] = (double)pv[g][j];
+  for (int j = l; j < n; j += COOP_LANES) Gg[sc.nmax * sc.nmax + j] = (double)pv[j];
 }
 
 // K3 (per cell).  mode 0: tangent A from the 9 solutions, F cotangent g for
@@ -232,7 +252,7 @@ __global__ void adj_atmul_kernel(long N, const double* __restrict__ A, const dou
 #define ADJ_RB 256
 __global__ void __launch_bounds__(ADJ_RB) adj_phase_sum_kernel(const double* __restrict__ v, long N,
                                                                const int32_t* __restrict__ phase, int n_phases,
-                                                               int nslot, double* __restrict__ out) {
+                                                               int nslot, int accumulate, double* __restrict__ out) {
   __shared__ double sh[DMK_MAX_PHASES][ADJ_RB];
   const int s = blockIdx.x;
   double acc[DMK_MAX_PHASES];
@@ -251,7 +271,7 @@ __global__ void __launch_bounds__(ADJ_RB) adj_phase_sum_kernel(const double* __r
     __syncthreads();
   }
   if (threadIdx.x == 0)
-    for (int p = 0; p < n_phases; p++) out[(size_t)p * nslot + s] = sh[p][0];
+    for (int p = 0; p < n_phases; p++) out[(size_t)p * nslot + s] = sh[p][0] + (accumulate ? out[(size_t)p * nslot + s] : 0.0);
 }
 
 // out[0] = max over N (NaN if any entry is NaN)
@@ -308,7 +328,9 @@ int dmk_adj_local(const AdjKArgs* a, const DmkPhaseDev* ph, hipStream_t s) {
   if (a->mode == 0) {
     hipLaunchKernelGGL(adj_assemble_kernel, dim3(nb), dim3(ADJ_WG), 0, s, *a, ph);
     unsigned nc = (unsigned)((a->N + COOP_CELLS - 1) / COOP_CELLS);
-    hipLaunchKernelGGL(adj_lu_kernel, dim3(nc), dim3(COOP_CELLS * COOP_LANES), 0, s, *a);
+    const int n = 9 + a->nsl;
+    size_t lds = sizeof(double) * COOP_CELLS * n * coop_ls(n) + sizeof(int) * COOP_CELLS * n;
+    hipLaunchKernelGGL(adj_lu_kernel, dim3(nc), dim3(COOP_CELLS * COOP_LANES), lds, s, *a);
   }
   hipLaunchKernelGGL(adj_vjp_kernel, dim3(nb), dim3(ADJ_WG), 0, s, *a, ph);
   return hipGetLastError() == hipSuccess ? 0 : 2;
@@ -319,10 +341,11 @@ int dmk_adj_atmul(long N, const double* A, const double* mu, double* y, hipStrea
   return hipGetLastError() == hipSuccess ? 0 : 2;
 }
 
-int dmk_adj_phase_sum(const double* v, long N, const int32_t* phase, int n_phases, int nslot, double* out,
-                      hipStream_t s) {
+int dmk_adj_phase_sum(const double* v, long N, const int32_t* phase, int n_phases, int nslot, int accumulate,
+                      double* out, hipStream_t s) {
   if (n_phases > DMK_MAX_PHASES) return 1;
-  hipLaunchKernelGGL(adj_phase_sum_kernel, dim3(nslot), dim3(ADJ_RB), 0, s, v, N, phase, n_phases, nslot, out);
+  hipLaunchKernelGGL(adj_phase_sum_kernel, dim3(nslot), dim3(ADJ_RB), 0, s, v, N, phase, n_phases, nslot, accumulate,
+                     out);
   return hipGetLastError() == hipSuccess ? 0 : 2;
 }
 
@@ -373,8 +396,10 @@ extern "C" int dmk_adjoint_point_host(const dmk_phase_desc* d, double dt, const 
   fld_to_rm(Pbar, Pr);
   if (Fpbar) fld_to_rm(Fpbar, Fpr);
   else for (int k = 0; k < 9; k++) Fpr[k] = 0.0;
-  AdjVec X0B{x0b, 1, 0}, TH{thbar, 1, 0};
+  double th[ADJ_TH_SYS * DMK_NSL_MAX + DMK_NH_MAX] = {0};
+  AdjVec X0B{x0b, 1, 0}, TH{th, 1, 0};
   adj_vjp(ph, pt, f, X, acc, Pr, Fpr, XB, Fb, true, Fp0b, X0B, TH, DMK_NSL_MAX);
+  for (int k = 0; k < ADJ_TH_SYS * DMK_NSL_MAX + DMK_NH_MAX; k++) thbar[k] += th[k];
   rm_to_fld(Fb, Fbar);
   rm_to_fld(Fp0b, Fp0bar);
   for (int a = 0; a < m; a++) xi0bar[a] = x0b[a];

@@ -256,10 +256,12 @@ DHI void adj_solve_t(const AdjAcc& acc, int n, double* x) {
 
 // dP/dF of the increment map with the internal variables re-equilibrated,
 // one direction dF = e_q0 (x) e_q1 (q = q0 + 3 q1) at a time:
-// right-hand side -dG/dF dF of G_y dy = -dG/dF dF ...
+// right-hand side -dG/dF dF of G_y dy = -dG/dF dF: L_p part in xl (9), xi
+// part in xs (m); w (m) is workspace.  The xi rows are formed one at a time
+// from the rate changes so that nothing accumulates through memory.
 DHI void adj_tangent_rhs(const DmkPhaseDev& ph, const AdjPoint& pt, const AdjFwd& f, const AdjAcc& acc, int q,
-                         double* x) {
-  const int m = ph.nsl, n = 9 + m, q0 = q % 3, q1 = q / 3;
+                         double* xl, double* xs, double* w) {
+  const int m = ph.nsl, q0 = q % 3, q1 = q / 3;
   const double dt = pt.dt;
   double dA0[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
   for (int j = 0; j < 3; j++) dA0[3 * q0 + j] = f.Fp0i[3 * q1 + j];
@@ -267,14 +269,22 @@ DHI void adj_tangent_rhs(const DmkPhaseDev& ph, const AdjPoint& pt, const AdjFwd
   m_mul(dA0, f.B, dFe);
   m_mul_tn(f.Fe, dFe, dE);
   adj_hooke(ph, dE, dS);
-  for (int k = 0; k < n; k++) x[k] = 0.0;
+  double x0 = 0, x1 = 0, x2 = 0, x3 = 0, x4 = 0, x5 = 0, x6 = 0, x7 = 0, x8 = 0;
   for (int b = 0; b < m; b++) {
-    const double* Pa = acc.v(AV_SG, b) > 0.0 ? ph.Pp[b] : ph.Pm[b];
-    double dg = adj_dot9(Pa, dS) * acc.v(AV_KT, b);     // d gdot_b
-    if (dg == 0.0) continue;
-    for (int k = 0; k < 9; k++) x[k] = fma(dg, ph.P[b][k], x[k]);
-    double s = acc.v(AV_SG, b) * dg;                     // d |gdot_b|
-    for (int a = 0; a < m; a++) x[9 + a] = fma(dt * ph.hf[a] * acc.v(AV_PHI, a) * ph.H[a][b], s, x[9 + a]);
+    const double sg = acc.v(AV_SG, b);
+    const double* Pa = sg > 0.0 ? ph.Pp[b] : ph.Pm[b];
+    const double dg = adj_dot9(Pa, dS) * acc.v(AV_KT, b);     // d gdot_b
+    const double* P = ph.P[b];
+    x0 = fma(dg, P[0], x0); x1 = fma(dg, P[1], x1); x2 = fma(dg, P[2], x2);
+    x3 = fma(dg, P[3], x3); x4 = fma(dg, P[4], x4); x5 = fma(dg, P[5], x5);
+    x6 = fma(dg, P[6], x6); x7 = fma(dg, P[7], x7); x8 = fma(dg, P[8], x8);
+    w[b] = sg * dg;                                            // d |gdot_b|
+  }
+  xl[0] = x0; xl[1] = x1; xl[2] = x2; xl[3] = x3; xl[4] = x4; xl[5] = x5; xl[6] = x6; xl[7] = x7; xl[8] = x8;
+  for (int a = 0; a < m; a++) {
+    double s = 0.0;
+    for (int b = 0; b < m; b++) s = fma(ph.H[a][b], w[b], s);
+    xs[a] = dt * ph.hf[a] * acc.v(AV_PHI, a) * s;
   }
 }
 
@@ -317,8 +327,8 @@ DHI void adj_tangent_col(const DmkPhaseDev& ph, const AdjPoint& pt, const AdjFwd
 
 DHI void adj_tangent(const DmkPhaseDev& ph, const AdjPoint& pt, const AdjFwd& f, const AdjAcc& acc, double* A) {
   for (int q = 0; q < 9; q++) {
-    double x[9 + DMK_NSL_MAX];
-    adj_tangent_rhs(ph, pt, f, acc, q, x);
+    double x[9 + DMK_NSL_MAX], w[DMK_NSL_MAX];
+    adj_tangent_rhs(ph, pt, f, acc, q, x, x + 9, w);
     adj_solve(acc, 9 + ph.nsl, x);
     adj_tangent_col(ph, pt, f, q, x, A);
   }
@@ -383,9 +393,9 @@ DHI void adj_vjp_rhs(const DmkPhaseDev& ph, const AdjPoint& pt, const AdjFwd& f,
 }
 
 // Second half, with lam = G_y^-T (do/dy)^T o_bar: F_bar -= G_F^T lam, and with
-// params the cotangents of F_p0 (Fp0b), xi_0 (xi0b, written) and the
-// parameters (thb, accumulated; slot * ldth + system, interaction types at
-// ADJ_TH_SYS * ldth + t).
+// params the cotangents of F_p0 (Fp0b), xi_0 (xi0b) and the parameters (thb:
+// slot * ldth + system, interaction types at ADJ_TH_SYS * ldth + t); every
+// slot of the phase is written once.
 DHI void adj_vjp_finish(const DmkPhaseDev& ph, const AdjPoint& pt, const AdjFwd& f, const AdjVec& xi,
                         const AdjAcc& acc, const double* lam, double* Fb, double* Fp0ib, bool params, double* Fp0b,
                         const AdjVec& xi0b, const AdjVec& thb, int ldth) {
@@ -403,10 +413,8 @@ DHI void adj_vjp_finish(const DmkPhaseDev& ph, const AdjPoint& pt, const AdjFwd&
     for (int k = 0; k < 9; k++) SbG[k] = fma(w, Pa[k], SbG[k]);
     if (params) {
       double gd = acc.v(AV_GD, c);
-      if (gd != 0.0) {
-        thb(ADJ_TH_N * ldth + c) += Gc * gd * log(acc.v(AV_TM, c) / xi(c));
-        thb(ADJ_TH_G0 * ldth + c) += Gc * gd / ph.g0[c];
-      }
+      thb(ADJ_TH_N * ldth + c) = gd != 0.0 ? Gc * gd * log(acc.v(AV_TM, c) / xi(c)) : 0.0;
+      thb(ADJ_TH_G0 * ldth + c) = gd != 0.0 ? Gc * gd / ph.g0[c] : 0.0;
     }
   }
   adj_back_S(ph, pt.F, f, SbG, Fb, Fp0ib, nullptr, -1.0);
@@ -421,11 +429,20 @@ DHI void adj_vjp_finish(const DmkPhaseDev& ph, const AdjPoint& pt, const AdjFwd&
     double phi = acc.v(AV_PHI, a), r = acc.v(AV_R, a), x = xi(a);
     double u = 1.0 - x / ph.xi_sat[a], au = fabs(u);
     double base = dt * l2 * r;
-    thb(ADJ_TH_XISAT * ldth + a) += base * ph.hf[a] * ph.a[a] * pow(au, ph.a[a] - 1.0) * x / (ph.xi_sat[a] * ph.xi_sat[a]);
-    thb(ADJ_TH_H0 * ldth + a) += base * phi * (ph.h0[a] != 0.0 ? ph.hf[a] / ph.h0[a] : 1.0);
-    if (au > 0.0) thb(ADJ_TH_A * ldth + a) += base * ph.hf[a] * phi * log(au);
-    double c = dt * l2 * ph.hf[a] * phi;
-    for (int b = 0; b < m; b++) thb(ADJ_TH_SYS * ldth + ph.Ht[a][b]) += c * fabs(acc.v(AV_GD, b));
+    thb(ADJ_TH_XISAT * ldth + a) = base * ph.hf[a] * ph.a[a] * pow(au, ph.a[a] - 1.0) * x / (ph.xi_sat[a] * ph.xi_sat[a]);
+    thb(ADJ_TH_H0 * ldth + a) = base * phi * (ph.h0[a] != 0.0 ? ph.hf[a] / ph.h0[a] : 1.0);
+    thb(ADJ_TH_A * ldth + a) = au > 0.0 ? base * ph.hf[a] * phi * log(au) : 0.0;
+  }
+  // interaction types: sum over the (reacting, acting) pairs of each type
+  for (int t = 0; t < ph.n_h; t++) {
+    double st = 0.0;
+    for (int a = 0; a < m; a++) {
+      double ca = 0.0;
+      for (int b = 0; b < m; b++)
+        if (ph.Ht[a][b] == t) ca += fabs(acc.v(AV_GD, b));
+      st = fma(dt * lam[9 + a] * ph.hf[a] * acc.v(AV_PHI, a), ca, st);
+    }
+    thb(ADJ_TH_SYS * ldth + t) = st;
   }
 }
 

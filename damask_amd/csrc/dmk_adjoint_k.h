@@ -33,8 +33,8 @@ inline size_t dmk_adj_scratch_slots(int mmax) {
 //         (scr), tangent (A), F cotangent (g) for the output cotangent
 //         (Pu, Fpb_in, xib_in);
 // mode 1: with the LU of mode 0, the cotangents of F_p0 (Fpb_out), xi_0
-//         (xib_out) and the parameters (thb, accumulated) for the output
-//         cotangent (Pu - mu, Fpb_in, xib_in).
+//         (xib_out) and the parameters of this increment (thb, every slot of
+//         the phase written) for the output cotangent (Pu - mu, Fpb_in, xib_in).
 // Fields component-major ([c][N], c = i + 3 j); per-system arrays [a][N].
 struct AdjKArgs {
   int N, phase_id, mode, mmax, nsl;    // nsl: slip systems of this phase
@@ -58,8 +58,9 @@ struct GsArgs {
 
 int dmk_adj_local(const AdjKArgs* a, const DmkPhaseDev* ph, hipStream_t s);
 int dmk_adj_atmul(long N, const double* A, const double* mu, double* y, hipStream_t s);
-int dmk_adj_phase_sum(const double* v, long N, const int32_t* phase, int n_phases, int nslot, double* out,
-                      hipStream_t s);
+// out[p * nslot + s] (+)= sum over the cells of phase p of v[s * N + e]
+int dmk_adj_phase_sum(const double* v, long N, const int32_t* phase, int n_phases, int nslot, int accumulate,
+                      double* out, hipStream_t s);
 int dmk_adj_max(const double* v, long N, double* out, hipStream_t s);
 int dmk_adj_imax(const int32_t* v, long N, double* out, hipStream_t s);
 int dmk_adj_gs(long n, double* w, const double* V, long ldV, const GsArgs* c, hipStream_t s);

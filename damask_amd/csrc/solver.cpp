@@ -848,6 +848,7 @@ extern "C" int dmk_adjoint_backward(dmk_handle* h, const double* P_av_bar, doubl
   CK(hipMemsetAsync(a.Fpb[0], 0, sizeof(double) * n9, h->st));
   CK(hipMemsetAsync(a.xib[0], 0, sizeof(double) * mmax * N, h->st));
   CK(hipMemsetAsync(a.thb, 0, sizeof(double) * nslot * N, h->st));
+  CK(hipMemsetAsync(a.red, 0, sizeof(double) * h->n_phases * (nslot + mmax), h->st));
   CK(hipMemsetAsync(a.mu, 0, sizeof(double) * n9, h->st));
   int cur = 0;
   for (int k = a.n; k >= 1; k--) {
@@ -926,6 +927,8 @@ extern "C" int dmk_adjoint_backward(dmk_handle* h, const double* P_av_bar, doubl
       ka.nsl = h->ph[p].nsl;
       CKL(dmk_adj_local(&ka, h->ph_dev + p, h->st));
     }
+    // parameter cotangents of this increment, summed per phase on the device
+    CKL(dmk_adj_phase_sum(a.thb, N, h->mp.phase, h->n_phases, nslot, 1, a.red + (size_t)h->n_phases * mmax, h->st));
     cur = 1 - cur;
     if (stats) {
       stats[4 * (k - 1) + 0] = its;
@@ -935,10 +938,7 @@ extern "C" int dmk_adjoint_backward(dmk_handle* h, const double* P_av_bar, doubl
     }
   }
   // xi_0 enters as the initial state (phenopowerlaw.f90:259-262)
-  double* red_xi = a.red;
-  double* red_th = a.red + (size_t)h->n_phases * mmax;
-  CKL(dmk_adj_phase_sum(a.xib[cur], N, h->mp.phase, h->n_phases, mmax, red_xi, h->st));
-  CKL(dmk_adj_phase_sum(a.thb, N, h->mp.phase, h->n_phases, nslot, red_th, h->st));
+  CKL(dmk_adj_phase_sum(a.xib[cur], N, h->mp.phase, h->n_phases, mmax, 0, a.red, h->st));
   std::vector<double> hr((size_t)h->n_phases * (nslot + mmax));
   CK(hipMemcpyAsync(hr.data(), a.red, sizeof(double) * hr.size(), hipMemcpyDeviceToHost, h->st));
   CK(hipStreamSynchronize(h->st));
