@@ -50,6 +50,8 @@ def parse():
                     help="also measure adjoint / forward cost on the 32^3 differentiation workload")
     ap.add_argument("--adjoint-rtol", type=float, default=1e-6,
                     help="adjoint GMRES tolerance (1e-6: gradient within 2e-7 of the 1e-12 solve at 32^3)")
+    ap.add_argument("--only-adjoint", type=int, default=0,
+                    help="only run the adjoint / forward measurement and print it as one JSON line")
     ap.add_argument("--copy-calibration", type=int, default=0,
                     help="only run a plain device copy of this many MiB (counter calibration)")
     return ap.parse_args()
@@ -146,6 +148,9 @@ def main():
         sys.exit(2)
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)
+        return
+    if args.only_adjoint:
+        print(json.dumps({"adjoint": adjoint_cost(args)}))
         return
     import numpy as np
     import torch
