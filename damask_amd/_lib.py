@@ -51,7 +51,7 @@ EXPORTS = ["dmk_mech_init", "dmk_mech_forward", "dmk_mech_materialpoint_forward"
            "dmk_gamma_convolution", "dmk_divergence_rms", "dmk_fft_forward", "dmk_fft_backward",
            "dmk_mech_profile", "dmk_mech_kernel_time", "dmk_slip_system", "dmk_adjoint_tape",
            "dmk_adjoint_record", "dmk_adjoint_info", "dmk_adjoint_backward", "dmk_adjoint_release",
-           "dmk_adjoint_point_host"]
+           "dmk_adjoint_point_host", "dmk_mech_restart_response", "dmk_grid_set_restart"]
 
 _lib = None
 
@@ -123,6 +123,10 @@ def load(path: str = LIB_PATH):
     L.dmk_adjoint_info.argtypes = [vp, ip, dp, dp]
     L.dmk_adjoint_backward.argtypes = [vp, dp, C.c_double, dp, dp]
     L.dmk_adjoint_release.argtypes = [vp]
+    L.dmk_mech_restart_response.restype = C.c_int
+    L.dmk_mech_restart_response.argtypes = [vp]
+    L.dmk_grid_set_restart.restype = C.c_int
+    L.dmk_grid_set_restart.argtypes = [vp, C.c_int32]
     _lib = L
     return L
 

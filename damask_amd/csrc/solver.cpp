@@ -211,6 +211,7 @@ struct dmk_handle {
   // driver state (DAMASK_grid.f90:78-100)
   double t = 0.0, Delta_t = 1.0, Delta_t_prev = 0.0;
   int cutBack = 0, cutBackLevel = 0, totalIncs = 0;
+  int restartInc = -1;               // increments up to this one are skipped (restart)
   int64_t n_updates = 0, n_residuals = 0;
   // optional per-kernel timing with HIP events on the solver stream
   int profile = 0;
@@ -1373,6 +1374,7 @@ extern "C" int dmk_mech_get_scalars(dmk_handle* h, int32_t what, double* dst) {
     case 6: std::memcpy(dst, h->F_aimDot, 9 * sizeof(double)); break;
     case 7: std::memcpy(dst, h->F_aim_lastinc, 9 * sizeof(double)); break;
     case 8: std::memcpy(dst, h->C_volAvgLastInc, 81 * sizeof(double)); break;
+    case 9: std::memcpy(dst, h->C_minMaxAvgLastInc, 81 * sizeof(double)); break;
     default: dmk_set_error("unknown scalar %d", what); return DMK_ERR_ARG;
   }
   return 0;
@@ -1388,6 +1390,7 @@ extern "C" int dmk_mech_set_scalars(dmk_handle* h, int32_t what, const double* s
     case 6: std::memcpy(h->F_aimDot, src, 9 * sizeof(double)); break;
     case 7: std::memcpy(h->F_aim_lastinc, src, 9 * sizeof(double)); break;
     case 8: std::memcpy(h->C_volAvgLastInc, src, 81 * sizeof(double)); break;
+    case 9: std::memcpy(h->C_minMaxAvgLastInc, src, 81 * sizeof(double)); break;
     default: dmk_set_error("scalar %d cannot be set", what); return DMK_ERR_ARG;
   }
   return 0;
@@ -1420,6 +1423,20 @@ extern "C" int dmk_grid_run_loadcase(dmk_handle* h, const dmk_loadcase* lc, doub
     h->totalIncs++;
     h->Delta_t_prev = h->Delta_t;
     h->Delta_t = lc->t * geometric_fraction(lc->r, inc, lc->N) * std::pow(2.0, -(double)h->cutBackLevel);
+    if (h->totalIncs <= h->restartInc) {
+      // already computed before the restart: advance the time only (DAMASK_grid.f90:300-303)
+      h->t += h->Delta_t;
+      guess = 1;
+      if (stats) {
+        stats[4 * (inc - 1) + 0] = h->t;
+        stats[4 * (inc - 1) + 1] = h->cutBackLevel;
+        stats[4 * (inc - 1) + 2] = 0;
+        stats[4 * (inc - 1) + 3] = 0;
+      }
+      if (P_av_hist) std::memcpy(P_av_hist + 9 * (inc - 1), h->P_av, 9 * sizeof(double));
+      if (F_aim_hist) std::memcpy(F_aim_hist + 9 * (inc - 1), h->F_aim, 9 * sizeof(double));
+      continue;
+    }
     int stepFraction = 0, its = 0, nf_total = 0;
     while (stepFraction < (1 << h->cutBackLevel)) {
       double t_remaining = lc->t + t_0 - h->t;
@@ -1463,6 +1480,25 @@ extern "C" int dmk_grid_run_loadcase(dmk_handle* h, const dmk_loadcase* lc, doub
   }
   if (t_io) *t_io = h->t;
   if (n_updates) *n_updates = h->n_updates - u0;
+  return 0;
+}
+
+// Restart (grid_mech_spectral_basic.f90:198-232, DAMASK_grid.f90:300-303).
+// After the fields and the solver scalars of the restart increment are set,
+// the solver evaluates the material at the restored F with dt = 0 (the
+// reference's init does so after reading the restart data); the stiffness
+// averages are then set from the restart data.  Increments up to restart_inc
+// of the following dmk_grid_run_loadcase calls only advance the time.
+extern "C" int dmk_mech_restart_response(dmk_handle* h) {
+  if (!h) { dmk_set_error("null handle"); return DMK_ERR_ARG; }
+  int rc = constitutive(h, h->F, 0.0, false, true);
+  if (rc) return rc;
+  return dmk_mech_update_coords(h, nullptr, nullptr);
+}
+
+extern "C" int dmk_grid_set_restart(dmk_handle* h, int32_t restart_inc) {
+  if (!h) { dmk_set_error("null handle"); return DMK_ERR_ARG; }
+  h->restartInc = restart_inc;
   return 0;
 }
 

@@ -24,7 +24,12 @@ from .geom import Grid
 FIELDS = {"F": 0, "P": 1, "F_e": 2, "F_p": 3, "L_p": 4, "S": 5, "F_i": 6, "O": 7, "xi_sl": 8,
           "gamma_sl": 9, "dPdF": 10, "F_lastinc": 11}
 SCALARS = {"F_aim": 0, "P_av": 1, "P_aim": 2, "C_volAvg": 3, "C_minMaxAvg": 4, "err": 5, "F_aimDot": 6,
-           "F_aim_lastinc": 7, "C_volAvgLastInc": 8}
+           "F_aim_lastinc": 7, "C_volAvgLastInc": 8, "C_minMaxAvgLastInc": 9}
+# restart data of grid_mech_spectral_basic (restartWrite, :378-416) and of the
+# phase (phase_mechanical.f90:1248-1280)
+RESTART_FIELDS = ("F", "F_lastinc", "F_p", "L_p", "S", "xi_sl", "gamma_sl")
+RESTART_SCALARS = ("P_aim", "F_aim", "F_aim_lastinc", "F_aimDot")
+RESTART_STIFFNESS = ("C_volAvg", "C_volAvgLastInc", "C_minMaxAvg")
 
 
 @dataclasses.dataclass
@@ -165,7 +170,8 @@ class GridMechSpectralBasic:
         _lib.check(self.lib.dmk_mech_set_field(self.h, FIELDS[name], _lib.dptr(v)), "dmk_mech_set_field")
 
     def scalar(self, name: str) -> np.ndarray:
-        n = {"C_volAvg": 81, "C_minMaxAvg": 81, "C_volAvgLastInc": 81, "err": 2}.get(name, 9)
+        n = {"C_volAvg": 81, "C_minMaxAvg": 81, "C_volAvgLastInc": 81, "C_minMaxAvgLastInc": 81,
+             "err": 2}.get(name, 9)
         out = np.zeros(n)
         _lib.check(self.lib.dmk_mech_get_scalars(self.h, SCALARS[name], _lib.dptr(out)), "dmk_mech_get_scalars")
         if n == 9:
@@ -223,6 +229,27 @@ class GridMechSpectralBasic:
             _lib.check(rc, "dmk_grid_run_loadcase")
         return {"rc": rc, "t": float(t[0]), "stats": stats.reshape(-1, 4), "P_av": P.reshape(-1, 9),
                 "F_aim": Fa.reshape(-1, 9), "n_updates": nu.value}
+
+    # --- restart ------------------------------------------------------------
+    def restart_write(self) -> dict:
+        """The data restartWrite stores (grid_mech_spectral_basic.f90:378-416)."""
+        d = {k: self.field(k) for k in RESTART_FIELDS}
+        d.update({k: self.scalar(k) for k in RESTART_SCALARS + RESTART_STIFFNESS})
+        return d
+
+    def restart_read(self, d: dict, restart_inc: int) -> None:
+        """Restart read of the solver init (grid_mech_spectral_basic.f90:198-232): fields
+        and aims, a dt = 0 response at the restored F, then the stiffness averages; the
+        native driver then skips increments 1..restart_inc."""
+        for k in RESTART_FIELDS:
+            self.set_field(k, d[k])
+        for k in RESTART_SCALARS:
+            self.set_scalar(k, d[k])
+        _lib.check(self.lib.dmk_mech_restart_response(self.h), "dmk_mech_restart_response")
+        for k in RESTART_STIFFNESS:
+            self.set_scalar(k, d[k])
+        self.set_scalar("C_minMaxAvgLastInc", d["C_minMaxAvg"])
+        _lib.check(self.lib.dmk_grid_set_restart(self.h, int(restart_inc)), "dmk_grid_set_restart")
 
     # --- adjoint ------------------------------------------------------------
     def adjoint_tape(self, max_increments: int) -> None:

@@ -131,9 +131,17 @@ int dmk_mech_set_field(dmk_handle* h, int32_t what, const double* src);
 
 /* solver scalars: 0 F_aim (9), 1 P_av (9), 2 P_aim (9), 3 C_volAvg (81),
  * 4 C_minMaxAvg (81), 5 {err_div, err_BC}, 6 F_aimDot (9), 7 F_aim_lastinc (9),
- * 8 C_volAvgLastInc (81) */
+ * 8 C_volAvgLastInc (81), 9 C_minMaxAvgLastInc (81) */
 int dmk_mech_get_scalars(dmk_handle* h, int32_t what, double* dst);
 int dmk_mech_set_scalars(dmk_handle* h, int32_t what, const double* src);
+/* restart read (grid_mech_spectral_basic.f90:198-232): after setting the
+ * fields and F_aim, F_aim_lastinc, F_aimDot, P_aim, call this for the
+ * material response at the restored F with dt = 0, then set C_volAvg,
+ * C_volAvgLastInc and C_minMaxAvg */
+int dmk_mech_restart_response(dmk_handle* h);
+/* increments 1..restart_inc of the following dmk_grid_run_loadcase calls only
+ * advance the time, with the rate guess on (DAMASK_grid.f90:300-303) */
+int dmk_grid_set_restart(dmk_handle* h, int32_t restart_inc);
 
 /* DAMASK_grid increment/cutback loop for one load case (DAMASK_grid.f90:288-460).
  * stats: 4 doubles per increment (t, cutback level, iterations, residual
@@ -193,7 +201,8 @@ int dmk_adjoint_info(dmk_handle* h, int32_t* n, double* P_av, double* dt);
  * slip system), n, xi_inf, h_0, a, dot_gamma_0 (24 each, per slip system;
  * sum over a family for the yaml value) and h_sl-sl (32, per interaction
  * type).  stats (4 n, optional): GMRES iterations, final relative residual,
- * largest local consistency residual, largest sub-step count. */
+ * largest local consistency residual (in units of the convergence
+ * tolerances), largest sub-step count. */
 int dmk_adjoint_backward(dmk_handle* h, const double* P_av_bar, double rtol, double* theta_bar, double* stats);
 int dmk_adjoint_release(dmk_handle* h);
 /* The local reverse step of one cell on the host (no GPU): tangent A (81,

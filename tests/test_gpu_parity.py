@@ -216,3 +216,40 @@ def test_full_fcc_golden_run_against_reference_result(gpu_lib):
         assert np.abs(g.field("xi_sl") - xr).max() < 2e-5 * xr.max()
         Pr = ref[f"P_{inc_end}"].reshape(-1, 3, 3).mean(0)
         assert np.abs(g.field("P").mean(0) - Pr).max() < 1e-3 * np.abs(Pr).max()
+
+
+def test_restart_continues_the_run(gpu_lib):
+    """restartWrite after increment 4, a fresh solver reading it
+    (grid_mech_spectral_basic.f90:198-232) and skipping increments 1-4
+    (DAMASK_grid.f90:300-303) continues the uninterrupted run: increments 5-8
+    agree at the (tight) solver tolerance."""
+    import copy as _copy
+    from damask_amd import config, workloads
+    from damask_amd.grid import GridMechSpectralBasic
+    grid, mat = workloads.polycrystal([8, 8, 8], 5, 7)
+    mat = config.parse_material(mat)
+    _, lcs = config.parse_load(workloads.tension_x(8.0, 8))
+    num = config.Numerics(eps_abs_div=1e-8, eps_rel_div=1e-11, eps_abs_P=1e-3, eps_rel_P=1e-11, N_iter_max=400,
+                          snes_max_it=400, eps_rel_state=1e-12, eps_rel_Lp=1e-11, eps_abs_Lp=1e-16)
+    lc = lcs[0]
+    straight = GridMechSpectralBasic(grid, mat, num)
+    r_all = straight.run_loadcase(lc)
+    assert r_all["rc"] == 0
+    first = GridMechSpectralBasic(grid, mat, num)
+    half = _copy.copy(lc)
+    half.t, half.N = lc.t / 2, lc.N // 2
+    assert first.run_loadcase(half)["rc"] == 0
+    data = first.restart_write()
+    first.close()
+    again = GridMechSpectralBasic(grid, mat, num)
+    again.restart_read(data, restart_inc=half.N)
+    r_cont = again.run_loadcase(lc)
+    assert r_cont["rc"] == 0
+    assert np.allclose(r_cont["stats"][:, 0], r_all["stats"][:, 0], rtol=1e-14)       # same times
+    P1, P2 = r_all["P_av"][half.N:], r_cont["P_av"][half.N:]
+    assert np.abs(P2 - P1).max() < 1e-8 * np.abs(P1).max()
+    for k in ("F", "F_p", "xi_sl"):
+        a, b = straight.field(k), again.field(k)
+        assert np.abs(a - b).max() < 1e-8 * np.abs(a).max(), k
+    straight.close()
+    again.close()
