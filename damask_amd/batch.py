@@ -1,0 +1,96 @@
+"""Many independent RVEs on one GPU.
+
+Each RVE is a GridMechSpectralBasic (one device handle with its own HIP
+stream).  ``dmk_batch_run`` / ``dmk_batch_adjoint`` drive all of them from a
+pool of native host threads, so the kernels and host synchronisations of
+different RVEs overlap on the device.  ``random_batch`` generates the batched
+workload: per RVE a cubic grid edge, a grain count, uniform random
+orientations, its own xi_0 and n, and uniaxial tension along a random
+direction (the load frame rotated by a uniform random rotation).
+"""
+from __future__ import annotations
+
+import copy
+import ctypes as C
+import os
+import time
+
+import numpy as np
+
+from . import _lib, adjoint, config, workloads
+from .grid import ADJ_NOUT, GridMechSpectralBasic, loadcase_struct
+
+
+def random_batch(n_rve: int, master_seed: int = 0, edges=(8, 12, 16, 20, 24, 32), grains=(10, 50),
+                 xi_0=(20.0e6, 45.0e6), n_exp=(10.0, 30.0), strain: float = 0.02, increments: int = 40):
+    """[(grid, material, loadcase)] of n_rve RVEs, every draw from one seeded generator."""
+    rng = np.random.default_rng(master_seed)
+    out = []
+    for _ in range(n_rve):
+        edge = int(rng.choice(edges))
+        ng = int(rng.integers(grains[0], grains[1] + 1))
+        phase = copy.deepcopy(workloads.FCC_PHASE)
+        pl = phase["mechanical"]["plastic"]
+        pl["xi_0_sl"] = [float(rng.uniform(*xi_0))]
+        pl["n_sl"] = [float(rng.uniform(*n_exp))]
+        grid, mat = workloads.polycrystal([edge] * 3, ng, int(rng.integers(2**31 - 1)), phase=phase)
+        _, lcs = config.parse_load(workloads.tension_x(strain / 1.0e-3, increments))
+        q = rng.normal(size=4)
+        q /= np.linalg.norm(q)
+        lc = lcs[0]
+        lc.q_R = q if q[0] >= 0.0 else -q
+        out.append((grid, config.parse_material(mat), lc))
+    return out
+
+
+class Batch:
+    """Solvers of a batch of RVEs and the batched forward / adjoint drivers."""
+
+    def __init__(self, items, numerics: config.Numerics | None = None):
+        self.items = items
+        self.lib = _lib.load()
+        self.solvers = [GridMechSpectralBasic(g, m, numerics) for g, m, _ in items]
+        self.cells = np.asarray([s.N for s in self.solvers], np.int64)
+        self._h = (C.c_void_p * len(self.solvers))(*[s.h.value for s in self.solvers])
+        self.threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+
+    def tape(self):
+        for s, (_, _, lc) in zip(self.solvers, self.items):
+            s.adjoint_tape(2 * lc.N + 16)
+
+    def run(self, threads: int | None = None) -> dict:
+        lcs = (_lib.LoadCase * len(self.items))(*[loadcase_struct(lc) for _, _, lc in self.items])
+        n = len(self.solvers)
+        rc = np.zeros(n, np.int32)
+        wall = np.zeros(n)
+        t0 = time.perf_counter()
+        r = self.lib.dmk_batch_run(self._h, n, lcs, self.cells.ctypes.data_as(C.POINTER(C.c_int64)),
+                                   threads or self.threads, _lib.iptr(rc), _lib.dptr(wall))
+        t = time.perf_counter() - t0
+        _lib.check(r, "dmk_batch_run")
+        return {"wall_s": t, "rve_wall_s": wall}
+
+    def curves(self) -> list[np.ndarray]:
+        return [adjoint._P33(s.adjoint_info()["P_av"]) for s in self.solvers]
+
+    def gradients(self, P_bar: list[np.ndarray], rtol: float = 1e-6, threads: int | None = None):
+        """Per RVE: gradient of sum(P_bar * P_av) with respect to its phase's parameters."""
+        n = len(self.solvers)
+        pb = [adjoint._P9(p) for p in P_bar]
+        tb = [np.zeros((1, ADJ_NOUT)) for _ in range(n)]
+        PB = (C.POINTER(C.c_double) * n)(*[_lib.dptr(p) for p in pb])
+        TB = (C.POINTER(C.c_double) * n)(*[_lib.dptr(t) for t in tb])
+        rc = np.zeros(n, np.int32)
+        wall = np.zeros(n)
+        t0 = time.perf_counter()
+        r = self.lib.dmk_batch_adjoint(self._h, n, PB, float(rtol), TB,
+                                       self.cells.ctypes.data_as(C.POINTER(C.c_int64)), threads or self.threads,
+                                       _lib.iptr(rc), _lib.dptr(wall))
+        t = time.perf_counter() - t0
+        _lib.check(r, "dmk_batch_adjoint")
+        grads = [adjoint.reduce_cotangent(s.material.phases[0], tb[i][0]) for i, s in enumerate(self.solvers)]
+        return grads, {"wall_s": t, "rve_wall_s": wall}
+
+    def close(self):
+        for s in self.solvers:
+            s.close()

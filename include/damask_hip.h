@@ -214,6 +214,20 @@ int dmk_adjoint_point_host(const dmk_phase_desc* d, double dt, const double F[9]
                            const double Fpbar[9], const double* xibar, double Fbar[9], double Fp0bar[9],
                            double* xi0bar, double* thbar, double A[81], double* residual);
 
+/* ---- Many RVEs.  No counterpart in the reference (the batched
+ * configuration of independent RVEs).  Each RVE is one handle with its own
+ * stream; a pool of n_threads host threads runs dmk_grid_run_loadcase(h[i],
+ * &lc[i]) for every i, largest cells[i] first (cells may be NULL).  rc_out
+ * and wall_s (optional, n entries) receive each RVE's return code and wall
+ * time.  Failures of single RVEs are reported through rc_out; the call then
+ * returns 950. */
+int dmk_batch_run(dmk_handle** h, int32_t n, const dmk_loadcase* lc, const int64_t* cells, int32_t n_threads,
+                  int32_t* rc_out, double* wall_s);
+/* dmk_adjoint_backward(h[i], P_av_bar[i], rtol, theta_bar[i], NULL) for every i, concurrently */
+int dmk_batch_adjoint(dmk_handle** h, int32_t n, const double* const* P_av_bar, double rtol,
+                      double* const* theta_bar, const int64_t* cells, int32_t n_threads, int32_t* rc_out,
+                      double* wall_s);
+
 void dmk_mech_destroy(dmk_handle* h);
 const char* dmk_last_error(void);
 int dmk_nsl(const dmk_handle* h);
