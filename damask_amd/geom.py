@@ -84,57 +84,211 @@ _DTYPES = {"Int8": np.int8, "UInt8": np.uint8, "Int16": np.int16, "UInt16": np.u
            "Float32": np.float32, "Float64": np.float64}
 
 
-def _decode(text: str, dtype, header_t, compressed: bool) -> np.ndarray:
-    hsz = np.dtype(header_t).itemsize
-    text = re.sub(r"\s+", "", text)
-    if not compressed:
-        raw = base64.b64decode(text)
-        n = int(np.frombuffer(raw[:hsz], header_t)[0])
-        return np.frombuffer(raw[hsz:hsz + n], dtype).copy()
-    # header: n_blocks, block size, last block size, compressed sizes; its own base64 run
-    head0 = base64.b64decode(text[:(((3 * hsz) + 2) // 3) * 4])
-    nb = int(np.frombuffer(head0[:hsz], header_t)[0])
-    hlen = (3 + nb) * hsz
-    hchars = ((hlen + 2) // 3) * 4
-    head = np.frombuffer(base64.b64decode(text[:hchars])[:hlen], header_t)
-    csz = head[3:3 + nb].astype(np.int64)
-    payload = base64.b64decode(text[hchars:])
-    out, off = [], 0
-    for k in range(nb):
-        out.append(zlib.decompress(payload[off:off + csz[k]]))
-        off += int(csz[k])
-    return np.frombuffer(b"".join(out), dtype).copy()
+class VTIError(ValueError):
+    """IO_error 844 (VTK) of the reference's VTI reader (src/grid/VTI.cpp:43)."""
+
+    def __init__(self, msg: str):
+        super().__init__(f"IO_error 844: {msg}")
+        self.code = 844
+        self.msg = msg
+
+
+class VTI:
+    """VTK ImageData (.vti) reader with the behaviour of src/grid/VTI.cpp: binary
+    DataArrays, base64 with UInt32/UInt64 headers, uncompressed or
+    vtkZLibDataCompressor blocks, little endian only."""
+
+    def __init__(self, text_or_path):
+        if isinstance(text_or_path, (bytes, bytearray)):
+            text = text_or_path.decode()
+        elif "<VTKFile" in str(text_or_path):
+            text = str(text_or_path)
+        else:
+            try:
+                with open(text_or_path) as f:
+                    text = f.read()
+            except OSError:
+                raise VTIError(f"cannot open file '{text_or_path}'") from None
+        try:
+            self.root = ET.fromstring(text)
+        except ET.ParseError as e:
+            raise VTIError(f"invalid XML ({e})") from None
+        if self.root.tag != "VTKFile":
+            raise VTIError("missing <VTKFile> element")
+        self.word = 8 if self.root.get("header_type", "UInt32") == "UInt64" else 4
+
+    @staticmethod
+    def read_word(b: bytes, n_bytes_per_word: int) -> int:
+        """little-endian unsigned word of 4 or 8 bytes (VTI.cpp:55-59)"""
+        return int.from_bytes(bytes(b[:n_bytes_per_word]), "little")
+
+    @staticmethod
+    def decode_b64(text: str) -> bytes:
+        """whitespace-free base64, possibly several padded runs back to back (VTK writes
+        the header of compressed data as a run of its own, VTI.cpp:315-350)"""
+        clean = re.sub(r"\s+", "", text).lstrip("=")
+        out = bytearray()
+        for run in re.findall(r"[^=]+=*", clean):
+            try:
+                out += base64.b64decode(run, validate=True)
+            except (ValueError, TypeError):
+                raise VTIError("base64 decode failed (invalid character or malformed input)") from None
+        return bytes(out)
+
+    @classmethod
+    def decode_uncompressed_vti(cls, text: str, n_bytes_per_word: int) -> bytes:
+        """blocks of (byte count, bytes) until a zero count or the end (VTI.cpp:111-131)"""
+        d = cls.decode_b64(text)
+        out, off = bytearray(), 0
+        while off + n_bytes_per_word <= len(d):
+            n = cls.read_word(d[off:], n_bytes_per_word)
+            off += n_bytes_per_word
+            if n == 0:
+                break
+            if off + n > len(d):
+                raise VTIError("VTI uncompressed: data block exceeds payload")
+            out += d[off:off + n]
+            off += n
+        return bytes(out)
+
+    @classmethod
+    def decode_compressed_vti(cls, text: str, n_bytes_per_word: int) -> bytes:
+        """header (blocks, block size, last block size, compressed sizes) followed by
+        the zlib blocks, all in one base64 run (VTI.cpp:61-109)"""
+        d = cls.decode_b64(text)
+        w = n_bytes_per_word
+        if len(d) < 3 * w:
+            raise VTIError("header for compressed VTI too short")
+        nb, bs, last = (cls.read_word(d[k * w:], w) for k in range(3))
+        hsz = 3 * w + nb * w
+        if len(d) < hsz:
+            raise VTIError("missing header for compressed VTI")
+        sizes = [cls.read_word(d[3 * w + k * w:], w) for k in range(nb)]
+        payload = d[hsz:]
+        out, off = bytearray(), 0
+        for k in range(nb):
+            if off + sizes[k] > len(payload):
+                raise VTIError(f"invalid VTI, defined size overflowing for block {k}")
+            want = last if (k + 1 == nb and last > 0) else bs
+            try:
+                blk = zlib.decompress(payload[off:off + sizes[k]])
+            except zlib.error:
+                raise VTIError(f"zlib inflate failed on block {k}") from None
+            if len(blk) != want:
+                raise VTIError(f"zlib inflate failed on block {k}")
+            out += blk
+            off += sizes[k]
+        return bytes(out)
+
+    def _check_format(self):
+        """VTI.cpp:149-161"""
+        r = self.root
+        if r.get("type", "") != "ImageData":
+            raise VTIError(f"type is not ImageData (got '{r.get('type', '')}')")
+        if r.get("byte_order", "") != "LittleEndian":
+            raise VTIError(f"byte_order must be 'LittleEndian' (got '{r.get('byte_order', '')}')")
+        comp = r.get("compressor", "")
+        if comp and comp != "vtkZLibDataCompressor":
+            raise VTIError(f"compressor is not vtkZLibDataCompressor (got '{comp}')")
+
+    def parse_cell_data_array(self, name: str) -> tuple[str, bytes]:
+        """(VTK type, raw bytes) of the cell DataArray `name` (VTI.cpp:265-300)"""
+        self._check_format()
+        img = self.root.find("ImageData")
+        if img is None:
+            raise VTIError("missing <ImageData> element")
+        cd = img.find("Piece/CellData")
+        if cd is None:
+            raise VTIError("missing <CellData> element")
+        arr = next((a for a in cd.findall("DataArray") if a.get("Name") == name), None)
+        if arr is None:
+            raise VTIError(f"no DataArray with Name='{name}' found")
+        if arr.get("format") != "binary":
+            raise VTIError(f"DataArray '{name}' is not binary")
+        vtk_type = arr.get("type")
+        if not vtk_type:
+            raise VTIError("DataArray missing 'type' attribute")
+        text = arr.text or ""
+        raw = (self.decode_compressed_vti(text, self.word) if self.root.get("compressor")
+               else self.decode_uncompressed_vti(text, self.word))
+        return vtk_type, raw
+
+    def read_geometry(self):
+        """cells, size = Spacing * cells, origin and the cell-data labels (VTI.cpp:163-250)"""
+        self._check_format()
+        img = self.root.find("ImageData")
+        if img is None:
+            raise VTIError("missing ImageData element")
+        d = img.get("Direction", "")
+        if d and d.split() != "1 0 0 0 1 0 0 0 1".split():
+            raise VTIError(f"unsupported 'Direction' (got '{d}')")
+        ext_s = img.get("WholeExtent", "")
+        if not ext_s:
+            raise VTIError("missing 'WholeExtent'")
+        try:
+            ext = [int(v) for v in ext_s.split()]
+        except ValueError:
+            ext = []
+        if len(ext) != 6 or ext[0] != 0 or ext[2] != 0 or ext[4] != 0:
+            raise VTIError(f"invalid 'WholeExtent' (got '{ext_s}')")
+
+        def three(field, s):
+            try:
+                v = [float(x) for x in s.split()[:3]]
+            except ValueError:
+                v = []
+            if len(v) != 3:
+                raise VTIError(f"bad numeric field for '{field}' (got '{s}')")
+            return np.array(v)
+        spacing = three("ImageData@Spacing", img.get("Spacing", ""))
+        origin = three("ImageData@Origin", img.get("Origin", ""))
+        cells = np.array([ext[1], ext[3], ext[5]], np.int64)
+        size = spacing * cells
+        if np.any(size <= 0):
+            raise VTIError("one or more entries <= 0 for 'size'")
+        if np.any(cells < 1):
+            raise VTIError("one or more entries < 1 for 'cells'")
+        labels = []
+        cd = img.find("Piece/CellData")
+        if cd is not None:
+            for a in cd.findall("DataArray"):
+                nm = a.get("Name", "")
+                if not nm:
+                    continue
+                if nm in labels:
+                    raise VTIError(f"repeated label '{nm}'")
+                labels.append(nm)
+        return cells, size, origin, labels
+
+    def read_dataset_int(self, name: str, dtype=np.int64) -> np.ndarray:
+        vtk_type, raw = self.parse_cell_data_array(name)
+        if vtk_type not in _DTYPES or _DTYPES[vtk_type] in (np.float32, np.float64):
+            raise VTIError(f"unsupported integer type for dataset '{name}'")
+        src = np.dtype(_DTYPES[vtk_type])
+        if len(raw) % src.itemsize:
+            raise VTIError("size mismatch")
+        return np.frombuffer(raw, src).astype(dtype)
+
+    def read_dataset_real(self, name: str) -> np.ndarray:
+        vtk_type, raw = self.parse_cell_data_array(name)
+        if vtk_type not in _DTYPES:
+            raise VTIError(f"unknown VTK type '{vtk_type}'")
+        src = np.dtype(_DTYPES[vtk_type])
+        if len(raw) % src.itemsize:
+            raise VTIError("size mismatch")
+        return np.frombuffer(raw, src).astype(np.float64)
 
 
 def read_vti(text_or_path) -> Grid:
-    if isinstance(text_or_path, (bytes, bytearray)):
-        text = text_or_path.decode()
-    elif "<VTKFile" in str(text_or_path):
-        text = str(text_or_path)
-    else:
-        with open(text_or_path) as f:
-            text = f.read()
-    root = ET.fromstring(text)
-    header_t = np.uint64 if root.get("header_type", "UInt32") == "UInt64" else np.uint32
-    compressed = root.get("compressor") is not None
-    img = root.find("ImageData")
-    ext = [int(v) for v in img.get("WholeExtent").split()]
-    cells = np.array([ext[1] - ext[0], ext[3] - ext[2], ext[5] - ext[4]], np.int64)
-    spacing = np.array([float(v) for v in img.get("Spacing").split()])
-    origin = np.array([float(v) for v in img.get("Origin").split()])
-    mat = None
-    for arr in img.iter("DataArray"):
-        if arr.get("Name") == "material":
-            dtype = _DTYPES[arr.get("type")]
-            if arr.get("format") == "ascii":
-                mat = np.array(arr.text.split(), dtype)
-            else:
-                mat = _decode(arr.text, dtype, header_t, compressed)
-    if mat is None:
-        raise ValueError("no 'material' cell data in VTI")
+    """Geometry and the 'material' cell data (DAMASK_grid's .vti input)."""
+    v = VTI(text_or_path)
+    cells, size, origin, labels = v.read_geometry()
+    if "material" not in labels:
+        raise VTIError("no DataArray with Name='material' found")
+    mat = v.read_dataset_int("material")
     if mat.size != int(np.prod(cells)):
-        raise ValueError("material array does not match the cell count")
-    return Grid(cells, cells * spacing, origin, mat.astype(np.int64))
+        raise VTIError("size mismatch")
+    return Grid(cells, size, origin, mat)
 
 
 def write_vti(grid: Grid, path=None, compress: bool = True) -> str:
