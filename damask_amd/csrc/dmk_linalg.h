@@ -4,10 +4,12 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+// XA = XM (1 - sum alpha) + sum_k alpha_k X_k
 struct CombineArgs {
   int l;
   double atot;
   double alpha[32];
+  const double* X[32];
 };
 
 struct ForwardArgs {
@@ -50,8 +52,7 @@ int dmk_colsum(const double* base, long ld, long n, int ncols, double* partial, 
 int dmk_mdot(const double* x, const double* Y, long ldY, int l, int mode, long n, double* partial, double* out,
              hipStream_t s);
 int dmk_axpby(long n, double a, const double* x, double b, const double* y, double* z, hipStream_t s);
-int dmk_combine(long n, const double* XM, const double* Xh, long ldX, const CombineArgs* c, double* XA,
-                hipStream_t s);
+int dmk_combine(long n, const double* XM, const CombineArgs* c, double* XA, hipStream_t s);
 // C_minMaxAvg from the per-cell norms nrm (written by the tangent kernel):
 // partial extrema, global extrema, per-chunk first qualifying cell, first overall, gather
 int dmk_cminmax(const double* dPdF, long N, double band, const double* nrm, double* pmax, double* pmin,

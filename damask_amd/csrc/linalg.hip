@@ -122,12 +122,11 @@ __global__ void axpby_kernel(long n, double a, const double* __restrict__ x, dou
   if (i < n) z[i] = a * x[i] + b * y[i];
 }
 
-__global__ void combine_kernel(long n, const double* __restrict__ XM, const double* __restrict__ Xh, long ldX,
-                               CombineArgs c, double* __restrict__ XA) {
+__global__ void combine_kernel(long n, const double* __restrict__ XM, CombineArgs c, double* __restrict__ XA) {
   long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   double v = XM[i] * (1.0 - c.atot);
-  for (int k = 0; k < c.l; k++) v = fma(c.alpha[k], Xh[k * ldX + i], v);
+  for (int k = 0; k < c.l; k++) v = fma(c.alpha[k], c.X[k][i], v);
   XA[i] = v;
 }
 
@@ -365,9 +364,8 @@ int dmk_axpby(long n, double a, const double* x, double b, const double* y, doub
   return hipGetLastError() == hipSuccess ? 0 : 2;
 }
 
-int dmk_combine(long n, const double* XM, const double* Xh, long ldX, const CombineArgs* c, double* XA,
-                hipStream_t s) {
-  hipLaunchKernelGGL(combine_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, XM, Xh, ldX, *c, XA);
+int dmk_combine(long n, const double* XM, const CombineArgs* c, double* XA, hipStream_t s) {
+  hipLaunchKernelGGL(combine_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, XM, *c, XA);
   return hipGetLastError() == hipSuccess ? 0 : 2;
 }
 

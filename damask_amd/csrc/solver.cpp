@@ -207,6 +207,7 @@ struct dmk_handle {
   double dt = 0;
   GammaArgs ga;
   std::vector<double> Q;
+  std::vector<double*> poolX, poolF;  // NGMRES iterate / residual buffers (M + 3 each)
   struct AdjState* adj = nullptr;    // adjoint tape and work buffers (dmk_adjoint_*)
   // driver state (DAMASK_grid.f90:78-100)
   double t = 0.0, Delta_t = 1.0, Delta_t_prev = 0.0;
@@ -464,9 +465,23 @@ static void add_dot(DotJobs* J, const double* x, const double* y, int mode) {
 // PETSc SNES NGMRES (snes/impls/ngmres, difference selection and restart,
 // basic line search on x^M = x - r(x)), restated; the dot products each step
 // needs are computed with its residual so that a residual evaluation costs
-// one host synchronisation.  X, F and the trial vectors swap by pointer.
+// one host synchronisation.  Iterates and residuals live in two fixed pools
+// of M + 3 buffers: the history slots, the current pair and the two trial
+// pairs are pointers into them, so a step moves no field (the current pair
+// enters the history by reference; trial buffers are whichever pool buffers
+// neither the current pair nor the history holds).
+static double* pool_free(const std::vector<double*>& pool, const double* cur, double* const* hist, int l,
+                         const double* also) {
+  for (double* b : pool) {
+    if (b == cur || b == also) continue;
+    bool used = false;
+    for (int i = 0; i < l && !used; i++) used = hist[i] == b;
+    if (!used) return b;
+  }
+  return nullptr;
+}
+
 static int solve_ngmres(dmk_handle* h, int* reason_out) {
-  const long n = h->n9;
   const int M = h->num.ngmres_m;
   const double gammaA = 2.0, gammaC = 2.0, epsB = 0.1, deltaB = 0.9;
   const int restart_it = 2;
@@ -480,18 +495,24 @@ static int solve_ngmres(dmk_handle* h, int* reason_out) {
   int reason = converged(h);
   if (reason || fnorm != fnorm) { *reason_out = reason ? reason : -1; return 0; }
   double fminnorm = fnorm;
-  // history slot 0 <- (X, F)
-  CK(hipMemcpyAsync(h->Xh, h->F, sizeof(double) * n, hipMemcpyDeviceToDevice, h->st));
-  CK(hipMemcpyAsync(h->Fh, h->R, sizeof(double) * n, hipMemcpyDeviceToDevice, h->st));
+  std::vector<double*> Xs(M, nullptr), Fs(M, nullptr);
+  // history slot 0 <- (X, F), by reference
+  Xs[0] = h->F;
+  Fs[0] = h->R;
   h->Q[0] = d[0];
   int k_restart = 1, l = 1, restart_count = 0;
   for (int k = 1; k <= h->num.snes_max_it; k++) {
     int ivec = k_restart % M;
-    CKL(dmk_axpby(n, 1.0, h->F, -1.0, h->R, h->XM, h->st));
+    double* XM = pool_free(h->poolX, h->F, Xs.data(), l, nullptr);
+    double* FM = pool_free(h->poolF, h->R, Fs.data(), l, nullptr);
+    double* XA = pool_free(h->poolX, h->F, Xs.data(), l, XM);
+    double* FA = pool_free(h->poolF, h->R, Fs.data(), l, FM);
+    if (!XM || !FM || !XA || !FA) { dmk_set_error("NGMRES buffer pool exhausted"); return DMK_ERR_CONFIG; }
+    CKL(dmk_axpby(h->n9, 1.0, h->F, -1.0, h->R, XM, h->st));
     J.n = 0;
-    add_dot(&J, h->FM, h->FM, 0);
-    for (int i = 0; i < l; i++) add_dot(&J, h->FM, h->Fh + (size_t)i * n, 0);
-    if ((rc = residual_enqueue(h, h->XM, h->FM, k - 1)) || (rc = residual_finish(h, &J, d))) return rc;
+    add_dot(&J, FM, FM, 0);
+    for (int i = 0; i < l; i++) add_dot(&J, FM, Fs[i], 0);
+    if ((rc = residual_enqueue(h, XM, FM, k - 1)) || (rc = residual_finish(h, &J, d))) return rc;
     double fMnorm = std::sqrt(d[0]);
     double xi[32];
     for (int i = 0; i < l; i++) xi[i] = d[1 + i];
@@ -504,14 +525,14 @@ static int solve_ngmres(dmk_handle* h, int* reason_out) {
     CombineArgs ca;
     ca.l = l;
     ca.atot = 0.0;
-    for (int i = 0; i < l; i++) { ca.alpha[i] = alpha[i]; ca.atot += alpha[i]; }
-    CKL(dmk_combine(n, h->XM, h->Xh, n, &ca, h->XA, h->st));
+    for (int i = 0; i < l; i++) { ca.alpha[i] = alpha[i]; ca.atot += alpha[i]; ca.X[i] = Xs[i]; }
+    CKL(dmk_combine(h->n9, XM, &ca, XA, h->st));
     J.n = 0;
-    add_dot(&J, h->FA, h->FA, 0);
-    add_dot(&J, h->XA, h->XM, 1);
-    for (int i = 0; i < l; i++) add_dot(&J, h->XA, h->Xh + (size_t)i * n, 1);
-    for (int i = 0; i < l; i++) add_dot(&J, h->FA, h->Fh + (size_t)i * n, 0);
-    if ((rc = residual_enqueue(h, h->XA, h->FA, k - 1)) || (rc = residual_finish(h, &J, d))) return rc;
+    add_dot(&J, FA, FA, 0);
+    add_dot(&J, XA, XM, 1);
+    for (int i = 0; i < l; i++) add_dot(&J, XA, Xs[i], 1);
+    for (int i = 0; i < l; i++) add_dot(&J, FA, Fs[i], 0);
+    if ((rc = residual_enqueue(h, XA, FA, k - 1)) || (rc = residual_finish(h, &J, d))) return rc;
     double fAnorm = std::sqrt(d[0]);
     double dnorm = std::sqrt(d[1]);
     double dminnorm = -1.0, eta[32];
@@ -523,40 +544,31 @@ static int solve_ngmres(dmk_handle* h, int* reason_out) {
     bool selectA = true;
     if (fAnorm >= gammaA * fminnorm) selectA = false;
     if (!(epsB * dnorm < dminnorm || std::sqrt(fnorm) < deltaB * std::sqrt(fminnorm))) selectA = false;
-    // the M pair before the swap (needed by a restart)
-    double *XMv = h->XM, *FMv = h->FM;
     const double* fsel_dots = selectA ? eta : xi;
     double fsel2 = selectA ? fAnorm * fAnorm : nu;
-    if (selectA) {
-      std::swap(h->F, h->XA);
-      std::swap(h->R, h->FA);
-      fnorm = fAnorm;
-    } else {
-      std::swap(h->F, h->XM);
-      std::swap(h->R, h->FM);
-      fnorm = fMnorm;
-      XMv = h->F;
-      FMv = h->R;
-    }
+    h->F = selectA ? XA : XM;
+    h->R = selectA ? FA : FM;
+    fnorm = selectA ? fAnorm : fMnorm;
     bool selectRestart = false;
     if (epsB * dnorm > dminnorm && std::sqrt(fAnorm) > deltaB * std::sqrt(fminnorm) && l > 0) selectRestart = true;
     if (std::sqrt(fAnorm) > gammaC * std::sqrt(fminnorm)) selectRestart = true;
     restart_count = selectRestart ? restart_count + 1 : 0;
     if (restart_count >= restart_it) {
+      // history restarts from the M pair
       restart_count = 0;
       k_restart = 1;
       l = 1;
-      CK(hipMemcpyAsync(h->Xh, XMv, sizeof(double) * n, hipMemcpyDeviceToDevice, h->st));
-      CK(hipMemcpyAsync(h->Fh, FMv, sizeof(double) * n, hipMemcpyDeviceToDevice, h->st));
+      Xs[0] = XM;
+      Fs[0] = FM;
       h->Q[0] = nu;
     } else {
       int l_old = l;
       if (l < M) l++;
       k_restart++;
       if (fminnorm > fnorm) fminnorm = fnorm;
-      // history slot ivec <- (X, F); Gram entries against the other slots
-      CK(hipMemcpyAsync(h->Xh + (size_t)ivec * n, h->F, sizeof(double) * n, hipMemcpyDeviceToDevice, h->st));
-      CK(hipMemcpyAsync(h->Fh + (size_t)ivec * n, h->R, sizeof(double) * n, hipMemcpyDeviceToDevice, h->st));
+      // history slot ivec <- (X, F), by reference; Gram entries against the other slots
+      Xs[ivec] = h->F;
+      Fs[ivec] = h->R;
       for (int i = 0; i < l; i++) {
         double v = i == ivec ? fsel2 : (i < l_old ? fsel_dots[i] : 0.0);
         h->Q[ivec + M * i] = v;
@@ -1055,6 +1067,12 @@ static int init_impl(dmk_handle* h, const int32_t cells[3], const double size[3]
   for (auto pp : sol) if ((rc = dalloc(h, pp, n9))) return rc;
   if ((rc = dalloc(h, &h->Xh, (size_t)h->num.ngmres_m * n9))) return rc;
   if ((rc = dalloc(h, &h->Fh, (size_t)h->num.ngmres_m * n9))) return rc;
+  h->poolX = {h->F, h->XM, h->XA};
+  h->poolF = {h->R, h->FM, h->FA};
+  for (int i = 0; i < h->num.ngmres_m; i++) {
+    h->poolX.push_back(h->Xh + (size_t)i * n9);
+    h->poolF.push_back(h->Fh + (size_t)i * n9);
+  }
   if ((rc = dalloc(h, &h->Z, 9 * h->Nk))) return rc;
   if ((rc = dalloc(h, &h->U, 3 * h->Nk))) return rc;
   if ((rc = dalloc(h, &h->u3, 3 * N))) return rc;
