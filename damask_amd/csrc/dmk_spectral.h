@@ -9,11 +9,14 @@ struct FftPlan {
   int radix[16];
 };
 
+// tile shapes: gx[0] = line pairs per block of the x pass, gx[1], gx[2] =
+// lines per block of the y and z passes (picked by dmk_fft_tune)
 struct DmkFFT {
   int n[3];
   int h;
   FftPlan plan[3];
   double2* tw[3] = {nullptr, nullptr, nullptr};
+  int gx[3] = {0, 0, 0};
 };
 
 struct GammaArgs {
@@ -35,6 +38,9 @@ struct CoordsArgs {
 };
 
 int dmk_fft_setup(DmkFFT* fft, const int cells[3]);
+// time the candidate tile shapes on the device (scratch: 9 Nk complex and 9 N
+// real doubles, overwritten) and keep the fastest per pass
+int dmk_fft_tune(DmkFFT* fft, double2* cscratch, double* rscratch, hipStream_t s);
 void dmk_fft_free(DmkFFT* fft);
 int dmk_fft_r2c(const DmkFFT* fft, int howmany, const double* in, double2* out, hipStream_t s);
 int dmk_fft_c2r(const DmkFFT* fft, int howmany, double2* in, double* out, hipStream_t s);

@@ -1074,6 +1074,9 @@ static int init_impl(dmk_handle* h, const int32_t cells[3], const double size[3]
     h->poolF.push_back(h->Fh + (size_t)i * n9);
   }
   if ((rc = dalloc(h, &h->Z, 9 * h->Nk))) return rc;
+  // FFT tile shapes for this grid, timed on the device (no effect on results:
+  // every line is transformed with the same operations whatever the tiling)
+  if (dmk_fft_tune(&h->fft, h->Z, h->Fext, h->st)) { dmk_set_error("FFT tuning failed"); return DMK_ERR_HIP; }
   if ((rc = dalloc(h, &h->U, 3 * h->Nk))) return rc;
   if ((rc = dalloc(h, &h->u3, 3 * N))) return rc;
   long nn = (long)(cells[0] + 1) * (cells[1] + 1) * (cells[2] + 1);

@@ -438,6 +438,15 @@ static FftPlan make_plan(int L) {
   return p;
 }
 
+static int x_pairs_per_block(int L) { int p = 1024 / L; return p < 1 ? 1 : p; }
+// lines per block of a strided pass covering h lines: the chunks are as even
+// as possible (h = 33 in groups of 16 would leave a block with one line)
+static int gx_even(int h, int gmax) {
+  int nch = (h + gmax - 1) / gmax;
+  return (h + nch - 1) / nch;
+}
+static int gx_for(int L) { int g = 1024 / L; if (g > 32) g = 32; return g < 1 ? 1 : g; }
+
 int dmk_fft_setup(DmkFFT* fft, const int cells[3]) {
   for (int d = 0; d < 3; d++) {
     fft->n[d] = cells[d];
@@ -454,6 +463,8 @@ int dmk_fft_setup(DmkFFT* fft, const int cells[3]) {
     free(host);
   }
   fft->h = cells[0] / 2 + 1;
+  fft->gx[0] = x_pairs_per_block(cells[0]);
+  for (int d = 1; d < 3; d++) fft->gx[d] = gx_even(fft->h, gx_for(cells[d]));
   return 0;
 }
 
@@ -461,57 +472,97 @@ void dmk_fft_free(DmkFFT* fft) {
   for (int d = 0; d < 3; d++) if (fft->tw[d]) hipFree(fft->tw[d]);
 }
 
-static int x_pairs_per_block(int L) { int p = 1024 / L; return p < 1 ? 1 : p; }
-static int gx_for(int L) { int g = 1024 / L; if (g > 32) g = 32; return g < 1 ? 1 : g; }
+static const size_t FFT_LDS_MAX = 64 * 1024;
 
-int dmk_fft_r2c(const DmkFFT* fft, int howmany, const double* in, double2* out, hipStream_t s) {
-  const int N1 = fft->n[0], N2 = fft->n[1], N3 = fft->n[2], h = fft->h;
-  const long Nk = (long)h * N2 * N3;
-  int nl = howmany * N2 * N3;
-  int ppb = x_pairs_per_block(N1);
+static void launch_x_r2c(const DmkFFT* fft, int howmany, const double* in, double2* out, int ppb, hipStream_t s) {
+  const int N1 = fft->n[0];
+  int nl = howmany * fft->n[1] * fft->n[2];
   int nblk = (nl + 2 * ppb - 1) / (2 * ppb);
   size_t lds = sizeof(double2) * 2 * ppb * (N1 + 1);
   hipLaunchKernelGGL(fft_x_r2c, dim3(nblk), dim3(256), lds, s, in, out, nl, fft->plan[0], fft->tw[0], ppb);
-  if (N2 > 1) {
-    int gx = gx_for(N2);
-    int nch = (h + gx - 1) / gx;
-    size_t l2 = sizeof(double2) * 2 * gx * (N2 + 1);
-    hipLaunchKernelGGL(fft_strided, dim3(howmany * N3 * nch), dim3(256), l2, s, out, h, gx, N3, Nk, (long)h * N2,
-                       (long)h, fft->plan[1], fft->tw[1], -1);
+}
+
+static void launch_x_c2r(const DmkFFT* fft, int howmany, const double2* in, double* out, int ppb, hipStream_t s) {
+  const int N1 = fft->n[0];
+  int nl = howmany * fft->n[1] * fft->n[2];
+  int nblk = (nl + 2 * ppb - 1) / (2 * ppb);
+  size_t lds = sizeof(double2) * 2 * ppb * (N1 + 1);
+  hipLaunchKernelGGL(fft_x_c2r, dim3(nblk), dim3(256), lds, s, in, out, nl, fft->plan[0], fft->tw[0], ppb);
+}
+
+// y (axis 1) or z (axis 2) pass over the half spectra, in place
+static void launch_strided(const DmkFFT* fft, int howmany, double2* data, int axis, int gx, int sign, hipStream_t s) {
+  const int N2 = fft->n[1], N3 = fft->n[2], h = fft->h;
+  const long Nk = (long)h * N2 * N3;
+  const int L = fft->n[axis];
+  int nch = (h + gx - 1) / gx;
+  size_t lds = sizeof(double2) * 2 * gx * (L + 1);
+  if (axis == 1)
+    hipLaunchKernelGGL(fft_strided, dim3(howmany * N3 * nch), dim3(256), lds, s, data, h, gx, N3, Nk, (long)h * N2,
+                       (long)h, fft->plan[1], fft->tw[1], sign);
+  else
+    hipLaunchKernelGGL(fft_strided, dim3(howmany * N2 * nch), dim3(256), lds, s, data, h, gx, N2, Nk, (long)h,
+                       (long)h * N2, fft->plan[2], fft->tw[2], sign);
+}
+
+int dmk_fft_tune(DmkFFT* fft, double2* cscratch, double* rscratch, hipStream_t s) {
+  const int howmany = 9;
+  const long Nk = (long)fft->h * fft->n[1] * fft->n[2], N = (long)fft->n[0] * fft->n[1] * fft->n[2];
+  if (hipMemsetAsync(cscratch, 0, sizeof(double2) * howmany * Nk, s) != hipSuccess) return 2;
+  if (hipMemsetAsync(rscratch, 0, sizeof(double) * howmany * N, s) != hipSuccess) return 2;
+  hipEvent_t e0, e1;
+  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) return 2;
+  auto timed = [&](auto&& launch) {
+    launch();                                    // warm-up
+    hipEventRecord(e0, s);
+    for (int r = 0; r < 3; r++) launch();
+    hipEventRecord(e1, s);
+    hipEventSynchronize(e1);
+    float ms = 0.0f;
+    hipEventElapsedTime(&ms, e0, e1);
+    return ms;
+  };
+  // x pass: line pairs per block
+  {
+    const int L = fft->n[0];
+    float best = 1e30f;
+    for (int ppb : {1024 / L, 512 / L, 256 / L, 2048 / L}) {
+      if (ppb < 1 || sizeof(double2) * 2 * ppb * (L + 1) > FFT_LDS_MAX) continue;
+      float t = timed([&] { launch_x_r2c(fft, howmany, rscratch, cscratch, ppb, s); });
+      if (t < best) { best = t; fft->gx[0] = ppb; }
+    }
   }
-  if (N3 > 1) {
-    int gx = gx_for(N3);
-    int nch = (h + gx - 1) / gx;
-    size_t l3 = sizeof(double2) * 2 * gx * (N3 + 1);
-    hipLaunchKernelGGL(fft_strided, dim3(howmany * N2 * nch), dim3(256), l3, s, out, h, gx, N2, Nk, (long)h,
-                       (long)h * N2, fft->plan[2], fft->tw[2], -1);
+  // strided passes: even chunkings of the h lines
+  for (int axis = 1; axis <= 2; axis++) {
+    const int L = fft->n[axis];
+    if (L <= 1) continue;
+    float best = 1e30f;
+    int last = -1;
+    for (int nch = 1; nch <= fft->h && nch <= 16; nch++) {
+      int gx = (fft->h + nch - 1) / nch;
+      if (gx == last || sizeof(double2) * 2 * gx * (L + 1) > FFT_LDS_MAX) continue;
+      last = gx;
+      float t = timed([&] { launch_strided(fft, howmany, cscratch, axis, gx, -1, s); });
+      if (t < best) { best = t; fft->gx[axis] = gx; }
+    }
   }
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  return hipGetLastError() == hipSuccess ? 0 : 2;
+}
+
+int dmk_fft_r2c(const DmkFFT* fft, int howmany, const double* in, double2* out, hipStream_t s) {
+  launch_x_r2c(fft, howmany, in, out, fft->gx[0], s);
+  if (fft->n[1] > 1) launch_strided(fft, howmany, out, 1, fft->gx[1], -1, s);
+  if (fft->n[2] > 1) launch_strided(fft, howmany, out, 2, fft->gx[2], -1, s);
   return hipGetLastError() == hipSuccess ? 0 : 2;
 }
 
 // in is overwritten (the y and z passes run in place)
 int dmk_fft_c2r(const DmkFFT* fft, int howmany, double2* in, double* out, hipStream_t s) {
-  const int N1 = fft->n[0], N2 = fft->n[1], N3 = fft->n[2], h = fft->h;
-  const long Nk = (long)h * N2 * N3;
-  if (N3 > 1) {
-    int gx = gx_for(N3);
-    int nch = (h + gx - 1) / gx;
-    size_t l3 = sizeof(double2) * 2 * gx * (N3 + 1);
-    hipLaunchKernelGGL(fft_strided, dim3(howmany * N2 * nch), dim3(256), l3, s, in, h, gx, N2, Nk, (long)h,
-                       (long)h * N2, fft->plan[2], fft->tw[2], +1);
-  }
-  if (N2 > 1) {
-    int gx = gx_for(N2);
-    int nch = (h + gx - 1) / gx;
-    size_t l2 = sizeof(double2) * 2 * gx * (N2 + 1);
-    hipLaunchKernelGGL(fft_strided, dim3(howmany * N3 * nch), dim3(256), l2, s, in, h, gx, N3, Nk, (long)h * N2,
-                       (long)h, fft->plan[1], fft->tw[1], +1);
-  }
-  int nl = howmany * N2 * N3;
-  int ppb = x_pairs_per_block(N1);
-  int nblk = (nl + 2 * ppb - 1) / (2 * ppb);
-  size_t lds = sizeof(double2) * 2 * ppb * (N1 + 1);
-  hipLaunchKernelGGL(fft_x_c2r, dim3(nblk), dim3(256), lds, s, in, out, nl, fft->plan[0], fft->tw[0], ppb);
+  if (fft->n[2] > 1) launch_strided(fft, howmany, in, 2, fft->gx[2], +1, s);
+  if (fft->n[1] > 1) launch_strided(fft, howmany, in, 1, fft->gx[1], +1, s);
+  launch_x_c2r(fft, howmany, in, out, fft->gx[0], s);
   return hipGetLastError() == hipSuccess ? 0 : 2;
 }
 
