@@ -80,3 +80,39 @@ def test_autograd_curve_loss(gpu_lib):
     g, _ = adjoint.curve_gradient(s, 0, Pb)
     s.close()
     assert np.allclose(theta.grad.numpy(), g, rtol=1e-8, atol=1e-12 * np.abs(g).max())
+
+
+def test_adjoint_gradient_matches_oracle_differences(gpu_lib):
+    """The GPU adjoint against central differences of the CPU oracle's own run
+    (its restated NGMRES solver and material point), an implementation that
+    shares no code with the GPU path"""
+    from damask_amd import adjoint
+    from oracle import pyoracle
+    grid, mat, lcs, num = _case()
+    s, P = adjoint.run_curve(grid, mat, lcs, num)
+    n = P.shape[0]
+    w = np.linspace(1.0, 2.0, n) / 1e6
+    g, _ = adjoint.curve_gradient(s, 0, _Pbar(n, w), rtol=1e-12)
+    s.close()
+    theta0 = adjoint.get_params(mat.phases[0])
+
+    def oracle_loss(th):
+        m2 = copy.deepcopy(mat)
+        m2.phases[0] = adjoint.set_params(mat.phases[0], th)
+        pr = pyoracle.Problem(grid, m2, num, threads=8)
+        r = pr.run_loadcase(lcs[0])
+        pr.close()
+        assert r["rc"] == 0
+        return _loss(adjoint._P33(r["P_av"]), w)
+
+    fd = np.zeros_like(theta0)
+    for k in range(theta0.size):
+        tp, tm = theta0.copy(), theta0.copy()
+        tp[k] *= 1.0 + 1e-4
+        tm[k] *= 1.0 - 1e-4
+        fd[k] = (oracle_loss(tp) - oracle_loss(tm)) / (2e-4 * theta0[k])
+    names = adjoint.param_names(mat.phases[0])
+    for k in range(theta0.size):
+        print(f"{names[k]:12s} adjoint {g[k]: .10e}  oracle fd {fd[k]: .10e}  rel {abs(g[k] - fd[k]) / abs(fd[k]):.2e}")
+    scale = np.abs(fd * theta0).max()
+    assert np.all(np.abs(g - fd) * np.abs(theta0) <= 1e-5 * np.abs(fd * theta0) + 1e-7 * scale)
