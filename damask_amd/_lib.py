@@ -34,7 +34,9 @@ class Numerics(C.Structure):
                 ("r_cutback_min", C.c_double), ("r_cutback", C.c_double), ("r_increase", C.c_double),
                 ("eps_rel_state", C.c_double), ("N_iter_state_max", C.c_int32), ("N_iter_Lp_max", C.c_int32),
                 ("r_linesearch_Lp", C.c_double), ("eps_rel_Lp", C.c_double), ("eps_abs_Lp", C.c_double),
-                ("f_update_jacobi_Lp", C.c_int32), ("cref_rule", C.c_int32)]
+                ("f_update_jacobi_Lp", C.c_int32), ("cref_rule", C.c_int32),
+                ("scheme", C.c_int32), ("alpha", C.c_double), ("beta", C.c_double),
+                ("eps_abs_curl", C.c_double), ("eps_rel_curl", C.c_double)]
 
 
 class LoadCase(C.Structure):
@@ -52,7 +54,7 @@ EXPORTS = ["dmk_mech_init", "dmk_mech_forward", "dmk_mech_materialpoint_forward"
            "dmk_mech_profile", "dmk_mech_kernel_time", "dmk_slip_system", "dmk_adjoint_tape",
            "dmk_adjoint_record", "dmk_adjoint_info", "dmk_adjoint_backward", "dmk_adjoint_release",
            "dmk_adjoint_point_host", "dmk_mech_restart_response", "dmk_grid_set_restart", "dmk_batch_run",
-           "dmk_batch_adjoint"]
+           "dmk_batch_adjoint", "dmk_curl_rms", "dmk_inv_sym3333"]
 
 _lib = None
 
@@ -106,6 +108,10 @@ def load(path: str = LIB_PATH):
     L.dmk_gamma_convolution.argtypes = [vp, dp, dp, dp]
     L.dmk_divergence_rms.restype = C.c_int
     L.dmk_divergence_rms.argtypes = [vp, dp, dp]
+    L.dmk_curl_rms.restype = C.c_int
+    L.dmk_curl_rms.argtypes = [vp, dp, dp]
+    L.dmk_inv_sym3333.restype = C.c_int
+    L.dmk_inv_sym3333.argtypes = [dp, dp]
     L.dmk_fft_forward.restype = C.c_int
     L.dmk_fft_forward.argtypes = [vp, C.c_int32, dp, dp]
     L.dmk_fft_backward.restype = C.c_int

@@ -15,7 +15,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libdamask_hip.so")
-SOURCES = ["constitutive.hip", "spectral.hip", "linalg.hip", "adjoint.hip", "crystal.cpp", "solver.cpp", "batch.cpp"]
+SOURCES = ["constitutive.hip", "spectral.hip", "linalg.hip", "adjoint.hip", "polarization.hip", "crystal.cpp", "solver.cpp",
+           "batch.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]

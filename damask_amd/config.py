@@ -328,6 +328,11 @@ class Numerics:
     eps_abs_P: float = 1.0e3
     eps_rel_P: float = 1.0e-3
     update_gamma: bool = False
+    # spectral_polarization only (grid_mech_spectral_polarization.f90:136-147)
+    eps_abs_curl: float = 1.0e-10
+    eps_rel_curl: float = 5.0e-4
+    alpha: float = 1.0
+    beta: float = 1.0
     # solver:grid (DAMASK_grid.f90:147-150, spectral_utilities.f90:176-197)
     N_cutback_max: int = 3
     N_staggered_iter_max: int = 10
@@ -360,6 +365,10 @@ class Numerics:
         if self.eps_rel_div <= 0: bad.append("eps_rel_div(P)")
         if self.eps_abs_P <= 0: bad.append("eps_abs_P")
         if self.eps_rel_P <= 0: bad.append("eps_rel_P")
+        if self.eps_abs_curl <= 0: bad.append("eps_abs_curl(F)")
+        if self.eps_rel_curl <= 0: bad.append("eps_rel_curl(F)")
+        if self.alpha <= 0 or self.alpha > 2: bad.append("alpha")
+        if self.beta < 0 or self.beta > 2: bad.append("beta")
         if self.N_iter_max < 1: bad.append("N_iter_max")
         if self.N_iter_min > self.N_iter_max or self.N_iter_min < 1: bad.append("N_iter_min")
         if self.r_cutback_min <= 0: bad.append("r_cutback_min")
@@ -386,7 +395,9 @@ def parse_numerics(text_or_dict=None) -> Numerics:
     for key, attr, typ in (("N_iter_min", "N_iter_min", int), ("N_iter_max", "N_iter_max", int),
                            ("eps_abs_div(P)", "eps_abs_div", float), ("eps_rel_div(P)", "eps_rel_div", float),
                            ("eps_abs_P", "eps_abs_P", float), ("eps_rel_P", "eps_rel_P", float),
-                           ("update_gamma", "update_gamma", bool)):
+                           ("update_gamma", "update_gamma", bool),
+                           ("eps_abs_curl(F)", "eps_abs_curl", float), ("eps_rel_curl(F)", "eps_rel_curl", float),
+                           ("alpha", "alpha", float), ("beta", "beta", float)):
         if key in gm:
             setattr(n, attr, typ(gm[key]))
     petsc = str(gm.get("PETSc_options", ""))

@@ -39,7 +39,7 @@ struct AimArgs {
   int32_t rotate;           // rotate P_av by q_R
   int32_t update;           // update F_aim and the aim (0: report P_av and flag only)
 };
-enum { REP_PAV = 0, REP_FAIM = 9, REP_ERRBC = 18, REP_FLAG = 19, REP_DIV2 = 20, REP_DOTS = 24, REP_SIZE = 24 + DMK_MAX_DOTS };
+enum { REP_PAV = 0, REP_FAIM = 9, REP_ERRBC = 18, REP_FLAG = 19, REP_DIV2 = 20, REP_CURL2 = 21, REP_DOTS = 24, REP_SIZE = 24 + DMK_MAX_DOTS };
 
 int dmk_nchunk(long n);
 int dmk_dots(const DotJobs* jobs, long n, double* partial, double* out, hipStream_t s);

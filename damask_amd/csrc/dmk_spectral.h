@@ -46,5 +46,8 @@ int dmk_fft_r2c(const DmkFFT* fft, int howmany, const double* in, double2* out, 
 int dmk_fft_c2r(const DmkFFT* fft, int howmany, double2* in, double* out, hipStream_t s);
 int dmk_gamma_div(double2* Z, const GammaArgs* g, double* div_partial, hipStream_t s);
 int dmk_gamma_div_blocks(const GammaArgs* g);
+// sum of squared curl amplitudes of a forward transform, per block of 256
+// frequencies (dmk_gamma_div_blocks partial sums)
+int dmk_curl_partial(const double2* Z, const GammaArgs* g, double* partial, hipStream_t s);
 int dmk_coords_freq(const double2* Fh, double2* U, const GammaArgs* g, hipStream_t s);
 int dmk_coords_nodes(const double* u, const CoordsArgs* c, double* x_n, double* x_p, hipStream_t s);

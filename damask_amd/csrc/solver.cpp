@@ -26,6 +26,7 @@
 #include "dmk_rot.h"
 #include "dmk_linalg.h"
 #include "dmk_spectral.h"
+#include "dmk_polar.h"
 #include "../../include/damask_hip.h"
 
 int dmk_build_phase(const dmk_phase_desc* d, DmkPhaseDev* p);
@@ -200,7 +201,14 @@ struct dmk_handle {
   // solver state
   double F_aimDot[9], F_aim[9], F_aim_lastinc[9], P_av[9], P_aim[9];
   double C_volAvg[81], C_volAvgLastInc[81], C_minMaxAvg[81], C_minMaxAvgLastInc[81], S[81];
-  double err_div = 0, err_BC = 0;
+  double C_minMaxAvgRestart[81];     // reference stiffness of Gamma, what restartWrite stores
+  double err_div = 0, err_BC = 0, err_curl = 0;
+  // spectral-polarization scheme (scheme 1): the iterate is [F; F_tau], 2 n9
+  // doubles; C_scale / S_scale and alpha, beta travel in pol
+  int scheme = 0;
+  long nx = 0;
+  PolArgs pol{};
+  double* T = nullptr;
   int status = 0, totalIter = 0, nfuncs = 0;
   int stress_mask[9];
   double q_R[4];
@@ -254,7 +262,8 @@ static int dalloc(dmk_handle* h, T** p, size_t n) {
 enum { SM_PAV = 0, SM_CVOL = 9, SM_CMM = 90, SM_DOTS = 180, SM_REP = 256, SM_FAIM = 352, SM_AIM = 368,
        // adjoint: aim bookkeeping, report, mean, dot products, maxima, tangent average
        SM_AFAIM = 384, SM_AAIM = 400, SM_AREP = 416, SM_APAV = 504, SM_ADOTS = 520, SM_AMAX = 584, SM_ACAV = 592,
-       SM_SIZE = 680 };
+       // polarization: mean of the compatible field F
+       SM_FAV = 680, SM_SIZE = 696 };
 static_assert(SM_AREP + REP_SIZE <= SM_APAV, "adjoint report region");
 static_assert(SM_ADOTS + DMK_MAX_DOTS <= SM_AMAX, "adjoint dot region");
 static_assert(SM_REP + REP_SIZE <= SM_FAIM, "report region");
@@ -381,14 +390,25 @@ static int masked_compliance(dmk_handle* h) {
   return masked_compliance_of(h->C_volAvg, h->stress_mask, h->q_R, h->S);
 }
 
+// C_scale = C_minMaxAvg, S_scale = invSym3333(C_minMaxAvg)
+// (grid_mech_spectral_polarization.f90:266-268, :301-305)
+static int set_scale(dmk_handle* h) {
+  std::memcpy(h->pol.C, h->C_minMaxAvg, sizeof h->pol.C);
+  if (dmk_inv_sym3333(h->C_minMaxAvg, h->pol.S)) { dmk_set_error("matrix inversion error (S_scale)"); return DMK_ERR_CONFIG; }
+  return 0;
+}
+
 // formResidual (grid_mech_spectral_basic.f90:464-521), enqueued: response,
 // stress-BC update of F_aim and the aim on the device, one forward FFT for
 // the divergence RMS and the Gamma convolution, inverse FFT into r.
+static int residual_enqueue_pol(dmk_handle* h, const double* X, double* r);
+
 static int residual_enqueue(dmk_handle* h, const double* X, double* r, int iter) {
   if (h->nfuncs == 0 && iter == 0) h->totalIter = -1;
   if (h->totalIter <= iter) h->totalIter++;
   h->nfuncs++;
   h->n_residuals++;
+  if (h->scheme == 1) return residual_enqueue_pol(h, X, r);
   int rc = response_enqueue(h, X, h->dt);
   if (rc) return rc;
   AimArgs a;
@@ -412,12 +432,66 @@ static int residual_enqueue(dmk_handle* h, const double* X, double* r, int iter)
   return 0;
 }
 
+// formResidual of the polarization scheme (grid_mech_spectral_polarization.f90
+// :530-634), enqueued.  X = [F; F_tau], r = [r_F; r_F_tau]:
+//   r_F_tau = beta F - Gamma(beta C:(F - I) - alpha F (C:(F_tau - F - I))),
+//             the convolution's mean set to rotate(beta F_aim);
+//   response at F - r_F_tau / beta (written into r_F's slot), curl RMS of F,
+//   divergence RMS of P, consistent tangent of every cell;
+//   r_F = invSym3333(dPdF + C):(P - F (C:(F_tau - F - I))) + r_F_tau;
+//   F_aim -= S:(P_av - P_aim) and err_BC on the device.
+static int residual_enqueue_pol(dmk_handle* h, const double* X, double* r) {
+  const long N = h->N, n9 = h->n9;
+  const double* F = X;
+  const double* Ft = X + n9;
+  double* rF = r;
+  double* rt = r + n9;
+  CKL(dmk_colsum(F, N, N, 9, h->partial, h->small + SM_FAV, h->wgt, h->st));
+  CKL(dmk_pol_aim_pre(h->q_R, h->pol.beta, h->small + SM_FAIM, h->small + SM_AIM, h->st));
+  CKL(dmk_pol_tau(N, &h->pol, F, Ft, h->T, h->st));
+  kt_begin(h, KT_FFT_FWD);
+  CKL(dmk_fft_r2c(&h->fft, 9, h->T, h->Z, h->st));
+  kt_end(h, KT_FFT_FWD);
+  GammaArgs g = h->ga;
+  g.do_div = 0;
+  g.do_gamma = 1;
+  g.aim_dev = h->small + SM_AIM;
+  kt_begin(h, KT_GAMMA);
+  CKL(dmk_gamma_div(h->Z, &g, h->partial, h->st));
+  kt_end(h, KT_GAMMA);
+  kt_begin(h, KT_FFT_INV);
+  CKL(dmk_fft_c2r(&h->fft, 9, h->Z, h->T, h->st));
+  kt_end(h, KT_FFT_INV);
+  CKL(dmk_pol_stage2(N, h->pol.beta, F, h->T, rt, rF, h->st));
+  int rc = response_enqueue(h, rF, h->dt);
+  if (rc) return rc;
+  int nb = dmk_gamma_div_blocks(&h->ga);
+  CKL(dmk_fft_r2c(&h->fft, 9, F, h->Z, h->st));
+  CKL(dmk_curl_partial(h->Z, &h->ga, h->partial, h->st));
+  CKL(dmk_colsum(h->partial, nb, nb, 1, h->partial + nb, h->small + SM_REP + REP_CURL2, 1.0, h->st));
+  CKL(dmk_fft_r2c(&h->fft, 9, h->P_hom, h->Z, h->st));
+  g.do_div = 1;
+  g.do_gamma = 0;
+  CKL(dmk_gamma_div(h->Z, &g, h->partial, h->st));
+  CKL(dmk_colsum(h->partial, nb, nb, 1, h->partial + nb, h->small + SM_REP + REP_DIV2, 1.0, h->st));
+  kt_begin(h, KT_TANGENT);
+  for (int p = 0; p < h->n_phases; p++)
+    CKL(dmk_launch_tangent(&h->mp, h->ph_dev + p, &h->ph[p], p, h->dt, h->flag, h->dPdF, h->nrm, h->st));
+  kt_end(h, KT_TANGENT);
+  CKL(dmk_pol_rF(N, &h->pol, F, Ft, h->P_hom, h->dPdF, rt, rF, h->st));
+  AimArgs a;
+  aim_args(h, true, true, &a);
+  CKL(dmk_pol_residual_aim(&a, &h->pol, h->small + SM_PAV, h->small + SM_FAV, h->small + SM_FAIM, h->flag,
+                           h->small + SM_REP, h->st));
+  return 0;
+}
+
 // The one synchronisation of a residual evaluation: optional dot products
 // (written after the report), report to the host, solver scalars updated.
 static int residual_finish(dmk_handle* h, const DotJobs* jobs, double* dots_out) {
   int nd = jobs ? jobs->n : 0;
   if (nd > DMK_MAX_DOTS) { dmk_set_error("too many dot products"); return DMK_ERR_ARG; }
-  if (nd) CKL(dmk_dots(jobs, h->n9, h->partial, h->small + SM_REP + REP_DOTS, h->st));
+  if (nd) CKL(dmk_dots(jobs, h->nx, h->partial, h->small + SM_REP + REP_DOTS, h->st));
   CK(hipMemcpyAsync(h->hsmall + SM_REP, h->small + SM_REP, sizeof(double) * (REP_DOTS + nd), hipMemcpyDeviceToHost,
                     h->st));
   CK(hipStreamSynchronize(h->st));
@@ -426,6 +500,7 @@ static int residual_finish(dmk_handle* h, const DotJobs* jobs, double* dots_out)
   kt_collect(h, KT_FFT_FWD);
   kt_collect(h, KT_GAMMA);
   kt_collect(h, KT_FFT_INV);
+  if (h->scheme == 1) kt_collect(h, KT_TANGENT);
   const double* rep = h->hsmall + SM_REP;
   std::memcpy(h->P_av, rep + REP_PAV, sizeof h->P_av);
   std::memcpy(h->F_aim, rep + REP_FAIM, sizeof h->F_aim);
@@ -433,6 +508,10 @@ static int residual_finish(dmk_handle* h, const DotJobs* jobs, double* dots_out)
   h->status = (int)rep[REP_FLAG];
   h->err_div = std::sqrt(rep[REP_DIV2]) * h->wgt;
   if (h->cells[0] == 1) h->err_div *= 0.5;
+  if (h->scheme == 1) {
+    h->err_curl = std::sqrt(rep[REP_CURL2]) * h->wgt;
+    if (h->cells[0] == 1) h->err_curl *= 0.5;
+  }
   if (nd) std::memcpy(dots_out, rep + REP_DOTS, sizeof(double) * nd);
   return 0;
 }
@@ -442,13 +521,21 @@ static int form_residual(dmk_handle* h, const double* X, double* r, int iter) {
   return rc ? rc : residual_finish(h, nullptr, nullptr);
 }
 
-// grid_mech_spectral_basic.f90:422-458 converged
+// grid_mech_spectral_basic.f90:422-458 converged; the polarization scheme
+// adds the curl criterion (grid_mech_spectral_polarization.f90:482-506)
 static int converged(const dmk_handle* h) {
   double pmax = 0.0;
   for (int k = 0; k < 9; k++) pmax = std::fmax(pmax, std::fabs(h->P_av[k]));
   double divTol = std::fmax(pmax * h->num.eps_rel_div, h->num.eps_abs_div);
   double BCTol = std::fmax(pmax * h->num.eps_rel_P, h->num.eps_abs_P);
-  if (h->totalIter >= h->num.N_iter_min && h->err_div / divTol < 1.0 && h->err_BC / BCTol < 1.0 &&
+  bool curl_ok = true;
+  if (h->scheme == 1) {
+    double fmax = 0.0;
+    for (int k = 0; k < 9; k++) fmax = std::fmax(fmax, std::fabs(h->F_aim[k] - (k % 4 == 0 ? 1.0 : 0.0)));
+    double curlTol = std::fmax(fmax * h->num.eps_rel_curl, h->num.eps_abs_curl);
+    curl_ok = h->err_curl / curlTol < 1.0;
+  }
+  if (h->totalIter >= h->num.N_iter_min && h->err_div / divTol < 1.0 && curl_ok && h->err_BC / BCTol < 1.0 &&
       h->status == ST_OK)
     return 1;
   if (h->totalIter >= h->num.N_iter_max) return -1;
@@ -508,7 +595,7 @@ static int solve_ngmres(dmk_handle* h, int* reason_out) {
     double* XA = pool_free(h->poolX, h->F, Xs.data(), l, XM);
     double* FA = pool_free(h->poolF, h->R, Fs.data(), l, FM);
     if (!XM || !FM || !XA || !FA) { dmk_set_error("NGMRES buffer pool exhausted"); return DMK_ERR_CONFIG; }
-    CKL(dmk_axpby(h->n9, 1.0, h->F, -1.0, h->R, XM, h->st));
+    CKL(dmk_axpby(h->nx, 1.0, h->F, -1.0, h->R, XM, h->st));
     J.n = 0;
     add_dot(&J, FM, FM, 0);
     for (int i = 0; i < l; i++) add_dot(&J, FM, Fs[i], 0);
@@ -526,7 +613,7 @@ static int solve_ngmres(dmk_handle* h, int* reason_out) {
     ca.l = l;
     ca.atot = 0.0;
     for (int i = 0; i < l; i++) { ca.alpha[i] = alpha[i]; ca.atot += alpha[i]; ca.X[i] = Xs[i]; }
-    CKL(dmk_combine(h->n9, XM, &ca, XA, h->st));
+    CKL(dmk_combine(h->nx, XM, &ca, XA, h->st));
     J.n = 0;
     add_dot(&J, FA, FA, 0);
     add_dot(&J, XA, XM, 1);
@@ -588,7 +675,7 @@ static int solve_richardson(dmk_handle* h, int* reason_out) {
     if (rc) return rc;
     int reason = converged(h);
     if (reason) { *reason_out = reason; return 0; }
-    CKL(dmk_axpby(h->n9, 1.0, h->F, -1.0, h->R, h->F, h->st));
+    CKL(dmk_axpby(h->nx, 1.0, h->F, -1.0, h->R, h->F, h->st));
   }
   *reason_out = -1;
   return 0;
@@ -656,6 +743,7 @@ static int tape_store(dmk_handle* h, int k) {
 
 extern "C" int dmk_adjoint_tape(dmk_handle* h, int32_t max_increments) {
   if (!h || max_increments < 1) { dmk_set_error("dmk_adjoint_tape: invalid argument"); return DMK_ERR_ARG; }
+  if (h->scheme != 0) { dmk_set_error("dmk_adjoint_tape: the adjoint differentiates the spectral-basic scheme"); return DMK_ERR_CONFIG; }
   adj_free(h);
   h->adj = new AdjState;
   AdjState& a = *h->adj;
@@ -1007,12 +1095,27 @@ static int init_impl(dmk_handle* h, const int32_t cells[3], const double size[3]
     dmk_set_error("IO_error 301: N_iter_min / N_iter_max");
     return DMK_ERR_CONFIG;
   }
+  if (h->num.scheme != 0 && h->num.scheme != 1) { dmk_set_error("IO_error 601: not a valid grid solver type"); return DMK_ERR_CONFIG; }
+  h->scheme = h->num.scheme;
+  if (h->scheme == 1) {
+    // grid_mech_spectral_polarization.f90:141-147; beta = 0, which the
+    // reference accepts, divides by zero in its residual (:601), so it is refused
+    const dmk_numerics& n = h->num;
+    if (!(n.eps_abs_curl > 0.0) || !(n.eps_rel_curl > 0.0) || !(n.alpha > 0.0 && n.alpha <= 2.0) ||
+        !(n.beta > 0.0 && n.beta <= 2.0)) {
+      dmk_set_error("IO_error 301: eps_abs_curl(F) / eps_rel_curl(F) / alpha / beta");
+      return DMK_ERR_CONFIG;
+    }
+    h->pol.alpha = n.alpha;
+    h->pol.beta = n.beta;
+  }
   h->N = (long)cells[0] * cells[1] * cells[2];
   // per-cell kernels address a field through a 32-bit byte offset of the cell
   if (h->N >= (1L << 29)) { dmk_set_error("grid of %ld cells exceeds the 2^29 - 1 cells of one device", h->N); return DMK_ERR_ARG; }
   h->h = cells[0] / 2 + 1;
   h->Nk = (long)h->h * cells[1] * cells[2];
   h->n9 = 9 * h->N;
+  h->nx = h->scheme == 1 ? 2 * h->n9 : h->n9;
   h->wgt = 1.0 / (double)h->N;
   CK(hipStreamCreate(&h->st));
   if (dmk_fft_setup(&h->fft, cells)) { dmk_set_error("FFT setup failed"); return DMK_ERR_HIP; }
@@ -1063,15 +1166,19 @@ static int init_impl(dmk_handle* h, const int32_t cells[3], const double size[3]
   if ((rc = dalloc(h, &mp.n_sub, N))) return rc;
   if ((rc = dalloc(h, &h->P_hom, n9))) return rc;
   if ((rc = dalloc(h, &h->dPdF, 81 * N))) return rc;
-  double** sol[] = {&h->F, &h->F_lastinc, &h->Fdot, &h->R, &h->XM, &h->FM, &h->XA, &h->FA, &h->Fext, &h->Pext};
-  for (auto pp : sol) if ((rc = dalloc(h, pp, n9))) return rc;
-  if ((rc = dalloc(h, &h->Xh, (size_t)h->num.ngmres_m * n9))) return rc;
-  if ((rc = dalloc(h, &h->Fh, (size_t)h->num.ngmres_m * n9))) return rc;
+  // solver iterates: [F] (basic) or [F; F_tau] (polarization), nx doubles each
+  const long nx = h->nx;
+  double** sol[] = {&h->F, &h->F_lastinc, &h->Fdot, &h->R, &h->XM, &h->FM, &h->XA, &h->FA};
+  for (auto pp : sol) if ((rc = dalloc(h, pp, nx))) return rc;
+  if ((rc = dalloc(h, &h->Fext, n9)) || (rc = dalloc(h, &h->Pext, n9))) return rc;
+  if (h->scheme == 1 && (rc = dalloc(h, &h->T, n9))) return rc;
+  if ((rc = dalloc(h, &h->Xh, (size_t)h->num.ngmres_m * nx))) return rc;
+  if ((rc = dalloc(h, &h->Fh, (size_t)h->num.ngmres_m * nx))) return rc;
   h->poolX = {h->F, h->XM, h->XA};
   h->poolF = {h->R, h->FM, h->FA};
   for (int i = 0; i < h->num.ngmres_m; i++) {
-    h->poolX.push_back(h->Xh + (size_t)i * n9);
-    h->poolF.push_back(h->Fh + (size_t)i * n9);
+    h->poolX.push_back(h->Xh + (size_t)i * nx);
+    h->poolF.push_back(h->Fh + (size_t)i * nx);
   }
   if ((rc = dalloc(h, &h->Z, 9 * h->Nk))) return rc;
   // FFT tile shapes for this grid, timed on the device (no effect on results:
@@ -1082,7 +1189,7 @@ static int init_impl(dmk_handle* h, const int32_t cells[3], const double size[3]
   long nn = (long)(cells[0] + 1) * (cells[1] + 1) * (cells[2] + 1);
   if ((rc = dalloc(h, &h->xn_dev, 3 * nn))) return rc;
   if ((rc = dalloc(h, &h->xp_dev, 3 * N))) return rc;
-  long np = 82L * dmk_nchunk(std::max(n9, 9 * h->Nk)) + 4096;
+  long np = 82L * dmk_nchunk(std::max(nx, 9 * h->Nk)) + 4096;
   if ((rc = dalloc(h, &h->partial, np))) return rc;
   if ((rc = dalloc(h, &h->small, SM_SIZE))) return rc;
   if ((rc = dalloc(h, &h->nrm, N))) return rc;
@@ -1125,6 +1232,12 @@ static int init_impl(dmk_handle* h, const int32_t cells[3], const double size[3]
     CK(hipMemcpy(mp.F0, I.data(), sizeof(double) * n9, hipMemcpyHostToDevice));
     CK(hipMemcpy(h->F, I.data(), sizeof(double) * n9, hipMemcpyHostToDevice));
     CK(hipMemcpy(h->F_lastinc, I.data(), sizeof(double) * n9, hipMemcpyHostToDevice));
+    if (h->scheme == 1) {
+      // F_tau = 2 F, F_tau_lastinc = 2 F_lastinc (grid_mech_spectral_polarization.f90:243-244)
+      for (auto& v : I) v *= 2.0;
+      CK(hipMemcpy(h->F + n9, I.data(), sizeof(double) * n9, hipMemcpyHostToDevice));
+      CK(hipMemcpy(h->F_lastinc + n9, I.data(), sizeof(double) * n9, hipMemcpyHostToDevice));
+    }
     CK(hipMemcpy(mp.xi, xi.data(), sizeof(double) * ns * N, hipMemcpyHostToDevice));
     CK(hipMemcpy(mp.xi0, xi.data(), sizeof(double) * ns * N, hipMemcpyHostToDevice));
     CK(hipMemcpy(mp.O, O.data(), sizeof(double) * 4 * N, hipMemcpyHostToDevice));
@@ -1162,6 +1275,8 @@ static int init_impl(dmk_handle* h, const int32_t cells[3], const double size[3]
   // init: F = I, response at dt = 0, C_ref (grid_mech_spectral_basic.f90:213-235)
   if ((rc = constitutive(h, h->F, 0.0, false, true))) return rc;
   set_gamma_coef(h, h->C_minMaxAvg);
+  std::memcpy(h->C_minMaxAvgRestart, h->C_minMaxAvg, sizeof h->C_minMaxAvg);
+  if (h->scheme == 1 && (rc = set_scale(h))) return rc;
   return 0;
 }
 
@@ -1227,6 +1342,20 @@ extern "C" int dmk_mech_forward(dmk_handle* h, int32_t cutBack, int32_t guess, d
   rotate33(bc->q_R, h->F_aim, true, aim);
   for (int k = 0; k < 9; k++) fa.shift[k] = h->hsmall[SM_DOTS + k] - aim[k];
   CKL(dmk_shift_field(h->N, &fa, h->F, h->st));
+  if (h->scheme == 1) {
+    // F_tau (grid_mech_spectral_polarization.f90:370-377,389-399): its rate and
+    // last-increment copy as F's; with a guess forwarded by its own rate (no
+    // mean correction), otherwise F_tau = I + S_scale:(F_l (C_scale:(F_l^T F_l - I)) / 2) + F
+    // with F_l = F_tau - F at the new F
+    const long n9 = h->n9;
+    double* Ft = h->F + n9;
+    if (guess) {
+      CKL(dmk_forward_field(h->N, 1, dt_old, dt, &fa, Ft, h->F_lastinc + n9, h->Fdot + n9, Ft, h->st));
+    } else {
+      if (!cutBack) CKL(dmk_forward_field(h->N, 0, dt_old, dt, &fa, Ft, h->F_lastinc + n9, h->Fdot + n9, h->T, h->st));
+      CKL(dmk_pol_lambda(h->N, &h->pol, h->F, Ft, h->st));
+    }
+  }
   for (int k = 0; k < 9; k++) h->stress_mask[k] = bc->stress_type < 0 ? 1 : bc->stress_mask[k];
   std::memcpy(h->q_R, bc->q_R, sizeof h->q_R);
   h->dt = dt;
@@ -1249,7 +1378,10 @@ extern "C" int dmk_mech_solution(dmk_handle* h, int32_t* iterations, int32_t* co
   if (!h) { dmk_set_error("null handle"); return DMK_ERR_ARG; }
   int rc = masked_compliance(h);
   if (rc) return rc;
-  if (h->num.update_gamma) set_gamma_coef(h, h->C_minMaxAvg);
+  if (h->num.update_gamma) {
+    set_gamma_coef(h, h->C_minMaxAvg);
+    if (h->scheme == 1 && (rc = set_scale(h))) return rc;
+  }
   h->nfuncs = 0;
   int reason = 0;
   // F_aim is updated on the device during the solve (residual_aim_kernel)
@@ -1331,6 +1463,8 @@ static int field_dev(dmk_handle* h, int what, const double** p, int* w) {
     case 9: *p = mp.gam; *w = h->nsl_max; return 0;
     case 10: *p = h->dPdF; *w = 81; return 0;
     case 11: *p = h->F_lastinc; return 0;
+    case 12: if (h->scheme != 1) return 1; *p = h->F + h->n9; return 0;
+    case 13: if (h->scheme != 1) return 1; *p = h->F_lastinc + h->n9; return 0;
     default: return 1;
   }
 }
@@ -1392,6 +1526,12 @@ extern "C" int dmk_mech_get_scalars(dmk_handle* h, int32_t what, double* dst) {
     case 3: std::memcpy(dst, h->C_volAvg, 81 * sizeof(double)); break;
     case 4: std::memcpy(dst, h->C_minMaxAvg, 81 * sizeof(double)); break;
     case 5: dst[0] = h->err_div; dst[1] = h->err_BC; break;
+    case 10: dst[0] = h->err_curl; break;
+    case 11:
+      // grid_mech_spectral_basic.f90:388,408: the stiffness Gamma was built from
+      // (the current C_minMaxAvg when update_gamma is set)
+      std::memcpy(dst, h->num.update_gamma ? h->C_minMaxAvg : h->C_minMaxAvgRestart, 81 * sizeof(double));
+      break;
     case 6: std::memcpy(dst, h->F_aimDot, 9 * sizeof(double)); break;
     case 7: std::memcpy(dst, h->F_aim_lastinc, 9 * sizeof(double)); break;
     case 8: std::memcpy(dst, h->C_volAvgLastInc, 81 * sizeof(double)); break;
@@ -1407,7 +1547,15 @@ extern "C" int dmk_mech_set_scalars(dmk_handle* h, int32_t what, const double* s
     case 0: std::memcpy(h->F_aim, src, 9 * sizeof(double)); break;
     case 2: std::memcpy(h->P_aim, src, 9 * sizeof(double)); break;
     case 3: std::memcpy(h->C_volAvg, src, 81 * sizeof(double)); break;
-    case 4: std::memcpy(h->C_minMaxAvg, src, 81 * sizeof(double)); set_gamma_coef(h, src); break;
+    case 4: {
+      // as the solver init after reading restart data: Gamma operator and,
+      // for the polarization scheme, C_scale / S_scale from C_minMaxAvg
+      std::memcpy(h->C_minMaxAvg, src, 81 * sizeof(double));
+      std::memcpy(h->C_minMaxAvgRestart, src, 81 * sizeof(double));
+      set_gamma_coef(h, src);
+      if (h->scheme == 1) { int rc = set_scale(h); if (rc) return rc; }
+      break;
+    }
     case 6: std::memcpy(h->F_aimDot, src, 9 * sizeof(double)); break;
     case 7: std::memcpy(h->F_aim_lastinc, src, 9 * sizeof(double)); break;
     case 8: std::memcpy(h->C_volAvgLastInc, src, 81 * sizeof(double)); break;
@@ -1554,6 +1702,23 @@ extern "C" int dmk_divergence_rms(dmk_handle* h, const double* field, double* ou
   g.do_gamma = 0;
   CKL(dmk_gamma_div(h->Z, &g, h->partial, h->st));
   int nb = dmk_gamma_div_blocks(&g);
+  CKL(dmk_colsum(h->partial, nb, nb, 1, h->partial + nb, h->small + SM_DOTS, 1.0, h->st));
+  CK(hipMemcpyAsync(h->hsmall + SM_DOTS, h->small + SM_DOTS, sizeof(double), hipMemcpyDeviceToHost, h->st));
+  CK(hipStreamSynchronize(h->st));
+  double r = std::sqrt(h->hsmall[SM_DOTS]) * h->wgt;
+  if (h->cells[0] == 1) r *= 0.5;
+  *out = r;
+  return 0;
+}
+
+// utilities_curlRMS (spectral_utilities.f90:697-757)
+extern "C" int dmk_curl_rms(dmk_handle* h, const double* field, double* out) {
+  if (!h || !field || !out) { dmk_set_error("null argument"); return DMK_ERR_ARG; }
+  CK(hipMemcpyAsync(h->Pext, field, sizeof(double) * h->n9, hipMemcpyHostToDevice, h->st));
+  CKL(dmk_aos_to_soa(h->Pext, h->N, 9, h->Fext, h->st));
+  CKL(dmk_fft_r2c(&h->fft, 9, h->Fext, h->Z, h->st));
+  CKL(dmk_curl_partial(h->Z, &h->ga, h->partial, h->st));
+  int nb = dmk_gamma_div_blocks(&h->ga);
   CKL(dmk_colsum(h->partial, nb, nb, 1, h->partial + nb, h->small + SM_DOTS, 1.0, h->st));
   CK(hipMemcpyAsync(h->hsmall + SM_DOTS, h->small + SM_DOTS, sizeof(double), hipMemcpyDeviceToHost, h->st));
   CK(hipStreamSynchronize(h->st));

@@ -11,6 +11,7 @@
 //   same forward transform, which the reference computes twice.
 //   With the continuous derivative xi = i a, Gamma_hat:P_hat collapses to
 //   (A^-1 (P_hat a)) (x) a with the acoustic tensor A_lm = C_ref,ljmq a_j a_q.
+// * curl_kernel: the RMS curl of the polarization scheme (:697-757).
 // * coords kernels: FFT integration of F for the node and cell positions
 //   (:857-968).
 #include "dmk_common.h"
@@ -361,6 +362,60 @@ __global__ void __launch_bounds__(256) gamma_div_kernel(double2* __restrict__ Z,
   }
 }
 
+// utilities_curlRMS (spectral_utilities.f90:697-757): per frequency the
+// squared curl of F_hat with xi = i a (rescaled like the divergence), interior
+// x planes counted twice.  On the x = 0 and x = N1/2 planes the reference
+// assigns each curl expression to the whole 3x3 array, so what it sums there
+// is 9 |F_hat(3,2) xi_1 - F_hat(3,1) xi_2|^2 (the last assignment); this kernel
+// sums the same.  For N1 = 1 that plane is visited twice.
+__global__ void __launch_bounds__(256) curl_kernel(const double2* __restrict__ Z, GammaArgs g,
+                                                   double* __restrict__ partial) {
+  const long Nk = (long)g.h * g.n[1] * g.n[2];
+  const long f = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  double contrib = 0.0;
+  if (f < Nk) {
+    int kx = (int)(f % g.h);
+    long r = f / g.h;
+    int ky = (int)(r % g.n[1]);
+    int kz = (int)(r / g.n[1]);
+    const double TAU = 6.283185307179586;
+    double w[3];
+    w[0] = TAU * (double)kx / g.size[0];
+    w[1] = TAU * wrap_freq(ky, g.n[1]) / g.size[1];
+    w[2] = TAU * wrap_freq(kz, g.n[2]) / g.size[2];
+    if (g.n[0] % 2 == 0 && 2 * kx == g.n[0]) w[0] = 0.0;
+    if (g.n[1] % 2 == 0 && 2 * ky == g.n[1]) w[1] = 0.0;
+    if (g.n[2] % 2 == 0 && 2 * kz == g.n[2]) w[2] = 0.0;
+#pragma unroll
+    for (int d = 0; d < 3; d++) w[d] *= g.rg[d];
+    auto cdiff = [](double2 p, double wp, double2 q, double wq) {
+      double re = p.x * wp - q.x * wq, im = p.y * wp - q.y * wq;
+      return re * re + im * im;
+    };
+    if (kx >= 1 && kx <= g.h - 2) {
+      double s = 0.0;
+#pragma unroll
+      for (int l = 0; l < 3; l++) {
+        double2 F0 = Z[(l + 0) * Nk + f], F1 = Z[(l + 3) * Nk + f], F2 = Z[(l + 6) * Nk + f];
+        s += cdiff(F2, w[1], F1, w[2]) + cdiff(F0, w[2], F2, w[0]) + cdiff(F1, w[0], F0, w[1]);
+      }
+      contrib = 2.0 * s;
+    } else {
+      double2 F20 = Z[2 * Nk + f], F21 = Z[5 * Nk + f];
+      double s = 9.0 * cdiff(F21, w[0], F20, w[1]);
+      contrib = g.h == 1 ? 2.0 * s : s;
+    }
+  }
+  __shared__ double red[256];
+  red[threadIdx.x] = contrib;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
 // updateCoords: per frequency u_hat = F_hat xi / (conj(-xi) . xi), xi second-derivative set
 __global__ void coords_freq_kernel(const double2* __restrict__ Fh, double2* __restrict__ U, GammaArgs g) {
   const long Nk = (long)g.h * g.n[1] * g.n[2];
@@ -576,6 +631,11 @@ int dmk_gamma_div(double2* Z, const GammaArgs* g, double* div_partial, hipStream
 int dmk_gamma_div_blocks(const GammaArgs* g) {
   const long Nk = (long)g->h * g->n[1] * g->n[2];
   return (int)((Nk + 255) / 256);
+}
+
+int dmk_curl_partial(const double2* Z, const GammaArgs* g, double* partial, hipStream_t s) {
+  hipLaunchKernelGGL(curl_kernel, dim3(dmk_gamma_div_blocks(g)), dim3(256), 0, s, Z, *g, partial);
+  return hipGetLastError() == hipSuccess ? 0 : 2;
 }
 
 int dmk_coords_freq(const double2* Fh, double2* U, const GammaArgs* g, hipStream_t s) {

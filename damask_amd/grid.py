@@ -22,9 +22,10 @@ from .config import LoadCase, Material, Numerics
 from .geom import Grid
 
 FIELDS = {"F": 0, "P": 1, "F_e": 2, "F_p": 3, "L_p": 4, "S": 5, "F_i": 6, "O": 7, "xi_sl": 8,
-          "gamma_sl": 9, "dPdF": 10, "F_lastinc": 11}
+          "gamma_sl": 9, "dPdF": 10, "F_lastinc": 11, "F_tau": 12, "F_tau_lastinc": 13}
 SCALARS = {"F_aim": 0, "P_av": 1, "P_aim": 2, "C_volAvg": 3, "C_minMaxAvg": 4, "err": 5, "F_aimDot": 6,
-           "F_aim_lastinc": 7, "C_volAvgLastInc": 8, "C_minMaxAvgLastInc": 9}
+           "F_aim_lastinc": 7, "C_volAvgLastInc": 8, "C_minMaxAvgLastInc": 9, "err_curl": 10,
+           "C_minMaxAvgRestart": 11}
 # restart data of grid_mech_spectral_basic (restartWrite, :378-416) and of the
 # phase (phase_mechanical.f90:1248-1280)
 RESTART_FIELDS = ("F", "F_lastinc", "F_p", "L_p", "S", "xi_sl", "gamma_sl")
@@ -69,11 +70,13 @@ def phase_desc(p) -> _lib.PhaseDesc:
     return d
 
 
-def numerics_struct(n: Numerics, cref_rule: int = 1) -> _lib.Numerics:
+def numerics_struct(n: Numerics, cref_rule: int = 1, scheme: int = 0) -> _lib.Numerics:
     s = _lib.Numerics()
     for f, _ in _lib.Numerics._fields_:
         if f == "cref_rule":
             s.cref_rule = cref_rule
+        elif f == "scheme":
+            s.scheme = scheme
         elif hasattr(n, f):
             v = getattr(n, f)
             setattr(s, f, int(v) if isinstance(getattr(s, f), int) else float(v))
@@ -106,6 +109,8 @@ ADJ_NOUT = 6 * ADJ_NSYS + 32
 class GridMechSpectralBasic:
     """One RVE on the GPU behind the spectral-basic plug-in interface."""
     taping = False
+    SCHEME = 0
+    RESTART_FIELDS = RESTART_FIELDS
 
     def __init__(self, grid: Grid, material: Material, numerics: Numerics | None = None, cref_rule: int = 1):
         self.lib = _lib.load()
@@ -121,7 +126,7 @@ class GridMechSpectralBasic:
         size = np.ascontiguousarray(grid.size, float)
         phase = np.ascontiguousarray(self.maps["phase"], np.int32)
         q0 = np.ascontiguousarray(self.maps["O"], float)
-        num = numerics_struct(numerics, cref_rule)
+        num = numerics_struct(numerics, cref_rule, self.SCHEME)
         h = C.c_void_p()
         _lib.check(self.lib.dmk_mech_init(_lib.iptr(cells), _lib.dptr(size), len(material.phases), descs,
                                           _lib.iptr(phase), _lib.dptr(q0), C.byref(num), C.byref(h)),
@@ -171,7 +176,7 @@ class GridMechSpectralBasic:
 
     def scalar(self, name: str) -> np.ndarray:
         n = {"C_volAvg": 81, "C_minMaxAvg": 81, "C_volAvgLastInc": 81, "C_minMaxAvgLastInc": 81,
-             "err": 2}.get(name, 9)
+             "C_minMaxAvgRestart": 81, "err": 2, "err_curl": 1}.get(name, 9)
         out = np.zeros(n)
         _lib.check(self.lib.dmk_mech_get_scalars(self.h, SCALARS[name], _lib.dptr(out)), "dmk_mech_get_scalars")
         if n == 9:
@@ -233,15 +238,17 @@ class GridMechSpectralBasic:
     # --- restart ------------------------------------------------------------
     def restart_write(self) -> dict:
         """The data restartWrite stores (grid_mech_spectral_basic.f90:378-416)."""
-        d = {k: self.field(k) for k in RESTART_FIELDS}
+        d = {k: self.field(k) for k in self.RESTART_FIELDS}
         d.update({k: self.scalar(k) for k in RESTART_SCALARS + RESTART_STIFFNESS})
+        # stored as C_minMaxAvg: the stiffness the Gamma operator was built from (:388,408)
+        d["C_minMaxAvg"] = self.scalar("C_minMaxAvgRestart")
         return d
 
     def restart_read(self, d: dict, restart_inc: int) -> None:
         """Restart read of the solver init (grid_mech_spectral_basic.f90:198-232): fields
         and aims, a dt = 0 response at the restored F, then the stiffness averages; the
         native driver then skips increments 1..restart_inc."""
-        for k in RESTART_FIELDS:
+        for k in self.RESTART_FIELDS:
             self.set_field(k, d[k])
         for k in RESTART_SCALARS:
             self.set_scalar(k, d[k])
@@ -293,6 +300,35 @@ class GridMechSpectralBasic:
             self.close()
         except Exception:
             pass
+
+
+class GridMechSpectralPolarization(GridMechSpectralBasic):
+    """One RVE on the GPU behind the spectral-polarization plug-in interface
+    (``src/grid/grid_mech_spectral_polarization.f90``): the iterate is the pair
+    (F, F_tau), convergence adds the curl RMS of F; numerics ``alpha``, ``beta``,
+    ``eps_abs_curl``, ``eps_rel_curl``.  The same methods as the basic scheme;
+    restart data adds F_tau and F_tau_lastinc (:430-470)."""
+    SCHEME = 1
+    RESTART_FIELDS = RESTART_FIELDS + ("F_tau", "F_tau_lastinc")
+
+    def adjoint_tape(self, max_increments: int) -> None:
+        raise NotImplementedError("the adjoint differentiates the spectral-basic scheme")
+
+
+# load.yaml solver:mechanical (DAMASK_grid.f90:171-204)
+MECHANICAL_SOLVERS = {"spectral_basic": GridMechSpectralBasic,
+                      "spectral_polarization": GridMechSpectralPolarization}
+
+
+def grid_mechanical(name: str):
+    """Solver class for a load file's ``solver: mechanical`` entry."""
+    try:
+        return MECHANICAL_SOLVERS[name]
+    except KeyError:
+        from .config import ConfigError
+        if name in ("spectral_Galerkin", "FEM"):
+            raise ConfigError(601, f"{name} is a valid grid solver type of the reference, not built here") from None
+        raise ConfigError(601, f"{name} is not a valid grid solver type") from None
 
 
 class DamaskGrid:

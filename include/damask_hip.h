@@ -52,7 +52,10 @@ typedef struct {
 } dmk_phase_desc;
 
 /* numerics.yaml: solver:grid:mechanical, solver:grid, phase:mechanical
- * (src/grid/grid_mech_spectral_basic.f90:130-136, src/phase_mechanical.f90:281-295) */
+ * (src/grid/grid_mech_spectral_basic.f90:130-136, src/phase_mechanical.f90:281-295),
+ * and the scheme of load.yaml solver:mechanical (src/grid/DAMASK_grid.f90:171-186)
+ * with the spectral-polarization parameters
+ * (src/grid/grid_mech_spectral_polarization.f90:136-147) */
 typedef struct {
   int32_t N_iter_min, N_iter_max;
   double eps_abs_div, eps_rel_div, eps_abs_P, eps_rel_P;
@@ -67,6 +70,9 @@ typedef struct {
   double r_linesearch_Lp, eps_rel_Lp, eps_abs_Lp;
   int32_t f_update_jacobi_Lp;
   int32_t cref_rule;               /* 0 strict first-extremum rule, 1 with 1e-12 dead band */
+  int32_t scheme;                  /* 0 spectral_basic, 1 spectral_polarization */
+  double alpha, beta;              /* polarization: 0 < alpha <= 2, 0 < beta <= 2 */
+  double eps_abs_curl, eps_rel_curl;
 } dmk_numerics;
 
 /* mechanical boundary conditions of one load step (src/grid/DAMASK_grid.f90:496-600) */
@@ -119,19 +125,27 @@ int dmk_constitutive_device(dmk_handle* h, double dt, const double* F_dev, doubl
                             int32_t* status);
 
 /* per-cell fields to host buffers: 0 F, 1 P, 2 F_e, 3 F_p, 4 L_p, 5 S, 6 F_i,
- * 7 O (4/cell), 8 xi_sl (nsl/cell), 9 gamma_sl (nsl/cell), 10 dPdF (81/cell).
+ * 7 O (4/cell), 8 xi_sl (nsl/cell), 9 gamma_sl (nsl/cell), 10 dPdF (81/cell),
+ * 11 F_lastinc; spectral-polarization handles also 12 F_tau, 13 F_tau_lastinc
+ * (grid_mech_spectral_polarization.f90:452-455).
  * O and dPdF are those of the last response of the last completed
  * dmk_mech_solution or dmk_constitutive call: the solver evaluates them once
  * per solve, since grid_mech_spectral_basic reads the stiffness only between
  * solves (grid_mech_spectral_basic.f90:261-262,306-311). */
 int dmk_mech_get_field(dmk_handle* h, int32_t what, double* dst);
 /* restart data (grid_mech_spectral_basic.f90:198-232, phase_mechanical.f90:1248-1280):
- * 0 F, 3 F_p, 4 L_p, 5 S, 8 xi_sl, 9 gamma_sl, 11 F_lastinc */
+ * 0 F, 3 F_p, 4 L_p, 5 S, 8 xi_sl, 9 gamma_sl, 11 F_lastinc, and for the
+ * polarization scheme 12 F_tau, 13 F_tau_lastinc (grid_mech_spectral_polarization.f90:226-233) */
 int dmk_mech_set_field(dmk_handle* h, int32_t what, const double* src);
 
 /* solver scalars: 0 F_aim (9), 1 P_av (9), 2 P_aim (9), 3 C_volAvg (81),
  * 4 C_minMaxAvg (81), 5 {err_div, err_BC}, 6 F_aimDot (9), 7 F_aim_lastinc (9),
- * 8 C_volAvgLastInc (81), 9 C_minMaxAvgLastInc (81) */
+ * 8 C_volAvgLastInc (81), 9 C_minMaxAvgLastInc (81), 10 err_curl (1, polarization),
+ * 11 C_minMaxAvgRestart (81, get only): the stiffness the Gamma operator was
+ * built from, which restartWrite stores as C_minMaxAvg
+ * (grid_mech_spectral_basic.f90:235,388,408).  Setting 4 also sets the Gamma
+ * operator, C_minMaxAvgRestart and, for the polarization scheme, C_scale and
+ * S_scale, as the solver init does after reading restart data. */
 int dmk_mech_get_scalars(dmk_handle* h, int32_t what, double* dst);
 int dmk_mech_set_scalars(dmk_handle* h, int32_t what, const double* src);
 /* restart read (grid_mech_spectral_basic.f90:198-232): after setting the
@@ -156,6 +170,16 @@ int dmk_grid_run_loadcase(dmk_handle* h, const dmk_loadcase* lc, double* t_io, d
 int dmk_gamma_convolution(dmk_handle* h, const double* field, const double aim[9], double* out);
 /* utilities_divergenceRMS (spectral_utilities.f90:650-691) of a (3,3,N) host field */
 int dmk_divergence_rms(dmk_handle* h, const double* field, double* out);
+
+/* utilities_curlRMS (src/grid/spectral_utilities.f90:697-757) of a (3,3,N)
+ * field, the convergence measure of the spectral-polarization scheme; on the
+ * x = 0 and x = N1/2 frequency planes it sums what the reference sums there
+ * (nine times the squared (3,3) curl entry of the last row) */
+int dmk_curl_rms(dmk_handle* h, const double* field, double* out);
+
+/* math_invSym3333 (src/math.f90:511-532): inverse through the Mandel 6x6 form
+ * (:826-904) of a column-major 3x3x3x3; nonzero on a zero pivot */
+int dmk_inv_sym3333(const double* A, double* out);
 /* unnormalised 3-D r2c / c2r of howmany (<= 9) interleaved fields, FFTW's
  * conventions (spectral_utilities.f90:239-261): real (howmany, N1, N2, N3) <->
  * complex (howmany, N1/2+1, N2, N3) as (re, im) pairs; c2r ignores the

@@ -106,6 +106,7 @@ void orc_rotate33(const double q[4], const double T[9], int active, double out[9
 void orc_rotate3333(const double q[4], const double T[81], int active, double out[81]);
 void orc_rotational_part(const double F[9], double R[9]);
 int  orc_invert(int n, const double *A, double *invA);            /* column-major */
+int  orc_inv_sym3333(const double *A, double *invA);                /* math_invSym3333 */
 void orc_schmid_slip(int lattice, int n_fam, const int *N_sl, int sense,
                      int n_nS, const double *a_nS, double *P /* [nsl][9] */);
 void orc_interaction_slipslip(int lattice, int n_fam, const int *N_sl,
@@ -158,6 +159,7 @@ void orc_fft_c2r(const orc_grid *g, int howmany, const double *in, double *out);
 void orc_update_gamma(orc_grid *g, const double C[81]);
 void orc_gamma_convolution(orc_grid *g, const double *field, const double aim[9], double *out);
 double orc_divergence_rms(orc_grid *g, const double *field);
+double orc_curl_rms(orc_grid *g, const double *field);
 void orc_update_coords(orc_grid *g, const double *F, double *x_p, double *x_n);
 
 /* ---- grid mechanical spectral basic solver + DAMASK_grid increment loop ---- */

@@ -151,6 +151,39 @@ int orc_invert(int n, const double *A, double *invA) {
   return 0;
 }
 
+/* math.f90:826-845 math_sym3333to66: Mandel 6x6 (column-major) of a 3x3x3x3;
+ * Voigt order 11 22 33 23 13 12, weights 1 and sqrt(2) (constants.f90:19) */
+static const int MV_I[6] = {0, 1, 2, 1, 0, 0}, MV_J[6] = {0, 1, 2, 2, 2, 1};
+static void sym3333to66(const double *T, double *m) {
+  const double w[6] = {1.0, 1.0, 1.0, sqrt(2.0), sqrt(2.0), sqrt(2.0)};
+  for (int q = 0; q < 6; q++)
+    for (int p = 0; p < 6; p++) m[p + 6 * q] = T[I4(MV_I[p], MV_J[p], MV_I[q], MV_J[q])] * w[p] * w[q];
+}
+
+/* math.f90:851-904 math_66toSym3333 */
+static void m66toSym3333(const double *m, double *T) {
+  const double w[6] = {1.0, 1.0, 1.0, sqrt(2.0), sqrt(2.0), sqrt(2.0)};
+  for (int q = 0; q < 6; q++)
+    for (int p = 0; p < 6; p++) {
+      double v = m[p + 6 * q] / (w[p] * w[q]);
+      int i = MV_I[p], j = MV_J[p], k = MV_I[q], l = MV_J[q];
+      T[I4(i, j, k, l)] = v;
+      T[I4(j, i, k, l)] = v;
+      T[I4(i, j, l, k)] = v;
+      T[I4(j, i, l, k)] = v;
+    }
+}
+
+/* math.f90:511-532 math_invSym3333 (dgetrf + dgetri on the Mandel form).
+ * Returns 1 on a singular matrix. */
+int orc_inv_sym3333(const double *A, double *invA) {
+  double m[36], mi[36];
+  sym3333to66(A, m);
+  if (orc_invert(6, m, mi)) return 1;
+  m66toSym3333(mi, invA);
+  return 0;
+}
+
 /* math.f90:405-418 math_mul3333xx3333 */
 void t3333_mul(const double *A, const double *B, double *C) {
   double T[81];

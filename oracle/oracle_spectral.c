@@ -4,7 +4,7 @@
  * Restatement of src/grid/spectral_utilities.f90 for the spectral-basic path:
  * unnormalised multi-field r2c/c2r FFTs (own mixed-radix FFT in place of
  * FFTW), the continuous-derivative frequency tables, the memory-efficient
- * Gamma convolution, the RMS divergence and the FFT coordinate update.
+ * Gamma convolution, the RMS divergence and curl and the FFT coordinate update.
  * Fourier data are kept un-transposed, (howmany, N1/2+1, N2, N3), component
  * fastest; FFTW's transposed-out layout only changes memory order.
  */
@@ -296,6 +296,53 @@ double orc_divergence_rms(orc_grid *g, const double *field) {
         for (int l = 0; l < 3; l++) s_inner += creal(v[l]) * creal(v[l]) + cimag(v[l]) * cimag(v[l]);
       }
       total += s_inner;
+    }
+  double r = sqrt(total) * g->wgt;
+  if (g->n[0] == 1) r *= 0.5;
+  return r;
+}
+
+/* spectral_utilities.f90:697-757 utilities_curlRMS.  On the DC and last x
+ * planes the reference assigns each of the three curl expressions to the whole
+ * 3x3 curl array inside the loop over l, so the sum it takes there is nine
+ * times |F(3,2) xi_1 - F(3,1) xi_2|^2 of l = 3; that is restated as written. */
+double orc_curl_rms(orc_grid *g, const double *field) {
+  int h = g->h, N2 = g->n[1], N3 = g->n[2];
+  cplx *c = g->work;
+  r2c(g, 9, field, c);
+  double rg[3];
+  for (int d = 0; d < 3; d++) rg[d] = g->size[d] / g->scaled[d];
+  double total = 0.0;
+  for (int y = 0; y < N2; y++)
+    for (int z = 0; z < N3; z++) {
+      for (int kx = 1; kx <= h - 2; kx++) {
+        cplx xi[3], cu[3][3];
+        xi_at(g, kx, y, z, 1, xi);
+        const cplx *Fh = c + (size_t)9 * (kx + (size_t)h * (y + (size_t)N2 * z));
+        for (int l = 0; l < 3; l++) {
+          cu[l][0] = Fh[I2(l, 2)] * xi[1] * rg[1] - Fh[I2(l, 1)] * xi[2] * rg[2];
+          cu[l][1] = Fh[I2(l, 0)] * xi[2] * rg[2] - Fh[I2(l, 2)] * xi[0] * rg[0];
+          cu[l][2] = Fh[I2(l, 1)] * xi[0] * rg[0] - Fh[I2(l, 0)] * xi[1] * rg[1];
+        }
+        double s = 0.0;
+        for (int l = 0; l < 3; l++)
+          for (int m = 0; m < 3; m++) s += creal(cu[l][m]) * creal(cu[l][m]) + cimag(cu[l][m]) * cimag(cu[l][m]);
+        total += 2.0 * s;
+      }
+      /* the DC plane, then the plane kx = h - 1 (the same plane when h == 1) */
+      int planes[2] = {0, h - 1};
+      for (int pi = 0; pi < 2; pi++) {
+        int kx = planes[pi];
+        cplx xi[3], v = 0;
+        xi_at(g, kx, y, z, 1, xi);
+        const cplx *Fh = c + (size_t)9 * (kx + (size_t)h * (y + (size_t)N2 * z));
+        for (int l = 0; l < 3; l++) {
+          v = Fh[I2(l, 2)] * xi[1] * rg[1] - Fh[I2(l, 1)] * xi[2] * rg[2];
+          v = Fh[I2(l, 0)] * xi[2] * rg[2] - Fh[I2(l, 2)] * xi[0] * rg[0];
+          v = Fh[I2(l, 1)] * xi[0] * rg[0] - Fh[I2(l, 0)] * xi[1] * rg[1];
+        }
+        total += 9.0 * (creal(v) * creal(v) + cimag(v) * cimag(v));
+      }
     }
   double r = sqrt(total) * g->wgt;
   if (g->n[0] == 1) r *= 0.5;

@@ -116,6 +116,10 @@ def lib():
         L.orc_gamma_convolution.argtypes = [vp, dp, dp, dp]
         L.orc_divergence_rms.restype = C.c_double
         L.orc_divergence_rms.argtypes = [vp, dp]
+        L.orc_curl_rms.restype = C.c_double
+        L.orc_curl_rms.argtypes = [vp, dp]
+        L.orc_inv_sym3333.restype = C.c_int
+        L.orc_inv_sym3333.argtypes = [dp, dp]
         L.orc_update_coords.argtypes = [vp, dp, dp, dp]
         L.orc_solver_create.restype = vp
         L.orc_solver_create.argtypes = [vp, vp, C.POINTER(NumGrid)]

@@ -1,0 +1,99 @@
+"""spectral_polarization (grid_mech_spectral_polarization.f90) on the GPU,
+through the C-ABI, against the spectral-basic scheme.
+
+* The reference's own solver-consistency test
+  (tests/integration/test_grid_mech_consistency.py, its g_444 / tensionX /
+  material_plast resources): with the default numerics, F, F_e, F_p, L_p and O
+  at every output increment agree between the schemes within the test's
+  bound (atol 1e-2, rtol 1e-5).
+* At tight tolerances both schemes converge to the same equilibrium: volume
+  averaged P within 1e-7 of max |P| and per-cell F within 1e-8.
+* The curl RMS of the device against the oracle's restatement of
+  utilities_curlRMS (spectral_utilities.f90:697-757) on random fields.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN
+
+pytestmark = pytest.mark.gpu
+
+OUT = ("F", "F_e", "F_p", "L_p", "O")
+
+
+def _inputs():
+    from damask_amd import config, geom
+    grid = geom.read_vti(os.path.join(GOLDEN, "consistency_g_444.vti"))
+    mat = config.parse_material(open(os.path.join(GOLDEN, "consistency_material_plast.yaml")).read())
+    _, lcs = config.parse_load(open(os.path.join(GOLDEN, "consistency_tensionX.yaml")).read())
+    return grid, mat, lcs
+
+
+def _run(name, grid, mat, lcs, num):
+    from damask_amd import grid as G
+    s = G.grid_mechanical(name)(grid, mat, num)
+    d = G.DamaskGrid(s)
+    rec = []
+
+    def out(inc, t):
+        r = {k: s.field(k) for k in OUT}
+        r["P_av"] = s.scalar("P_av")
+        r["err_curl"] = float(s.scalar("err_curl")[0]) if name == "spectral_polarization" else 0.0
+        rec.append(r)
+
+    d.run(lcs, on_output=out)
+    its = sum(x[4] for x in d.sta)
+    s.close()
+    return rec, its
+
+
+def test_consistency_with_basic_at_reference_bar(gpu_lib):
+    from damask_amd import config
+    grid, mat, lcs = _inputs()
+    num = config.Numerics()
+    rb, itb = _run("spectral_basic", grid, mat, lcs, num)
+    rp, itp = _run("spectral_polarization", grid, mat, lcs, num)
+    print(f"iterations: basic {itb}, polarization {itp}")
+    assert len(rb) == len(rp) == lcs[0].N // lcs[0].f_out
+    assert all(r["err_curl"] > 0.0 for r in rp)
+    for a, b in zip(rb, rp):
+        for k in OUT:
+            np.testing.assert_allclose(b[k], a[k], rtol=1e-5, atol=1e-2, err_msg=k)
+
+
+def test_polarization_converges_to_basic_solution(gpu_lib):
+    import copy
+    from damask_amd import config
+    grid, mat, lcs = _inputs()
+    lc = copy.copy(lcs[0])
+    lc.N, lc.t, lc.f_out = 30, 4.5, 5
+    num = config.Numerics(eps_abs_div=1e-8, eps_rel_div=1e-11, eps_abs_P=1e-3, eps_rel_P=1e-11,
+                          eps_abs_curl=1e-13, eps_rel_curl=1e-11, N_iter_max=400, snes_max_it=400)
+    rb, _ = _run("spectral_basic", grid, mat, [lc], num)
+    rp, itp = _run("spectral_polarization", grid, mat, [lc], num)
+    print("polarization iterations", itp)
+    for a, b in zip(rb, rp):
+        pmax = np.abs(a["P_av"]).max()
+        assert np.abs(b["P_av"] - a["P_av"]).max() < 1e-7 * pmax
+        assert np.abs(b["F"] - a["F"]).max() < 1e-8
+        assert np.abs(b["F_p"] - a["F_p"]).max() < 1e-8
+
+
+@pytest.mark.parametrize("cells", [[6, 7, 8], [8, 8, 8], [5, 6, 4]])
+def test_curl_rms_against_oracle(gpu_lib, oracle, cells):
+    from damask_amd import _lib, grid as G, workloads, config
+    g, mat = workloads.polycrystal(cells, 3, 4, size=(1.0, 1.25, 0.75))
+    s = G.GridMechSpectralPolarization(g, config.parse_material(mat))
+    o = oracle.Problem(g, config.parse_material(mat))
+    N = g.n_cells
+    rng = np.random.default_rng(3)
+    for _ in range(2):
+        F = np.tile(np.eye(3).reshape(-1), N) + 1e-3 * rng.standard_normal(9 * N)
+        r_g = np.zeros(1)
+        _lib.check(s.lib.dmk_curl_rms(s.h, _lib.dptr(F), _lib.dptr(r_g)))
+        r_o = oracle.lib().orc_curl_rms(o.grid, oracle.ptr(F))
+        assert abs(r_g[0] - r_o) < 1e-12 * r_o
+    s.close()
+    o.close()

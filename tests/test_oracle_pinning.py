@@ -7,6 +7,13 @@ tolerance, against what the mathematics fixes:
   puts the aim on the DC term;
 * divergence RMS (spectral_utilities.f90:650-691): the closed form for a
   plane sine stress field;
+* curl RMS (spectral_utilities.f90:697-757): zero for a band-limited
+  compatible field, the closed form of a plane sine varying along x
+  (interior frequency planes), and three times the true value for one varying
+  along y only (the reference's sum on the x = 0 plane is nine times the last
+  row's (3,3) entry), halved again for N1 = 1;
+* invSym3333 (math.f90:511-532): the inverse on symmetric second-order
+  tensors of a major- and minor-symmetric stiffness, and an involution;
 * updateCoords (spectral_utilities.f90:857-968): the displacements DAMASK
   stored in the golden result for the stored F field;
 * dPdF (phase_mechanical.f90:1121-1242): central differences of the stress
@@ -194,3 +201,104 @@ def test_dPdF_matches_central_differences(oracle):
             fd[k + 3 * l] = (P_of(F33 + E) - P_of(F33 - E)) / (2 * h)
     L.orc_mp_destroy(mp)
     assert np.abs(fd - T).max() < 1e-6 * np.abs(T).max()
+
+
+def _field(F):
+    """(N3, N2, N1, 3, 3) [.., i, j] -> (3,3,N) column-major flat"""
+    return np.ascontiguousarray(F.transpose(0, 1, 2, 4, 3).reshape(-1))
+
+
+@pytest.mark.parametrize("cells,size", [([8, 5, 6], [1.0, 1.0, 1.0]), ([7, 6, 4], [2.0, 1.5, 1.0]),
+                                        ([12, 4, 5], [1.0, 0.5, 2.0])])
+def test_curl_rms_closed_forms(oracle, cells, size):
+    L, g = _grid(oracle, cells, size, correction=2)
+    N1, N2, N3 = cells
+    A = 2.0e-3
+    x = (np.arange(N1) + 0.5) * size[0] / N1
+    y = (np.arange(N2) + 0.5) * size[1] / N2
+    X, Y, Z = np.meshgrid(x, y, (np.arange(N3) + 0.5) * size[2] / N3, indexing="ij")
+    X, Y, Z = (a.transpose(2, 1, 0) for a in (X, Y, Z))
+    kx, ky = 2 * np.pi / size[0], 2 * np.pi / size[1]
+    # F_12 varying along x: interior x planes only, the plain RMS curl
+    F = np.zeros((N3, N2, N1, 3, 3)) + np.eye(3)
+    F[..., 0, 1] += A * np.sin(kx * X)
+    r = L.orc_curl_rms(g, oracle.ptr(_field(F)))
+    assert abs(r - A * kx / np.sqrt(2.0)) < 1e-12 * r
+    # F_31 varying along y: on the x = 0 plane, nine times the squared (3,3) entry
+    F = np.zeros((N3, N2, N1, 3, 3)) + np.eye(3)
+    F[..., 2, 0] += A * np.sin(ky * Y)
+    r = L.orc_curl_rms(g, oracle.ptr(_field(F)))
+    assert abs(r - 3.0 * A * ky / np.sqrt(2.0)) < 1e-12 * r
+    # a band-limited compatible field I + grad u has no curl
+    rng = np.random.default_rng(5)
+    F = np.zeros((N3, N2, N1, 3, 3)) + np.eye(3)
+    ref = 0.0
+    for _ in range(4):
+        # wave numbers below the Nyquist frequency of every axis
+        k = np.array([rng.integers(-((n - 1) // 2), (n - 1) // 2 + 1) for n in cells]) * 2 * np.pi / np.asarray(size)
+        a, ph = A * rng.standard_normal(3), rng.random() * 2 * np.pi
+        arg = k[0] * X + k[1] * Y + k[2] * Z + ph
+        for i in range(3):
+            for j in range(3):
+                F[..., i, j] += a[i] * k[j] * np.cos(arg)
+        ref = max(ref, np.abs(a).max() * np.abs(k).max())
+    r = L.orc_curl_rms(g, oracle.ptr(_field(F)))
+    assert r < 1e-12 * ref
+    L.orc_grid_destroy(g)
+
+
+def test_curl_rms_single_x_layer(oracle):
+    # N1 = 1: the one x plane is both the DC and the last plane, the sum is halved
+    cells, size = [1, 6, 4], [1.0, 1.0, 1.0]
+    L, g = _grid(oracle, cells, size, correction=2)
+    A = 1.0e-3
+    y = (np.arange(6) + 0.5) / 6
+    F = np.zeros((4, 6, 1, 3, 3)) + np.eye(3)
+    F[..., 2, 0] += A * np.sin(2 * np.pi * y)[None, :, None]
+    r = L.orc_curl_rms(g, oracle.ptr(_field(F)))
+    assert abs(r - 1.5 * A * 2 * np.pi) < 1e-12 * r
+    L.orc_grid_destroy(g)
+
+
+def _cubic(c11, c12, c44):
+    C4 = np.zeros((3, 3, 3, 3))
+    for i in range(3):
+        for j in range(3):
+            for k in range(3):
+                for l in range(3):
+                    C4[i, j, k, l] = (c12 * (i == j) * (k == l) + c44 * ((i == k) * (j == l) + (i == l) * (j == k))
+                                      + (c11 - c12 - 2 * c44) * (i == j == k == l))
+    return C4
+
+
+def _f4(C4):
+    return np.ascontiguousarray(C4.reshape(-1, order="F"))
+
+
+def test_inv_sym3333(oracle):
+    L = oracle.lib()
+    rng = np.random.default_rng(2)
+    R = np.linalg.qr(rng.standard_normal((3, 3)))[0]
+    C4 = np.einsum("ai,bj,ck,dl,ijkl->abcd", R, R, R, R, _cubic(106.75e9, 60.41e9, 28.34e9))
+    S = np.zeros(81)
+    assert L.orc_inv_sym3333(oracle.ptr(_f4(C4)), oracle.ptr(S)) == 0
+    S4 = S.reshape(3, 3, 3, 3, order="F")
+    for _ in range(3):
+        E = rng.standard_normal((3, 3))
+        E = E + E.T
+        back = np.einsum("ijkl,kl->ij", S4, np.einsum("ijkl,kl->ij", C4, E))
+        assert np.abs(back - E).max() < 1e-12 * np.abs(E).max()
+    # minor symmetries of the result and involution
+    assert np.abs(S4 - S4.transpose(1, 0, 2, 3)).max() == 0.0
+    assert np.abs(S4 - S4.transpose(0, 1, 3, 2)).max() == 0.0
+    C2 = np.zeros(81)
+    assert L.orc_inv_sym3333(oracle.ptr(S), oracle.ptr(C2)) == 0
+    assert np.abs(C2 - _f4(C4)).max() < 1e-12 * np.abs(C4).max()
+    # isotropic closed form: S = 1/(2 mu) (I_sym - lambda / (2 mu + 3 lambda) I x I)
+    lam, mu = 60.0e9, 26.0e9
+    Ciso = _cubic(lam + 2 * mu, lam, mu)
+    assert L.orc_inv_sym3333(oracle.ptr(_f4(Ciso)), oracle.ptr(S)) == 0
+    I = np.eye(3)
+    Isym = 0.5 * (np.einsum("ik,jl->ijkl", I, I) + np.einsum("il,jk->ijkl", I, I))
+    Siso = (Isym - lam / (2 * mu + 3 * lam) * np.einsum("ij,kl->ijkl", I, I)) / (2 * mu)
+    assert np.abs(S - _f4(Siso)).max() < 1e-12 * np.abs(Siso).max()
