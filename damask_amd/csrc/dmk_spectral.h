@@ -2,6 +2,7 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+#include <stdint.h>
 
 struct FftPlan {
   int L;
@@ -30,6 +31,17 @@ struct GammaArgs {
   int do_div, do_gamma;
 };
 
+// utilities_G_Convolution options: residual mode sets the DC term to
+// dc_dev[c] * dc_scale, Jacobian mode keeps it; components with stress_mask
+// set (stress not prescribed) get a zero DC term either way
+struct GalerkinArgs {
+  int do_div;
+  int residual;
+  int32_t stress_mask[9];
+  const double* dc_dev;
+  double dc_scale;
+};
+
 struct CoordsArgs {
   int n[3];
   double step[3];
@@ -48,6 +60,7 @@ int dmk_gamma_div(double2* Z, const GammaArgs* g, double* div_partial, hipStream
 int dmk_gamma_div_blocks(const GammaArgs* g);
 // sum of squared curl amplitudes of a forward transform, per block of 256
 // frequencies (dmk_gamma_div_blocks partial sums)
+int dmk_galerkin(double2* Z, const GammaArgs* g, const GalerkinArgs* ga, double* div_partial, hipStream_t s);
 int dmk_curl_partial(const double2* Z, const GammaArgs* g, double* partial, hipStream_t s);
 int dmk_coords_freq(const double2* Fh, double2* U, const GammaArgs* g, hipStream_t s);
 int dmk_coords_nodes(const double* u, const CoordsArgs* c, double* x_n, double* x_p, hipStream_t s);

@@ -27,6 +27,7 @@
 #include "dmk_linalg.h"
 #include "dmk_spectral.h"
 #include "dmk_polar.h"
+#include "dmk_galerkin.h"
 #include "../../include/damask_hip.h"
 
 int dmk_build_phase(const dmk_phase_desc* d, DmkPhaseDev* p);
@@ -203,8 +204,9 @@ struct dmk_handle {
   double C_volAvg[81], C_volAvgLastInc[81], C_minMaxAvg[81], C_minMaxAvgLastInc[81], S[81];
   double C_minMaxAvgRestart[81];     // reference stiffness of Gamma, what restartWrite stores
   double err_div = 0, err_BC = 0, err_curl = 0;
-  // spectral-polarization scheme (scheme 1): the iterate is [F; F_tau], 2 n9
-  // doubles; C_scale / S_scale and alpha, beta travel in pol
+  // scheme 0 spectral_basic, 1 spectral_polarization (the iterate is [F; F_tau],
+  // 2 n9 doubles; C_scale / S_scale and alpha, beta travel in pol),
+  // 2 spectral_Galerkin
   int scheme = 0;
   long nx = 0;
   PolArgs pol{};
@@ -262,8 +264,8 @@ static int dalloc(dmk_handle* h, T** p, size_t n) {
 enum { SM_PAV = 0, SM_CVOL = 9, SM_CMM = 90, SM_DOTS = 180, SM_REP = 256, SM_FAIM = 352, SM_AIM = 368,
        // adjoint: aim bookkeeping, report, mean, dot products, maxima, tangent average
        SM_AFAIM = 384, SM_AAIM = 400, SM_AREP = 416, SM_APAV = 504, SM_ADOTS = 520, SM_AMAX = 584, SM_ACAV = 592,
-       // polarization: mean of the compatible field F
-       SM_FAV = 680, SM_SIZE = 696 };
+       // polarization: mean of the compatible field F; Galerkin: DC of the residual
+       SM_FAV = 680, SM_GDC = 690, SM_SIZE = 704 };
 static_assert(SM_AREP + REP_SIZE <= SM_APAV, "adjoint report region");
 static_assert(SM_ADOTS + DMK_MAX_DOTS <= SM_AMAX, "adjoint dot region");
 static_assert(SM_REP + REP_SIZE <= SM_FAIM, "report region");
@@ -402,6 +404,7 @@ static int set_scale(dmk_handle* h) {
 // stress-BC update of F_aim and the aim on the device, one forward FFT for
 // the divergence RMS and the Gamma convolution, inverse FFT into r.
 static int residual_enqueue_pol(dmk_handle* h, const double* X, double* r);
+static int residual_enqueue_gk(dmk_handle* h, const double* X, double* r);
 
 static int residual_enqueue(dmk_handle* h, const double* X, double* r, int iter) {
   if (h->nfuncs == 0 && iter == 0) h->totalIter = -1;
@@ -409,6 +412,7 @@ static int residual_enqueue(dmk_handle* h, const double* X, double* r, int iter)
   h->nfuncs++;
   h->n_residuals++;
   if (h->scheme == 1) return residual_enqueue_pol(h, X, r);
+  if (h->scheme == 2) return residual_enqueue_gk(h, X, r);
   int rc = response_enqueue(h, X, h->dt);
   if (rc) return rc;
   AimArgs a;
@@ -486,6 +490,42 @@ static int residual_enqueue_pol(dmk_handle* h, const double* X, double* r) {
   return 0;
 }
 
+// formResidual of the Galerkin scheme (grid_mech_spectral_Galerkin.f90:543-601),
+// enqueued: response and its consistent tangent (read by the matrix-free
+// Jacobian at this iterate), the stress-BC term dP, the divergence RMS and the
+// G convolution of P from one forward FFT, DC term dP / wgt on the
+// stress-controlled components and zero on the others, inverse FFT into r.
+// F_aim is not touched here: it changes once per Newton step (set_F_aim).
+static int residual_enqueue_gk(dmk_handle* h, const double* X, double* r) {
+  int rc = response_enqueue(h, X, h->dt);
+  if (rc) return rc;
+  kt_begin(h, KT_TANGENT);
+  for (int p = 0; p < h->n_phases; p++)
+    CKL(dmk_launch_tangent(&h->mp, h->ph_dev + p, &h->ph[p], p, h->dt, h->flag, h->dPdF, h->nrm, h->st));
+  kt_end(h, KT_TANGENT);
+  AimArgs a;
+  aim_args(h, true, false, &a);
+  CKL(dmk_gk_bc(&a, h->small + SM_PAV, h->flag, h->small + SM_REP, h->small + SM_GDC, h->st));
+  kt_begin(h, KT_FFT_FWD);
+  CKL(dmk_fft_r2c(&h->fft, 9, h->P_hom, h->Z, h->st));
+  kt_end(h, KT_FFT_FWD);
+  GalerkinArgs ga{};
+  ga.do_div = 1;
+  ga.residual = 1;
+  for (int k = 0; k < 9; k++) ga.stress_mask[k] = h->stress_mask[k];
+  ga.dc_dev = h->small + SM_GDC;
+  ga.dc_scale = 1.0 / h->wgt;
+  kt_begin(h, KT_GAMMA);
+  CKL(dmk_galerkin(h->Z, &h->ga, &ga, h->partial, h->st));
+  kt_end(h, KT_GAMMA);
+  int nb = dmk_gamma_div_blocks(&h->ga);
+  CKL(dmk_colsum(h->partial, nb, nb, 1, h->partial + nb, h->small + SM_REP + REP_DIV2, 1.0, h->st));
+  kt_begin(h, KT_FFT_INV);
+  CKL(dmk_fft_c2r(&h->fft, 9, h->Z, r, h->st));
+  kt_end(h, KT_FFT_INV);
+  return 0;
+}
+
 // The one synchronisation of a residual evaluation: optional dot products
 // (written after the report), report to the host, solver scalars updated.
 static int residual_finish(dmk_handle* h, const DotJobs* jobs, double* dots_out) {
@@ -500,10 +540,10 @@ static int residual_finish(dmk_handle* h, const DotJobs* jobs, double* dots_out)
   kt_collect(h, KT_FFT_FWD);
   kt_collect(h, KT_GAMMA);
   kt_collect(h, KT_FFT_INV);
-  if (h->scheme == 1) kt_collect(h, KT_TANGENT);
+  if (h->scheme != 0) kt_collect(h, KT_TANGENT);
   const double* rep = h->hsmall + SM_REP;
   std::memcpy(h->P_av, rep + REP_PAV, sizeof h->P_av);
-  std::memcpy(h->F_aim, rep + REP_FAIM, sizeof h->F_aim);
+  if (h->scheme != 2) std::memcpy(h->F_aim, rep + REP_FAIM, sizeof h->F_aim);
   h->err_BC = rep[REP_ERRBC];
   h->status = (int)rep[REP_FLAG];
   h->err_div = std::sqrt(rep[REP_DIV2]) * h->wgt;
@@ -681,6 +721,254 @@ static int solve_richardson(dmk_handle* h, int* reason_out) {
   return 0;
 }
 
+// ------------------------------------------------------------------ Galerkin
+// PETSc SNES "newtonls" with "-ksp_type gmres -snes_linesearch_type bt"
+// (grid_mech_spectral_Galerkin.f90:217-218) on the matrix-free Jacobian GK_op
+// (:607-660), restated from the published methods with PETSc's defaults:
+// Newton's method with the backtracking line search of Dennis & Schnabel,
+// "Numerical Methods for Unconstrained Optimization and Nonlinear Equations"
+// (1983), algorithm A6.3.1 (cubic order, alpha = 1e-4, minimum step 1e-12,
+// maximum step 1e8, 40 backtracking steps, initial slope from the Jacobian
+// applied to the step), one failed line search ending the solve; linear
+// solves by restarted GMRES (Saad & Schultz, SIAM J. Sci. Stat. Comput. 7(3)
+// 1986): restart 30, classical Gram-Schmidt, zero initial guess, Givens
+// residual estimate, relative tolerance 1e-5, no preconditioner (a shell
+// matrix gets none).  SNESSetUpdate's set_F_aim (:666-681) runs at the start
+// of every Newton step.
+
+static int sync_dots(dmk_handle* h, const DotJobs& J, double* out) {
+  CKL(dmk_dots(&J, h->nx, h->partial, h->small + SM_ADOTS, h->st));
+  CK(hipMemcpyAsync(h->hsmall + SM_ADOTS, h->small + SM_ADOTS, sizeof(double) * J.n, hipMemcpyDeviceToHost, h->st));
+  CK(hipStreamSynchronize(h->st));
+  std::memcpy(out, h->hsmall + SM_ADOTS, sizeof(double) * J.n);
+  return 0;
+}
+
+// GK_op: out = G * (K : v), DC term kept on the stress-controlled components
+static int gk_jv(dmk_handle* h, const double* v, double* out) {
+  CKL(dmk_gk_apply(h->N, h->dPdF, v, h->T, h->st));
+  CKL(dmk_fft_r2c(&h->fft, 9, h->T, h->Z, h->st));
+  GalerkinArgs ga{};
+  for (int k = 0; k < 9; k++) ga.stress_mask[k] = h->stress_mask[k];
+  CKL(dmk_galerkin(h->Z, &h->ga, &ga, h->partial, h->st));
+  CKL(dmk_fft_c2r(&h->fft, 9, h->Z, out, h->st));
+  return 0;
+}
+
+// Krylov basis vector i < 31 (the Galerkin scheme allocates Xh with at least 31 vectors)
+static double* gk_basis(dmk_handle* h, int i) { return h->Xh + (size_t)i * h->n9; }
+
+// J x = b, x overwritten; returns the iteration count in *its, *ok = converged
+static int gk_gmres(dmk_handle* h, const double* b, double* x, int* its, bool* ok) {
+  const int m = 30, maxit = 10000;
+  const double rtol = 1.0e-5, atol = 1.0e-50;
+  const long n = h->n9;
+  DotJobs J;
+  J.n = 0;
+  add_dot(&J, b, b, 0);
+  double d[DMK_MAX_DOTS];
+  int rc;
+  if ((rc = sync_dots(h, J, d))) return rc;
+  const double bnorm = std::sqrt(d[0]);
+  CK(hipMemsetAsync(x, 0, sizeof(double) * n, h->st));
+  *its = 0;
+  *ok = std::isfinite(bnorm);
+  if (bnorm == 0.0 || !*ok) return 0;
+  const double tol = std::fmax(rtol * bnorm, atol);
+  std::vector<double> H((size_t)(m + 1) * m), g(m + 1), cs(m), sn(m), y(m);
+  int total = 0;
+  bool first = true;
+  for (;;) {
+    double* v0 = gk_basis(h, 0);
+    double beta;
+    if (first) {
+      beta = bnorm;
+      CKL(dmk_axpby(n, 1.0 / beta, b, 0.0, b, v0, h->st));
+    } else {
+      if ((rc = gk_jv(h, x, v0))) return rc;
+      CKL(dmk_axpby(n, 1.0, b, -1.0, v0, v0, h->st));
+      J.n = 0;
+      add_dot(&J, v0, v0, 0);
+      if ((rc = sync_dots(h, J, d))) return rc;
+      beta = std::sqrt(d[0]);
+      if (beta <= tol) break;
+      CKL(dmk_axpby(n, 1.0 / beta, v0, 0.0, v0, v0, h->st));
+    }
+    first = false;
+    std::fill(g.begin(), g.end(), 0.0);
+    g[0] = beta;
+    int k = 0;
+    bool done = false;
+    for (int j = 0; j < m; j++) {
+      double* w = gk_basis(h, j + 1);
+      if ((rc = gk_jv(h, gk_basis(h, j), w))) return rc;
+      J.n = 0;
+      for (int i = 0; i <= j; i++) add_dot(&J, w, gk_basis(h, i), 0);
+      if ((rc = sync_dots(h, J, d))) return rc;
+      CombineArgs ca;
+      ca.l = j + 1;
+      ca.atot = 0.0;
+      for (int i = 0; i <= j; i++) {
+        H[i + (m + 1) * j] = d[i];
+        ca.alpha[i] = -d[i];
+        ca.X[i] = gk_basis(h, i);
+      }
+      CKL(dmk_lincomb(n, 1.0, w, &ca, w, h->st));
+      J.n = 0;
+      add_dot(&J, w, w, 0);
+      if ((rc = sync_dots(h, J, d))) return rc;
+      double hn = std::sqrt(d[0]);
+      if (!std::isfinite(hn)) { *ok = false; *its = total; return 0; }
+      H[j + 1 + (m + 1) * j] = hn;
+      if (hn > 0.0) CKL(dmk_axpby(n, 1.0 / hn, w, 0.0, w, w, h->st));
+      for (int i = 0; i < j; i++) {
+        double a = H[i + (m + 1) * j], c = H[i + 1 + (m + 1) * j];
+        H[i + (m + 1) * j] = cs[i] * a + sn[i] * c;
+        H[i + 1 + (m + 1) * j] = -sn[i] * a + cs[i] * c;
+      }
+      double a = H[j + (m + 1) * j], c = H[j + 1 + (m + 1) * j];
+      double r = std::hypot(a, c);
+      cs[j] = r > 0.0 ? a / r : 1.0;
+      sn[j] = r > 0.0 ? c / r : 0.0;
+      H[j + (m + 1) * j] = r;
+      H[j + 1 + (m + 1) * j] = 0.0;
+      g[j + 1] = -sn[j] * g[j];
+      g[j] = cs[j] * g[j];
+      total++;
+      k = j + 1;
+      if (std::fabs(g[j + 1]) <= tol || hn == 0.0) { done = true; break; }
+      if (total >= maxit) break;
+    }
+    for (int i = k - 1; i >= 0; i--) {
+      double s = g[i];
+      for (int q = i + 1; q < k; q++) s -= H[i + (m + 1) * q] * y[q];
+      y[i] = s / H[i + (m + 1) * i];
+    }
+    CombineArgs ca;
+    ca.l = k;
+    ca.atot = 0.0;
+    for (int i = 0; i < k; i++) { ca.alpha[i] = y[i]; ca.X[i] = gk_basis(h, i); }
+    CKL(dmk_lincomb(n, 1.0, x, &ca, x, h->st));
+    if (done) break;
+    if (total >= maxit) { *ok = false; break; }
+  }
+  *its = total;
+  return 0;
+}
+
+// trial point W = X - lambda Y and its residual G; g = |G|^2
+static int gk_trial(dmk_handle* h, double lambda, const double* Y, double* W, double* G, int iter, double* g) {
+  CKL(dmk_axpby(h->n9, 1.0, h->F, -lambda, Y, W, h->st));
+  DotJobs J;
+  J.n = 0;
+  add_dot(&J, G, G, 0);
+  int rc;
+  if ((rc = residual_enqueue(h, W, G, iter)) || (rc = residual_finish(h, &J, g))) return rc;
+  return 0;
+}
+
+// backtracking line search from X = h->F, F = h->R along -Y; on success the
+// accepted pair becomes (h->F, h->R) and *fnorm its residual norm
+static int gk_linesearch(dmk_handle* h, double* Y, int iter, double* fnorm, bool* ok) {
+  const double alpha = 1.0e-4, minlambda = 1.0e-12, maxstep = 1.0e8;
+  const int max_its = 40;
+  double* W = pool_free(h->poolX, h->F, nullptr, 0, Y);
+  double* G = pool_free(h->poolF, h->R, nullptr, 0, nullptr);
+  double* JY = pool_free(h->poolF, h->R, nullptr, 0, G);
+  if (!W || !G || !JY) { dmk_set_error("Galerkin buffer pool exhausted"); return DMK_ERR_CONFIG; }
+  DotJobs J;
+  J.n = 0;
+  add_dot(&J, Y, Y, 0);
+  double d[DMK_MAX_DOTS];
+  int rc;
+  if ((rc = sync_dots(h, J, d))) return rc;
+  double ynorm = std::sqrt(d[0]);
+  *ok = false;
+  if (ynorm == 0.0) return 0;
+  if (ynorm > maxstep) {
+    CKL(dmk_axpby(h->n9, maxstep / ynorm, Y, 0.0, Y, Y, h->st));
+    ynorm = maxstep;
+  }
+  const double f = (*fnorm) * (*fnorm);
+  if ((rc = gk_jv(h, Y, JY))) return rc;
+  J.n = 0;
+  add_dot(&J, h->R, JY, 0);
+  if ((rc = sync_dots(h, J, d))) return rc;
+  double initslope = d[0];
+  if (initslope > 0.0) initslope = -initslope;
+  if (initslope == 0.0) initslope = -1.0;
+  double lambda = 1.0, g;
+  for (;;) {
+    if ((rc = gk_trial(h, lambda, Y, W, G, iter, &g))) return rc;
+    if (std::isfinite(g)) break;
+    lambda *= 0.5;
+    if (lambda <= minlambda) return 0;
+  }
+  bool accept = 0.5 * g <= 0.5 * f + lambda * alpha * initslope;
+  if (!accept) {
+    double lambdatemp = -initslope / (g - f - 2.0 * lambda * initslope);
+    double lambdaprev = lambda, gprev = g;
+    if (lambdatemp > 0.5 * lambda) lambdatemp = 0.5 * lambda;
+    lambda = lambdatemp <= 0.1 * lambda ? 0.1 * lambda : lambdatemp;
+    if ((rc = gk_trial(h, lambda, Y, W, G, iter, &g))) return rc;
+    accept = 0.5 * g < 0.5 * f + lambda * alpha * initslope;
+    for (int count = 0; !accept && count < max_its; count++) {
+      if (lambda <= minlambda) return 0;
+      double t1 = 0.5 * (g - f) - lambda * initslope;
+      double t2 = 0.5 * (gprev - f) - lambdaprev * initslope;
+      double a = (t1 / (lambda * lambda) - t2 / (lambdaprev * lambdaprev)) / (lambda - lambdaprev);
+      double b = (-lambdaprev * t1 / (lambda * lambda) + lambda * t2 / (lambdaprev * lambdaprev)) / (lambda - lambdaprev);
+      double dd = b * b - 3.0 * a * initslope;
+      if (dd < 0.0) dd = 0.0;
+      lambdatemp = a == 0.0 ? -initslope / (2.0 * b) : (-b + std::sqrt(dd)) / (3.0 * a);
+      lambdaprev = lambda;
+      gprev = g;
+      if (lambdatemp > 0.5 * lambda) lambdatemp = 0.5 * lambda;
+      lambda = lambdatemp <= 0.1 * lambda ? 0.1 * lambda : lambdatemp;
+      if ((rc = gk_trial(h, lambda, Y, W, G, iter, &g))) return rc;
+      accept = 0.5 * g < 0.5 * f + lambda * alpha * initslope;
+    }
+  }
+  if (!accept) return 0;
+  h->F = W;
+  h->R = G;
+  *fnorm = std::sqrt(g);
+  *ok = true;
+  return 0;
+}
+
+static int solve_newtonls(dmk_handle* h, int* reason_out) {
+  int rc;
+  DotJobs J;
+  J.n = 0;
+  add_dot(&J, h->R, h->R, 0);
+  double d[DMK_MAX_DOTS];
+  if ((rc = residual_enqueue(h, h->F, h->R, 0)) || (rc = residual_finish(h, &J, d))) return rc;
+  double fnorm = std::sqrt(d[0]);
+  int reason = converged(h);
+  if (reason || fnorm != fnorm) { *reason_out = reason ? reason : -1; return 0; }
+  for (int i = 0; i < h->num.snes_max_it; i++) {
+    // set_F_aim (grid_mech_spectral_Galerkin.f90:666-681)
+    for (int q = 0; q < 9; q++) {
+      double s = 0.0;
+      for (int p = 0; p < 9; p++) s += h->S[q + 9 * p] * (h->P_av[p] - h->P_aim[p]);
+      h->F_aim[q] -= s;
+    }
+    double* Y = pool_free(h->poolX, h->F, nullptr, 0, nullptr);
+    if (!Y) { dmk_set_error("Galerkin buffer pool exhausted"); return DMK_ERR_CONFIG; }
+    int lits;
+    bool kok, lok;
+    if ((rc = gk_gmres(h, h->R, Y, &lits, &kok))) return rc;
+    if (!kok) { *reason_out = -1; return 0; }
+    if ((rc = gk_linesearch(h, Y, i, &fnorm, &lok))) return rc;
+    if (!lok) { *reason_out = -1; return 0; }
+    reason = converged(h);
+    if (reason || fnorm != fnorm) { *reason_out = reason ? reason : -1; return 0; }
+  }
+  *reason_out = -1;
+  return 0;
+}
+
 // ------------------------------------------------------------------ adjoint
 // Reverse sweep over recorded increments.  This has no counterpart in the
 // reference: it differentiates the converged spectral-basic increments with
@@ -793,13 +1081,7 @@ extern "C" int dmk_adjoint_release(dmk_handle* h) {
   return 0;
 }
 
-static int adj_dots(dmk_handle* h, const DotJobs& J, double* out) {
-  CKL(dmk_dots(&J, h->n9, h->partial, h->small + SM_ADOTS, h->st));
-  CK(hipMemcpyAsync(h->hsmall + SM_ADOTS, h->small + SM_ADOTS, sizeof(double) * J.n, hipMemcpyDeviceToHost, h->st));
-  CK(hipStreamSynchronize(h->st));
-  std::memcpy(out, h->hsmall + SM_ADOTS, sizeof(double) * J.n);
-  return 0;
-}
+static int adj_dots(dmk_handle* h, const DotJobs& J, double* out) { return sync_dots(h, J, out); }
 
 // out = Phi(y): Gamma_0 convolution with the mean rule of the increment
 static int adj_phi(dmk_handle* h, const AimArgs& aa, const double* y, double* out) {
@@ -1095,7 +1377,7 @@ static int init_impl(dmk_handle* h, const int32_t cells[3], const double size[3]
     dmk_set_error("IO_error 301: N_iter_min / N_iter_max");
     return DMK_ERR_CONFIG;
   }
-  if (h->num.scheme != 0 && h->num.scheme != 1) { dmk_set_error("IO_error 601: not a valid grid solver type"); return DMK_ERR_CONFIG; }
+  if (h->num.scheme < 0 || h->num.scheme > 2) { dmk_set_error("IO_error 601: not a valid grid solver type"); return DMK_ERR_CONFIG; }
   h->scheme = h->num.scheme;
   if (h->scheme == 1) {
     // grid_mech_spectral_polarization.f90:141-147; beta = 0, which the
@@ -1171,12 +1453,18 @@ static int init_impl(dmk_handle* h, const int32_t cells[3], const double size[3]
   double** sol[] = {&h->F, &h->F_lastinc, &h->Fdot, &h->R, &h->XM, &h->FM, &h->XA, &h->FA};
   for (auto pp : sol) if ((rc = dalloc(h, pp, nx))) return rc;
   if ((rc = dalloc(h, &h->Fext, n9)) || (rc = dalloc(h, &h->Pext, n9))) return rc;
-  if (h->scheme == 1 && (rc = dalloc(h, &h->T, n9))) return rc;
-  if ((rc = dalloc(h, &h->Xh, (size_t)h->num.ngmres_m * nx))) return rc;
-  if ((rc = dalloc(h, &h->Fh, (size_t)h->num.ngmres_m * nx))) return rc;
+  if (h->scheme != 0 && (rc = dalloc(h, &h->T, n9))) return rc;
+  // NGMRES history (M pairs), or for the Galerkin scheme the 31 Krylov vectors of GMRES(30)
+  if (h->scheme == 2) {
+    if ((rc = dalloc(h, &h->Xh, (size_t)31 * nx))) return rc;
+    h->Fh = nullptr;
+  } else {
+    if ((rc = dalloc(h, &h->Xh, (size_t)h->num.ngmres_m * nx))) return rc;
+    if ((rc = dalloc(h, &h->Fh, (size_t)h->num.ngmres_m * nx))) return rc;
+  }
   h->poolX = {h->F, h->XM, h->XA};
   h->poolF = {h->R, h->FM, h->FA};
-  for (int i = 0; i < h->num.ngmres_m; i++) {
+  for (int i = 0; h->scheme != 2 && i < h->num.ngmres_m; i++) {
     h->poolX.push_back(h->Xh + (size_t)i * nx);
     h->poolF.push_back(h->Fh + (size_t)i * nx);
   }
@@ -1386,7 +1674,8 @@ extern "C" int dmk_mech_solution(dmk_handle* h, int32_t* iterations, int32_t* co
   int reason = 0;
   // F_aim is updated on the device during the solve (residual_aim_kernel)
   CK(hipMemcpyAsync(h->small + SM_FAIM, h->F_aim, sizeof h->F_aim, hipMemcpyHostToDevice, h->st));
-  rc = h->num.solver == 1 ? solve_richardson(h, &reason) : solve_ngmres(h, &reason);
+  if (h->scheme == 2) rc = solve_newtonls(h, &reason);
+  else rc = h->num.solver == 1 ? solve_richardson(h, &reason) : solve_ngmres(h, &reason);
   if (rc) return rc;
   if ((rc = tangent_pass(h, h->dt))) return rc;
   if (iterations) *iterations = h->totalIter;
@@ -1530,7 +1819,9 @@ extern "C" int dmk_mech_get_scalars(dmk_handle* h, int32_t what, double* dst) {
     case 11:
       // grid_mech_spectral_basic.f90:388,408: the stiffness Gamma was built from
       // (the current C_minMaxAvg when update_gamma is set)
-      std::memcpy(dst, h->num.update_gamma ? h->C_minMaxAvg : h->C_minMaxAvgRestart, 81 * sizeof(double));
+      // the Galerkin scheme never assigns it and stores zeros (grid_mech_spectral_Galerkin.f90:82,488)
+      if (h->scheme == 2) std::memset(dst, 0, 81 * sizeof(double));
+      else std::memcpy(dst, h->num.update_gamma ? h->C_minMaxAvg : h->C_minMaxAvgRestart, 81 * sizeof(double));
       break;
     case 6: std::memcpy(dst, h->F_aimDot, 9 * sizeof(double)); break;
     case 7: std::memcpy(dst, h->F_aim_lastinc, 9 * sizeof(double)); break;

@@ -12,6 +12,7 @@
 //   With the continuous derivative xi = i a, Gamma_hat:P_hat collapses to
 //   (A^-1 (P_hat a)) (x) a with the acoustic tensor A_lm = C_ref,ljmq a_j a_q.
 // * curl_kernel: the RMS curl of the polarization scheme (:697-757).
+// * galerkin_kernel: the G convolution of the Galerkin scheme (:507-589).
 // * coords kernels: FFT integration of F for the node and cell positions
 //   (:857-968).
 #include "dmk_common.h"
@@ -416,6 +417,93 @@ __global__ void __launch_bounds__(256) curl_kernel(const double2* __restrict__ Z
   if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
 }
 
+// utilities_G_Convolution (spectral_utilities.f90:544-589) with G_hat of
+// G_hat_init (:507-533): G_lmno = delta_ln conj(-xi_m) xi_o / |xi|^2, i.e. with
+// xi = i a, (G:F_hat)_lm = -a_m (F_hat a)_l / |a|^2 (zero where |a|^2 <= 1e-16).
+// DC term: residual mode sets it to the aim (dc_dev * dc_scale, the reference's
+// fieldAim / wgt), Jacobian mode keeps the transform's own; both then zero the
+// components whose stress is not prescribed.  The optional RMS divergence reads
+// the transform before it is overwritten (same sum as gamma_div_kernel).
+__global__ void __launch_bounds__(256) galerkin_kernel(double2* __restrict__ Z, GammaArgs g, GalerkinArgs ga,
+                                                       double* __restrict__ div_partial) {
+  const long Nk = (long)g.h * g.n[1] * g.n[2];
+  const long f = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  double contrib = 0.0;
+  if (f < Nk) {
+    int kx = (int)(f % g.h);
+    long r = f / g.h;
+    int ky = (int)(r % g.n[1]);
+    int kz = (int)(r / g.n[1]);
+    const double TAU = 6.283185307179586;
+    double a[3];
+    a[0] = TAU * (double)kx / g.size[0];
+    a[1] = TAU * wrap_freq(ky, g.n[1]) / g.size[1];
+    a[2] = TAU * wrap_freq(kz, g.n[2]) / g.size[2];
+    if (g.n[0] % 2 == 0 && 2 * kx == g.n[0]) a[0] = 0.0;
+    if (g.n[1] % 2 == 0 && 2 * ky == g.n[1]) a[1] = 0.0;
+    if (g.n[2] % 2 == 0 && 2 * kz == g.n[2]) a[2] = 0.0;
+    double2 P[9];
+#pragma unroll
+    for (int c = 0; c < 9; c++) P[c] = Z[c * Nk + f];
+    if (ga.do_div) {
+      double s = 0.0;
+#pragma unroll
+      for (int l = 0; l < 3; l++) {
+        double2 u = make_double2(0.0, 0.0);
+#pragma unroll
+        for (int m = 0; m < 3; m++) {
+          double w = a[m] * g.rg[m];
+          u.x = fma(P[l + 3 * m].x, w, u.x);
+          u.y = fma(P[l + 3 * m].y, w, u.y);
+        }
+        s = fma(u.x, u.x, fma(u.y, u.y, s));
+      }
+      double wgt;
+      if (g.h == 1) wgt = 2.0;
+      else wgt = (kx >= 1 && kx <= g.h - 2) ? 2.0 : 1.0;
+      contrib = wgt * s;
+    }
+    if (f == 0) {
+#pragma unroll
+      for (int c = 0; c < 9; c++) {
+        double2 v = ga.residual ? make_double2(ga.dc_dev[c] * ga.dc_scale, 0.0) : P[c];
+        Z[c * Nk] = ga.stress_mask[c] ? make_double2(0.0, 0.0) : v;
+      }
+    } else {
+      const double n2 = a[0] * a[0] + a[1] * a[1] + a[2] * a[2];
+      if (n2 > 1.0e-16) {
+#pragma unroll
+        for (int l = 0; l < 3; l++) {
+          double2 v = make_double2(0.0, 0.0);
+#pragma unroll
+          for (int o = 0; o < 3; o++) {
+            v.x = fma(P[l + 3 * o].x, a[o], v.x);
+            v.y = fma(P[l + 3 * o].y, a[o], v.y);
+          }
+#pragma unroll
+          for (int m = 0; m < 3; m++) {
+            double w = -a[m] / n2;
+            Z[(l + 3 * m) * Nk + f] = make_double2(v.x * w, v.y * w);
+          }
+        }
+      } else {
+#pragma unroll
+        for (int c = 0; c < 9; c++) Z[c * Nk + f] = make_double2(0.0, 0.0);
+      }
+    }
+  }
+  if (ga.do_div) {
+    __shared__ double red[256];
+    red[threadIdx.x] = contrib;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+      if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) div_partial[blockIdx.x] = red[0];
+  }
+}
+
 // updateCoords: per frequency u_hat = F_hat xi / (conj(-xi) . xi), xi second-derivative set
 __global__ void coords_freq_kernel(const double2* __restrict__ Fh, double2* __restrict__ U, GammaArgs g) {
   const long Nk = (long)g.h * g.n[1] * g.n[2];
@@ -635,6 +723,11 @@ int dmk_gamma_div_blocks(const GammaArgs* g) {
 
 int dmk_curl_partial(const double2* Z, const GammaArgs* g, double* partial, hipStream_t s) {
   hipLaunchKernelGGL(curl_kernel, dim3(dmk_gamma_div_blocks(g)), dim3(256), 0, s, Z, *g, partial);
+  return hipGetLastError() == hipSuccess ? 0 : 2;
+}
+
+int dmk_galerkin(double2* Z, const GammaArgs* g, const GalerkinArgs* ga, double* div_partial, hipStream_t s) {
+  hipLaunchKernelGGL(galerkin_kernel, dim3(dmk_gamma_div_blocks(g)), dim3(256), 0, s, Z, *g, *ga, div_partial);
   return hipGetLastError() == hipSuccess ? 0 : 2;
 }
 

@@ -315,9 +315,22 @@ class GridMechSpectralPolarization(GridMechSpectralBasic):
         raise NotImplementedError("the adjoint differentiates the spectral-basic scheme")
 
 
+class GridMechSpectralGalerkin(GridMechSpectralBasic):
+    """One RVE on the GPU behind the spectral-Galerkin plug-in interface
+    (``src/grid/grid_mech_spectral_Galerkin.f90``): Newton steps with a
+    backtracking line search, each solving G*(K:dF) = G*P by GMRES with the
+    matrix-free Jacobian (:607-660); F_aim follows the stress BC once per Newton
+    step (:666-681).  Restart data as the basic scheme's."""
+    SCHEME = 2
+
+    def adjoint_tape(self, max_increments: int) -> None:
+        raise NotImplementedError("the adjoint differentiates the spectral-basic scheme")
+
+
 # load.yaml solver:mechanical (DAMASK_grid.f90:171-204)
 MECHANICAL_SOLVERS = {"spectral_basic": GridMechSpectralBasic,
-                      "spectral_polarization": GridMechSpectralPolarization}
+                      "spectral_polarization": GridMechSpectralPolarization,
+                      "spectral_Galerkin": GridMechSpectralGalerkin}
 
 
 def grid_mechanical(name: str):
@@ -326,7 +339,7 @@ def grid_mechanical(name: str):
         return MECHANICAL_SOLVERS[name]
     except KeyError:
         from .config import ConfigError
-        if name in ("spectral_Galerkin", "FEM"):
+        if name == "FEM":
             raise ConfigError(601, f"{name} is a valid grid solver type of the reference, not built here") from None
         raise ConfigError(601, f"{name} is not a valid grid solver type") from None
 

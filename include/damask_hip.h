@@ -70,7 +70,7 @@ typedef struct {
   double r_linesearch_Lp, eps_rel_Lp, eps_abs_Lp;
   int32_t f_update_jacobi_Lp;
   int32_t cref_rule;               /* 0 strict first-extremum rule, 1 with 1e-12 dead band */
-  int32_t scheme;                  /* 0 spectral_basic, 1 spectral_polarization */
+  int32_t scheme;                  /* 0 spectral_basic, 1 spectral_polarization, 2 spectral_Galerkin */
   double alpha, beta;              /* polarization: 0 < alpha <= 2, 0 < beta <= 2 */
   double eps_abs_curl, eps_rel_curl;
 } dmk_numerics;

@@ -1,12 +1,13 @@
-"""spectral_polarization (grid_mech_spectral_polarization.f90) on the GPU,
-through the C-ABI, against the spectral-basic scheme.
+"""spectral_polarization (grid_mech_spectral_polarization.f90) and
+spectral_Galerkin (grid_mech_spectral_Galerkin.f90) on the GPU, through the
+C-ABI, against the spectral-basic scheme.
 
 * The reference's own solver-consistency test
   (tests/integration/test_grid_mech_consistency.py, its g_444 / tensionX /
   material_plast resources): with the default numerics, F, F_e, F_p, L_p and O
   at every output increment agree between the schemes within the test's
   bound (atol 1e-2, rtol 1e-5).
-* At tight tolerances both schemes converge to the same equilibrium: volume
+* At tight tolerances the schemes converge to the same equilibrium: volume
   averaged P within 1e-7 of max |P| and per-cell F within 1e-8.
 * The curl RMS of the device against the oracle's restatement of
   utilities_curlRMS (spectral_utilities.f90:697-757) on random fields.
@@ -49,21 +50,34 @@ def _run(name, grid, mat, lcs, num):
     return rec, its
 
 
-def test_consistency_with_basic_at_reference_bar(gpu_lib):
+SCHEMES = ("spectral_polarization", "spectral_Galerkin")
+_basic = {}
+
+
+def _basic_run(key, grid, mat, lcs, num):
+    if key not in _basic:
+        _basic[key] = _run("spectral_basic", grid, mat, lcs, num)
+    return _basic[key]
+
+
+@pytest.mark.parametrize("scheme", SCHEMES)
+def test_consistency_with_basic_at_reference_bar(gpu_lib, scheme):
     from damask_amd import config
     grid, mat, lcs = _inputs()
     num = config.Numerics()
-    rb, itb = _run("spectral_basic", grid, mat, lcs, num)
-    rp, itp = _run("spectral_polarization", grid, mat, lcs, num)
-    print(f"iterations: basic {itb}, polarization {itp}")
+    rb, itb = _basic_run("default", grid, mat, lcs, num)
+    rp, itp = _run(scheme, grid, mat, lcs, num)
+    print(f"iterations: basic {itb}, {scheme} {itp}")
     assert len(rb) == len(rp) == lcs[0].N // lcs[0].f_out
-    assert all(r["err_curl"] > 0.0 for r in rp)
+    if scheme == "spectral_polarization":
+        assert all(r["err_curl"] > 0.0 for r in rp)
     for a, b in zip(rb, rp):
         for k in OUT:
             np.testing.assert_allclose(b[k], a[k], rtol=1e-5, atol=1e-2, err_msg=k)
 
 
-def test_polarization_converges_to_basic_solution(gpu_lib):
+@pytest.mark.parametrize("scheme", SCHEMES)
+def test_scheme_converges_to_basic_solution(gpu_lib, scheme):
     import copy
     from damask_amd import config
     grid, mat, lcs = _inputs()
@@ -71,9 +85,9 @@ def test_polarization_converges_to_basic_solution(gpu_lib):
     lc.N, lc.t, lc.f_out = 30, 4.5, 5
     num = config.Numerics(eps_abs_div=1e-8, eps_rel_div=1e-11, eps_abs_P=1e-3, eps_rel_P=1e-11,
                           eps_abs_curl=1e-13, eps_rel_curl=1e-11, N_iter_max=400, snes_max_it=400)
-    rb, _ = _run("spectral_basic", grid, mat, [lc], num)
-    rp, itp = _run("spectral_polarization", grid, mat, [lc], num)
-    print("polarization iterations", itp)
+    rb, _ = _basic_run("tight", grid, mat, [lc], num)
+    rp, itp = _run(scheme, grid, mat, [lc], num)
+    print(scheme, "iterations", itp)
     for a, b in zip(rb, rp):
         pmax = np.abs(a["P_av"]).max()
         assert np.abs(b["P_av"] - a["P_av"]).max() < 1e-7 * pmax

@@ -1,4 +1,4 @@
-"""Host side of the spectral-polarization scheme (no GPU): the library's
+"""Host side of the spectral-polarization and -Galerkin schemes (no GPU): the library's
 invSym3333 (math.f90:511-532) against the oracle on tensors without minor
 symmetry (as dPdF + C_scale is), the numerics keys and checks of
 grid_mech_spectral_polarization.f90:136-153, and the solver selection of
@@ -46,7 +46,8 @@ def test_solver_selection():
     name, _ = config.parse_load(d)
     assert grid.grid_mechanical(name) is grid.GridMechSpectralPolarization
     assert grid.grid_mechanical("spectral_basic") is grid.GridMechSpectralBasic
-    for bad in ("spectral_Galerkin", "FEM", "n/a"):
+    assert grid.grid_mechanical("spectral_Galerkin") is grid.GridMechSpectralGalerkin
+    for bad in ("FEM", "n/a"):
         with pytest.raises(config.ConfigError) as e:
             grid.grid_mechanical(bad)
         assert e.value.code == 601
