@@ -218,30 +218,36 @@ def test_full_fcc_golden_run_against_reference_result(gpu_lib):
         assert np.abs(g.field("P").mean(0) - Pr).max() < 1e-3 * np.abs(Pr).max()
 
 
-def test_restart_continues_the_run(gpu_lib):
+@pytest.mark.parametrize("scheme", ["spectral_basic", "spectral_polarization", "spectral_Galerkin"])
+def test_restart_continues_the_run(gpu_lib, scheme):
     """restartWrite after increment 4, a fresh solver reading it
-    (grid_mech_spectral_basic.f90:198-232) and skipping increments 1-4
-    (DAMASK_grid.f90:300-303) continues the uninterrupted run: increments 5-8
-    agree at the (tight) solver tolerance."""
+    (grid_mech_spectral_basic.f90:198-232, grid_mech_spectral_polarization.f90
+    :222-270 with F_tau, grid_mech_spectral_Galerkin.f90:283-318) and skipping
+    increments 1-4 (DAMASK_grid.f90:300-303) continues the uninterrupted run:
+    increments 5-8 agree at the (tight) solver tolerance."""
     import copy as _copy
     from damask_amd import config, workloads
-    from damask_amd.grid import GridMechSpectralBasic
+    from damask_amd.grid import grid_mechanical
+    Solver = grid_mechanical(scheme)
     grid, mat = workloads.polycrystal([8, 8, 8], 5, 7)
     mat = config.parse_material(mat)
     _, lcs = config.parse_load(workloads.tension_x(8.0, 8))
     num = config.Numerics(eps_abs_div=1e-8, eps_rel_div=1e-11, eps_abs_P=1e-3, eps_rel_P=1e-11, N_iter_max=400,
-                          snes_max_it=400, eps_rel_state=1e-12, eps_rel_Lp=1e-11, eps_abs_Lp=1e-16)
+                          snes_max_it=400, eps_rel_state=1e-12, eps_rel_Lp=1e-11, eps_abs_Lp=1e-16,
+                          eps_abs_curl=1e-13, eps_rel_curl=1e-11)
     lc = lcs[0]
-    straight = GridMechSpectralBasic(grid, mat, num)
+    straight = Solver(grid, mat, num)
     r_all = straight.run_loadcase(lc)
     assert r_all["rc"] == 0
-    first = GridMechSpectralBasic(grid, mat, num)
+    first = Solver(grid, mat, num)
     half = _copy.copy(lc)
     half.t, half.N = lc.t / 2, lc.N // 2
     assert first.run_loadcase(half)["rc"] == 0
     data = first.restart_write()
     first.close()
-    again = GridMechSpectralBasic(grid, mat, num)
+    if scheme == "spectral_polarization":
+        assert {"F_tau", "F_tau_lastinc"} <= set(data)
+    again = Solver(grid, mat, num)
     again.restart_read(data, restart_inc=half.N)
     r_cont = again.run_loadcase(lc)
     assert r_cont["rc"] == 0

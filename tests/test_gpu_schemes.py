@@ -95,6 +95,26 @@ def test_scheme_converges_to_basic_solution(gpu_lib, scheme):
         assert np.abs(b["F_p"] - a["F_p"]).max() < 1e-8
 
 
+def test_polarization_update_gamma(gpu_lib):
+    """update_gamma: Gamma, C_scale and S_scale follow C_minMaxAvg at every
+    solution (grid_mech_spectral_polarization.f90:301-305); same equilibrium as
+    the basic scheme with update_gamma"""
+    import copy
+    from damask_amd import config
+    grid, mat, lcs = _inputs()
+    lc = copy.copy(lcs[0])
+    lc.N, lc.t, lc.f_out = 20, 3.0, 5
+    num = config.Numerics(eps_abs_div=1e-8, eps_rel_div=1e-11, eps_abs_P=1e-3, eps_rel_P=1e-11,
+                          eps_abs_curl=1e-13, eps_rel_curl=1e-11, N_iter_max=400, snes_max_it=400,
+                          update_gamma=True)
+    rb, _ = _run("spectral_basic", grid, mat, [lc], num)
+    rp, itp = _run("spectral_polarization", grid, mat, [lc], num)
+    print("iterations", itp)
+    for a, b in zip(rb, rp):
+        assert np.abs(b["P_av"] - a["P_av"]).max() < 1e-7 * np.abs(a["P_av"]).max()
+        assert np.abs(b["F"] - a["F"]).max() < 1e-8
+
+
 @pytest.mark.parametrize("cells", [[6, 7, 8], [8, 8, 8], [5, 6, 4]])
 def test_curl_rms_against_oracle(gpu_lib, oracle, cells):
     from damask_amd import _lib, grid as G, workloads, config
