@@ -15,7 +15,16 @@ struct CombineArgs {
 struct ForwardArgs {
   int cutback;
   double avRate[9];   // column-major rate for the homogeneous guess
-  double shift[9];    // mean correction (column-major)
+  double aim[9];      // mean the forwarded field is corrected to (column-major)
+  const double* mean; // device: the forwarded field's mean (the correction is mean - aim)
+};
+
+// up to 8 device-to-device copies in one launch (blockIdx.y = copy)
+struct CopyJobs {
+  const double* src[8];
+  double* dst[8];
+  long n[8];
+  int k;
 };
 
 // several dot products in one launch: out[j] = x_j . y_j (mode 0) or
@@ -63,6 +72,7 @@ int dmk_status(const int32_t* st, const int32_t* nn, const int32_t* nf, long N, 
 int dmk_forward_field(long N, int guess, double dt_old, double dt, const ForwardArgs* fa, const double* F,
                       double* F_lastinc, double* Fdot, double* Fnew, hipStream_t s);
 int dmk_shift_field(long N, const ForwardArgs* fa, double* F, hipStream_t s);
+int dmk_copy_jobs(const CopyJobs* j, hipStream_t s);
 int dmk_aos_to_soa(const double* a, long N, int w, double* soa, hipStream_t s);
 int dmk_soa_to_aos(const double* soa, long N, int w, double* a, hipStream_t s);
 

@@ -45,7 +45,8 @@ struct GalerkinArgs {
 struct CoordsArgs {
   int n[3];
   double step[3];
-  double Favg[9];          // column-major
+  const double2* Fdc;      // forward transform of F: mean F_c = Fdc[c * Nk].x * wgt
+  long Nk;
   double wgt;
 };
 

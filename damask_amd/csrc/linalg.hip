@@ -9,6 +9,8 @@
 
 #include <float.h>
 
+#include <algorithm>
+
 #define RB 256
 #define PER_BLOCK 2048
 
@@ -302,7 +304,16 @@ __global__ void forward_field_kernel(long N, int guess, double dt_old, double dt
 __global__ void shift_kernel(long N, ForwardArgs fa, double* __restrict__ F) {
   long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= 9 * N) return;
-  F[i] -= fa.shift[i / N];
+  int c = (int)(i / N);
+  F[i] -= fa.mean[c] - fa.aim[c];
+}
+
+__global__ void copy_jobs_kernel(CopyJobs j) {
+  const int q = blockIdx.y;
+  const double* __restrict__ src = j.src[q];
+  double* __restrict__ dst = j.dst[q];
+  const long n = j.n[q];
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) dst[i] = src[i];
 }
 
 // (9,N) cell-major <-> component-major
@@ -397,6 +408,15 @@ int dmk_forward_field(long N, int guess, double dt_old, double dt, const Forward
 
 int dmk_shift_field(long N, const ForwardArgs* fa, double* F, hipStream_t s) {
   hipLaunchKernelGGL(shift_kernel, dim3((unsigned)((9 * N + 255) / 256)), dim3(256), 0, s, N, *fa, F);
+  return hipGetLastError() == hipSuccess ? 0 : 2;
+}
+
+int dmk_copy_jobs(const CopyJobs* j, hipStream_t s) {
+  if (j->k < 1 || j->k > 8) return 1;
+  long mx = 0;
+  for (int q = 0; q < j->k; q++) mx = j->n[q] > mx ? j->n[q] : mx;
+  unsigned nb = (unsigned)std::min<long>((mx + 255) / 256, 4096);
+  hipLaunchKernelGGL(copy_jobs_kernel, dim3(nb ? nb : 1, (unsigned)j->k), dim3(256), 0, s, *j);
   return hipGetLastError() == hipSuccess ? 0 : 2;
 }
 

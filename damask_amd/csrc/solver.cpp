@@ -235,15 +235,18 @@ struct dmk_handle {
 enum { KT_STRESS = 0, KT_TANGENT = 1, KT_REDUCE = 2, KT_FFT_FWD = 3, KT_GAMMA = 4, KT_FFT_INV = 5, KT_CAVG = 6,
        KT_N = 7 };
 
+// profile level 1 times the stress kernel only, level 2 every group (each
+// event record costs the stream a few microseconds between kernels)
+static bool kt_on(const dmk_handle* h, int k) { return h->profile >= (k == KT_STRESS ? 1 : 2); }
 static void kt_begin(dmk_handle* h, int k) {
-  if (h->profile) hipEventRecord(h->ev[k][0], h->st);
+  if (kt_on(h, k)) hipEventRecord(h->ev[k][0], h->st);
 }
 static void kt_end(dmk_handle* h, int k) {
-  if (h->profile) hipEventRecord(h->ev[k][1], h->st);
+  if (kt_on(h, k)) hipEventRecord(h->ev[k][1], h->st);
 }
 // after a stream synchronisation: accumulate the recorded intervals
 static void kt_collect(dmk_handle* h, int k) {
-  if (!h->profile) return;
+  if (!kt_on(h, k)) return;
   float ms = 0.0f;
   if (hipEventElapsedTime(&ms, h->ev[k][0], h->ev[k][1]) == hipSuccess) {
     h->kt_ms[k] += ms;
@@ -1624,11 +1627,8 @@ extern "C" int dmk_mech_forward(dmk_handle* h, int32_t cutBack, int32_t guess, d
   // F = F_lastinc + Fdot dt, mean corrected to the rotated aim
   CKL(dmk_forward_field(h->N, guess, dt_old, dt, &fa, h->F, h->F_lastinc, h->Fdot, h->F, h->st));
   CKL(dmk_colsum(h->F, h->N, h->N, 9, h->partial, h->small + SM_DOTS, h->wgt, h->st));
-  CK(hipMemcpyAsync(h->hsmall + SM_DOTS, h->small + SM_DOTS, sizeof(double) * 9, hipMemcpyDeviceToHost, h->st));
-  CK(hipStreamSynchronize(h->st));
-  double aim[9];
-  rotate33(bc->q_R, h->F_aim, true, aim);
-  for (int k = 0; k < 9; k++) fa.shift[k] = h->hsmall[SM_DOTS + k] - aim[k];
+  rotate33(bc->q_R, h->F_aim, true, fa.aim);
+  fa.mean = h->small + SM_DOTS;
   CKL(dmk_shift_field(h->N, &fa, h->F, h->st));
   if (h->scheme == 1) {
     // F_tau (grid_mech_spectral_polarization.f90:370-377,389-399): its rate and
@@ -1650,15 +1650,17 @@ extern "C" int dmk_mech_forward(dmk_handle* h, int32_t cutBack, int32_t guess, d
   return 0;
 }
 
+// the six state copies in one launch
 extern "C" int dmk_mech_materialpoint_forward(dmk_handle* h) {
   DmkMPDev& mp = h->mp;
-  const size_t b9 = sizeof(double) * h->n9, bs = sizeof(double) * h->nsl_max * h->N;
-  CK(hipMemcpyAsync(mp.Fp0, mp.Fp, b9, hipMemcpyDeviceToDevice, h->st));
-  CK(hipMemcpyAsync(mp.F0, mp.Fphase, b9, hipMemcpyDeviceToDevice, h->st));
-  CK(hipMemcpyAsync(mp.Lp0, mp.Lp, b9, hipMemcpyDeviceToDevice, h->st));
-  CK(hipMemcpyAsync(mp.S0, mp.S, b9, hipMemcpyDeviceToDevice, h->st));
-  CK(hipMemcpyAsync(mp.xi0, mp.xi, bs, hipMemcpyDeviceToDevice, h->st));
-  CK(hipMemcpyAsync(mp.gam0, mp.gam, bs, hipMemcpyDeviceToDevice, h->st));
+  const long n9 = h->n9, ns = (long)h->nsl_max * h->N;
+  CopyJobs j;
+  const double* src[6] = {mp.Fp, mp.Fphase, mp.Lp, mp.S, mp.xi, mp.gam};
+  double* dst[6] = {mp.Fp0, mp.F0, mp.Lp0, mp.S0, mp.xi0, mp.gam0};
+  const long n[6] = {n9, n9, n9, n9, ns, ns};
+  j.k = 6;
+  for (int q = 0; q < 6; q++) { j.src[q] = src[q]; j.dst[q] = dst[q]; j.n[q] = n[q]; }
+  CKL(dmk_copy_jobs(&j, h->st));
   return 0;
 }
 
@@ -1685,23 +1687,21 @@ extern "C" int dmk_mech_solution(dmk_handle* h, int32_t* iterations, int32_t* co
   return 0;
 }
 
+// enqueued; synchronises only when positions are copied out
 extern "C" int dmk_mech_update_coords(dmk_handle* h, double* x_p, double* x_n) {
   CKL(dmk_fft_r2c(&h->fft, 9, h->F, h->Z, h->st));
-  double2 dc[9];
-  for (int c = 0; c < 9; c++)
-    CK(hipMemcpyAsync(&dc[c], h->Z + (size_t)c * h->Nk, sizeof(double2), hipMemcpyDeviceToHost, h->st));
   CKL(dmk_coords_freq(h->Z, h->U, &h->ga, h->st));
   CKL(dmk_fft_c2r(&h->fft, 3, h->U, h->u3, h->st));
-  CK(hipStreamSynchronize(h->st));
   CoordsArgs ca;
   for (int d = 0; d < 3; d++) { ca.n[d] = h->cells[d]; ca.step[d] = h->size[d] / h->cells[d]; }
-  for (int c = 0; c < 9; c++) ca.Favg[c] = dc[c].x * h->wgt;
+  ca.Fdc = h->Z;
+  ca.Nk = h->Nk;
   ca.wgt = h->wgt;
   CKL(dmk_coords_nodes(h->u3, &ca, h->xn_dev, h->xp_dev, h->st));
   long nn = (long)(h->cells[0] + 1) * (h->cells[1] + 1) * (h->cells[2] + 1);
   if (x_n) CK(hipMemcpyAsync(x_n, h->xn_dev, sizeof(double) * 3 * nn, hipMemcpyDeviceToHost, h->st));
   if (x_p) CK(hipMemcpyAsync(x_p, h->xp_dev, sizeof(double) * 3 * h->N, hipMemcpyDeviceToHost, h->st));
-  CK(hipStreamSynchronize(h->st));
+  if (x_n || x_p) CK(hipStreamSynchronize(h->st));
   return 0;
 }
 
@@ -2054,7 +2054,8 @@ extern "C" int dmk_fft_backward(dmk_handle* h, int32_t howmany, const double* in
   return 0;
 }
 
-// per-kernel timing: enable (1) / disable (0); totals reset on enable
+// per-kernel timing: 0 off, 1 the stress kernel only, 2 every kernel group;
+// totals reset when switched on
 extern "C" int dmk_mech_profile(dmk_handle* h, int32_t enable) {
   if (!h) { dmk_set_error("null handle"); return DMK_ERR_ARG; }
   if (enable && !h->ev_ready) {
@@ -2066,7 +2067,7 @@ extern "C" int dmk_mech_profile(dmk_handle* h, int32_t enable) {
   }
   if (enable)
     for (int k = 0; k < 8; k++) { h->kt_ms[k] = 0.0; h->kt_n[k] = 0; }
-  h->profile = enable ? 1 : 0;
+  h->profile = enable < 0 ? 0 : (enable > 2 ? 2 : enable);
   return 0;
 }
 

@@ -540,6 +540,9 @@ __global__ void coords_nodes_kernel(const double* __restrict__ u, CoordsArgs c, 
   const long N = (long)N1 * N2 * N3;
   const long nn = (long)(N1 + 1) * (N2 + 1) * (N3 + 1);
   long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  // mean F from the DC terms of the forward transform (component c at Fdc[c * Nk])
+  double Favg[9];
+  for (int q = 0; q < 9; q++) Favg[q] = c.Fdc[q * c.Nk].x * c.wgt;
   if (t < nn) {
     int i = (int)(t % (N1 + 1));
     long r = t / (N1 + 1);
@@ -547,7 +550,7 @@ __global__ void coords_nodes_kernel(const double* __restrict__ u, CoordsArgs c, 
     int k = (int)(r / (N2 + 1));
     double pos[3] = {c.step[0] * i, c.step[1] * j, c.step[2] * k};
     double x[3];
-    for (int a = 0; a < 3; a++) x[a] = c.Favg[a] * pos[0] + c.Favg[a + 3] * pos[1] + c.Favg[a + 6] * pos[2];
+    for (int a = 0; a < 3; a++) x[a] = Favg[a] * pos[0] + Favg[a + 3] * pos[1] + Favg[a + 6] * pos[2];
     for (int nb = 0; nb < 8; nb++) {
       int dx = (nb == 1 || nb == 2 || nb == 5 || nb == 6), dy = (nb == 2 || nb == 3 || nb == 6 || nb == 7), dz = nb >= 4;
       int mx = (i + dx - 1 + N1) % N1, my = (j + dy - 1 + N2) % N2, mz = (k + dz - 1 + N3) % N3;
@@ -563,7 +566,7 @@ __global__ void coords_nodes_kernel(const double* __restrict__ u, CoordsArgs c, 
     int k = (int)(r / N2);
     double pos[3] = {c.step[0] * (i + 1 - 0.5), c.step[1] * (j + 1 - 0.5), c.step[2] * (k + 1 - 0.5)};
     for (int a = 0; a < 3; a++)
-      x_p[3 * t + a] = u[a * N + t] * c.wgt + (c.Favg[a] * pos[0] + c.Favg[a + 3] * pos[1] + c.Favg[a + 6] * pos[2]);
+      x_p[3 * t + a] = u[a * N + t] * c.wgt + (Favg[a] * pos[0] + Favg[a + 3] * pos[1] + Favg[a + 6] * pos[2]);
   }
 }
 

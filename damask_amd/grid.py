@@ -208,8 +208,9 @@ class GridMechSpectralBasic:
 
     KERNEL_GROUPS = ("stress", "tangent", "reduce", "fft_fwd", "gamma_div", "fft_inv", "c_avg")
 
-    def profile(self, enable: bool = True) -> None:
-        _lib.check(self.lib.dmk_mech_profile(self.h, int(enable)), "dmk_mech_profile")
+    def profile(self, level: int = 2) -> None:
+        """Per-kernel HIP-event timing: 0 off, 1 the stress kernel only, 2 every group."""
+        _lib.check(self.lib.dmk_mech_profile(self.h, int(level)), "dmk_mech_profile")
 
     def kernel_times(self) -> dict:
         out = {}

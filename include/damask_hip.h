@@ -188,7 +188,8 @@ int dmk_fft_forward(dmk_handle* h, int32_t howmany, const double* in, double* ou
 int dmk_fft_backward(dmk_handle* h, int32_t howmany, const double* in, double* out);
 
 /* timing of the kernel groups with HIP events on the solver stream:
- * dmk_mech_profile(h, 1) enables (and resets), 0 disables.  Groups: 0 stress
+ * dmk_mech_profile(h, level) with level 1 times group 0 only, 2 every group
+ * (both reset the totals), 0 disables.  Groups: 0 stress
  * integration, 1 orientation + tangent, 2 response reductions (status, P_av),
  * 3 forward FFT, 4 Gamma convolution + div RMS, 5 inverse FFT, 6 stiffness
  * averages C_volAvg / C_minMaxAvg */
