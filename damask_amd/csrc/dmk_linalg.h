@@ -50,8 +50,26 @@ struct AimArgs {
 };
 enum { REP_PAV = 0, REP_FAIM = 9, REP_ERRBC = 18, REP_FLAG = 19, REP_DIV2 = 20, REP_CURL2 = 21, REP_DOTS = 24, REP_SIZE = 24 + DMK_MAX_DOTS };
 
+// sums of partial results: out[j][0] = scale[j] * sum of p[j][0..n[j])
+#define DMK_MAX_SUMS (DMK_MAX_DOTS + 2)
+struct SumJobs {
+  const double* p[DMK_MAX_SUMS];
+  double* out[DMK_MAX_SUMS];
+  double scale[DMK_MAX_SUMS];
+  int n[DMK_MAX_SUMS];
+  int k;
+};
+
 int dmk_nchunk(long n);
 int dmk_dots(const DotJobs* jobs, long n, double* partial, double* out, hipStream_t s);
+// flag, iteration counters, raw P_av (kept when a cell failed) and, with a,
+// the stress-BC bookkeeping of dmk_residual_aim, in two launches; partial
+// needs 12 * dmk_nchunk(N) doubles
+int dmk_response_reduce(const int32_t* st, const int32_t* nn, const int32_t* nf, const double* P, long N,
+                        double wgt, double* partial, int* flag, unsigned long long* cnt, double* pav,
+                        const AimArgs* a, double* faim, double* report, double* aim, hipStream_t s);
+// the dot products of jobs (out[j]) and the sums of extra, with one finalize launch
+int dmk_dots_sums(const DotJobs* jobs, long n, double* partial, double* out, const SumJobs* extra, hipStream_t s);
 int dmk_residual_aim(const AimArgs* a, const double* pav_raw, double* faim, const int* flag, double* report,
                      double* aim, hipStream_t s);
 // P_new <- P_old where the response failed (flag[0] != 0); nothing otherwise

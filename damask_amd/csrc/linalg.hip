@@ -89,14 +89,13 @@ __global__ void accept_P_kernel(const int* __restrict__ flag, const double* __re
 }
 
 // one thread: the order of operations is that of formResidual (:478-496)
-__global__ void residual_aim_kernel(AimArgs a, const double* __restrict__ pav_raw, double* __restrict__ faim,
-                                    const int* __restrict__ flag, double* __restrict__ rep, double* __restrict__ aim) {
-  if (threadIdx.x != 0) return;
+__device__ void aim_update(const AimArgs& a, const double* __restrict__ pav_raw, double* __restrict__ faim,
+                           int flagv, double* __restrict__ rep, double* __restrict__ aim) {
   double P[9];
   for (int k = 0; k < 9; k++) P[k] = pav_raw[k];
   if (a.rotate) dmk_rotate33(a.q_R, P, false, P);
   for (int k = 0; k < 9; k++) rep[REP_PAV + k] = P[k];
-  rep[REP_FLAG] = (double)flag[0];
+  rep[REP_FLAG] = (double)flagv;
   if (!a.update) return;
   double dP[9], dFa[9];
   for (int k = 0; k < 9; k++) dP[k] = P[k] - a.P_aim[k];
@@ -116,6 +115,113 @@ __global__ void residual_aim_kernel(AimArgs a, const double* __restrict__ pav_ra
   }
   rep[REP_ERRBC] = e;
   dmk_rotate33(a.q_R, dFa, true, aim);
+}
+
+__global__ void residual_aim_kernel(AimArgs a, const double* __restrict__ pav_raw, double* __restrict__ faim,
+                                    const int* __restrict__ flag, double* __restrict__ rep, double* __restrict__ aim) {
+  if (threadIdx.x != 0) return;
+  aim_update(a, pav_raw, faim, flag[0], rep, aim);
+}
+
+// The reductions of one constitutive response in two launches.  Partial
+// (blockIdx.y 0: status, 1-9: the columns of P): per chunk of cells the
+// any-failed flag, the Newton and state iteration counts and the nine column
+// sums of the new P (each as status_partial /
+// colsum_partial compute them, so the sums are bitwise those).  Finalize (one
+// block): the convergence flag and counters; the raw volume average of P when
+// every cell converged (otherwise P stays the previous response's, and so does
+// its average); then, when asked, the stress-BC bookkeeping of aim_update.
+__global__ void __launch_bounds__(RB) response_partial(const int32_t* __restrict__ st, const int32_t* __restrict__ nn,
+                                                       const int32_t* __restrict__ nf, const double* __restrict__ P,
+                                                       long N, double* __restrict__ part, int nchunk) {
+  __shared__ double sb[RB], s1[RB], s2[RB];
+  long s = (long)blockIdx.x * PER_BLOCK, e = min(s + PER_BLOCK, N);
+  if (blockIdx.y > 0) {
+    // column c = blockIdx.y - 1 of P
+    const int c = blockIdx.y - 1;
+    const double* x = P + (long)c * N;
+    double acc = 0.0;
+    for (long i = s + threadIdx.x; i < e; i += RB) acc += x[i];
+    double r = block_sum(acc);
+    if (threadIdx.x == 0) part[(long)(3 + c) * nchunk + blockIdx.x] = r;
+    return;
+  }
+  double bad = 0.0, a = 0.0, b = 0.0;
+  for (long i = s + threadIdx.x; i < e; i += RB) {
+    if (st[i] != ST_OK) bad = 1.0;
+    a += (double)nn[i];
+    b += (double)nf[i];
+  }
+  sb[threadIdx.x] = bad; s1[threadIdx.x] = a; s2[threadIdx.x] = b;
+  __syncthreads();
+  for (int k = RB / 2; k > 0; k >>= 1) {
+    if (threadIdx.x < k) {
+      sb[threadIdx.x] = fmax(sb[threadIdx.x], sb[threadIdx.x + k]);
+      s1[threadIdx.x] += s1[threadIdx.x + k];
+      s2[threadIdx.x] += s2[threadIdx.x + k];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    part[blockIdx.x] = sb[0];
+    part[nchunk + blockIdx.x] = s1[0];
+    part[2 * nchunk + blockIdx.x] = s2[0];
+  }
+}
+
+__global__ void __launch_bounds__(RB) response_finalize(const double* __restrict__ part, int nchunk, double wgt,
+                                                        int* __restrict__ flag, unsigned long long* __restrict__ cnt,
+                                                        double* __restrict__ pav, AimArgs a, int do_aim,
+                                                        double* __restrict__ faim, double* __restrict__ rep,
+                                                        double* __restrict__ aim) {
+  __shared__ double sb[RB], s1[RB], s2[RB];
+  __shared__ int ok;
+  double bad = 0.0, x = 0.0, y = 0.0;
+  for (int i = threadIdx.x; i < nchunk; i += RB) {
+    bad = fmax(bad, part[i]);
+    x += part[nchunk + i];
+    y += part[2 * nchunk + i];
+  }
+  sb[threadIdx.x] = bad; s1[threadIdx.x] = x; s2[threadIdx.x] = y;
+  __syncthreads();
+  for (int k = RB / 2; k > 0; k >>= 1) {
+    if (threadIdx.x < k) {
+      sb[threadIdx.x] = fmax(sb[threadIdx.x], sb[threadIdx.x + k]);
+      s1[threadIdx.x] += s1[threadIdx.x + k];
+      s2[threadIdx.x] += s2[threadIdx.x + k];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    int f = sb[0] > 0.0 ? ST_FAIL_MECH : ST_OK;
+    flag[0] = f;
+    ok = f == ST_OK;
+    cnt[0] += (unsigned long long)s1[0];
+    cnt[1] += (unsigned long long)s2[0];
+  }
+  __syncthreads();
+  double sums[9];
+  for (int c = 0; c < 9; c++) {
+    const double* p = part + (long)(3 + c) * nchunk;
+    double acc = 0.0;
+    for (int i = threadIdx.x; i < nchunk; i += RB) acc += p[i];
+    sums[c] = block_sum(acc);
+  }
+  if (threadIdx.x == 0) {
+    if (ok)
+      for (int c = 0; c < 9; c++) pav[c] = sums[c] * wgt;
+    if (do_aim) aim_update(a, pav, faim, ok ? ST_OK : ST_FAIL_MECH, rep, aim);
+  }
+}
+
+// several sums of partial results in one launch (blockIdx.x = job)
+__global__ void __launch_bounds__(RB) finalize_jobs(SumJobs J) {
+  const int j = blockIdx.x;
+  const double* p = J.p[j];
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < J.n[j]; i += RB) acc += p[i];
+  double r = block_sum(acc);
+  if (threadIdx.x == 0) J.out[j][0] = r * J.scale[j];
 }
 
 __global__ void axpby_kernel(long n, double a, const double* __restrict__ x, double b, const double* __restrict__ y,
@@ -356,6 +462,37 @@ int dmk_dots(const DotJobs* jobs, long n, double* partial, double* out, hipStrea
   int nc = dmk_nchunk(n);
   hipLaunchKernelGGL(dot_jobs_partial, dim3(nc, jobs->n), dim3(RB), 0, s, *jobs, n, partial, nc);
   hipLaunchKernelGGL(finalize_sum, dim3(jobs->n), dim3(RB), 0, s, partial, nc, 1.0, out);
+  return hipGetLastError() == hipSuccess ? 0 : 2;
+}
+
+int dmk_response_reduce(const int32_t* st, const int32_t* nn, const int32_t* nf, const double* P, long N,
+                        double wgt, double* partial, int* flag, unsigned long long* cnt, double* pav,
+                        const AimArgs* a, double* faim, double* report, double* aim, hipStream_t s) {
+  int nc = dmk_nchunk(N);
+  hipLaunchKernelGGL(response_partial, dim3(nc, 10), dim3(RB), 0, s, st, nn, nf, P, N, partial, nc);
+  AimArgs none{};
+  hipLaunchKernelGGL(response_finalize, dim3(1), dim3(RB), 0, s, partial, nc, wgt, flag, cnt, pav, a ? *a : none,
+                     a ? 1 : 0, faim, report, aim);
+  return hipGetLastError() == hipSuccess ? 0 : 2;
+}
+
+int dmk_dots_sums(const DotJobs* jobs, long n, double* partial, double* out, const SumJobs* extra, hipStream_t s) {
+  SumJobs J = extra ? *extra : SumJobs{};
+  if (!extra) J.k = 0;
+  int nc = dmk_nchunk(n);
+  if (jobs && jobs->n > 0) {
+    hipLaunchKernelGGL(dot_jobs_partial, dim3(nc, jobs->n), dim3(RB), 0, s, *jobs, n, partial, nc);
+    for (int j = 0; j < jobs->n; j++) {
+      if (J.k >= DMK_MAX_SUMS) return 1;
+      J.p[J.k] = partial + (long)j * nc;
+      J.n[J.k] = nc;
+      J.scale[J.k] = 1.0;
+      J.out[J.k] = out + j;
+      J.k++;
+    }
+  }
+  if (J.k == 0) return 0;
+  hipLaunchKernelGGL(finalize_jobs, dim3(J.k), dim3(RB), 0, s, J);
   return hipGetLastError() == hipSuccess ? 0 : 2;
 }
 

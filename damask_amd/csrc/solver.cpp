@@ -195,6 +195,8 @@ struct dmk_handle {
   double2 *Z, *U;
   double *u3, *xn_dev, *xp_dev;
   double *partial, *small, *nrm, *pmax, *pmin, *ext;
+  double* divp = nullptr;            // per-block divergence sums of the basic residual
+  bool div_pending = false;          // divp holds sums the next residual_finish adds up
   long *sel, *csel;
   int* flag;
   unsigned long long* cnt;
@@ -320,20 +322,23 @@ static int tangent_pass(dmk_handle* h, double dt) {
 
 // utilities_constitutiveResponse on the device field F (component-major),
 // enqueued without synchronisation: stress integration of every cell,
-// convergence flag, accepted P and its raw volume average.  The stress kernel
-// writes into the spare P buffer; when a cell failed, the previous P is copied
-// back into it (the reference's skipped second loop leaves P as it was,
-// homogenization.f90:349), and the buffers swap roles.
-static int response_enqueue(dmk_handle* h, const double* F, double dt) {
+// convergence flag, accepted P and its raw volume average, and with a the
+// report and stress-BC bookkeeping of dmk_residual_aim.  The stress kernel
+// writes into the spare P buffer; when a cell failed, the previous P (and its
+// average) stays: it is copied back into that buffer (the reference's skipped
+// second loop leaves P as it was, homogenization.f90:349), and the buffers
+// swap roles.
+static int response_enqueue(dmk_handle* h, const double* F, double dt, const AimArgs* a) {
   kt_begin(h, KT_STRESS);
   for (int p = 0; p < h->n_phases; p++)
     CKL(dmk_launch_stress(&h->mp, F, h->ph_dev + p, &h->ph[p], p, &h->nm, dt, h->st));
   kt_end(h, KT_STRESS);
   kt_begin(h, KT_REDUCE);
-  CKL(dmk_status(h->mp.status, h->mp.n_newton, h->mp.n_fpi, h->N, h->partial, h->flag, h->cnt, h->st));
+  CKL(dmk_response_reduce(h->mp.status, h->mp.n_newton, h->mp.n_fpi, h->mp.Pph, h->N, h->wgt, h->partial, h->flag,
+                          h->cnt, h->small + SM_PAV, a, h->small + SM_FAIM, h->small + SM_REP, h->small + SM_AIM,
+                          h->st));
   CKL(dmk_accept_P(h->flag, h->P_hom, h->mp.Pph, h->n9, h->st));
   std::swap(h->P_hom, h->mp.Pph);
-  CKL(dmk_colsum(h->P_hom, h->N, h->N, 9, h->partial, h->small + SM_PAV, h->wgt, h->st));
   kt_end(h, KT_REDUCE);
   h->n_updates += h->N;
   return 0;
@@ -350,12 +355,10 @@ static void aim_args(const dmk_handle* h, bool rotate, bool update, AimArgs* a) 
 
 // constitutive response with the report read back (init, dmk_constitutive*)
 static int constitutive(dmk_handle* h, const double* F, double dt, bool rotate, bool tangent) {
-  int rc = response_enqueue(h, F, dt);
-  if (rc) return rc;
   AimArgs a;
   aim_args(h, rotate, false, &a);
-  CKL(dmk_residual_aim(&a, h->small + SM_PAV, h->small + SM_FAIM, h->flag, h->small + SM_REP,
-                       h->small + SM_AIM, h->st));
+  int rc = response_enqueue(h, F, dt, &a);
+  if (rc) return rc;
   CK(hipMemcpyAsync(h->hsmall + SM_REP, h->small + SM_REP, sizeof(double) * REP_DIV2, hipMemcpyDeviceToHost,
                     h->st));
   CK(hipStreamSynchronize(h->st));
@@ -416,12 +419,10 @@ static int residual_enqueue(dmk_handle* h, const double* X, double* r, int iter)
   h->n_residuals++;
   if (h->scheme == 1) return residual_enqueue_pol(h, X, r);
   if (h->scheme == 2) return residual_enqueue_gk(h, X, r);
-  int rc = response_enqueue(h, X, h->dt);
-  if (rc) return rc;
   AimArgs a;
   aim_args(h, true, true, &a);
-  CKL(dmk_residual_aim(&a, h->small + SM_PAV, h->small + SM_FAIM, h->flag, h->small + SM_REP,
-                       h->small + SM_AIM, h->st));
+  int rc = response_enqueue(h, X, h->dt, &a);
+  if (rc) return rc;
   kt_begin(h, KT_FFT_FWD);
   CKL(dmk_fft_r2c(&h->fft, 9, h->P_hom, h->Z, h->st));
   kt_end(h, KT_FFT_FWD);
@@ -429,10 +430,9 @@ static int residual_enqueue(dmk_handle* h, const double* X, double* r, int iter)
   h->ga.do_gamma = 1;
   h->ga.aim_dev = h->small + SM_AIM;
   kt_begin(h, KT_GAMMA);
-  CKL(dmk_gamma_div(h->Z, &h->ga, h->partial, h->st));
+  CKL(dmk_gamma_div(h->Z, &h->ga, h->divp, h->st));
   kt_end(h, KT_GAMMA);
-  int nb = dmk_gamma_div_blocks(&h->ga);
-  CKL(dmk_colsum(h->partial, nb, nb, 1, h->partial + nb, h->small + SM_REP + REP_DIV2, 1.0, h->st));
+  h->div_pending = true;            // summed with the dot products in residual_finish
   kt_begin(h, KT_FFT_INV);
   CKL(dmk_fft_c2r(&h->fft, 9, h->Z, r, h->st));
   kt_end(h, KT_FFT_INV);
@@ -470,7 +470,7 @@ static int residual_enqueue_pol(dmk_handle* h, const double* X, double* r) {
   CKL(dmk_fft_c2r(&h->fft, 9, h->Z, h->T, h->st));
   kt_end(h, KT_FFT_INV);
   CKL(dmk_pol_stage2(N, h->pol.beta, F, h->T, rt, rF, h->st));
-  int rc = response_enqueue(h, rF, h->dt);
+  int rc = response_enqueue(h, rF, h->dt, nullptr);
   if (rc) return rc;
   int nb = dmk_gamma_div_blocks(&h->ga);
   CKL(dmk_fft_r2c(&h->fft, 9, F, h->Z, h->st));
@@ -500,7 +500,7 @@ static int residual_enqueue_pol(dmk_handle* h, const double* X, double* r) {
 // stress-controlled components and zero on the others, inverse FFT into r.
 // F_aim is not touched here: it changes once per Newton step (set_F_aim).
 static int residual_enqueue_gk(dmk_handle* h, const double* X, double* r) {
-  int rc = response_enqueue(h, X, h->dt);
+  int rc = response_enqueue(h, X, h->dt, nullptr);
   if (rc) return rc;
   kt_begin(h, KT_TANGENT);
   for (int p = 0; p < h->n_phases; p++)
@@ -534,7 +534,17 @@ static int residual_enqueue_gk(dmk_handle* h, const double* X, double* r) {
 static int residual_finish(dmk_handle* h, const DotJobs* jobs, double* dots_out) {
   int nd = jobs ? jobs->n : 0;
   if (nd > DMK_MAX_DOTS) { dmk_set_error("too many dot products"); return DMK_ERR_ARG; }
-  if (nd) CKL(dmk_dots(jobs, h->nx, h->partial, h->small + SM_REP + REP_DOTS, h->st));
+  SumJobs extra;
+  extra.k = 0;
+  if (h->div_pending) {
+    extra.p[0] = h->divp;
+    extra.n[0] = dmk_gamma_div_blocks(&h->ga);
+    extra.scale[0] = 1.0;
+    extra.out[0] = h->small + SM_REP + REP_DIV2;
+    extra.k = 1;
+    h->div_pending = false;
+  }
+  if (nd || extra.k) CKL(dmk_dots_sums(jobs, h->nx, h->partial, h->small + SM_REP + REP_DOTS, &extra, h->st));
   CK(hipMemcpyAsync(h->hsmall + SM_REP, h->small + SM_REP, sizeof(double) * (REP_DOTS + nd), hipMemcpyDeviceToHost,
                     h->st));
   CK(hipStreamSynchronize(h->st));
@@ -1483,6 +1493,7 @@ static int init_impl(dmk_handle* h, const int32_t cells[3], const double size[3]
   long np = 82L * dmk_nchunk(std::max(nx, 9 * h->Nk)) + 4096;
   if ((rc = dalloc(h, &h->partial, np))) return rc;
   if ((rc = dalloc(h, &h->small, SM_SIZE))) return rc;
+  if ((rc = dalloc(h, &h->divp, (size_t)(h->Nk + 255) / 256))) return rc;
   if ((rc = dalloc(h, &h->nrm, N))) return rc;
   if ((rc = dalloc(h, &h->pmax, dmk_nchunk(N)))) return rc;
   if ((rc = dalloc(h, &h->pmin, dmk_nchunk(N)))) return rc;
