@@ -11,13 +11,16 @@ struct FftPlan {
 };
 
 // tile shapes: gx[0] = line pairs per block of the x pass, gx[1], gx[2] =
-// lines per block of the y and z passes (picked by dmk_fft_tune)
+// lines per block of the y and z passes; xy_threads > 0: the x and y passes
+// run fused, one block of that many threads per (field, z) plane (all picked
+// by dmk_fft_tune)
 struct DmkFFT {
   int n[3];
   int h;
   FftPlan plan[3];
   double2* tw[3] = {nullptr, nullptr, nullptr};
   int gx[3] = {0, 0, 0};
+  int xy_threads = 0;
 };
 
 struct GammaArgs {

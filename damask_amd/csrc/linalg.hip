@@ -127,8 +127,8 @@ __global__ void residual_aim_kernel(AimArgs a, const double* __restrict__ pav_ra
 // (blockIdx.y 0: status, 1-9: the columns of P): per chunk of cells the
 // any-failed flag, the Newton and state iteration counts and the nine column
 // sums of the new P (each as status_partial /
-// colsum_partial compute them, so the sums are bitwise those).  Finalize (one
-// block): the convergence flag and counters; the raw volume average of P when
+// colsum_partial compute them).  Finalize (one block, a wavefront per reduced
+// quantity): the convergence flag and counters; the raw volume average of P when
 // every cell converged (otherwise P stays the previous response's, and so does
 // its average); then, when asked, the stress-BC bookkeeping of aim_update.
 __global__ void __launch_bounds__(RB) response_partial(const int32_t* __restrict__ st, const int32_t* __restrict__ nn,
@@ -169,47 +169,45 @@ __global__ void __launch_bounds__(RB) response_partial(const int32_t* __restrict
   }
 }
 
-__global__ void __launch_bounds__(RB) response_finalize(const double* __restrict__ part, int nchunk, double wgt,
-                                                        int* __restrict__ flag, unsigned long long* __restrict__ cnt,
-                                                        double* __restrict__ pav, AimArgs a, int do_aim,
-                                                        double* __restrict__ faim, double* __restrict__ rep,
-                                                        double* __restrict__ aim) {
-  __shared__ double sb[RB], s1[RB], s2[RB];
-  __shared__ int ok;
-  double bad = 0.0, x = 0.0, y = 0.0;
-  for (int i = threadIdx.x; i < nchunk; i += RB) {
-    bad = fmax(bad, part[i]);
-    x += part[nchunk + i];
-    y += part[2 * nchunk + i];
+// sum over the 64 lanes of a wavefront (fixed butterfly order)
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+  return v;
+}
+
+// 12 wavefronts, one per reduced quantity: 0 the failure flag, 1-2 the
+// iteration counts, 3-11 the columns of P; then one lane does the rest
+__global__ void __launch_bounds__(768) response_finalize(const double* __restrict__ part, int nchunk, double wgt,
+                                                         int* __restrict__ flag, unsigned long long* __restrict__ cnt,
+                                                         double* __restrict__ pav, AimArgs a, int do_aim,
+                                                         double* __restrict__ faim, double* __restrict__ rep,
+                                                         double* __restrict__ aim) {
+  __shared__ double res[12];
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const double* p = part + (long)w * nchunk;
+  double v = 0.0;
+  if (w == 0) {
+    for (int i = lane; i < nchunk; i += 64) v = fmax(v, p[i]);
+    v = wave_max(v);
+  } else {
+    for (int i = lane; i < nchunk; i += 64) v += p[i];
+    v = wave_sum(v);
   }
-  sb[threadIdx.x] = bad; s1[threadIdx.x] = x; s2[threadIdx.x] = y;
+  if (lane == 0) res[w] = v;
   __syncthreads();
-  for (int k = RB / 2; k > 0; k >>= 1) {
-    if (threadIdx.x < k) {
-      sb[threadIdx.x] = fmax(sb[threadIdx.x], sb[threadIdx.x + k]);
-      s1[threadIdx.x] += s1[threadIdx.x + k];
-      s2[threadIdx.x] += s2[threadIdx.x + k];
-    }
-    __syncthreads();
-  }
   if (threadIdx.x == 0) {
-    int f = sb[0] > 0.0 ? ST_FAIL_MECH : ST_OK;
-    flag[0] = f;
-    ok = f == ST_OK;
-    cnt[0] += (unsigned long long)s1[0];
-    cnt[1] += (unsigned long long)s2[0];
-  }
-  __syncthreads();
-  double sums[9];
-  for (int c = 0; c < 9; c++) {
-    const double* p = part + (long)(3 + c) * nchunk;
-    double acc = 0.0;
-    for (int i = threadIdx.x; i < nchunk; i += RB) acc += p[i];
-    sums[c] = block_sum(acc);
-  }
-  if (threadIdx.x == 0) {
+    const bool ok = !(res[0] > 0.0);
+    flag[0] = ok ? ST_OK : ST_FAIL_MECH;
+    cnt[0] += (unsigned long long)res[1];
+    cnt[1] += (unsigned long long)res[2];
     if (ok)
-      for (int c = 0; c < 9; c++) pav[c] = sums[c] * wgt;
+      for (int c = 0; c < 9; c++) pav[c] = res[3 + c] * wgt;
     if (do_aim) aim_update(a, pav, faim, ok ? ST_OK : ST_FAIL_MECH, rep, aim);
   }
 }
@@ -471,7 +469,7 @@ int dmk_response_reduce(const int32_t* st, const int32_t* nn, const int32_t* nf,
   int nc = dmk_nchunk(N);
   hipLaunchKernelGGL(response_partial, dim3(nc, 10), dim3(RB), 0, s, st, nn, nf, P, N, partial, nc);
   AimArgs none{};
-  hipLaunchKernelGGL(response_finalize, dim3(1), dim3(RB), 0, s, partial, nc, wgt, flag, cnt, pav, a ? *a : none,
+  hipLaunchKernelGGL(response_finalize, dim3(1), dim3(768), 0, s, partial, nc, wgt, flag, cnt, pav, a ? *a : none,
                      a ? 1 : 0, faim, report, aim);
   return hipGetLastError() == hipSuccess ? 0 : 2;
 }

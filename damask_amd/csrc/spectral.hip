@@ -1,7 +1,7 @@
 // Spectral operators of the basic scheme on MI355X (src/grid/spectral_utilities.f90).
 //
 // * Batched 3-D r2c / c2r FFTs of component-major fields, built from 1-D
-//   mixed-radix Stockham passes staged in LDS:
+//   mixed-radix (8, 4, 2, 3, 5, 7, generic) Stockham passes staged in LDS:
 //     x pass: two real lines packed into one complex line (r2c/c2r split),
 //     y and z passes: c2c on groups of lines adjacent in x so that every
 //     global load and store is coalesced across the group.
@@ -48,6 +48,25 @@ __device__ __forceinline__ void butterfly(double2* v, const double2* __restrict_
     v[2] = csub(s0, s2);
     v[1] = cadd(s1, r3);
     v[3] = csub(s1, r3);
+  } else if constexpr (R == 8) {
+    // radix 2 x 4: a_j = x_j + x_{j+4} gives the even outputs, (x_j - x_{j+4}) w8^j the odd ones
+    const double c = 0.70710678118654752440;
+    double2 a[4], b[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) { a[j] = cadd(v[j], v[j + 4]); b[j] = csub(v[j], v[j + 4]); }
+    if (sign < 0) {
+      b[1] = make_double2((b[1].x + b[1].y) * c, (b[1].y - b[1].x) * c);
+      b[2] = make_double2(b[2].y, -b[2].x);
+      b[3] = make_double2((b[3].y - b[3].x) * c, -(b[3].x + b[3].y) * c);
+    } else {
+      b[1] = make_double2((b[1].x - b[1].y) * c, (b[1].y + b[1].x) * c);
+      b[2] = make_double2(-b[2].y, b[2].x);
+      b[3] = make_double2(-(b[3].x + b[3].y) * c, (b[3].x - b[3].y) * c);
+    }
+    butterfly<4>(a, tw, L, sign);
+    butterfly<4>(b, tw, L, sign);
+#pragma unroll
+    for (int j = 0; j < 4; j++) { v[2 * j] = a[j]; v[2 * j + 1] = b[j]; }
   } else {
     double2 w[R];
     const int st = L / R;
@@ -130,6 +149,7 @@ __device__ double2* lds_fft(double2* b0, double2* b1, int nlines, int L, int ldl
       case 4: stockham_stage<4>(in, out, nlines, L, ldl, Ns, tw, sign); break;
       case 5: stockham_stage<5>(in, out, nlines, L, ldl, Ns, tw, sign); break;
       case 7: stockham_stage<7>(in, out, nlines, L, ldl, Ns, tw, sign); break;
+      case 8: stockham_stage<8>(in, out, nlines, L, ldl, Ns, tw, sign); break;
       default: stockham_stage_generic(in, out, nlines, L, ldl, Ns, R, tw, sign); break;
     }
     __syncthreads();
@@ -217,6 +237,104 @@ __global__ void __launch_bounds__(256) fft_x_c2r(const double2* __restrict__ in,
     if (line >= nlines_here) continue;
     double2 z = r[(line >> 1) * ldl + pos];
     out[(l0 + line) * L + pos] = (line & 1) ? z.y : z.x;
+  }
+}
+
+// Fused x and y passes, one block per (field, z) plane: the real plane and its
+// half-spectrum plane are each contiguous in memory, so every global access
+// is coalesced and the plane stays in LDS between the passes.  The arithmetic
+// is that of fft_x_r2c / fft_x_c2r followed or preceded by fft_strided along
+// y (same line packing, same Stockham stages), so for even N2 results are
+// bitwise those of the separate passes (for odd N2 the x pass pairs the last
+// line of a plane with zeros instead of the next plane's first line: equal to
+// rounding).  LDS: two buffers of max(ceil(N2/2) (N1+1), h (N2+1)) complex
+// values.
+__device__ __forceinline__ int xy_buf(int N1, int N2) {
+  const int h = N1 / 2 + 1, np = (N2 + 1) / 2;
+  int a = np * (N1 + 1), b = h * (N2 + 1);
+  return a > b ? a : b;
+}
+
+__global__ void fft_xy_r2c(const double* __restrict__ in, double2* __restrict__ out, int N1, int N2, int N3,
+                           FftPlan px, const double2* __restrict__ twx, FftPlan py,
+                           const double2* __restrict__ twy) {
+  extern __shared__ double2 lds[];
+  const int h = N1 / 2 + 1, np = (N2 + 1) / 2, lx = N1 + 1, ly = N2 + 1;
+  double2* A = lds;
+  double2* B = lds + xy_buf(N1, N2);
+  const long plane = blockIdx.x;                 // field * N3 + z
+  const double* src = in + plane * N1 * N2;
+  for (int i = threadIdx.x; i < 2 * np * N1; i += blockDim.x) {
+    int line = i / N1, pos = i - line * N1;
+    double v = line < N2 ? src[(long)line * N1 + pos] : 0.0;
+    double2* c = &A[(line >> 1) * lx + pos];
+    if (line & 1) c->y = v; else c->x = v;
+  }
+  __syncthreads();
+  double2* r = lds_fft(A, B, np, N1, lx, px, twx, -1);
+  double2* t = r == A ? B : A;
+  for (int i = threadIdx.x; i < N2 * h; i += blockDim.x) {
+    int line = i / h, k = i - line * h;
+    const double2* z = r + (line >> 1) * lx;
+    double2 a = z[k], b = cconj(z[(N1 - k) % N1]);
+    double2 v;
+    if (line & 1) {
+      double2 d = csub(a, b);
+      v = make_double2(0.5 * d.y, -0.5 * d.x);
+    } else {
+      double2 s = cadd(a, b);
+      v = make_double2(0.5 * s.x, 0.5 * s.y);
+    }
+    t[k * ly + line] = v;
+  }
+  __syncthreads();
+  double2* q = lds_fft(t, r, h, N2, ly, py, twy, -1);
+  double2* dst = out + (blockIdx.x / N3) * ((long)h * N2 * N3) + (long)(blockIdx.x % N3) * h * N2;
+  for (int i = threadIdx.x; i < N2 * h; i += blockDim.x) {
+    int ky = i / h, kx = i - ky * h;
+    dst[i] = q[kx * ly + ky];
+  }
+}
+
+__global__ void fft_yx_c2r(const double2* __restrict__ in, double* __restrict__ out, int N1, int N2, int N3,
+                           FftPlan px, const double2* __restrict__ twx, FftPlan py,
+                           const double2* __restrict__ twy) {
+  extern __shared__ double2 lds[];
+  const int h = N1 / 2 + 1, np = (N2 + 1) / 2, lx = N1 + 1, ly = N2 + 1;
+  double2* A = lds;
+  double2* B = lds + xy_buf(N1, N2);
+  const double2* src = in + (blockIdx.x / N3) * ((long)h * N2 * N3) + (long)(blockIdx.x % N3) * h * N2;
+  for (int i = threadIdx.x; i < N2 * h; i += blockDim.x) {
+    int ky = i / h, kx = i - ky * h;
+    A[kx * ly + ky] = src[i];
+  }
+  __syncthreads();
+  double2* r = lds_fft(A, B, h, N2, ly, py, twy, +1);
+  double2* t = r == A ? B : A;
+  // pairs of y lines into one complex line, Hermitian-extended in x; the
+  // imaginary parts of the DC and Nyquist bins do not enter
+  for (int i = threadIdx.x; i < np * N1; i += blockDim.x) {
+    int p = i / N1, k = i - p * N1;
+    int kk = (k < h) ? k : N1 - k;
+    bool cj = k >= h;
+    double2 a = r[kk * ly + 2 * p];
+    if (kk == 0 || 2 * kk == N1) a.y = 0.0;
+    if (cj) a.y = -a.y;
+    double2 b = make_double2(0.0, 0.0);
+    if (2 * p + 1 < N2) {
+      b = r[kk * ly + 2 * p + 1];
+      if (kk == 0 || 2 * kk == N1) b.y = 0.0;
+      if (cj) b.y = -b.y;
+    }
+    t[p * lx + k] = make_double2(a.x - b.y, a.y + b.x);
+  }
+  __syncthreads();
+  double2* q = lds_fft(t, r, np, N1, lx, px, twx, +1);
+  double* dst = out + (long)blockIdx.x * N1 * N2;
+  for (int i = threadIdx.x; i < N1 * N2; i += blockDim.x) {
+    int line = i / N1, pos = i - line * N1;
+    double2 z = q[(line >> 1) * lx + pos];
+    dst[i] = (line & 1) ? z.y : z.x;
   }
 }
 
@@ -577,6 +695,7 @@ static FftPlan make_plan(int L) {
   FftPlan p{};
   p.L = L;
   int n = L;
+  while (n % 8 == 0 && n > 1) { p.radix[p.nstages++] = 8; n /= 8; }
   while (n % 4 == 0 && n > 1) { p.radix[p.nstages++] = 4; n /= 4; }
   while (n % 2 == 0) { p.radix[p.nstages++] = 2; n /= 2; }
   for (int f = 3; n > 1; f += 2)
@@ -619,6 +738,24 @@ void dmk_fft_free(DmkFFT* fft) {
 }
 
 static const size_t FFT_LDS_MAX = 64 * 1024;
+// the fused x-y kernels may take more (the MI355X allows 160 KB per block)
+static const size_t FFT_XY_LDS_MAX = 160 * 1024;
+
+static size_t xy_lds(const DmkFFT* fft) {
+  const int N1 = fft->n[0], N2 = fft->n[1], h = fft->h, np = (N2 + 1) / 2;
+  size_t a = (size_t)np * (N1 + 1), b = (size_t)h * (N2 + 1);
+  return 2 * sizeof(double2) * (a > b ? a : b);
+}
+
+static void launch_xy_r2c(const DmkFFT* fft, int howmany, const double* in, double2* out, int threads, hipStream_t s) {
+  hipLaunchKernelGGL(fft_xy_r2c, dim3(howmany * fft->n[2]), dim3(threads), xy_lds(fft), s, in, out, fft->n[0],
+                     fft->n[1], fft->n[2], fft->plan[0], fft->tw[0], fft->plan[1], fft->tw[1]);
+}
+
+static void launch_yx_c2r(const DmkFFT* fft, int howmany, const double2* in, double* out, int threads, hipStream_t s) {
+  hipLaunchKernelGGL(fft_yx_c2r, dim3(howmany * fft->n[2]), dim3(threads), xy_lds(fft), s, in, out, fft->n[0],
+                     fft->n[1], fft->n[2], fft->plan[0], fft->tw[0], fft->plan[1], fft->tw[1]);
+}
 
 static void launch_x_r2c(const DmkFFT* fft, int howmany, const double* in, double2* out, int ppb, hipStream_t s) {
   const int N1 = fft->n[0];
@@ -679,6 +816,7 @@ int dmk_fft_tune(DmkFFT* fft, double2* cscratch, double* rscratch, hipStream_t s
     }
   }
   // strided passes: even chunkings of the h lines
+  float t_y = 0.0f;
   for (int axis = 1; axis <= 2; axis++) {
     const int L = fft->n[axis];
     if (L <= 1) continue;
@@ -691,6 +829,19 @@ int dmk_fft_tune(DmkFFT* fft, double2* cscratch, double* rscratch, hipStream_t s
       float t = timed([&] { launch_strided(fft, howmany, cscratch, axis, gx, -1, s); });
       if (t < best) { best = t; fft->gx[axis] = gx; }
     }
+    if (axis == 1) t_y = best;
+  }
+  // fused x-y passes against the separate ones (forward direction);
+  // DMK_FFT_XY=0 / 1 in the environment forces the separate / fused passes
+  fft->xy_threads = 0;
+  const char* force = getenv("DMK_FFT_XY");
+  if (fft->n[0] >= 2 && fft->n[1] >= 2 && xy_lds(fft) <= FFT_XY_LDS_MAX && !(force && force[0] == '0')) {
+    float sep = timed([&] { launch_x_r2c(fft, howmany, rscratch, cscratch, fft->gx[0], s); }) + t_y;
+    if (force && force[0] == '1') sep = 1e30f;
+    for (int th : {256, 512, 1024}) {
+      float t = timed([&] { launch_xy_r2c(fft, howmany, rscratch, cscratch, th, s); });
+      if (t < sep) { sep = t; fft->xy_threads = th; }
+    }
   }
   hipEventDestroy(e0);
   hipEventDestroy(e1);
@@ -698,8 +849,12 @@ int dmk_fft_tune(DmkFFT* fft, double2* cscratch, double* rscratch, hipStream_t s
 }
 
 int dmk_fft_r2c(const DmkFFT* fft, int howmany, const double* in, double2* out, hipStream_t s) {
-  launch_x_r2c(fft, howmany, in, out, fft->gx[0], s);
-  if (fft->n[1] > 1) launch_strided(fft, howmany, out, 1, fft->gx[1], -1, s);
+  if (fft->xy_threads) {
+    launch_xy_r2c(fft, howmany, in, out, fft->xy_threads, s);
+  } else {
+    launch_x_r2c(fft, howmany, in, out, fft->gx[0], s);
+    if (fft->n[1] > 1) launch_strided(fft, howmany, out, 1, fft->gx[1], -1, s);
+  }
   if (fft->n[2] > 1) launch_strided(fft, howmany, out, 2, fft->gx[2], -1, s);
   return hipGetLastError() == hipSuccess ? 0 : 2;
 }
@@ -707,8 +862,12 @@ int dmk_fft_r2c(const DmkFFT* fft, int howmany, const double* in, double2* out, 
 // in is overwritten (the y and z passes run in place)
 int dmk_fft_c2r(const DmkFFT* fft, int howmany, double2* in, double* out, hipStream_t s) {
   if (fft->n[2] > 1) launch_strided(fft, howmany, in, 2, fft->gx[2], +1, s);
-  if (fft->n[1] > 1) launch_strided(fft, howmany, in, 1, fft->gx[1], +1, s);
-  launch_x_c2r(fft, howmany, in, out, fft->gx[0], s);
+  if (fft->xy_threads) {
+    launch_yx_c2r(fft, howmany, in, out, fft->xy_threads, s);
+  } else {
+    if (fft->n[1] > 1) launch_strided(fft, howmany, in, 1, fft->gx[1], +1, s);
+    launch_x_c2r(fft, howmany, in, out, fft->gx[0], s);
+  }
   return hipGetLastError() == hipSuccess ? 0 : 2;
 }
 

@@ -88,10 +88,13 @@ def test_constitutive_response(gpu_lib, oracle, lattice, eps):
         L.orc_mp_forward(o.mp)
 
 
+@pytest.mark.parametrize("xy", ["0", "1"])
 @pytest.mark.parametrize("cells", [[6, 7, 8], [8, 8, 8], [10, 6, 5], [7, 9, 3], [16, 12, 20]])
-def test_fft_roundtrip_and_parity(gpu_lib, oracle, cells):
+def test_fft_roundtrip_and_parity(gpu_lib, oracle, cells, xy, monkeypatch):
+    """separate passes (DMK_FFT_XY=0) and the fused x-y passes (1)"""
     import ctypes as C
     from damask_amd import _lib
+    monkeypatch.setenv("DMK_FFT_XY", xy)
     grid, mat = _poly(cells, 4, 1)
     g, o = _pair(grid, mat, oracle=oracle)
     N = grid.n_cells
@@ -113,9 +116,11 @@ def test_fft_roundtrip_and_parity(gpu_lib, oracle, cells):
         assert np.abs(y / N - x).max() < 5e-15 * max(1.0, np.log2(N))
 
 
+@pytest.mark.parametrize("xy", ["0", "1"])
 @pytest.mark.parametrize("cells", [[6, 7, 8], [8, 8, 8], [5, 6, 4]])
-def test_gamma_and_div(gpu_lib, oracle, cells):
+def test_gamma_and_div(gpu_lib, oracle, cells, xy, monkeypatch):
     from damask_amd import _lib
+    monkeypatch.setenv("DMK_FFT_XY", xy)
     grid, mat = _poly(cells, 5, 2)
     g, o = _pair(grid, mat, oracle=oracle)
     N = grid.n_cells
