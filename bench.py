@@ -28,7 +28,7 @@ PEAK_HBM_GBS = 8000.0
 # HBM bytes per stress-kernel launch on this workload: rocprofv3 --pmc
 # FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 (factors from a plain device copy), see
 # profiles/kernels_64cube_fcc_tension.md
-STRESS_TRAFFIC_BYTES = 2297932462.0
+STRESS_TRAFFIC_BYTES = 2246378705.9
 
 
 def parse():
