@@ -322,18 +322,30 @@ __global__ void adj_gs_kernel(long n, double* __restrict__ w, const double* __re
   w[i] = v;
 }
 
-int dmk_adj_local(const AdjKArgs* a, const DmkPhaseDev* ph, hipStream_t s) {
+int dmk_adj_factor(const AdjKArgs* a, const DmkPhaseDev* ph, hipStream_t s) {
   if (a->nsl < 1 || a->nsl > a->mmax || a->mmax > DMK_NSL_MAX) return 1;
   unsigned nb = (unsigned)((a->N + ADJ_WG - 1) / ADJ_WG);
-  if (a->mode == 0) {
-    hipLaunchKernelGGL(adj_assemble_kernel, dim3(nb), dim3(ADJ_WG), 0, s, *a, ph);
-    unsigned nc = (unsigned)((a->N + COOP_CELLS - 1) / COOP_CELLS);
-    const int n = 9 + a->nsl;
-    size_t lds = sizeof(double) * COOP_CELLS * n * coop_ls(n) + sizeof(int) * COOP_CELLS * n;
-    hipLaunchKernelGGL(adj_lu_kernel, dim3(nc), dim3(COOP_CELLS * COOP_LANES), lds, s, *a);
-  }
+  hipLaunchKernelGGL(adj_assemble_kernel, dim3(nb), dim3(ADJ_WG), 0, s, *a, ph);
+  unsigned nc = (unsigned)((a->N + COOP_CELLS - 1) / COOP_CELLS);
+  const int n = 9 + a->nsl;
+  size_t lds = sizeof(double) * COOP_CELLS * n * coop_ls(n) + sizeof(int) * COOP_CELLS * n;
+  hipLaunchKernelGGL(adj_lu_kernel, dim3(nc), dim3(COOP_CELLS * COOP_LANES), lds, s, *a);
+  return hipGetLastError() == hipSuccess ? 0 : 2;
+}
+
+int dmk_adj_vjp(const AdjKArgs* a, const DmkPhaseDev* ph, hipStream_t s) {
+  if (a->nsl < 1 || a->nsl > a->mmax || a->mmax > DMK_NSL_MAX) return 1;
+  unsigned nb = (unsigned)((a->N + ADJ_WG - 1) / ADJ_WG);
   hipLaunchKernelGGL(adj_vjp_kernel, dim3(nb), dim3(ADJ_WG), 0, s, *a, ph);
   return hipGetLastError() == hipSuccess ? 0 : 2;
+}
+
+int dmk_adj_local(const AdjKArgs* a, const DmkPhaseDev* ph, hipStream_t s) {
+  if (a->mode == 0) {
+    int rc = dmk_adj_factor(a, ph, s);
+    if (rc) return rc;
+  }
+  return dmk_adj_vjp(a, ph, s);
 }
 
 int dmk_adj_atmul(long N, const double* A, const double* mu, double* y, hipStream_t s) {

@@ -57,6 +57,10 @@ struct GsArgs {
 };
 
 int dmk_adj_local(const AdjKArgs* a, const DmkPhaseDev* ph, hipStream_t s);
+// the two halves of dmk_adj_local: K1 + K2 (read the tape only; write scr and
+// res) and K3 (mode 0 or 1)
+int dmk_adj_factor(const AdjKArgs* a, const DmkPhaseDev* ph, hipStream_t s);
+int dmk_adj_vjp(const AdjKArgs* a, const DmkPhaseDev* ph, hipStream_t s);
 int dmk_adj_atmul(long N, const double* A, const double* mu, double* y, hipStream_t s);
 // out[p * nslot + s] (+)= sum over the cells of phase p of v[s * N + e]
 int dmk_adj_phase_sum(const double* v, long N, const int32_t* phase, int n_phases, int nslot, int accumulate,

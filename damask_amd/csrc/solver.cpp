@@ -1005,10 +1005,14 @@ struct AdjState {
   int M = 30;                      // GMRES restart length
   GammaArgs ga;                    // Gamma_0 of the adjoint solves (reference medium sym <A>)
   bool work_ready = false;
-  double *A = nullptr, *g = nullptr, *res = nullptr, *scr = nullptr, *Fpb[2] = {nullptr, nullptr},
-         *xib[2] = {nullptr, nullptr}, *thb = nullptr, *mu = nullptr, *y = nullptr, *b = nullptr, *r = nullptr,
-         *V = nullptr, *red = nullptr;
+  double *A = nullptr, *g = nullptr, *res[2] = {nullptr, nullptr}, *scr[2] = {nullptr, nullptr},
+         *Fpb[2] = {nullptr, nullptr}, *xib[2] = {nullptr, nullptr}, *thb = nullptr, *mu = nullptr, *y = nullptr,
+         *b = nullptr, *r = nullptr, *V = nullptr, *red = nullptr;
   std::vector<void*> work;
+  // the factorisation of increment k - 1 (tape only) runs on st2 while the
+  // GMRES of increment k runs on the solver stream; scr / res alternate
+  hipStream_t st2 = nullptr;
+  hipEvent_t fac[2] = {nullptr, nullptr};
 };
 
 enum { TP_F = 0, TP_FP = 9, TP_LP = 18, TP_XI = 27 };   // tape entry offsets, in fields of N
@@ -1018,6 +1022,12 @@ static double* tape_entry(const dmk_handle* h, int k) { return h->adj->tape + (s
 static void adj_free(dmk_handle* h) {
   if (!h->adj) return;
   if (h->st) hipStreamSynchronize(h->st);
+  if (h->adj->st2) {
+    hipStreamSynchronize(h->adj->st2);
+    hipStreamDestroy(h->adj->st2);
+  }
+  for (hipEvent_t e : h->adj->fac)
+    if (e) hipEventDestroy(e);
   if (h->adj->tape) hipFree(h->adj->tape);
   for (void* p : h->adj->work) hipFree(p);
   delete h->adj;
@@ -1231,14 +1241,18 @@ extern "C" int dmk_adjoint_backward(dmk_handle* h, const double* P_av_bar, doubl
   const int mmax = h->nsl_max, nslot = ADJ_TH_SYS * mmax + DMK_NH_MAX;
   int rc;
   if (!a.work_ready) {
-    if ((rc = adj_walloc(h, &a.A, 81 * N)) || (rc = adj_walloc(h, &a.g, n9)) || (rc = adj_walloc(h, &a.res, N)) ||
-        (rc = adj_walloc(h, &a.scr, dmk_adj_scratch_slots(mmax) * N)) || (rc = adj_walloc(h, &a.Fpb[0], n9)) ||
+    if ((rc = adj_walloc(h, &a.A, 81 * N)) || (rc = adj_walloc(h, &a.g, n9)) || (rc = adj_walloc(h, &a.res[0], N)) ||
+        (rc = adj_walloc(h, &a.res[1], N)) || (rc = adj_walloc(h, &a.scr[0], dmk_adj_scratch_slots(mmax) * N)) ||
+        (rc = adj_walloc(h, &a.scr[1], dmk_adj_scratch_slots(mmax) * N)) || (rc = adj_walloc(h, &a.Fpb[0], n9)) ||
         (rc = adj_walloc(h, &a.Fpb[1], n9)) || (rc = adj_walloc(h, &a.xib[0], (size_t)mmax * N)) ||
         (rc = adj_walloc(h, &a.xib[1], (size_t)mmax * N)) || (rc = adj_walloc(h, &a.thb, (size_t)nslot * N)) ||
         (rc = adj_walloc(h, &a.mu, n9)) || (rc = adj_walloc(h, &a.y, n9)) || (rc = adj_walloc(h, &a.b, n9)) ||
         (rc = adj_walloc(h, &a.r, n9)) || (rc = adj_walloc(h, &a.V, (size_t)(a.M + 1) * n9)) ||
         (rc = adj_walloc(h, &a.red, (size_t)h->n_phases * (nslot + mmax))))
       return rc;
+    CK(hipStreamCreate(&a.st2));
+    CK(hipEventCreateWithFlags(&a.fac[0], hipEventDisableTiming));
+    CK(hipEventCreateWithFlags(&a.fac[1], hipEventDisableTiming));
     a.work_ready = true;
   }
   CK(hipMemsetAsync(a.Fpb[0], 0, sizeof(double) * n9, h->st));
@@ -1246,11 +1260,10 @@ extern "C" int dmk_adjoint_backward(dmk_handle* h, const double* P_av_bar, doubl
   CK(hipMemsetAsync(a.thb, 0, sizeof(double) * nslot * N, h->st));
   CK(hipMemsetAsync(a.red, 0, sizeof(double) * h->n_phases * (nslot + mmax), h->st));
   CK(hipMemsetAsync(a.mu, 0, sizeof(double) * n9, h->st));
-  int cur = 0;
-  for (int k = a.n; k >= 1; k--) {
+  // the tape-only arguments of increment k, scratch buffer k % 2
+  auto local_args = [&](int k) {
     const double* E = tape_entry(h, k);
     const double* Ep = tape_entry(h, k - 1);
-    const double* qR = &a.qR[4 * (k - 1)];
     AdjKArgs ka{};
     ka.N = (int)N;
     ka.mmax = mmax;
@@ -1264,23 +1277,44 @@ extern "C" int dmk_adjoint_backward(dmk_handle* h, const double* P_av_bar, doubl
     ka.xi = E + TP_XI * N;
     ka.Fp0 = Ep + TP_FP * N;
     ka.xi0 = Ep + TP_XI * N;
+    ka.scr = a.scr[k % 2];
+    ka.res = a.res[k % 2];
+    return ka;
+  };
+  // K1 + K2 of increment k on st2; its buffers were last read by increment
+  // k + 2, whose work the solver stream finished before the enqueue
+  auto factor = [&](int k) -> int {
+    AdjKArgs ka = local_args(k);
+    for (int p = 0; p < h->n_phases; p++) {
+      ka.phase_id = p;
+      ka.nsl = h->ph[p].nsl;
+      CKL(dmk_adj_factor(&ka, h->ph_dev + p, a.st2));
+    }
+    CK(hipEventRecord(a.fac[k % 2], a.st2));
+    return 0;
+  };
+  CK(hipStreamSynchronize(h->st));
+  if ((rc = factor(a.n))) return rc;
+  int cur = 0;
+  for (int k = a.n; k >= 1; k--) {
+    const double* qR = &a.qR[4 * (k - 1)];
+    AdjKArgs ka = local_args(k);
     // P_av = R <P> R^T (passive rotation of the raw average): <P>_bar = R^T P_av_bar R
     double pr[9];
     rotate33(qR, P_av_bar + 9 * (k - 1), true, pr);
     for (int c = 0; c < 9; c++) ka.Pu[c] = h->wgt * pr[c];
     ka.Fpb_in = a.Fpb[cur];
     ka.xib_in = a.xib[cur];
-    ka.scr = a.scr;
     ka.A = a.A;
     ka.g = a.g;
-    ka.res = a.res;
     ka.mode = 0;
+    CK(hipStreamWaitEvent(h->st, a.fac[k % 2], 0));
     for (int p = 0; p < h->n_phases; p++) {
       ka.phase_id = p;
       ka.nsl = h->ph[p].nsl;
-      CKL(dmk_adj_local(&ka, h->ph_dev + p, h->st));
+      CKL(dmk_adj_vjp(&ka, h->ph_dev + p, h->st));
     }
-    CKL(dmk_adj_max(a.res, N, h->small + SM_AMAX, h->st));
+    CKL(dmk_adj_max(ka.res, N, h->small + SM_AMAX, h->st));
     CKL(dmk_colsum(a.A, N, N, 81, h->partial, h->small + SM_ACAV, h->wgt, h->st));
     CK(hipMemcpyAsync(h->hsmall + SM_AMAX, h->small + SM_AMAX, sizeof(double) * (SM_ACAV + 81 - SM_AMAX),
                       hipMemcpyDeviceToHost, h->st));
@@ -1290,6 +1324,7 @@ extern "C" int dmk_adjoint_backward(dmk_handle* h, const double* P_av_bar, doubl
       dmk_set_error("dmk_adjoint_backward: increment %d: singular increment Jacobian or non-finite state", k);
       return DMK_ERR_NOCONV;
     }
+    if (k > 1 && (rc = factor(k - 1))) return rc;
     // mean rule: masked compliance of <A^T>
     double Ct[81], Sadj[81];
     const double* Abar = h->hsmall + SM_ACAV;
@@ -1321,7 +1356,7 @@ extern "C" int dmk_adjoint_backward(dmk_handle* h, const double* P_av_bar, doubl
     for (int p = 0; p < h->n_phases; p++) {
       ka.phase_id = p;
       ka.nsl = h->ph[p].nsl;
-      CKL(dmk_adj_local(&ka, h->ph_dev + p, h->st));
+      CKL(dmk_adj_vjp(&ka, h->ph_dev + p, h->st));
     }
     // parameter cotangents of this increment, summed per phase on the device
     CKL(dmk_adj_phase_sum(a.thb, N, h->mp.phase, h->n_phases, nslot, 1, a.red + (size_t)h->n_phases * mmax, h->st));
