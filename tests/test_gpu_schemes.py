@@ -95,6 +95,27 @@ def test_scheme_converges_to_basic_solution(gpu_lib, scheme):
         assert np.abs(b["F_p"] - a["F_p"]).max() < 1e-8
 
 
+@pytest.mark.parametrize("scheme", SCHEMES)
+def test_scheme_bcc_odd_grid(gpu_lib, scheme):
+    """cI phase with N_sl = [12, 12] and non-Schmid terms on a 7 x 6 x 5 grid
+    (odd N1: no Nyquist plane in x; odd N3), uniaxial tension with lateral
+    stress BCs: same equilibrium as the basic scheme at tight tolerances"""
+    from damask_amd import config, workloads
+    g, m = workloads.polycrystal([7, 6, 5], 6, 21, size=(1.0, 0.9, 0.8), phase=workloads.BCC_PHASE)
+    mat = config.parse_material(m)
+    _, lcs = config.parse_load(workloads.tension_x(6.0, 12, 4))
+    num = config.Numerics(eps_abs_div=1e-8, eps_rel_div=1e-11, eps_abs_P=1e-3, eps_rel_P=1e-11,
+                          eps_abs_curl=1e-13, eps_rel_curl=1e-11, N_iter_max=400, snes_max_it=400)
+    rb, _ = _run("spectral_basic", g, mat, lcs, num)
+    rp, its = _run(scheme, g, mat, lcs, num)
+    print(scheme, "iterations", its)
+    assert len(rb) == len(rp) == 3
+    for a, b in zip(rb, rp):
+        assert np.abs(b["P_av"] - a["P_av"]).max() < 1e-7 * np.abs(a["P_av"]).max()
+        assert np.abs(b["F"] - a["F"]).max() < 1e-8
+        assert np.abs(b["F_p"] - a["F_p"]).max() < 1e-8
+
+
 def test_polarization_update_gamma(gpu_lib):
     """update_gamma: Gamma, C_scale and S_scale follow C_minMaxAvg at every
     solution (grid_mech_spectral_polarization.f90:301-305).  Default numerics
