@@ -21,6 +21,7 @@ struct DmkFFT {
   double2* tw[3] = {nullptr, nullptr, nullptr};
   int gx[3] = {0, 0, 0};
   int xy_threads = 0;
+  int zgz_g = 0, zgz_threads = 0;   // > 0: z passes and Gamma fused, zgz_g columns per block
 };
 
 struct GammaArgs {
@@ -59,6 +60,15 @@ int dmk_fft_setup(DmkFFT* fft, const int cells[3]);
 int dmk_fft_tune(DmkFFT* fft, double2* cscratch, double* rscratch, hipStream_t s);
 void dmk_fft_free(DmkFFT* fft);
 int dmk_fft_r2c(const DmkFFT* fft, int howmany, const double* in, double2* out, hipStream_t s);
+// the x and y passes of dmk_fft_r2c / dmk_fft_c2r only
+int dmk_fft_r2c_xy(const DmkFFT* fft, int howmany, const double* in, double2* out, hipStream_t s);
+int dmk_fft_c2r_xy(const DmkFFT* fft, int howmany, double2* in, double* out, hipStream_t s);
+// on the x-y transformed nine fields Z: forward z pass, Gamma convolution /
+// divergence per g, inverse z pass (fused or separate as tuned); *nb = number
+// of per-block divergence sums written to div_partial (at most
+// dmk_gamma_conv_partials)
+int dmk_gamma_conv(const DmkFFT* fft, double2* Z, const GammaArgs* g, double* div_partial, int* nb, hipStream_t s);
+long dmk_gamma_conv_partials(const DmkFFT* fft);
 int dmk_fft_c2r(const DmkFFT* fft, int howmany, double2* in, double* out, hipStream_t s);
 int dmk_gamma_div(double2* Z, const GammaArgs* g, double* div_partial, hipStream_t s);
 int dmk_gamma_div_blocks(const GammaArgs* g);

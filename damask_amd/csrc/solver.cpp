@@ -196,6 +196,7 @@ struct dmk_handle {
   double *u3, *xn_dev, *xp_dev;
   double *partial, *small, *nrm, *pmax, *pmin, *ext;
   double* divp = nullptr;            // per-block divergence sums of the basic residual
+  int div_nb = 0;                    // how many
   bool div_pending = false;          // divp holds sums the next residual_finish adds up
   long *sel, *csel;
   int* flag;
@@ -423,18 +424,20 @@ static int residual_enqueue(dmk_handle* h, const double* X, double* r, int iter)
   aim_args(h, true, true, &a);
   int rc = response_enqueue(h, X, h->dt, &a);
   if (rc) return rc;
+  // x-y passes, then the z passes with the Gamma convolution and the
+  // divergence between them (fused when the FFT tuner found that faster)
   kt_begin(h, KT_FFT_FWD);
-  CKL(dmk_fft_r2c(&h->fft, 9, h->P_hom, h->Z, h->st));
+  CKL(dmk_fft_r2c_xy(&h->fft, 9, h->P_hom, h->Z, h->st));
   kt_end(h, KT_FFT_FWD);
   h->ga.do_div = 1;
   h->ga.do_gamma = 1;
   h->ga.aim_dev = h->small + SM_AIM;
   kt_begin(h, KT_GAMMA);
-  CKL(dmk_gamma_div(h->Z, &h->ga, h->divp, h->st));
+  CKL(dmk_gamma_conv(&h->fft, h->Z, &h->ga, h->divp, &h->div_nb, h->st));
   kt_end(h, KT_GAMMA);
   h->div_pending = true;            // summed with the dot products in residual_finish
   kt_begin(h, KT_FFT_INV);
-  CKL(dmk_fft_c2r(&h->fft, 9, h->Z, r, h->st));
+  CKL(dmk_fft_c2r_xy(&h->fft, 9, h->Z, r, h->st));
   kt_end(h, KT_FFT_INV);
   return 0;
 }
@@ -538,7 +541,7 @@ static int residual_finish(dmk_handle* h, const DotJobs* jobs, double* dots_out)
   extra.k = 0;
   if (h->div_pending) {
     extra.p[0] = h->divp;
-    extra.n[0] = dmk_gamma_div_blocks(&h->ga);
+    extra.n[0] = h->div_nb;
     extra.scale[0] = 1.0;
     extra.out[0] = h->small + SM_REP + REP_DIV2;
     extra.k = 1;
@@ -1528,7 +1531,7 @@ static int init_impl(dmk_handle* h, const int32_t cells[3], const double size[3]
   long np = 82L * dmk_nchunk(std::max(nx, 9 * h->Nk)) + 4096;
   if ((rc = dalloc(h, &h->partial, np))) return rc;
   if ((rc = dalloc(h, &h->small, SM_SIZE))) return rc;
-  if ((rc = dalloc(h, &h->divp, (size_t)(h->Nk + 255) / 256))) return rc;
+  if ((rc = dalloc(h, &h->divp, (size_t)dmk_gamma_conv_partials(&h->fft)))) return rc;
   if ((rc = dalloc(h, &h->nrm, N))) return rc;
   if ((rc = dalloc(h, &h->pmax, dmk_nchunk(N)))) return rc;
   if ((rc = dalloc(h, &h->pmin, dmk_nchunk(N)))) return rc;
@@ -2014,14 +2017,16 @@ extern "C" int dmk_gamma_convolution(dmk_handle* h, const double* field, const d
   if (!h || !field || !aim || !out) { dmk_set_error("null argument"); return DMK_ERR_ARG; }
   CK(hipMemcpyAsync(h->Pext, field, sizeof(double) * h->n9, hipMemcpyHostToDevice, h->st));
   CKL(dmk_aos_to_soa(h->Pext, h->N, 9, h->Fext, h->st));
-  CKL(dmk_fft_r2c(&h->fft, 9, h->Fext, h->Z, h->st));
+  // the transform path of the basic residual (x-y passes, z passes with Gamma)
+  CKL(dmk_fft_r2c_xy(&h->fft, 9, h->Fext, h->Z, h->st));
   GammaArgs g = h->ga;
   g.do_div = 0;
   g.do_gamma = 1;
   g.aim_dev = nullptr;
   std::memcpy(g.aim, aim, sizeof g.aim);
-  CKL(dmk_gamma_div(h->Z, &g, h->partial, h->st));
-  CKL(dmk_fft_c2r(&h->fft, 9, h->Z, h->Fext, h->st));
+  int nb;
+  CKL(dmk_gamma_conv(&h->fft, h->Z, &g, h->divp, &nb, h->st));
+  CKL(dmk_fft_c2r_xy(&h->fft, 9, h->Z, h->Fext, h->st));
   CKL(dmk_soa_to_aos(h->Fext, h->N, 9, h->Pext, h->st));
   CK(hipMemcpyAsync(out, h->Pext, sizeof(double) * h->n9, hipMemcpyDeviceToHost, h->st));
   CK(hipStreamSynchronize(h->st));

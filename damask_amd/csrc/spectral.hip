@@ -369,6 +369,100 @@ __global__ void __launch_bounds__(256) fft_strided(double2* __restrict__ data, i
 // ---------------------------------------------------------------- Gamma + div
 __device__ __forceinline__ double wrap_freq(int k, int n) { return (double)(k > n / 2 ? k - n : k); }
 
+// divergence contribution and (with do_gamma) the Gamma convolution of one
+// frequency (kx, ky, kz), P in place; dc: the frequency is the DC term
+__device__ __forceinline__ double gamma_div_point(double2* P, int kx, int ky, int kz, bool dc, const GammaArgs& g) {
+  const double TAU = 6.283185307179586;
+  double a[3];
+  a[0] = TAU * (double)kx / g.size[0];
+  a[1] = TAU * wrap_freq(ky, g.n[1]) / g.size[1];
+  a[2] = TAU * wrap_freq(kz, g.n[2]) / g.size[2];
+  // first-derivative frequencies: Nyquist set to zero on even grids
+  if (g.n[0] % 2 == 0 && 2 * kx == g.n[0]) a[0] = 0.0;
+  if (g.n[1] % 2 == 0 && 2 * ky == g.n[1]) a[1] = 0.0;
+  if (g.n[2] % 2 == 0 && 2 * kz == g.n[2]) a[2] = 0.0;
+  double contrib = 0.0;
+  if (g.do_div) {
+    // |P_hat (conj(-xi) s)|^2 with conj(-xi) = i a
+    double s = 0.0;
+#pragma unroll
+    for (int l = 0; l < 3; l++) {
+      double2 u = make_double2(0.0, 0.0);
+#pragma unroll
+      for (int m = 0; m < 3; m++) {
+        double w = a[m] * g.rg[m];
+        u.x = fma(P[l + 3 * m].x, w, u.x);
+        u.y = fma(P[l + 3 * m].y, w, u.y);
+      }
+      s = fma(u.x, u.x, fma(u.y, u.y, s));
+    }
+    double wgt;
+    if (g.h == 1) wgt = 2.0;
+    else wgt = (kx >= 1 && kx <= g.h - 2) ? 2.0 : 1.0;
+    contrib = wgt * s;
+  }
+  if (g.do_gamma) {
+    if (dc) {
+#pragma unroll
+      for (int c = 0; c < 9; c++) P[c] = make_double2(g.aim_dev ? g.aim_dev[c] : g.aim[c], 0.0);
+    } else {
+      double aa[6] = {a[0] * a[0], a[1] * a[1], a[2] * a[2], a[1] * a[2], a[0] * a[2], a[0] * a[1]};
+      double A[9];
+#pragma unroll
+      for (int q = 0; q < 9; q++) {
+        double s = 0.0;
+#pragma unroll
+        for (int p = 0; p < 6; p++) s = fma(g.Ccoef[q][p], aa[p], s);
+        A[q] = s;                      // A_lm, q = 3 l + m
+      }
+      double detA = A[0] * (A[4] * A[8] - A[5] * A[7]) - A[1] * (A[3] * A[8] - A[5] * A[6])
+                  + A[2] * (A[3] * A[7] - A[4] * A[6]);
+      if (fabs(detA) > 1.0e-16) {
+        double inv[9];
+        double r = 1.0 / detA;
+        inv[0] = (A[4] * A[8] - A[5] * A[7]) * r;
+        inv[1] = (A[2] * A[7] - A[1] * A[8]) * r;
+        inv[2] = (A[1] * A[5] - A[2] * A[4]) * r;
+        inv[3] = (A[5] * A[6] - A[3] * A[8]) * r;
+        inv[4] = (A[0] * A[8] - A[2] * A[6]) * r;
+        inv[5] = (A[2] * A[3] - A[0] * A[5]) * r;
+        inv[6] = (A[3] * A[7] - A[4] * A[6]) * r;
+        inv[7] = (A[1] * A[6] - A[0] * A[7]) * r;
+        inv[8] = (A[0] * A[4] - A[1] * A[3]) * r;
+        double2 v[3], w[3];
+#pragma unroll
+        for (int n = 0; n < 3; n++) {
+          double2 s = make_double2(0.0, 0.0);
+#pragma unroll
+          for (int o2 = 0; o2 < 3; o2++) {
+            s.x = fma(P[n + 3 * o2].x, a[o2], s.x);
+            s.y = fma(P[n + 3 * o2].y, a[o2], s.y);
+          }
+          v[n] = s;
+        }
+#pragma unroll
+        for (int l = 0; l < 3; l++) {
+          double2 s = make_double2(0.0, 0.0);
+#pragma unroll
+          for (int n = 0; n < 3; n++) {
+            s.x = fma(inv[3 * l + n], v[n].x, s.x);
+            s.y = fma(inv[3 * l + n], v[n].y, s.y);
+          }
+          w[l] = s;
+        }
+#pragma unroll
+        for (int l = 0; l < 3; l++)
+#pragma unroll
+          for (int m = 0; m < 3; m++) P[l + 3 * m] = make_double2(w[l].x * a[m], w[l].y * a[m]);
+      } else {
+#pragma unroll
+        for (int c = 0; c < 9; c++) P[c] = make_double2(0.0, 0.0);
+      }
+    }
+  }
+  return contrib;
+}
+
 __global__ void __launch_bounds__(256) gamma_div_kernel(double2* __restrict__ Z, GammaArgs g, double* __restrict__ div_partial) {
   const long Nk = (long)g.h * g.n[1] * g.n[2];
   const long f = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -378,95 +472,13 @@ __global__ void __launch_bounds__(256) gamma_div_kernel(double2* __restrict__ Z,
     long r = f / g.h;
     int ky = (int)(r % g.n[1]);
     int kz = (int)(r / g.n[1]);
-    const double TAU = 6.283185307179586;
-    double a[3];
-    a[0] = TAU * (double)kx / g.size[0];
-    a[1] = TAU * wrap_freq(ky, g.n[1]) / g.size[1];
-    a[2] = TAU * wrap_freq(kz, g.n[2]) / g.size[2];
-    // first-derivative frequencies: Nyquist set to zero on even grids
-    if (g.n[0] % 2 == 0 && 2 * kx == g.n[0]) a[0] = 0.0;
-    if (g.n[1] % 2 == 0 && 2 * ky == g.n[1]) a[1] = 0.0;
-    if (g.n[2] % 2 == 0 && 2 * kz == g.n[2]) a[2] = 0.0;
     double2 P[9];
 #pragma unroll
     for (int c = 0; c < 9; c++) P[c] = Z[c * Nk + f];
-    if (g.do_div) {
-      // |P_hat (conj(-xi) s)|^2 with conj(-xi) = i a
-      double s = 0.0;
-#pragma unroll
-      for (int l = 0; l < 3; l++) {
-        double2 u = make_double2(0.0, 0.0);
-#pragma unroll
-        for (int m = 0; m < 3; m++) {
-          double w = a[m] * g.rg[m];
-          u.x = fma(P[l + 3 * m].x, w, u.x);
-          u.y = fma(P[l + 3 * m].y, w, u.y);
-        }
-        s = fma(u.x, u.x, fma(u.y, u.y, s));
-      }
-      double wgt;
-      if (g.h == 1) wgt = 2.0;
-      else wgt = (kx >= 1 && kx <= g.h - 2) ? 2.0 : 1.0;
-      contrib = wgt * s;
-    }
+    contrib = gamma_div_point(P, kx, ky, kz, f == 0, g);
     if (g.do_gamma) {
-      if (f == 0) {
 #pragma unroll
-        for (int c = 0; c < 9; c++) Z[c * Nk] = make_double2(g.aim_dev ? g.aim_dev[c] : g.aim[c], 0.0);
-      } else {
-        double aa[6] = {a[0] * a[0], a[1] * a[1], a[2] * a[2], a[1] * a[2], a[0] * a[2], a[0] * a[1]};
-        double A[9];
-#pragma unroll
-        for (int q = 0; q < 9; q++) {
-          double s = 0.0;
-#pragma unroll
-          for (int p = 0; p < 6; p++) s = fma(g.Ccoef[q][p], aa[p], s);
-          A[q] = s;                      // A_lm, q = 3 l + m
-        }
-        double detA = A[0] * (A[4] * A[8] - A[5] * A[7]) - A[1] * (A[3] * A[8] - A[5] * A[6])
-                    + A[2] * (A[3] * A[7] - A[4] * A[6]);
-        if (fabs(detA) > 1.0e-16) {
-          double inv[9];
-          double r = 1.0 / detA;
-          inv[0] = (A[4] * A[8] - A[5] * A[7]) * r;
-          inv[1] = (A[2] * A[7] - A[1] * A[8]) * r;
-          inv[2] = (A[1] * A[5] - A[2] * A[4]) * r;
-          inv[3] = (A[5] * A[6] - A[3] * A[8]) * r;
-          inv[4] = (A[0] * A[8] - A[2] * A[6]) * r;
-          inv[5] = (A[2] * A[3] - A[0] * A[5]) * r;
-          inv[6] = (A[3] * A[7] - A[4] * A[6]) * r;
-          inv[7] = (A[1] * A[6] - A[0] * A[7]) * r;
-          inv[8] = (A[0] * A[4] - A[1] * A[3]) * r;
-          double2 v[3], w[3];
-#pragma unroll
-          for (int n = 0; n < 3; n++) {
-            double2 s = make_double2(0.0, 0.0);
-#pragma unroll
-            for (int o2 = 0; o2 < 3; o2++) {
-              s.x = fma(P[n + 3 * o2].x, a[o2], s.x);
-              s.y = fma(P[n + 3 * o2].y, a[o2], s.y);
-            }
-            v[n] = s;
-          }
-#pragma unroll
-          for (int l = 0; l < 3; l++) {
-            double2 s = make_double2(0.0, 0.0);
-#pragma unroll
-            for (int n = 0; n < 3; n++) {
-              s.x = fma(inv[3 * l + n], v[n].x, s.x);
-              s.y = fma(inv[3 * l + n], v[n].y, s.y);
-            }
-            w[l] = s;
-          }
-#pragma unroll
-          for (int l = 0; l < 3; l++)
-#pragma unroll
-            for (int m = 0; m < 3; m++) Z[(l + 3 * m) * Nk + f] = make_double2(w[l].x * a[m], w[l].y * a[m]);
-        } else {
-#pragma unroll
-          for (int c = 0; c < 9; c++) Z[c * Nk + f] = make_double2(0.0, 0.0);
-        }
-      }
+      for (int c = 0; c < 9; c++) Z[c * Nk + f] = P[c];
     }
   }
   if (g.do_div) {
@@ -478,6 +490,60 @@ __global__ void __launch_bounds__(256) gamma_div_kernel(double2* __restrict__ Z,
       __syncthreads();
     }
     if (threadIdx.x == 0) div_partial[blockIdx.x] = red[0];
+  }
+}
+
+// z pass, Gamma convolution with the divergence RMS, and inverse z pass of
+// the nine fields in one kernel.  A block holds gz consecutive columns
+// m = kx + h ky (all z) of the nine fields in LDS, as lines (field, column)
+// of length N3: each frequency then has its nine components at hand, and the
+// spectrum makes one round trip through HBM instead of three.
+__global__ void fft_zgz(double2* __restrict__ Z, GammaArgs g, FftPlan pz, const double2* __restrict__ twz, int gz,
+                        double* __restrict__ div_partial) {
+  extern __shared__ double2 lds[];
+  __shared__ double red[1024];
+  const int h = g.h, N3 = g.n[2], ldl = N3 + 1;
+  const long M = (long)h * g.n[1], Nk = M * N3;
+  const long m0 = (long)blockIdx.x * gz;
+  const int nl = (int)min((long)gz, M - m0);
+  const int nlines = 9 * gz;
+  double2* b0 = lds;
+  double2* b1 = lds + nlines * ldl;
+  for (int i = threadIdx.x; i < 9 * N3 * gz; i += blockDim.x) {
+    int j = i % gz, t = i / gz, kz = t % N3, c = t / N3;
+    b0[(c * gz + j) * ldl + kz] = j < nl ? Z[c * Nk + (long)kz * M + m0 + j] : make_double2(0.0, 0.0);
+  }
+  __syncthreads();
+  double2* r = lds_fft(b0, b1, nlines, N3, ldl, pz, twz, -1);
+  double2* o = r == b0 ? b1 : b0;
+  double contrib = 0.0;
+  for (int i = threadIdx.x; i < gz * N3; i += blockDim.x) {
+    int j = i % gz, kz = i / gz;
+    if (j >= nl) continue;
+    const long m = m0 + j;
+    double2 P[9];
+#pragma unroll
+    for (int c = 0; c < 9; c++) P[c] = r[(c * gz + j) * ldl + kz];
+    contrib += gamma_div_point(P, (int)(m % h), (int)(m / h), kz, m == 0 && kz == 0, g);
+    if (g.do_gamma) {
+#pragma unroll
+      for (int c = 0; c < 9; c++) r[(c * gz + j) * ldl + kz] = P[c];
+    }
+  }
+  if (g.do_div) {
+    red[threadIdx.x] = contrib;
+    __syncthreads();
+    for (int k = blockDim.x / 2; k > 0; k >>= 1) {
+      if (threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) div_partial[blockIdx.x] = red[0];
+  }
+  __syncthreads();
+  double2* q = lds_fft(r, o, nlines, N3, ldl, pz, twz, +1);
+  for (int i = threadIdx.x; i < 9 * N3 * gz; i += blockDim.x) {
+    int j = i % gz, t = i / gz, kz = t % N3, c = t / N3;
+    if (j < nl) Z[c * Nk + (long)kz * M + m0 + j] = q[(c * gz + j) * ldl + kz];
   }
 }
 
@@ -788,6 +854,16 @@ static void launch_strided(const DmkFFT* fft, int howmany, double2* data, int ax
                        (long)h * N2, fft->plan[2], fft->tw[2], sign);
 }
 
+static size_t zgz_lds(const DmkFFT* fft, int gz) { return 2 * sizeof(double2) * 9 * gz * (fft->n[2] + 1); }
+
+static void launch_zgz(const DmkFFT* fft, double2* Z, const GammaArgs* g, int gz, int threads, double* div_partial,
+                       hipStream_t s) {
+  const long M = (long)fft->h * fft->n[1];
+  unsigned nb = (unsigned)((M + gz - 1) / gz);
+  hipLaunchKernelGGL(fft_zgz, dim3(nb), dim3(threads), zgz_lds(fft, gz), s, Z, *g, fft->plan[2], fft->tw[2], gz,
+                     div_partial);
+}
+
 int dmk_fft_tune(DmkFFT* fft, double2* cscratch, double* rscratch, hipStream_t s) {
   const int howmany = 9;
   const long Nk = (long)fft->h * fft->n[1] * fft->n[2], N = (long)fft->n[0] * fft->n[1] * fft->n[2];
@@ -843,8 +919,74 @@ int dmk_fft_tune(DmkFFT* fft, double2* cscratch, double* rscratch, hipStream_t s
       if (t < sep) { sep = t; fft->xy_threads = th; }
     }
   }
+  // fused z passes and Gamma against the separate kernels (a reference medium
+  // with an invertible acoustic tensor; DMK_FFT_ZGZ=0 / 1 forces separate / fused)
+  fft->zgz_g = 0;
+  const char* fz = getenv("DMK_FFT_ZGZ");
+  if (fft->n[2] >= 2 && !(fz && fz[0] == '0')) {
+    GammaArgs g{};
+    for (int d = 0; d < 3; d++) { g.n[d] = fft->n[d]; g.size[d] = 1.0; g.rg[d] = 1.0; }
+    g.h = fft->h;
+    for (int q = 0; q < 9; q++) g.Ccoef[q][0] = g.Ccoef[q][1] = g.Ccoef[q][2] = (q % 4 == 0) ? 1.0 : 0.0;
+    g.do_div = 1;
+    g.do_gamma = 1;
+    double* part = rscratch;      // per-block sums: rscratch holds 9 N >= h N2 doubles
+    float sep = timed([&] {
+      launch_strided(fft, howmany, cscratch, 2, fft->gx[2], -1, s);
+      hipLaunchKernelGGL(gamma_div_kernel, dim3((unsigned)((Nk + 255) / 256)), dim3(256), 0, s, cscratch, g, part);
+      launch_strided(fft, howmany, cscratch, 2, fft->gx[2], +1, s);
+    });
+    if (fz && fz[0] == '1') sep = 1e30f;
+    for (int gz : {1, 2, 4, 8}) {
+      if (zgz_lds(fft, gz) > FFT_XY_LDS_MAX) continue;
+      for (int th : {256, 512}) {
+        float t = timed([&] { launch_zgz(fft, cscratch, &g, gz, th, part, s); });
+        if (t < sep) { sep = t; fft->zgz_g = gz; fft->zgz_threads = th; }
+      }
+    }
+  }
   hipEventDestroy(e0);
   hipEventDestroy(e1);
+  return hipGetLastError() == hipSuccess ? 0 : 2;
+}
+
+int dmk_fft_r2c_xy(const DmkFFT* fft, int howmany, const double* in, double2* out, hipStream_t s) {
+  if (fft->xy_threads) {
+    launch_xy_r2c(fft, howmany, in, out, fft->xy_threads, s);
+  } else {
+    launch_x_r2c(fft, howmany, in, out, fft->gx[0], s);
+    if (fft->n[1] > 1) launch_strided(fft, howmany, out, 1, fft->gx[1], -1, s);
+  }
+  return hipGetLastError() == hipSuccess ? 0 : 2;
+}
+
+int dmk_fft_c2r_xy(const DmkFFT* fft, int howmany, double2* in, double* out, hipStream_t s) {
+  if (fft->xy_threads) {
+    launch_yx_c2r(fft, howmany, in, out, fft->xy_threads, s);
+  } else {
+    if (fft->n[1] > 1) launch_strided(fft, howmany, in, 1, fft->gx[1], +1, s);
+    launch_x_c2r(fft, howmany, in, out, fft->gx[0], s);
+  }
+  return hipGetLastError() == hipSuccess ? 0 : 2;
+}
+
+long dmk_gamma_conv_partials(const DmkFFT* fft) {
+  const long M = (long)fft->h * fft->n[1], Nk = M * fft->n[2];
+  long a = (Nk + 255) / 256;
+  return a > M ? a : M;
+}
+
+int dmk_gamma_conv(const DmkFFT* fft, double2* Z, const GammaArgs* g, double* div_partial, int* nb, hipStream_t s) {
+  const long M = (long)fft->h * fft->n[1], Nk = M * fft->n[2];
+  if (fft->zgz_g) {
+    launch_zgz(fft, Z, g, fft->zgz_g, fft->zgz_threads, div_partial, s);
+    *nb = (int)((M + fft->zgz_g - 1) / fft->zgz_g);
+  } else {
+    if (fft->n[2] > 1) launch_strided(fft, 9, Z, 2, fft->gx[2], -1, s);
+    hipLaunchKernelGGL(gamma_div_kernel, dim3((unsigned)((Nk + 255) / 256)), dim3(256), 0, s, Z, *g, div_partial);
+    if (fft->n[2] > 1) launch_strided(fft, 9, Z, 2, fft->gx[2], +1, s);
+    *nb = (int)((Nk + 255) / 256);
+  }
   return hipGetLastError() == hipSuccess ? 0 : 2;
 }
 

@@ -116,11 +116,14 @@ def test_fft_roundtrip_and_parity(gpu_lib, oracle, cells, xy, monkeypatch):
         assert np.abs(y / N - x).max() < 5e-15 * max(1.0, np.log2(N))
 
 
-@pytest.mark.parametrize("xy", ["0", "1"])
-@pytest.mark.parametrize("cells", [[6, 7, 8], [8, 8, 8], [5, 6, 4]])
-def test_gamma_and_div(gpu_lib, oracle, cells, xy, monkeypatch):
+@pytest.mark.parametrize("xy,zgz", [("0", "0"), ("1", "1"), ("0", "1")])
+@pytest.mark.parametrize("cells", [[6, 7, 8], [8, 8, 8], [5, 6, 4], [6, 5, 1]])
+def test_gamma_and_div(gpu_lib, oracle, cells, xy, zgz, monkeypatch):
+    """Gamma convolution through the residual's transform path: separate or
+    fused x-y passes (DMK_FFT_XY), separate or fused z passes + Gamma (DMK_FFT_ZGZ)"""
     from damask_amd import _lib
     monkeypatch.setenv("DMK_FFT_XY", xy)
+    monkeypatch.setenv("DMK_FFT_ZGZ", zgz)
     grid, mat = _poly(cells, 5, 2)
     g, o = _pair(grid, mat, oracle=oracle)
     N = grid.n_cells
@@ -138,8 +141,12 @@ def test_gamma_and_div(gpu_lib, oracle, cells, xy, monkeypatch):
     assert abs(d_g[0] - d_o) < 1e-12 * d_o
 
 
-def test_solver_increments_fcc_golden_grid(gpu_lib, oracle):
-    """the golden 6x7x8 case, first 8 increments, GPU vs oracle"""
+@pytest.mark.parametrize("fused", ["0", "1"])
+def test_solver_increments_fcc_golden_grid(gpu_lib, oracle, fused, monkeypatch):
+    """the golden 6x7x8 case, first 8 increments, GPU vs oracle, with the
+    separate and the fused FFT / Gamma kernels"""
+    monkeypatch.setenv("DMK_FFT_XY", fused)
+    monkeypatch.setenv("DMK_FFT_ZGZ", fused)
     grid, mat, lcs = fcc_golden_inputs()
     lc = copy.copy(lcs[0])
     lc.N = 8
