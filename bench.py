@@ -24,6 +24,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAK_FP64_TFLOPS = 78.6      # MI355X FP64 (vector and matrix)
+PEAK_HBM_GBS = 8000.0        # MI355X HBM3E
 PEAK_HBM_GBS = 8000.0
 # HBM bytes per stress-kernel launch on this workload: rocprofv3 --pmc
 # FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 (factors from a plain device copy), see
@@ -223,6 +224,25 @@ def kernel_breakdown(args, grid, mat, lc, n=10):
     out = {k: round(v[0] / n, 4) for k, v in kt.items()}
     out["wall_with_all_events"] = round(wall * 1e3 / n, 4)
     out["increments"] = f"{args.warmup + 1}-{args.warmup + n}"
+    return out, kt
+
+
+def spectral_roofline(kt, cells):
+    """Achieved HBM bandwidth of the residual's transform path (HIP events per
+    kernel group): algorithmic bytes (each field read and written once per
+    group, flops.spectral_bytes) x launches / time, against 8 TB/s"""
+    from damask_amd import flops
+    N, Nk = cells**3, (cells // 2 + 1) * cells**2
+    bts = flops.spectral_bytes(N, Nk)
+    names = {"fft_fwd": "x-y forward passes (fft_xy_r2c)",
+             "gamma_div": "z passes + Gamma convolution + divergence (fft_zgz, or z passes and gamma_div_kernel)",
+             "fft_inv": "y-x inverse passes (fft_yx_c2r)"}
+    out = {"bound": "hbm", "unit": "GB/s", "peak": PEAK_HBM_GBS, "kernels": {}}
+    for k in ("fft_fwd", "gamma_div", "fft_inv"):
+        ms, n = kt[k]
+        gbs = bts[k] * n / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
+        out["kernels"][k] = {"kernels": names[k], "bytes_per_launch": bts[k], "avg_launch_us": ms * 1e3 / max(n, 1),
+                             "achieved": gbs, "frac": gbs / PEAK_HBM_GBS}
     return out
 
 
@@ -285,7 +305,7 @@ def main():
     kt = s.kernel_times()
     s.profile(0)
     s.close()
-    breakdown = kernel_breakdown(args, grid, mat, lc)
+    breakdown, kt_all = kernel_breakdown(args, grid, mat, lc)
     updates = c1["updates"] - c0["updates"]
     newton = c1["newton"] - c0["newton"]
     state = c1["state"] - c0["state"]
@@ -329,6 +349,7 @@ def main():
                      "flops_per_launch": fl / max(stress_n, 1),
                      "avg_launch_ms": stress_ms / max(stress_n, 1)},
     }
+    line["roofline_spectral"] = spectral_roofline(kt_all, args.cells)
     if args.adjoint:
         line["adjoint"] = adjoint_cost(args)
     if args.batch:

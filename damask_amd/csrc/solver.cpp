@@ -747,7 +747,8 @@ static int solve_richardson(dmk_handle* h, int* reason_out) {
 // maximum step 1e8, 40 backtracking steps, initial slope from the Jacobian
 // applied to the step), one failed line search ending the solve; linear
 // solves by restarted GMRES (Saad & Schultz, SIAM J. Sci. Stat. Comput. 7(3)
-// 1986): restart 30, classical Gram-Schmidt, zero initial guess, Givens
+// 1986): restart 30, classical Gram-Schmidt (the new vector's norm from the
+// same reduction as its projections), zero initial guess, Givens
 // residual estimate, relative tolerance 1e-5, no preconditioner (a shell
 // matrix gets none).  SNESSetUpdate's set_F_aim (:666-681) runs at the start
 // of every Newton step.
@@ -816,27 +817,41 @@ static int gk_gmres(dmk_handle* h, const double* b, double* x, int* its, bool* o
     int k = 0;
     bool done = false;
     for (int j = 0; j < m; j++) {
+      // classical Gram-Schmidt with the norm from the same reduction,
+      // |w - V h|^2 = |w|^2 - |h|^2, so one synchronisation per step; the
+      // norm is recomputed when that difference cancels by more than 100x
       double* w = gk_basis(h, j + 1);
       if ((rc = gk_jv(h, gk_basis(h, j), w))) return rc;
       J.n = 0;
       for (int i = 0; i <= j; i++) add_dot(&J, w, gk_basis(h, i), 0);
+      add_dot(&J, w, w, 0);
       if ((rc = sync_dots(h, J, d))) return rc;
+      double hh = 0.0;
+      for (int i = 0; i <= j; i++) hh += d[i] * d[i];
+      double hn2 = d[j + 1] - hh;
       CombineArgs ca;
       ca.l = j + 1;
       ca.atot = 0.0;
       for (int i = 0; i <= j; i++) {
         H[i + (m + 1) * j] = d[i];
-        ca.alpha[i] = -d[i];
         ca.X[i] = gk_basis(h, i);
       }
-      CKL(dmk_lincomb(n, 1.0, w, &ca, w, h->st));
-      J.n = 0;
-      add_dot(&J, w, w, 0);
-      if ((rc = sync_dots(h, J, d))) return rc;
-      double hn = std::sqrt(d[0]);
+      double hn;
+      if (hn2 > 0.01 * d[j + 1]) {
+        hn = std::sqrt(hn2);
+        for (int i = 0; i <= j; i++) ca.alpha[i] = -d[i] / hn;
+        CKL(dmk_lincomb(n, 1.0 / hn, w, &ca, w, h->st));
+      } else {
+        for (int i = 0; i <= j; i++) ca.alpha[i] = -d[i];
+        CKL(dmk_lincomb(n, 1.0, w, &ca, w, h->st));
+        J.n = 0;
+        add_dot(&J, w, w, 0);
+        if ((rc = sync_dots(h, J, d))) return rc;
+        hn = std::sqrt(d[0]);
+        if (hn > 0.0) CKL(dmk_axpby(n, 1.0 / hn, w, 0.0, w, w, h->st));
+      }
       if (!std::isfinite(hn)) { *ok = false; *its = total; return 0; }
       H[j + 1 + (m + 1) * j] = hn;
-      if (hn > 0.0) CKL(dmk_axpby(n, 1.0 / hn, w, 0.0, w, w, h->st));
       for (int i = 0; i < j; i++) {
         double a = H[i + (m + 1) * j], c = H[i + 1 + (m + 1) * j];
         H[i + (m + 1) * j] = cs[i] * a + sn[i] * c;
