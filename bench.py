@@ -292,6 +292,7 @@ def main():
         r = s.run_loadcase(sub_loadcase(lc, args.warmup))
         assert r["rc"] == 0, "warm-up did not converge"
     c0 = s.counters()
+    fft_path = [int(v) for v in s.scalar("fft_path")]
     # HIP events around the stress kernel only: an event record between two
     # kernels leaves the stream idle for a few microseconds
     s.profile(1)
@@ -337,6 +338,8 @@ def main():
                    "newton_iterations_per_update": newton / max(updates, 1),
                    "state_iterations_per_update": state / max(updates, 1),
                    "init_s": t_init,
+                   "fft_path": {"xy_threads": fft_path[0], "xy_in_place": fft_path[1],
+                                "zgz_columns": fft_path[2], "zgz_threads": fft_path[3]},
                    "stress_kernel_ms": round(kt["stress"][0], 3),
                    "kernel_ms_per_increment": breakdown,
                    "cells_x_increments_per_s": args.cells**3 * args.steps / elapsed},

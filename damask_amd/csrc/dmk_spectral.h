@@ -21,6 +21,7 @@ struct DmkFFT {
   double2* tw[3] = {nullptr, nullptr, nullptr};
   int gx[3] = {0, 0, 0};
   int xy_threads = 0;
+  int xy_ip = 0;                    // 1: the fused x-y kernels' single-buffer (in-place) variant
   int zgz_g = 0, zgz_threads = 0;   // > 0: z passes and Gamma fused, zgz_g columns per block
 };
 

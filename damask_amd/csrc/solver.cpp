@@ -1891,6 +1891,10 @@ extern "C" int dmk_mech_get_scalars(dmk_handle* h, int32_t what, double* dst) {
     case 7: std::memcpy(dst, h->F_aim_lastinc, 9 * sizeof(double)); break;
     case 8: std::memcpy(dst, h->C_volAvgLastInc, 81 * sizeof(double)); break;
     case 9: std::memcpy(dst, h->C_minMaxAvgLastInc, 81 * sizeof(double)); break;
+    case 12:
+      dst[0] = h->fft.xy_threads; dst[1] = h->fft.xy_ip;
+      dst[2] = h->fft.zgz_g; dst[3] = h->fft.zgz_threads;
+      break;
     default: dmk_set_error("unknown scalar %d", what); return DMK_ERR_ARG;
   }
   return 0;

@@ -240,6 +240,165 @@ __global__ void __launch_bounds__(256) fft_x_c2r(const double2* __restrict__ in,
   }
 }
 
+// In-place variants for radix 2, 4 and 8 plans: each thread keeps its
+// butterflies (at most KMAX) in registers across a barrier and writes them
+// back into the same LDS buffer, so one buffer suffices and more blocks fit on
+// a CU.  Same arithmetic as stockham_stage / lds_fft.
+constexpr int IP_KMAX = 2;     // butterflies per thread and stage
+constexpr int IP_KU = 12;      // unpacked values per thread
+
+template <int R>
+__device__ void stockham_stage_ip(double2* buf, int nlines, int L, int ldl, int Ns, const double2* __restrict__ tw,
+                                  int sign) {
+  const int nb = L / R;
+  const int tot = nlines * nb;
+  const int tstep = L / (Ns * R);
+  double2 v[IP_KMAX][R];
+  int dst[IP_KMAX];
+#pragma unroll
+  for (int q = 0; q < IP_KMAX; q++) {
+    const int idx = threadIdx.x + q * blockDim.x;
+    dst[q] = -1;
+    if (idx < tot) {
+      const int line = idx / nb, j = idx - line * nb;
+      const double2* src = buf + line * ldl;
+#pragma unroll
+      for (int r = 0; r < R; r++) v[q][r] = src[j + r * nb];
+      const int k = j % Ns;
+      if (Ns > 1) {
+#pragma unroll
+        for (int r = 1; r < R; r++) v[q][r] = cmul(v[q][r], twid(tw, (r * k * tstep) % L, sign));
+      }
+      butterfly<R>(v[q], tw, L, sign);
+      dst[q] = line * ldl + (j / Ns) * Ns * R + k;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < IP_KMAX; q++)
+    if (dst[q] >= 0) {
+#pragma unroll
+      for (int r = 0; r < R; r++) buf[dst[q] + r * Ns] = v[q][r];
+    }
+  __syncthreads();
+}
+
+__device__ void lds_fft_ip(double2* buf, int nlines, int L, int ldl, const FftPlan& pl, const double2* __restrict__ tw,
+                           int sign) {
+  int Ns = 1;
+  for (int s = 0; s < pl.nstages; s++) {
+    const int R = pl.radix[s];
+    if (R == 8) stockham_stage_ip<8>(buf, nlines, L, ldl, Ns, tw, sign);
+    else if (R == 4) stockham_stage_ip<4>(buf, nlines, L, ldl, Ns, tw, sign);
+    else stockham_stage_ip<2>(buf, nlines, L, ldl, Ns, tw, sign);
+    Ns *= R;
+  }
+}
+
+__global__ __launch_bounds__(512) void fft_xy_r2c_ip(const double* __restrict__ in, double2* __restrict__ out, int N1, int N2, int N3,
+                              FftPlan px, const double2* __restrict__ twx, FftPlan py,
+                              const double2* __restrict__ twy) {
+  extern __shared__ double2 lds[];
+  const int h = N1 / 2 + 1, np = (N2 + 1) / 2, lx = N1 + 1, ly = N2 + 1;
+  double2* A = lds;
+  const double* src = in + (long)blockIdx.x * N1 * N2;
+  for (int i = threadIdx.x; i < 2 * np * N1; i += blockDim.x) {
+    int line = i / N1, pos = i - line * N1;
+    double v = line < N2 ? src[(long)line * N1 + pos] : 0.0;
+    double2* c = &A[(line >> 1) * lx + pos];
+    if (line & 1) c->y = v; else c->x = v;
+  }
+  __syncthreads();
+  lds_fft_ip(A, np, N1, lx, px, twx, -1);
+  // unpack the pairs into y lines (in registers across the barrier)
+  double2 u[IP_KU];
+#pragma unroll
+  for (int q = 0; q < IP_KU; q++) {
+    const int i = threadIdx.x + q * blockDim.x;
+    if (i < N2 * h) {
+      int line = i / h, k = i - line * h;
+      const double2* z = A + (line >> 1) * lx;
+      double2 a = z[k], b = cconj(z[(N1 - k) % N1]);
+      if (line & 1) {
+        double2 d = csub(a, b);
+        u[q] = make_double2(0.5 * d.y, -0.5 * d.x);
+      } else {
+        double2 s = cadd(a, b);
+        u[q] = make_double2(0.5 * s.x, 0.5 * s.y);
+      }
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < IP_KU; q++) {
+    const int i = threadIdx.x + q * blockDim.x;
+    if (i < N2 * h) {
+      int line = i / h, k = i - line * h;
+      A[k * ly + line] = u[q];
+    }
+  }
+  __syncthreads();
+  lds_fft_ip(A, h, N2, ly, py, twy, -1);
+  double2* dst = out + (blockIdx.x / N3) * ((long)h * N2 * N3) + (long)(blockIdx.x % N3) * h * N2;
+  for (int i = threadIdx.x; i < N2 * h; i += blockDim.x) {
+    int ky = i / h, kx = i - ky * h;
+    dst[i] = A[kx * ly + ky];
+  }
+}
+
+__global__ __launch_bounds__(512) void fft_yx_c2r_ip(const double2* __restrict__ in, double* __restrict__ out, int N1, int N2, int N3,
+                              FftPlan px, const double2* __restrict__ twx, FftPlan py,
+                              const double2* __restrict__ twy) {
+  extern __shared__ double2 lds[];
+  const int h = N1 / 2 + 1, np = (N2 + 1) / 2, lx = N1 + 1, ly = N2 + 1;
+  double2* A = lds;
+  const double2* src = in + (blockIdx.x / N3) * ((long)h * N2 * N3) + (long)(blockIdx.x % N3) * h * N2;
+  for (int i = threadIdx.x; i < N2 * h; i += blockDim.x) {
+    int ky = i / h, kx = i - ky * h;
+    A[kx * ly + ky] = src[i];
+  }
+  __syncthreads();
+  lds_fft_ip(A, h, N2, ly, py, twy, +1);
+  // pack pairs of y lines into complex x lines (registers across the barrier)
+  double2 u[IP_KU];
+#pragma unroll
+  for (int q = 0; q < IP_KU; q++) {
+    const int i = threadIdx.x + q * blockDim.x;
+    if (i < np * N1) {
+      int p = i / N1, k = i - p * N1;
+      int kk = (k < h) ? k : N1 - k;
+      bool cj = k >= h;
+      double2 a = A[kk * ly + 2 * p];
+      if (kk == 0 || 2 * kk == N1) a.y = 0.0;
+      if (cj) a.y = -a.y;
+      double2 b = make_double2(0.0, 0.0);
+      if (2 * p + 1 < N2) {
+        b = A[kk * ly + 2 * p + 1];
+        if (kk == 0 || 2 * kk == N1) b.y = 0.0;
+        if (cj) b.y = -b.y;
+      }
+      u[q] = make_double2(a.x - b.y, a.y + b.x);
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < IP_KU; q++) {
+    const int i = threadIdx.x + q * blockDim.x;
+    if (i < np * N1) {
+      int p = i / N1, k = i - p * N1;
+      A[p * lx + k] = u[q];
+    }
+  }
+  __syncthreads();
+  lds_fft_ip(A, np, N1, lx, px, twx, +1);
+  double* dst = out + (long)blockIdx.x * N1 * N2;
+  for (int i = threadIdx.x; i < N1 * N2; i += blockDim.x) {
+    int line = i / N1, pos = i - line * N1;
+    double2 z = A[(line >> 1) * lx + pos];
+    dst[i] = (line & 1) ? z.y : z.x;
+  }
+}
+
 // Fused x and y passes, one block per (field, z) plane: the real plane and its
 // half-spectrum plane are each contiguous in memory, so every global access
 // is coalesced and the plane stays in LDS between the passes.  The arithmetic
@@ -813,14 +972,40 @@ static size_t xy_lds(const DmkFFT* fft) {
   return 2 * sizeof(double2) * (a > b ? a : b);
 }
 
-static void launch_xy_r2c(const DmkFFT* fft, int howmany, const double* in, double2* out, int threads, hipStream_t s) {
-  hipLaunchKernelGGL(fft_xy_r2c, dim3(howmany * fft->n[2]), dim3(threads), xy_lds(fft), s, in, out, fft->n[0],
-                     fft->n[1], fft->n[2], fft->plan[0], fft->tw[0], fft->plan[1], fft->tw[1]);
+static void launch_xy_r2c(const DmkFFT* fft, int howmany, const double* in, double2* out, int threads, int ip,
+                          hipStream_t s) {
+  if (ip)
+    hipLaunchKernelGGL(fft_xy_r2c_ip, dim3(howmany * fft->n[2]), dim3(threads), xy_lds(fft) / 2, s, in, out,
+                       fft->n[0], fft->n[1], fft->n[2], fft->plan[0], fft->tw[0], fft->plan[1], fft->tw[1]);
+  else
+    hipLaunchKernelGGL(fft_xy_r2c, dim3(howmany * fft->n[2]), dim3(threads), xy_lds(fft), s, in, out, fft->n[0],
+                       fft->n[1], fft->n[2], fft->plan[0], fft->tw[0], fft->plan[1], fft->tw[1]);
 }
 
-static void launch_yx_c2r(const DmkFFT* fft, int howmany, const double2* in, double* out, int threads, hipStream_t s) {
-  hipLaunchKernelGGL(fft_yx_c2r, dim3(howmany * fft->n[2]), dim3(threads), xy_lds(fft), s, in, out, fft->n[0],
-                     fft->n[1], fft->n[2], fft->plan[0], fft->tw[0], fft->plan[1], fft->tw[1]);
+static void launch_yx_c2r(const DmkFFT* fft, int howmany, const double2* in, double* out, int threads, int ip,
+                          hipStream_t s) {
+  if (ip)
+    hipLaunchKernelGGL(fft_yx_c2r_ip, dim3(howmany * fft->n[2]), dim3(threads), xy_lds(fft) / 2, s, in, out,
+                       fft->n[0], fft->n[1], fft->n[2], fft->plan[0], fft->tw[0], fft->plan[1], fft->tw[1]);
+  else
+    hipLaunchKernelGGL(fft_yx_c2r, dim3(howmany * fft->n[2]), dim3(threads), xy_lds(fft), s, in, out, fft->n[0],
+                       fft->n[1], fft->n[2], fft->plan[0], fft->tw[0], fft->plan[1], fft->tw[1]);
+}
+
+// the in-place x-y kernels need radix 2/4/8 plans and every stage, unpack and
+// pack to fit the per-thread register slots at this block size
+static bool xy_ip_fits(const DmkFFT* fft, int threads) {
+  const int N1 = fft->n[0], N2 = fft->n[1], h = fft->h, np = (N2 + 1) / 2;
+  for (int d = 0; d < 2; d++) {
+    const FftPlan& p = fft->plan[d];
+    const int lines = d == 0 ? np : h, L = fft->n[d];
+    for (int st = 0; st < p.nstages; st++) {
+      const int R = p.radix[st];
+      if (R != 2 && R != 4 && R != 8) return false;
+      if ((long)lines * (L / R) > (long)IP_KMAX * threads) return false;
+    }
+  }
+  return (long)N2 * h <= (long)IP_KU * threads && (long)np * N1 <= (long)IP_KU * threads;
 }
 
 static void launch_x_r2c(const DmkFFT* fft, int howmany, const double* in, double2* out, int ppb, hipStream_t s) {
@@ -910,14 +1095,19 @@ int dmk_fft_tune(DmkFFT* fft, double2* cscratch, double* rscratch, hipStream_t s
   // fused x-y passes against the separate ones (forward direction);
   // DMK_FFT_XY=0 / 1 in the environment forces the separate / fused passes
   fft->xy_threads = 0;
+  fft->xy_ip = 0;
   const char* force = getenv("DMK_FFT_XY");
   if (fft->n[0] >= 2 && fft->n[1] >= 2 && xy_lds(fft) <= FFT_XY_LDS_MAX && !(force && force[0] == '0')) {
     float sep = timed([&] { launch_x_r2c(fft, howmany, rscratch, cscratch, fft->gx[0], s); }) + t_y;
-    if (force && force[0] == '1') sep = 1e30f;
-    for (int th : {256, 512, 1024}) {
-      float t = timed([&] { launch_xy_r2c(fft, howmany, rscratch, cscratch, th, s); });
-      if (t < sep) { sep = t; fft->xy_threads = th; }
-    }
+    if (force && (force[0] == '1' || force[0] == '2')) sep = 1e30f;
+    for (int ip = 0; ip <= 1; ip++)
+      for (int th : {256, 512, 1024}) {
+        if (ip && (th > 512 || !xy_ip_fits(fft, th))) continue;   // launch bounds of the in-place kernels
+        if (force && force[0] == '1' && ip) continue;       // DMK_FFT_XY=1: the two-buffer variant
+        if (force && force[0] == '2' && !ip) continue;      // DMK_FFT_XY=2: the in-place variant
+        float t = timed([&] { launch_xy_r2c(fft, howmany, rscratch, cscratch, th, ip, s); });
+        if (t < sep) { sep = t; fft->xy_threads = th; fft->xy_ip = ip; }
+      }
   }
   // fused z passes and Gamma against the separate kernels (a reference medium
   // with an invertible acoustic tensor; DMK_FFT_ZGZ=0 / 1 forces separate / fused)
@@ -952,7 +1142,7 @@ int dmk_fft_tune(DmkFFT* fft, double2* cscratch, double* rscratch, hipStream_t s
 
 int dmk_fft_r2c_xy(const DmkFFT* fft, int howmany, const double* in, double2* out, hipStream_t s) {
   if (fft->xy_threads) {
-    launch_xy_r2c(fft, howmany, in, out, fft->xy_threads, s);
+    launch_xy_r2c(fft, howmany, in, out, fft->xy_threads, fft->xy_ip, s);
   } else {
     launch_x_r2c(fft, howmany, in, out, fft->gx[0], s);
     if (fft->n[1] > 1) launch_strided(fft, howmany, out, 1, fft->gx[1], -1, s);
@@ -962,7 +1152,7 @@ int dmk_fft_r2c_xy(const DmkFFT* fft, int howmany, const double* in, double2* ou
 
 int dmk_fft_c2r_xy(const DmkFFT* fft, int howmany, double2* in, double* out, hipStream_t s) {
   if (fft->xy_threads) {
-    launch_yx_c2r(fft, howmany, in, out, fft->xy_threads, s);
+    launch_yx_c2r(fft, howmany, in, out, fft->xy_threads, fft->xy_ip, s);
   } else {
     if (fft->n[1] > 1) launch_strided(fft, howmany, in, 1, fft->gx[1], +1, s);
     launch_x_c2r(fft, howmany, in, out, fft->gx[0], s);
@@ -992,7 +1182,7 @@ int dmk_gamma_conv(const DmkFFT* fft, double2* Z, const GammaArgs* g, double* di
 
 int dmk_fft_r2c(const DmkFFT* fft, int howmany, const double* in, double2* out, hipStream_t s) {
   if (fft->xy_threads) {
-    launch_xy_r2c(fft, howmany, in, out, fft->xy_threads, s);
+    launch_xy_r2c(fft, howmany, in, out, fft->xy_threads, fft->xy_ip, s);
   } else {
     launch_x_r2c(fft, howmany, in, out, fft->gx[0], s);
     if (fft->n[1] > 1) launch_strided(fft, howmany, out, 1, fft->gx[1], -1, s);
@@ -1005,7 +1195,7 @@ int dmk_fft_r2c(const DmkFFT* fft, int howmany, const double* in, double2* out, 
 int dmk_fft_c2r(const DmkFFT* fft, int howmany, double2* in, double* out, hipStream_t s) {
   if (fft->n[2] > 1) launch_strided(fft, howmany, in, 2, fft->gx[2], +1, s);
   if (fft->xy_threads) {
-    launch_yx_c2r(fft, howmany, in, out, fft->xy_threads, s);
+    launch_yx_c2r(fft, howmany, in, out, fft->xy_threads, fft->xy_ip, s);
   } else {
     if (fft->n[1] > 1) launch_strided(fft, howmany, in, 1, fft->gx[1], +1, s);
     launch_x_c2r(fft, howmany, in, out, fft->gx[0], s);

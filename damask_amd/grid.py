@@ -25,7 +25,7 @@ FIELDS = {"F": 0, "P": 1, "F_e": 2, "F_p": 3, "L_p": 4, "S": 5, "F_i": 6, "O": 7
           "gamma_sl": 9, "dPdF": 10, "F_lastinc": 11, "F_tau": 12, "F_tau_lastinc": 13}
 SCALARS = {"F_aim": 0, "P_av": 1, "P_aim": 2, "C_volAvg": 3, "C_minMaxAvg": 4, "err": 5, "F_aimDot": 6,
            "F_aim_lastinc": 7, "C_volAvgLastInc": 8, "C_minMaxAvgLastInc": 9, "err_curl": 10,
-           "C_minMaxAvgRestart": 11}
+           "C_minMaxAvgRestart": 11, "fft_path": 12}
 # restart data of grid_mech_spectral_basic (restartWrite, :378-416) and of the
 # phase (phase_mechanical.f90:1248-1280)
 RESTART_FIELDS = ("F", "F_lastinc", "F_p", "L_p", "S", "xi_sl", "gamma_sl")
@@ -176,7 +176,7 @@ class GridMechSpectralBasic:
 
     def scalar(self, name: str) -> np.ndarray:
         n = {"C_volAvg": 81, "C_minMaxAvg": 81, "C_volAvgLastInc": 81, "C_minMaxAvgLastInc": 81,
-             "C_minMaxAvgRestart": 81, "err": 2, "err_curl": 1}.get(name, 9)
+             "C_minMaxAvgRestart": 81, "err": 2, "err_curl": 1, "fft_path": 4}.get(name, 9)
         out = np.zeros(n)
         _lib.check(self.lib.dmk_mech_get_scalars(self.h, SCALARS[name], _lib.dptr(out)), "dmk_mech_get_scalars")
         if n == 9:

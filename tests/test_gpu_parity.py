@@ -88,10 +88,12 @@ def test_constitutive_response(gpu_lib, oracle, lattice, eps):
         L.orc_mp_forward(o.mp)
 
 
-@pytest.mark.parametrize("xy", ["0", "1"])
-@pytest.mark.parametrize("cells", [[6, 7, 8], [8, 8, 8], [10, 6, 5], [7, 9, 3], [16, 12, 20]])
+@pytest.mark.parametrize("xy", ["0", "1", "2"])
+@pytest.mark.parametrize("cells", [[6, 7, 8], [8, 8, 8], [10, 6, 5], [7, 9, 3], [16, 12, 20], [32, 16, 6],
+                                   [64, 64, 3]])
 def test_fft_roundtrip_and_parity(gpu_lib, oracle, cells, xy, monkeypatch):
-    """separate passes (DMK_FFT_XY=0) and the fused x-y passes (1)"""
+    """separate passes (DMK_FFT_XY=0), the fused x-y passes (1) and their
+    single-buffer variant (2, taken where the x and y plans are radix 2/4/8)"""
     import ctypes as C
     from damask_amd import _lib
     monkeypatch.setenv("DMK_FFT_XY", xy)
@@ -100,6 +102,12 @@ def test_fft_roundtrip_and_parity(gpu_lib, oracle, cells, xy, monkeypatch):
     N = grid.n_cells
     h = cells[0] // 2 + 1
     Nk = h * cells[1] * cells[2]
+    path = g.scalar("fft_path")
+    pow2 = all(n & (n - 1) == 0 for n in cells[:2])
+    if xy == "2" and pow2:
+        assert path[0] > 0 and path[1] == 1, path
+    elif xy in ("0", "1"):
+        assert path[1] == 0 and (path[0] > 0) == (xy == "1"), path
     rng = np.random.default_rng(0)
     for howmany in (9, 3):
         x = rng.random(howmany * N)
@@ -116,8 +124,8 @@ def test_fft_roundtrip_and_parity(gpu_lib, oracle, cells, xy, monkeypatch):
         assert np.abs(y / N - x).max() < 5e-15 * max(1.0, np.log2(N))
 
 
-@pytest.mark.parametrize("xy,zgz", [("0", "0"), ("1", "1"), ("0", "1")])
-@pytest.mark.parametrize("cells", [[6, 7, 8], [8, 8, 8], [5, 6, 4], [6, 5, 1]])
+@pytest.mark.parametrize("xy,zgz", [("0", "0"), ("1", "1"), ("0", "1"), ("2", "1")])
+@pytest.mark.parametrize("cells", [[6, 7, 8], [8, 8, 8], [5, 6, 4], [6, 5, 1], [16, 32, 4]])
 def test_gamma_and_div(gpu_lib, oracle, cells, xy, zgz, monkeypatch):
     """Gamma convolution through the residual's transform path: separate or
     fused x-y passes (DMK_FFT_XY), separate or fused z passes + Gamma (DMK_FFT_ZGZ)"""
