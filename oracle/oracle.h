@@ -19,7 +19,7 @@
 
 #include <stdint.h>
 
-#ifdef __cplusplus
+#if defined(__cplusplus) && !defined(ORC_NO_EXTERN_C)
 extern "C" {
 #endif
 
@@ -193,7 +193,7 @@ int orc_run_loadcase(orc_solver *s, const orc_loadcase *lc, double *t_io,
                      double *stats /* N*4 */, double *P_av_hist /* N*9 */,
                      double *F_aim_hist /* N*9 */, int64_t *n_updates);
 
-#ifdef __cplusplus
+#if defined(__cplusplus) && !defined(ORC_NO_EXTERN_C)
 }
 #endif
 #endif

@@ -267,6 +267,16 @@ static double vdot(size_t n, const double *a, const double *b) {
   return s;
 }
 
+/* the tangent-linear build (oracle_fwd.cpp) waits for the derivative part of
+ * the iteration to settle as well, and differentiates the least squares
+ * solution in closed form instead of through the Jacobi sweeps */
+#ifndef ORC_TANGENT_CONVERGED
+#define ORC_TANGENT_CONVERGED(r, x, n) 1
+#endif
+#ifndef ORC_LSQ_TANGENT
+#define ORC_LSQ_TANGENT(m, H, b, lam, V, cut, a)
+#endif
+
 /* symmetric least squares H a = b via Jacobi eigen-decomposition with
  * singular values below eps*max*m treated as zero (LAPACK gelss rcond = -1) */
 static void lsq_sym(int m, const double *H, const double *b, double *a) {
@@ -283,8 +293,15 @@ static void lsq_sym(int m, const double *H, const double *b, double *a) {
       for (int q = p + 1; q < m; q++) {
         double apq = A[p + m * q];
         if (apq == 0.0) continue;
-        double theta = (A[q + m * q] - A[p + m * p]) / (2.0 * apq);
-        double t = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
+        /* rotation t = tan(phi); for a negligible pivot t = apq / (a_qq - a_pp)
+         * to first order, which also keeps theta^2 from overflowing */
+        double hd = A[q + m * q] - A[p + m * p], t;
+        if (fabs(apq) < 1e-150 * fabs(hd)) {
+          t = apq / hd;
+        } else {
+          double theta = hd / (2.0 * apq);
+          t = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
+        }
         double c = 1.0 / sqrt(t * t + 1.0), sn = t * c;
         for (int k = 0; k < m; k++) {
           double akp = A[k + m * p], akq = A[k + m * q];
@@ -315,6 +332,7 @@ static void lsq_sym(int m, const double *H, const double *b, double *a) {
     c /= lam;
     for (int i = 0; i < m; i++) a[i] += c * V[i + m * k];
   }
+  ORC_LSQ_TANGENT(m, H, b, A, V, cut, a);
 }
 
 static void update_subspace(orc_solver *s, int ivec, int l, const double *F, const double *X) {
@@ -338,6 +356,7 @@ static int solve_ngmres(orc_solver *s) {
   form_residual(s, X, F, 0);
   double fnorm = sqrt(vdot(n, F, F));
   int reason = converged(s);
+  if (reason > 0 && !ORC_TANGENT_CONVERGED(F, X, n)) reason = 0;
   if (reason) return reason;
   if (fnorm != fnorm) return -1;
   double fminnorm = fnorm;
@@ -404,6 +423,7 @@ static int solve_ngmres(orc_solver *s) {
       update_subspace(s, ivec, l, F, X);
     }
     reason = converged(s);
+    if (reason > 0 && !ORC_TANGENT_CONVERGED(F, X, n)) reason = 0;
     if (reason) return reason;
     if (fnorm != fnorm) return -1;
   }
@@ -416,6 +436,7 @@ static int solve_richardson(orc_solver *s) {
   for (int k = 0; k <= s->num.snes_max_it; k++) {
     form_residual(s, X, F, k);
     int reason = converged(s);
+    if (reason > 0 && !ORC_TANGENT_CONVERGED(F, X, n)) reason = 0;
     if (reason) return reason;
     for (size_t i = 0; i < n; i++) X[i] = X[i] - F[i];
   }

@@ -10,12 +10,15 @@
  */
 #include "oracle_internal.h"
 
-#include <complex.h>
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
 
+/* the tangent-linear build (oracle_fwd.cpp) supplies its own complex type */
+#ifndef ORC_CPLX_DEFINED
+#include <complex.h>
 typedef double complex cplx;
+#endif
 
 struct orc_grid {
   int n[3], h;            /* cells, N1/2+1 */

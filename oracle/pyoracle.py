@@ -13,6 +13,7 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "liboracle.so")
+LIB_FWD = os.path.join(HERE, "liboracle_fwd.so")      # tangent-linear (dual-number) build
 
 NSL_MAX = 48
 
@@ -282,3 +283,63 @@ class Problem:
         self.t = float(t[0])
         return {"rc": rc, "stats": stats.reshape(-1, 4), "P_av": P.reshape(-1, 9),
                 "F_aim": Fa.reshape(-1, 9), "n_updates": nu.value}
+
+
+_fwd = None
+
+
+def fwd_lib():
+    """the oracle's tangent-linear build (oracle_fwd.cpp)"""
+    global _fwd
+    if _fwd is None:
+        if not os.path.exists(LIB_FWD):
+            subprocess.run(["make", "-s", "-C", HERE, "liboracle_fwd.so"], check=True, stdout=subprocess.DEVNULL)
+        L = C.CDLL(LIB_FWD)
+        dp = C.POINTER(C.c_double)
+        L.orc_fwd_run_loadcase.restype = C.c_int
+        L.orc_fwd_run_loadcase.argtypes = [C.POINTER(C.c_int), dp, C.c_int, C.POINTER(PhaseDesc),
+                                           C.POINTER(PhaseDesc), C.POINTER(C.c_int32), dp, C.POINTER(NumMech),
+                                           C.POINTER(NumGrid), C.c_int, C.POINTER(LoadCase), C.c_double, dp,
+                                           dp, dp, C.POINTER(C.c_int64)]
+        _fwd = L
+    return _fwd
+
+
+def seed_desc(field: str, index: int = 0) -> PhaseDesc:
+    """unit tangent on one parameter of a phase descriptor (e.g. ("xi_0", 0),
+    ("h_sl_sl", 3), ("C_11", 0)); every other parameter held fixed"""
+    d = PhaseDesc()
+    v = getattr(d, field)
+    if isinstance(v, float):
+        setattr(d, field, 1.0)
+    else:
+        v[index] = 1.0
+    return d
+
+
+def tangent_loadcase(grid, material, numerics, lc, seeds, cref_rule: int = 1, threads: int = 0,
+                     tangent_rtol: float = 1e-10):
+    """One load case from a fresh state through the dual-number build: P_av per
+    increment (the plain oracle's values, up to the extra iterations the
+    derivative may take to settle) and dP_av, its derivative along the
+    parameter direction ``seeds`` (one PhaseDesc per phase, see seed_desc)."""
+    if threads:
+        lib().orc_set_threads(threads)
+    maps = material.cell_maps(grid.material)
+    cells = np.asarray(grid.cells, np.int32)
+    size = np.asarray(grid.size, float)
+    nph = len(material.phases)
+    descs = (PhaseDesc * nph)(*[phase_desc(p) for p in material.phases])
+    sd = (PhaseDesc * nph)(*seeds)
+    phase_of = np.ascontiguousarray(maps["phase"], np.int32)
+    q0 = np.ascontiguousarray(maps["O"], float)
+    nm, ng, c = num_mech(numerics), num_grid(numerics), loadcase(lc)
+    P = np.zeros(9 * lc.N)
+    dP = np.zeros(9 * lc.N)
+    stats = np.zeros(4 * lc.N)
+    nu = C.c_int64(0)
+    rc = fwd_lib().orc_fwd_run_loadcase(cells.ctypes.data_as(C.POINTER(C.c_int)), ptr(size), nph, descs, sd,
+                                        iptr(phase_of), ptr(q0), C.byref(nm), C.byref(ng), cref_rule,
+                                        C.byref(c), tangent_rtol, ptr(P), ptr(dP), ptr(stats), C.byref(nu))
+    return {"rc": rc, "P_av": P.reshape(-1, 9), "dP_av": dP.reshape(-1, 9), "stats": stats.reshape(-1, 4),
+            "n_updates": nu.value}

@@ -116,3 +116,39 @@ def test_adjoint_gradient_matches_oracle_differences(gpu_lib):
         print(f"{names[k]:12s} adjoint {g[k]: .10e}  oracle fd {fd[k]: .10e}  rel {abs(g[k] - fd[k]) / abs(fd[k]):.2e}")
     scale = np.abs(fd * theta0).max()
     assert np.all(np.abs(g - fd) * np.abs(theta0) <= 1e-5 * np.abs(fd * theta0) + 1e-7 * scale)
+
+
+def test_adjoint_gradient_matches_forward_mode(gpu_lib):
+    """The GPU adjoint against the FP64 forward-mode reference gradient: the
+    CPU oracle's tangent-linear (dual-number) build, one parameter direction
+    per run, exact up to the solvers' tolerances (no difference quotient)"""
+    from concurrent.futures import ThreadPoolExecutor
+
+    from damask_amd import adjoint
+    from oracle import pyoracle
+    grid, mat, lcs, num = _case(cells=4, grains=3, N=4)
+    s, P = adjoint.run_curve(grid, mat, lcs, num)
+    n = P.shape[0]
+    w = np.linspace(1.0, 2.0, n) / 1e6
+    g, _ = adjoint.curve_gradient(s, 0, _Pbar(n, w), rtol=1e-12)
+    s.close()
+    ph = mat.phases[0]
+    field = {"xi_0_sl": "xi_0", "xi_inf_sl": "xi_inf", "h_0_sl_sl": "h_0", "n_sl": "n", "a_sl": "a"}
+    seeds = [(field[attr], f) for attr, _, _ in adjoint.FAMILY_PARAMS for f in range(len(ph.N_sl))]
+    seeds += [("h_sl_sl", t) for t in range(len(ph.h_sl_sl))]
+    assert len(seeds) == g.size
+
+    def tangent(sd):
+        r = pyoracle.tangent_loadcase(grid, mat, num, lcs[0], [pyoracle.seed_desc(*sd)], threads=1)
+        assert r["rc"] == 0
+        return _loss(adjoint._P33(r["dP_av"]), w)      # the loss is linear in P_av
+
+    with ThreadPoolExecutor(min(len(seeds), 16)) as ex:
+        fm = np.array(list(ex.map(tangent, seeds)))
+    theta0 = adjoint.get_params(ph)
+    names = adjoint.param_names(ph)
+    for k in range(g.size):
+        print(f"{names[k]:12s} adjoint {g[k]: .10e}  forward mode {fm[k]: .10e}  "
+              f"rel {abs(g[k] - fm[k]) / abs(fm[k]):.2e}")
+    scale = np.abs(fm * theta0).max()
+    assert np.all(np.abs(g - fm) * np.abs(theta0) <= 1e-5 * np.abs(fm * theta0) + 1e-8 * scale)
