@@ -54,6 +54,8 @@ def parse():
     ap.add_argument("--batch", type=int, default=256,
                     help="RVEs of the batched forward + adjoint measurement (0: skip)")
     ap.add_argument("--batch-threads", type=int, default=16)
+    ap.add_argument("--lockstep-threads", type=int, default=32,
+                    help="RVEs in flight of the lockstep batched forward (one host thread each; 0: skip)")
     ap.add_argument("--only-batch", type=int, default=0,
                     help="only run the batched measurement and print it as one JSON line")
     ap.add_argument("--only-adjoint", type=int, default=0,
@@ -160,6 +162,15 @@ def batched_cost(args):
     updates = sum(s.counters()["updates"] for s in b.solvers)
     cells = int(b.cells.sum())
     b.close()
+    lock = None
+    if args.lockstep_threads:
+        b = batch.Batch(items)
+        rl = b.run(threads=args.lockstep_threads, lockstep=True)
+        ul = sum(s.counters()["updates"] for s in b.solvers)
+        b.close()
+        lock = {"threads": args.lockstep_threads, "forward_s": rl["wall_s"], "updates_per_s": ul / rl["wall_s"],
+                "note": "dmk_batch_run_lockstep: the stress integrations of the RVEs in flight share one launch "
+                        "per residual evaluation round"}
     b = batch.Batch(items)
     b.tape()
     rt = b.run(threads=args.batch_threads)
@@ -176,7 +187,8 @@ def batched_cost(args):
             "cells": cells, "threads": args.batch_threads, "forward_s": r["wall_s"],
             "updates_per_s": updates / r["wall_s"], "forward_with_tape_s": rt["wall_s"],
             "backward_s": rb["wall_s"], "adjoint_over_forward": (rt["wall_s"] + rb["wall_s"]) / r["wall_s"],
-            "gmres_rtol": args.adjoint_rtol, "gradient_norm": float(np.sqrt(sum(float(x @ x) for x in g)))}
+            "gmres_rtol": args.adjoint_rtol, "gradient_norm": float(np.sqrt(sum(float(x @ x) for x in g))),
+            "lockstep": lock}
 
 
 def schemes_cost(args):

@@ -53,7 +53,7 @@ EXPORTS = ["dmk_mech_init", "dmk_mech_forward", "dmk_mech_materialpoint_forward"
            "dmk_gamma_convolution", "dmk_divergence_rms", "dmk_fft_forward", "dmk_fft_backward",
            "dmk_mech_profile", "dmk_mech_kernel_time", "dmk_slip_system", "dmk_adjoint_tape",
            "dmk_adjoint_record", "dmk_adjoint_info", "dmk_adjoint_backward", "dmk_adjoint_release",
-           "dmk_adjoint_point_host", "dmk_mech_restart_response", "dmk_grid_set_restart", "dmk_batch_run",
+           "dmk_adjoint_point_host", "dmk_mech_restart_response", "dmk_grid_set_restart", "dmk_batch_run", "dmk_batch_run_lockstep",
            "dmk_batch_adjoint", "dmk_curl_rms", "dmk_inv_sym3333"]
 
 _lib = None
@@ -137,6 +137,8 @@ def load(path: str = LIB_PATH):
     L.dmk_batch_run.restype = C.c_int
     L.dmk_batch_run.argtypes = [C.POINTER(vp), C.c_int32, C.POINTER(LoadCase), C.POINTER(C.c_int64), C.c_int32,
                                 ip, dp]
+    L.dmk_batch_run_lockstep.restype = C.c_int
+    L.dmk_batch_run_lockstep.argtypes = L.dmk_batch_run.argtypes
     L.dmk_batch_adjoint.restype = C.c_int
     L.dmk_batch_adjoint.argtypes = [C.POINTER(vp), C.c_int32, C.POINTER(dp), C.c_double, C.POINTER(dp),
                                     C.POINTER(C.c_int64), C.c_int32, ip, dp]

@@ -58,16 +58,19 @@ class Batch:
         for s, (_, _, lc) in zip(self.solvers, self.items):
             s.adjoint_tape(2 * lc.N + 16)
 
-    def run(self, threads: int | None = None) -> dict:
+    def run(self, threads: int | None = None, lockstep: bool = False) -> dict:
+        """All load cases; with lockstep the RVEs in flight (one per thread)
+        share one stress-integration launch per residual evaluation."""
         lcs = (_lib.LoadCase * len(self.items))(*[loadcase_struct(lc) for _, _, lc in self.items])
         n = len(self.solvers)
         rc = np.zeros(n, np.int32)
         wall = np.zeros(n)
+        fn = self.lib.dmk_batch_run_lockstep if lockstep else self.lib.dmk_batch_run
         t0 = time.perf_counter()
-        r = self.lib.dmk_batch_run(self._h, n, lcs, self.cells.ctypes.data_as(C.POINTER(C.c_int64)),
-                                   threads or self.threads, _lib.iptr(rc), _lib.dptr(wall))
+        r = fn(self._h, n, lcs, self.cells.ctypes.data_as(C.POINTER(C.c_int64)), threads or self.threads,
+               _lib.iptr(rc), _lib.dptr(wall))
         t = time.perf_counter() - t0
-        _lib.check(r, "dmk_batch_run")
+        _lib.check(r, "dmk_batch_run_lockstep" if lockstep else "dmk_batch_run")
         return {"wall_s": t, "rve_wall_s": wall}
 
     def curves(self) -> list[np.ndarray]:

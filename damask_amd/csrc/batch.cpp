@@ -3,27 +3,157 @@
 // (dmk_grid_run_loadcase) and adjoint sweeps concurrently, so that the
 // kernels, copies and host synchronisations of different RVEs overlap.  Work
 // is handed out largest RVE first from a shared counter.
+//
+// Lockstep mode (dmk_batch_run_lockstep): the RVEs in flight also share their
+// stress integrations.  Each residual evaluation submits its (RVE, phase)
+// items to a DmkStressBatch; when every RVE still running has submitted, the
+// last one enqueues one stress_kernel_batch launch per slip-system variant
+// over all of them on the batch's stream, and every RVE's stream waits for it.
+// The many small per-RVE launches, which the few hardware queues serialise,
+// become one launch that fills the GPU.
 #include <algorithm>
 #include <atomic>
 #include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <mutex>
 #include <thread>
 #include <vector>
 
 #include "dmk_common.h"
+#include "dmk_batch.h"
+#include "dmk_linalg.h"
 #include "../../include/damask_hip.h"
 
+struct DmkStressBatch {
+  struct Req {
+    int slot, n;
+    DmkStressItem items[DMK_MAX_PHASES];
+    int key[DMK_MAX_PHASES];
+  };
+  static constexpr int NSTAGE = 4;   // staging buffers of the item tables in flight
+  std::mutex mu;
+  std::condition_variable cv;
+  int active = 0, waiting = 0;
+  uint64_t gen = 0;                  // batches launched
+  int rc[2] = {0, 0};                // launch status of the last two batches
+  hipStream_t st = nullptr;
+  hipEvent_t done[2] = {};           // batch g recorded into done[g & 1]
+  std::vector<hipEvent_t> ev_in;     // per slot: the submitter's input is ready
+  std::vector<Req> reqs;
+  DmkStressItem* host[NSTAGE] = {};
+  DmkStressItem* dev[NSTAGE] = {};
+  hipEvent_t staged[NSTAGE] = {};
+  int cap = 0, stage = 0;
+  int64_t launches = 0;
+};
+
 namespace {
+
+int batch_init(DmkStressBatch* b, int slots) {
+  b->cap = slots * DMK_MAX_PHASES;
+  b->reqs.reserve(slots);
+  if (hipStreamCreateWithFlags(&b->st, hipStreamNonBlocking) != hipSuccess) return DMK_ERR_HIP;
+  for (auto& e : b->done)
+    if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) return DMK_ERR_HIP;
+  b->ev_in.assign(slots, nullptr);
+  for (auto& e : b->ev_in)
+    if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) return DMK_ERR_HIP;
+  for (int s = 0; s < DmkStressBatch::NSTAGE; s++) {
+    if (hipHostMalloc((void**)&b->host[s], sizeof(DmkStressItem) * b->cap, hipHostMallocDefault) != hipSuccess ||
+        hipMalloc((void**)&b->dev[s], sizeof(DmkStressItem) * b->cap) != hipSuccess ||
+        hipEventCreateWithFlags(&b->staged[s], hipEventDisableTiming) != hipSuccess)
+      return DMK_ERR_HIP;
+  }
+  return 0;
+}
+
+void batch_free(DmkStressBatch* b) {
+  if (b->st) hipStreamSynchronize(b->st);
+  for (int s = 0; s < DmkStressBatch::NSTAGE; s++) {
+    if (b->host[s]) hipHostFree(b->host[s]);
+    if (b->dev[s]) hipFree(b->dev[s]);
+    if (b->staged[s]) hipEventDestroy(b->staged[s]);
+  }
+  for (auto& e : b->ev_in) if (e) hipEventDestroy(e);
+  for (auto& e : b->done) if (e) hipEventDestroy(e);
+  if (b->st) hipStreamDestroy(b->st);
+}
+
+// with b->mu held: one launch per slip-system variant over every waiting
+// request, then release the waiters
+void batch_launch_locked(DmkStressBatch* b) {
+  const int s = b->stage;
+  b->stage = (s + 1) % DmkStressBatch::NSTAGE;
+  int rc = 0;
+  if (hipEventSynchronize(b->staged[s]) != hipSuccess) rc = DMK_ERR_HIP;   // slot's previous upload done
+  for (auto& r : b->reqs)
+    if (!rc && hipStreamWaitEvent(b->st, b->ev_in[r.slot], 0) != hipSuccess) rc = DMK_ERR_HIP;
+  struct Group { int key, off, n, blocks; };
+  Group g[4];
+  int ng = 0, off = 0;
+  for (auto& r : b->reqs)
+    for (int i = 0; i < r.n; i++) {
+      int k = 0;
+      while (k < ng && g[k].key != r.key[i]) k++;
+      if (k == ng) {
+        if (ng == 4) { rc = DMK_ERR_CONFIG; break; }
+        g[ng++] = Group{r.key[i], 0, 0, 0};
+      }
+    }
+  for (int k = 0; k < ng && !rc; k++) {
+    g[k].off = off;
+    for (auto& r : b->reqs)
+      for (int i = 0; i < r.n; i++) {
+        if (r.key[i] != g[k].key) continue;
+        DmkStressItem it = r.items[i];
+        it.block0 = g[k].blocks;
+        g[k].blocks += (it.mp.N + DMK_STRESS_WG - 1) / DMK_STRESS_WG;
+        b->host[s][off + g[k].n++] = it;
+      }
+    off += g[k].n;
+  }
+  if (!rc && off > b->cap) rc = DMK_ERR_ARG;
+  if (!rc && off > 0) {
+    if (hipMemcpyAsync(b->dev[s], b->host[s], sizeof(DmkStressItem) * off, hipMemcpyHostToDevice, b->st) !=
+            hipSuccess ||
+        hipEventRecord(b->staged[s], b->st) != hipSuccess)
+      rc = DMK_ERR_HIP;
+    for (int k = 0; k < ng && !rc; k++)
+      if (dmk_launch_stress_batch(b->dev[s] + g[k].off, g[k].n, g[k].blocks, g[k].key / 2, g[k].key & 1, b->st))
+        rc = DMK_ERR_HIP;
+  }
+  if (!rc && hipEventRecord(b->done[b->gen & 1], b->st) != hipSuccess) rc = DMK_ERR_HIP;
+  if (rc) dmk_set_error("lockstep batch: stress launch failed");
+  b->rc[b->gen & 1] = rc;
+  b->gen++;
+  b->launches++;
+  b->reqs.clear();
+  b->waiting = 0;
+  b->cv.notify_all();
+}
+
+void batch_enter(DmkStressBatch* b) {
+  std::lock_guard<std::mutex> lk(b->mu);
+  b->active++;
+}
+
+void batch_leave(DmkStressBatch* b) {
+  std::lock_guard<std::mutex> lk(b->mu);
+  b->active--;
+  if (b->waiting > 0 && b->waiting == b->active) batch_launch_locked(b);
+}
 
 template <class F>
 int run_pool(int n, const std::vector<int>& order, int n_threads, F&& job, int32_t* rc_out, double* wall_s) {
   std::atomic<int> next{0};
-  auto worker = [&]() {
+  auto worker = [&](int slot) {
     for (;;) {
       int k = next.fetch_add(1);
       if (k >= n) break;
       int i = order[k];
       auto t0 = std::chrono::steady_clock::now();
-      int rc = job(i);
+      int rc = job(i, slot);
       auto t1 = std::chrono::steady_clock::now();
       if (rc_out) rc_out[i] = rc;
       if (wall_s) wall_s[i] = std::chrono::duration<double>(t1 - t0).count();
@@ -32,7 +162,7 @@ int run_pool(int n, const std::vector<int>& order, int n_threads, F&& job, int32
   int nt = std::max(1, std::min(n_threads, n));
   std::vector<std::thread> pool;
   pool.reserve(nt);
-  for (int t = 0; t < nt; t++) pool.emplace_back(worker);
+  for (int t = 0; t < nt; t++) pool.emplace_back(worker, t);
   for (auto& t : pool) t.join();
   return 0;
 }
@@ -46,21 +176,79 @@ std::vector<int> largest_first(dmk_handle** h, int n, const int64_t* cells) {
 
 }  // namespace
 
-extern "C" int dmk_batch_run(dmk_handle** h, int32_t n, const dmk_loadcase* lc, const int64_t* cells,
-                             int32_t n_threads, int32_t* rc_out, double* wall_s) {
-  if (!h || !lc || n < 1) { dmk_set_error("dmk_batch_run: invalid argument"); return DMK_ERR_ARG; }
+int dmk_stress_batch_submit(DmkStressBatch* b, int slot, const DmkStressItem* items, const int* key, int n,
+                            hipStream_t st) {
+  if (n < 1 || n > DMK_MAX_PHASES || slot < 0 || slot >= (int)b->ev_in.size()) {
+    dmk_set_error("lockstep batch: bad submission");
+    return DMK_ERR_ARG;
+  }
+  if (hipEventRecord(b->ev_in[slot], st) != hipSuccess) { dmk_set_error("lockstep batch: event record"); return DMK_ERR_HIP; }
+  std::unique_lock<std::mutex> lk(b->mu);
+  DmkStressBatch::Req r;
+  r.slot = slot;
+  r.n = n;
+  std::memcpy(r.items, items, sizeof(DmkStressItem) * n);
+  std::memcpy(r.key, key, sizeof(int) * n);
+  b->reqs.push_back(r);
+  b->waiting++;
+  const uint64_t my = b->gen;
+  if (b->waiting == b->active) batch_launch_locked(b);
+  else b->cv.wait(lk, [&] { return b->gen != my; });
+  // batch `my` is the one that held this request; done[my & 1] is recorded
+  // again only two batches later, which needs this thread's next submission
+  const hipEvent_t ev = b->done[my & 1];
+  const int rc = b->rc[my & 1];
+  lk.unlock();
+  if (rc) return rc;
+  if (hipStreamWaitEvent(st, ev, 0) != hipSuccess) { dmk_set_error("lockstep batch: stream wait"); return DMK_ERR_HIP; }
+  return 0;
+}
+
+static int batch_run(dmk_handle** h, int32_t n, const dmk_loadcase* lc, const int64_t* cells, int32_t n_threads,
+                     int32_t* rc_out, double* wall_s, bool lockstep, const char* name) {
+  if (!h || !lc || n < 1) { dmk_set_error("%s: invalid argument", name); return DMK_ERR_ARG; }
   std::vector<int> order = largest_first(h, n, cells);
   std::vector<int32_t> rc(n, 0);
-  run_pool(n, order, n_threads,
-           [&](int i) { return dmk_grid_run_loadcase(h[i], &lc[i], nullptr, nullptr, nullptr, nullptr, nullptr); },
+  const int nt = std::max(1, std::min<int>(n_threads, n));
+  DmkStressBatch b;
+  if (lockstep) {
+    int e = batch_init(&b, nt);
+    if (e) {
+      batch_free(&b);
+      dmk_set_error("%s: HIP resources of the lockstep batch", name);
+      return e;
+    }
+  }
+  run_pool(n, order, nt,
+           [&](int i, int slot) {
+             if (!lockstep)
+               return dmk_grid_run_loadcase(h[i], &lc[i], nullptr, nullptr, nullptr, nullptr, nullptr);
+             batch_enter(&b);
+             dmk_set_stress_batch(h[i], &b, slot);
+             int r = dmk_grid_run_loadcase(h[i], &lc[i], nullptr, nullptr, nullptr, nullptr, nullptr);
+             dmk_set_stress_batch(h[i], nullptr, -1);
+             batch_leave(&b);
+             return r;
+           },
            rc.data(), wall_s);
+  if (lockstep) batch_free(&b);
   int bad = 0;
   for (int i = 0; i < n; i++) {
     if (rc_out) rc_out[i] = rc[i];
     if (rc[i]) bad++;
   }
-  if (bad) { dmk_set_error("dmk_batch_run: %d of %d RVEs failed", bad, n); return DMK_ERR_NOCONV; }
+  if (bad) { dmk_set_error("%s: %d of %d RVEs failed", name, bad, n); return DMK_ERR_NOCONV; }
   return 0;
+}
+
+extern "C" int dmk_batch_run(dmk_handle** h, int32_t n, const dmk_loadcase* lc, const int64_t* cells,
+                             int32_t n_threads, int32_t* rc_out, double* wall_s) {
+  return batch_run(h, n, lc, cells, n_threads, rc_out, wall_s, false, "dmk_batch_run");
+}
+
+extern "C" int dmk_batch_run_lockstep(dmk_handle** h, int32_t n, const dmk_loadcase* lc, const int64_t* cells,
+                                      int32_t n_threads, int32_t* rc_out, double* wall_s) {
+  return batch_run(h, n, lc, cells, n_threads, rc_out, wall_s, true, "dmk_batch_run_lockstep");
 }
 
 extern "C" int dmk_batch_adjoint(dmk_handle** h, int32_t n, const double* const* P_av_bar, double rtol,
@@ -73,7 +261,7 @@ extern "C" int dmk_batch_adjoint(dmk_handle** h, int32_t n, const double* const*
   std::vector<int> order = largest_first(h, n, cells);
   std::vector<int32_t> rc(n, 0);
   run_pool(n, order, n_threads,
-           [&](int i) { return dmk_adjoint_backward(h[i], P_av_bar[i], rtol, theta_bar[i], nullptr); }, rc.data(),
+           [&](int i, int) { return dmk_adjoint_backward(h[i], P_av_bar[i], rtol, theta_bar[i], nullptr); }, rc.data(),
            wall_s);
   int bad = 0;
   for (int i = 0; i < n; i++) {

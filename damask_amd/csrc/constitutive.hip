@@ -28,7 +28,7 @@
 
 namespace {
 
-constexpr int SWG = 64;   // work-group of the per-cell kernels: one wavefront
+constexpr int SWG = DMK_STRESS_WG;   // work-group of the per-cell kernels: one wavefront
 
 // The stress kernel runs two waves per SIMD: measured on the 64^3 tension
 // workload this is 20 % faster than one wave per SIMD without spills, the
@@ -385,13 +385,12 @@ DI int integrate_state_fpi(const DmkPhaseDev* __restrict__ ph, const DmkNumMech&
   return status;
 }
 
+// one material point update per lane: block `blk` of the cells of mp
 template <int NSL, bool QP>
-__global__ void __launch_bounds__(SWG) __attribute__((amdgpu_waves_per_eu(DMK_STRESS_WAVES))) stress_kernel(DmkMPDev mp, const double* __restrict__ Ftarget,
-                                                     const DmkPhaseDev* __restrict__ ph, int phase_id,
-                                                     DmkNumMech num, double dt) {
-  __shared__ SlipLds<NSL> sh;
+DI void stress_body(const DmkMPDev& mp, const double* __restrict__ Ftarget, const DmkPhaseDev* __restrict__ ph,
+                    int phase_id, const DmkNumMech& num, double dt, int blk, SlipLds<NSL>& sh) {
   const int lane = threadIdx.x;
-  const int e = blockIdx.x * SWG + lane;
+  const int e = blk * SWG + lane;
   if (e >= mp.N) return;
   if (mp.phase[e] != phase_id) return;
   const Lane ln = make_lane(mp.N, e);
@@ -460,6 +459,33 @@ __global__ void __launch_bounds__(SWG) __attribute__((amdgpu_waves_per_eu(DMK_ST
   mp.n_newton[e] = newton;
   mp.n_fpi[e] = fpi;
   mp.n_sub[e] = nsub;
+}
+
+template <int NSL, bool QP>
+__global__ void __launch_bounds__(SWG) __attribute__((amdgpu_waves_per_eu(DMK_STRESS_WAVES)))
+stress_kernel(DmkMPDev mp, const double* __restrict__ Ftarget, const DmkPhaseDev* __restrict__ ph, int phase_id,
+              DmkNumMech num, double dt) {
+  __shared__ SlipLds<NSL> sh;
+  stress_body<NSL, QP>(mp, Ftarget, ph, phase_id, num, dt, blockIdx.x, sh);
+}
+
+// The stress integration of several RVEs (and phases) in one launch: item i
+// owns blocks [block0_i, block0_{i+1}) and runs them on its own material
+// point, phase tables, load and time step, exactly as stress_kernel would.
+template <int NSL, bool QP>
+__global__ void __launch_bounds__(SWG) __attribute__((amdgpu_waves_per_eu(DMK_STRESS_WAVES)))
+stress_kernel_batch(const DmkStressItem* __restrict__ items, int n_items) {
+  __shared__ SlipLds<NSL> sh;
+  const int b = blockIdx.x;
+  int lo = 0, hi = n_items - 1;                     // last item with block0 <= b (uniform)
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (items[mid].block0 <= b) lo = mid; else hi = mid - 1;
+  }
+  const DmkStressItem* it = items + lo;
+  const DmkMPDev mp = it->mp;
+  const DmkNumMech num = it->num;
+  stress_body<NSL, QP>(mp, it->F, it->ph, it->phase_id, num, it->dt, b - it->block0, sh);
 }
 
 // math_rotationalPart (src/math.f90:1004-1043)
@@ -726,6 +752,23 @@ extern "C" int dmk_launch_stress(const DmkMPDev* mp, const double* Ftarget, cons
                 : launch_stress<24, false>(mp, Ftarget, ph, phase_id, num, dt, s); break;
     default: return 1;
   }
+  return hipGetLastError() == hipSuccess ? 0 : 2;
+}
+
+extern "C" int dmk_launch_stress_batch(const DmkStressItem* items_dev, int n_items, int n_blocks, int nsl,
+                                       int quarter_pow, hipStream_t s) {
+  if (n_items < 1 || n_blocks < 1) return 0;
+  dim3 blk(SWG), grd(n_blocks);
+  if (nsl == 12 && quarter_pow)
+    hipLaunchKernelGGL((stress_kernel_batch<12, true>), grd, blk, 0, s, items_dev, n_items);
+  else if (nsl == 12)
+    hipLaunchKernelGGL((stress_kernel_batch<12, false>), grd, blk, 0, s, items_dev, n_items);
+  else if (nsl == 24 && quarter_pow)
+    hipLaunchKernelGGL((stress_kernel_batch<24, true>), grd, blk, 0, s, items_dev, n_items);
+  else if (nsl == 24)
+    hipLaunchKernelGGL((stress_kernel_batch<24, false>), grd, blk, 0, s, items_dev, n_items);
+  else
+    return 1;
   return hipGetLastError() == hipSuccess ? 0 : 2;
 }
 

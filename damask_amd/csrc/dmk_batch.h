@@ -1,0 +1,19 @@
+// Lockstep batches of RVEs (batch.cpp): the stress integrations of the
+// residual evaluations of all RVEs in flight run as one launch.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+struct DmkStressBatch;
+struct DmkStressItem;
+struct dmk_handle;
+
+// Hand the stress integration of one residual evaluation (its items: one per
+// phase, key = 2 nsl + quarter_pow) to the batch.  Called from the RVE's own
+// host thread after its input F has been enqueued on `st`; returns once the
+// batched launch holding these items is enqueued, with `st` made to wait for
+// it.  Blocks until every RVE of the batch still running has submitted.
+int dmk_stress_batch_submit(DmkStressBatch* b, int slot, const DmkStressItem* items, const int* key, int n,
+                            hipStream_t st);
+// attach (b != NULL) or detach a handle (solver.cpp)
+void dmk_set_stress_batch(dmk_handle* h, DmkStressBatch* b, int slot);

@@ -12,6 +12,7 @@
 
 #define DMK_NSL_MAX 24
 #define DMK_MAX_PHASES 8
+#define DMK_STRESS_WG 64       // work-group (one wavefront) of the per-cell kernels
 
 #define DMK_CHECK(x)                                                                   \
   do {                                                                                 \
@@ -106,4 +107,15 @@ struct DmkMPDev {
   double* last1;                 // state-rate history, ping-pong pair
   double* last2;
   double* sub;                   // [36][N]: sub-step F_0, F, F_p0, L_p0
+};
+
+// one (RVE, phase) of a batched stress launch (stress_kernel_batch)
+struct DmkStressItem {
+  DmkMPDev mp;
+  const double* F;               // target F of the RVE's cells (component-major)
+  const DmkPhaseDev* ph;         // device phase tables
+  DmkNumMech num;
+  double dt;
+  int phase_id;
+  int block0;                    // first block of the item in the launch
 };

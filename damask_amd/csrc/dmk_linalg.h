@@ -101,6 +101,11 @@ struct DmkNumMech;
 extern "C" int dmk_launch_stress(const DmkMPDev* mp, const double* Ftarget, const DmkPhaseDev* ph,
                                  const DmkPhaseDev* ph_host, int phase_id, const DmkNumMech* num, double dt,
                                  hipStream_t s);
+struct DmkStressItem;
+// stress integration of several RVEs / phases with the same slip-system count
+// and power evaluation in one launch (items on the device, block0 ascending)
+extern "C" int dmk_launch_stress_batch(const DmkStressItem* items_dev, int n_items, int n_blocks, int nsl,
+                                       int quarter_pow, hipStream_t s);
 extern "C" int dmk_launch_tangent(const DmkMPDev* mp, const DmkPhaseDev* ph, const DmkPhaseDev* ph_host,
                                   int phase_id, double dt, const int* flag, double* dPdF, double* nrm,
                                   hipStream_t s);

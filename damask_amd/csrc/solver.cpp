@@ -28,6 +28,7 @@
 #include "dmk_spectral.h"
 #include "dmk_polar.h"
 #include "dmk_galerkin.h"
+#include "dmk_batch.h"
 #include "../../include/damask_hip.h"
 
 int dmk_build_phase(const dmk_phase_desc* d, DmkPhaseDev* p);
@@ -233,7 +234,16 @@ struct dmk_handle {
   double kt_ms[8] = {0};
   int64_t kt_n[8] = {0};
   bool ev_ready = false;
+  // lockstep batch (batch.cpp): when set, the stress integration of each
+  // residual evaluation joins one launch with the other RVEs of the batch
+  DmkStressBatch* batch = nullptr;
+  int batch_slot = -1;
 };
+
+void dmk_set_stress_batch(dmk_handle* h, DmkStressBatch* b, int slot) {
+  h->batch = b;
+  h->batch_slot = slot;
+}
 
 enum { KT_STRESS = 0, KT_TANGENT = 1, KT_REDUCE = 2, KT_FFT_FWD = 3, KT_GAMMA = 4, KT_FFT_INV = 5, KT_CAVG = 6,
        KT_N = 7 };
@@ -331,8 +341,19 @@ static int tangent_pass(dmk_handle* h, double dt) {
 // swap roles.
 static int response_enqueue(dmk_handle* h, const double* F, double dt, const AimArgs* a) {
   kt_begin(h, KT_STRESS);
-  for (int p = 0; p < h->n_phases; p++)
-    CKL(dmk_launch_stress(&h->mp, F, h->ph_dev + p, &h->ph[p], p, &h->nm, dt, h->st));
+  if (h->batch) {
+    DmkStressItem items[DMK_MAX_PHASES];
+    int key[DMK_MAX_PHASES];
+    for (int p = 0; p < h->n_phases; p++) {
+      items[p] = DmkStressItem{h->mp, F, h->ph_dev + p, h->nm, dt, p, 0};
+      key[p] = 2 * h->ph[p].nsl + (h->ph[p].quarter_pow ? 1 : 0);
+    }
+    int rc = dmk_stress_batch_submit(h->batch, h->batch_slot, items, key, h->n_phases, h->st);
+    if (rc) return rc;
+  } else {
+    for (int p = 0; p < h->n_phases; p++)
+      CKL(dmk_launch_stress(&h->mp, F, h->ph_dev + p, &h->ph[p], p, &h->nm, dt, h->st));
+  }
   kt_end(h, KT_STRESS);
   kt_begin(h, KT_REDUCE);
   CKL(dmk_response_reduce(h->mp.status, h->mp.n_newton, h->mp.n_fpi, h->mp.Pph, h->N, h->wgt, h->partial, h->flag,

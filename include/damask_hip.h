@@ -251,6 +251,12 @@ int dmk_adjoint_point_host(const dmk_phase_desc* d, double dt, const double F[9]
  * returns 950. */
 int dmk_batch_run(dmk_handle** h, int32_t n, const dmk_loadcase* lc, const int64_t* cells, int32_t n_threads,
                   int32_t* rc_out, double* wall_s);
+/* as dmk_batch_run, with the RVEs in flight (one per host thread) in
+ * lockstep: the stress integration of every residual evaluation waits until
+ * each RVE still running has reached its own, and all of them run as one
+ * launch per slip-system variant.  Results are those of dmk_batch_run. */
+int dmk_batch_run_lockstep(dmk_handle** h, int32_t n, const dmk_loadcase* lc, const int64_t* cells,
+                           int32_t n_threads, int32_t* rc_out, double* wall_s);
 /* dmk_adjoint_backward(h[i], P_av_bar[i], rtol, theta_bar[i], NULL) for every i, concurrently */
 int dmk_batch_adjoint(dmk_handle** h, int32_t n, const double* const* P_av_bar, double rtol,
                       double* const* theta_bar, const int64_t* cells, int32_t n_threads, int32_t* rc_out,
