@@ -150,6 +150,11 @@ def adjoint_cost(args):
             "gradient_norm": float(np.linalg.norm(g))}
 
 
+def progress(msg):
+    """one line per finished measurement on stderr (the JSON line stays alone on stdout)"""
+    print(f"bench: {msg}", file=sys.stderr, flush=True)
+
+
 def batched_cost(args):
     """256 independent RVEs (edge 8-32, 10-50 grains, xi_0 and n drawn per RVE,
     tension along a random direction, 40 increments to 2 %): batched forward,
@@ -365,12 +370,16 @@ def main():
                      "avg_launch_ms": stress_ms / max(stress_n, 1)},
     }
     line["roofline_spectral"] = spectral_roofline(kt_all, args.cells)
+    progress(f"forward: {value / 1e6:.1f} M updates/s")
     if args.adjoint:
         line["adjoint"] = adjoint_cost(args)
+        progress(f"adjoint / forward: {line['adjoint']['adjoint_over_forward']:.2f}")
     if args.batch:
         line["batched"] = batched_cost(args)
+        progress(f"batched: {line['batched']['updates_per_s'] / 1e6:.1f} M updates/s")
     if args.schemes:
         line["schemes"] = schemes_cost(args)
+        progress("other schemes timed")
     if args.cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(args)
     else:
