@@ -60,6 +60,8 @@ def parse():
                     help="only run the batched measurement and print it as one JSON line")
     ap.add_argument("--only-adjoint", type=int, default=0,
                     help="only run the adjoint / forward measurement and print it as one JSON line")
+    ap.add_argument("--bcc", type=int, default=1,
+                    help="also time the same workload with the 24-system BCC phase")
     ap.add_argument("--schemes", type=int, default=1,
                     help="also time the spectral_polarization and spectral_Galerkin schemes on the same workload")
     ap.add_argument("--only-schemes", type=int, default=0, help="only the other schemes' timings")
@@ -82,9 +84,9 @@ def copy_calibration(mib):
                       "checksum": float(y[-1])}))
 
 
-def workload(cells, grains, seed, increments, strain):
+def workload(cells, grains, seed, increments, strain, phase=None):
     from damask_amd import config, workloads
-    grid, mat = workloads.polycrystal([cells] * 3, grains, seed)
+    grid, mat = workloads.polycrystal([cells] * 3, grains, seed, phase=phase)
     t = strain / 1.0e-3
     _, lcs = config.parse_load(workloads.tension_x(t, increments))
     return grid, config.parse_material(mat), lcs[0]
@@ -194,6 +196,30 @@ def batched_cost(args):
             "backward_s": rb["wall_s"], "adjoint_over_forward": (rt["wall_s"] + rb["wall_s"]) / r["wall_s"],
             "gmres_rtol": args.adjoint_rtol, "gradient_norm": float(np.sqrt(sum(float(x @ x) for x in g))),
             "lockstep": lock}
+
+
+def bcc_cost(args, n=20):
+    """The headline workload with the BCC phase (cI, N_sl = [12, 12]: 24 slip
+    systems with non-Schmid terms): warm-up, then n timed increments"""
+    from damask_amd import config, workloads
+    from damask_amd.grid import GridMechSpectralBasic
+    grid, mat, lc = workload(args.cells, args.grains, args.seed, args.increments, args.strain,
+                             phase=workloads.BCC_PHASE)
+    s = GridMechSpectralBasic(grid, mat, config.Numerics())
+    if args.warmup:
+        assert s.run_loadcase(sub_loadcase(lc, args.warmup))["rc"] == 0
+    c0 = s.counters()
+    t0 = time.perf_counter()
+    r = s.run_loadcase(sub_loadcase(lc, n))
+    elapsed = time.perf_counter() - t0
+    assert r["rc"] == 0, "bcc: timed increments did not converge"
+    c1 = s.counters()
+    s.close()
+    return {"workload": f"{args.cells}^3 BCC (cI, 24 slip systems, non-Schmid) phenopowerlaw tension, "
+                        f"{args.grains} grains, increments {args.warmup + 1}-{args.warmup + n}",
+            "value": (c1["updates"] - c0["updates"]) / elapsed, "unit": "material-point updates/s",
+            "ms_per_increment": elapsed / n * 1e3,
+            "residual_evaluations_per_increment": (c1["residuals"] - c0["residuals"]) / n}
 
 
 def schemes_cost(args):
@@ -380,6 +406,9 @@ def main():
     if args.schemes:
         line["schemes"] = schemes_cost(args)
         progress("other schemes timed")
+    if args.bcc:
+        line["bcc"] = bcc_cost(args)
+        progress(f"bcc: {line['bcc']['value'] / 1e6:.1f} M updates/s")
     if args.cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(args)
     else:
