@@ -166,7 +166,11 @@ DI void copy9(double* dst, const double* src, const Lane& l) {
 }
 
 // sub-step scratch of a cell ([36][N]): sub-step F_0, sub-step F, F_p0, L_p0
-enum { SUB_F0 = 0, SUB_F = 9, SUB_FP0 = 18, SUB_LP0 = 27 };
+// per-cell scratch of the stress integration ([DMK_SUB_ROWS][N]): sub-step
+// F_0, F, F_p0, L_p0, and the Newton line search's last accepted L_p and
+// last direction (kept out of the registers the 8x8 Newton system needs)
+enum { SUB_F0 = 0, SUB_F = 9, SUB_FP0 = 18, SUB_LP0 = 27, SUB_LSOLD = 36, SUB_LSDIR = 45 };
+static_assert(SUB_LSDIR + 9 == DMK_SUB_ROWS, "sub-step scratch rows");
 
 // integrateStress (src/phase_mechanical.f90:379-584) with F_i = I, L_i = 0.
 // F and F_p0 of the sub-step are read from the sub-step scratch.  On success
@@ -187,10 +191,11 @@ DI int integrate_stress(const DmkPhaseDev* __restrict__ ph, const DmkNumMech& nu
     m_load(Fsub_p, ln, F);
     m_mul(F, invFp0, A);
   }
-  double Lpg[9], Lpg_old[9], dLp[9];
+  double Lpg[9];
   m_load(mp.Lp, ln, Lpg);
-#pragma unroll
-  for (int k = 0; k < 9; k++) { Lpg_old[k] = Lpg[k]; dLp[k] = 0.0; }
+  // the first iteration is always accepted, so both are written before they are read
+  double* ls_old = mp.sub + SUB_LSOLD * ln.sN;
+  double* ls_dir = mp.sub + SUB_LSDIR * ln.sN;
   double step = 1.0, res_old = DBL_MAX;
   int it = 0;
   for (;;) {
@@ -218,12 +223,12 @@ DI int integrate_stress(const DmkPhaseDev* __restrict__ ph, const DmkNumMech& nu
     if (nres < res_old) {
       res_old = nres;
 #pragma unroll
-      for (int k = 0; k < 9; k++) Lpg_old[k] = Lpg[k];
+      for (int k = 0; k < 9; k++) st(ls_old, ln, k, Lpg[k]);
       step = 1.0;
     } else {
       step = num.r_linesearch_Lp * step;
 #pragma unroll
-      for (int k = 0; k < 9; k++) Lpg[k] = fma(dLp[k], step, Lpg_old[k]);
+      for (int k = 0; k < 9; k++) Lpg[k] = fma(ld(ls_dir, ln, k), step, ld(ls_old, ln, k));
       continue;
     }
     // Jacobian, rebuilt every iteration (f_update_jacobi_Lp = 1):
@@ -280,7 +285,11 @@ DI int integrate_stress(const DmkPhaseDev* __restrict__ ph, const DmkNumMech& nu
     if (dmk_qr_solve<8>(J, res)) return ST_FAIL_LP_DGESV;
     res[8] = tr - res[0] - res[4];
 #pragma unroll
-    for (int k = 0; k < 9; k++) { dLp[k] = -res[k]; Lpg[k] = fma(dLp[k], step, Lpg[k]); }
+    for (int k = 0; k < 9; k++) {
+      const double d = -res[k];
+      st(ls_dir, ln, k, d);
+      Lpg[k] = fma(d, step, Lpg[k]);
+    }
   }
 
   double invFp_new[9], Fp_new[9];

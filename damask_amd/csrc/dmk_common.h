@@ -13,6 +13,7 @@
 #define DMK_NSL_MAX 24
 #define DMK_MAX_PHASES 8
 #define DMK_STRESS_WG 64       // work-group (one wavefront) of the per-cell kernels
+#define DMK_SUB_ROWS 54        // rows of the per-cell stress-integration scratch (DmkMPDev::sub)
 
 #define DMK_CHECK(x)                                                                   \
   do {                                                                                 \
@@ -106,7 +107,7 @@ struct DmkMPDev {
   double* st0s;                  // state at the start of the sub-step (after the first sub-step)
   double* last1;                 // state-rate history, ping-pong pair
   double* last2;
-  double* sub;                   // [36][N]: sub-step F_0, F, F_p0, L_p0
+  double* sub;                   // [DMK_SUB_ROWS][N]: sub-step F_0, F, F_p0, L_p0, line-search state
 };
 
 // one (RVE, phase) of a batched stress launch (stress_kernel_batch)

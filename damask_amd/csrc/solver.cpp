@@ -1527,7 +1527,7 @@ static int init_impl(dmk_handle* h, const int32_t cells[3], const double size[3]
   for (auto pp : fs) if ((rc = dalloc(h, pp, (size_t)ns * N))) return rc;
   double** f2s[] = {&mp.st0s, &mp.last1, &mp.last2};
   for (auto pp : f2s) if ((rc = dalloc(h, pp, (size_t)2 * ns * N))) return rc;
-  if ((rc = dalloc(h, &mp.sub, 36 * N))) return rc;
+  if ((rc = dalloc(h, &mp.sub, (size_t)DMK_SUB_ROWS * N))) return rc;
   if ((rc = dalloc(h, &mp.O, 4 * N))) return rc;
   if ((rc = dalloc(h, &mp.status, N))) return rc;
   if ((rc = dalloc(h, &mp.n_newton, N))) return rc;
