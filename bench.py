@@ -25,11 +25,10 @@ sys.path.insert(0, ROOT)
 
 PEAK_FP64_TFLOPS = 78.6      # MI355X FP64 (vector and matrix)
 PEAK_HBM_GBS = 8000.0        # MI355X HBM3E
-PEAK_HBM_GBS = 8000.0
 # HBM bytes per stress-kernel launch on this workload: rocprofv3 --pmc
 # FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 (factors from a plain device copy), see
 # profiles/kernels_64cube_fcc_tension.md
-STRESS_TRAFFIC_BYTES = 2246378705.9
+STRESS_TRAFFIC_BYTES = 2194606584.7
 
 
 def parse():
