@@ -392,7 +392,12 @@ def main():
                      "traffic": args.traffic_bytes or None,
                      "algorithmic_bytes_per_launch": flops.stress_kernel_bytes(nsl) * args.cells**3,
                      "flops_per_launch": fl / max(stress_n, 1),
-                     "avg_launch_ms": stress_ms / max(stress_n, 1)},
+                     "avg_launch_ms": stress_ms / max(stress_n, 1),
+                     # measured HBM bytes per launch over the launch time, against 8 TB/s
+                     "traffic_GBs": (args.traffic_bytes / (stress_ms / max(stress_n, 1) * 1e-3) / 1e9
+                                     if args.traffic_bytes else None),
+                     "traffic_frac_of_hbm_peak": (args.traffic_bytes / (stress_ms / max(stress_n, 1) * 1e-3) / 1e9
+                                                  / PEAK_HBM_GBS if args.traffic_bytes else None)},
     }
     line["roofline_spectral"] = spectral_roofline(kt_all, args.cells)
     progress(f"forward: {value / 1e6:.1f} M updates/s")
