@@ -12,7 +12,11 @@
 // point, NGMRES or Richardson as the numerics choose), and each spectral
 // solve stops only once the derivative part of its residual has also fallen
 // below tangent_rtol of the derivative field: the derivative of a convergent
-// fixed-point iteration converges to the derivative of its fixed point.
+// fixed-point iteration converges to the derivative of its fixed point.  The
+// material point's own iterations stop on their values, so the derivative
+// residual levels off at a small fraction of the field, larger for parameters
+// of little influence (1e-8 for an h_sl-sl entry of the 24-system BCC phase);
+// 1e-7 is the default, ample for checks at 1e-5.
 #include <float.h>
 #include <math.h>
 #include <omp.h>
@@ -27,7 +31,7 @@
 namespace orc_fwd {
 #pragma omp declare reduction(+ : Dual : omp_out += omp_in) initializer(omp_priv = Dual(0.0))
 
-double tangent_rtol = 1e-10;
+double tangent_rtol = 1e-7;
 
 // max |r'| <= tangent_rtol max |x'| over the solver's field and residual
 inline int tangent_converged(const Dual* r, const Dual* x, size_t n) {

@@ -318,7 +318,7 @@ def seed_desc(field: str, index: int = 0) -> PhaseDesc:
 
 
 def tangent_loadcase(grid, material, numerics, lc, seeds, cref_rule: int = 1, threads: int = 0,
-                     tangent_rtol: float = 1e-10):
+                     tangent_rtol: float = 1e-7):
     """One load case from a fresh state through the dual-number build: P_av per
     increment (the plain oracle's values, up to the extra iterations the
     derivative may take to settle) and dP_av, its derivative along the

@@ -11,7 +11,8 @@ pytestmark = pytest.mark.gpu
 
 def _case(cells=8, grains=4, seed=3, t=8.0, N=5, lattice="cF"):
     from damask_amd import config, workloads
-    grid, mat = workloads.polycrystal([cells] * 3, grains, seed)
+    phase = workloads.BCC_PHASE if lattice == "cI" else None
+    grid, mat = workloads.polycrystal([cells] * 3, grains, seed, phase=phase)
     mat = config.parse_material(mat)
     mat.phases[0].atol_xi, mat.phases[0].atol_gamma = 1e-8, 1e-14
     _, lcs = config.parse_load(workloads.tension_x(t, N))
@@ -118,15 +119,17 @@ def test_adjoint_gradient_matches_oracle_differences(gpu_lib):
     assert np.all(np.abs(g - fd) * np.abs(theta0) <= 1e-5 * np.abs(fd * theta0) + 1e-7 * scale)
 
 
-def test_adjoint_gradient_matches_forward_mode(gpu_lib):
+@pytest.mark.parametrize("lattice", ["cF", "cI"])
+def test_adjoint_gradient_matches_forward_mode(gpu_lib, lattice):
     """The GPU adjoint against the FP64 forward-mode reference gradient: the
     CPU oracle's tangent-linear (dual-number) build, one parameter direction
-    per run, exact up to the solvers' tolerances (no difference quotient)"""
+    per run, exact up to the solvers' tolerances (no difference quotient).
+    cF: 12 systems; cI: two families of 12 with non-Schmid terms."""
     from concurrent.futures import ThreadPoolExecutor
 
     from damask_amd import adjoint
     from oracle import pyoracle
-    grid, mat, lcs, num = _case(cells=4, grains=3, N=4)
+    grid, mat, lcs, num = _case(cells=4, grains=3, N=4, lattice=lattice)
     s, P = adjoint.run_curve(grid, mat, lcs, num)
     n = P.shape[0]
     w = np.linspace(1.0, 2.0, n) / 1e6
