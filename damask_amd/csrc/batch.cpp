@@ -106,6 +106,7 @@ void batch_launch_locked(DmkStressBatch* b) {
     for (auto& r : b->reqs)
       for (int i = 0; i < r.n; i++) {
         if (r.key[i] != g[k].key) continue;
+        if (off + g[k].n >= b->cap) { rc = DMK_ERR_ARG; break; }   // at most one request per slot
         DmkStressItem it = r.items[i];
         it.block0 = g[k].blocks;
         g[k].blocks += (it.mp.N + DMK_STRESS_WG - 1) / DMK_STRESS_WG;
@@ -113,7 +114,6 @@ void batch_launch_locked(DmkStressBatch* b) {
       }
     off += g[k].n;
   }
-  if (!rc && off > b->cap) rc = DMK_ERR_ARG;
   if (!rc && off > 0) {
     if (hipMemcpyAsync(b->dev[s], b->host[s], sizeof(DmkStressItem) * off, hipMemcpyHostToDevice, b->st) !=
             hipSuccess ||
@@ -199,7 +199,10 @@ int dmk_stress_batch_submit(DmkStressBatch* b, int slot, const DmkStressItem* it
   const hipEvent_t ev = b->done[my & 1];
   const int rc = b->rc[my & 1];
   lk.unlock();
-  if (rc) return rc;
+  if (rc) {
+    dmk_set_error("lockstep batch: stress launch failed");
+    return rc;
+  }
   if (hipStreamWaitEvent(st, ev, 0) != hipSuccess) { dmk_set_error("lockstep batch: stream wait"); return DMK_ERR_HIP; }
   return 0;
 }
