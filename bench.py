@@ -42,8 +42,9 @@ def parse():
     ap.add_argument("--increments", type=int, default=100, help="increments of the full load case")
     ap.add_argument("--strain", type=float, default=0.05)
     ap.add_argument("--cpu-baseline", type=int, default=1)
-    ap.add_argument("--cpu-cells", type=int, default=32)
-    ap.add_argument("--cpu-increments", type=int, default=12)
+    ap.add_argument("--cpu-cells", type=int, default=64, help="grid of the CPU baseline (the headline grid)")
+    ap.add_argument("--cpu-increments", type=int, default=2,
+                    help="timed increments of the CPU baseline, after the same warm-up increments as the GPU")
     ap.add_argument("--traffic-bytes", type=float, default=STRESS_TRAFFIC_BYTES,
                     help="measured HBM bytes per stress-kernel launch (rocprofv3 --pmc); 0 for none")
     ap.add_argument("--adjoint", type=int, default=1,
@@ -199,7 +200,7 @@ def batched_cost(args):
 
 def bcc_cost(args, n=20):
     """The headline workload with the BCC phase (cI, N_sl = [12, 12]: 24 slip
-    systems with non-Schmid terms): warm-up, then n timed increments"""
+    systems, pure Schmid law): warm-up, then n timed increments"""
     from damask_amd import config, workloads
     from damask_amd.grid import GridMechSpectralBasic
     grid, mat, lc = workload(args.cells, args.grains, args.seed, args.increments, args.strain,
@@ -214,7 +215,7 @@ def bcc_cost(args, n=20):
     assert r["rc"] == 0, "bcc: timed increments did not converge"
     c1 = s.counters()
     s.close()
-    return {"workload": f"{args.cells}^3 BCC (cI, 24 slip systems, non-Schmid) phenopowerlaw tension, "
+    return {"workload": f"{args.cells}^3 BCC (cI, 24 slip systems, Schmid law) phenopowerlaw tension, "
                         f"{args.grains} grains, increments {args.warmup + 1}-{args.warmup + n}",
             "value": (c1["updates"] - c0["updates"]) / elapsed, "unit": "material-point updates/s",
             "ms_per_increment": elapsed / n * 1e3,
@@ -288,18 +289,78 @@ def spectral_roofline(kt, cells):
     return out
 
 
+def host_cpu():
+    """model, sockets and cores of this host (lscpu) and the CPUs this process may use"""
+    import subprocess
+    info = {}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=20).stdout
+        for line in out.splitlines():
+            k, _, v = line.partition(":")
+            info[k.strip()] = v.strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
+    avail = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    def num(k):
+        try:
+            return int(info.get(k, ""))
+        except ValueError:
+            return None
+    return {"model": info.get("Model name"), "sockets": num("Socket(s)"), "cores_per_socket": num("Core(s) per socket"),
+            "threads_per_core": num("Thread(s) per core"), "logical_cpus": num("CPU(s)"),
+            "cpus_available": avail}
+
+
 def cpu_baseline(args):
+    """The oracle (C/OpenMP restatement of the same algorithm: same NGMRES, same
+    C_ref rule, same update counter) on the headline grid, in the mode that
+    forms dPdF, the orientation and the stiffness averages once per solve as
+    the GPU path does.  It warms up over the bench's own warm-up increments and
+    times the next --cpu-increments, a slice of the GPU's timed increments.
+    The FFT share is the time of the oracle's transforms (two r2c and one c2r
+    of the nine components per residual evaluation) over the timed wall time."""
+    import ctypes as C
+
+    import numpy as np
     from damask_amd import config
     from oracle import pyoracle
-    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    cpu = host_cpu()
+    threads = min(16, cpu["cpus_available"])
+    if cpu["threads_per_core"] and cpu["threads_per_core"] > 1:
+        threads = max(1, min(threads, cpu["cpus_available"] // cpu["threads_per_core"]))
     grid, mat, lc = workload(args.cpu_cells, args.grains, args.seed, args.increments, args.strain)
-    pr = pyoracle.Problem(grid, mat, config.Numerics(), threads=threads)
+    pr = pyoracle.Problem(grid, mat, config.Numerics(), threads=threads, tangent_once=True)
+    if args.warmup:
+        assert pr.run_loadcase(sub_loadcase(lc, args.warmup))["rc"] == 0, "cpu baseline: warm-up did not converge"
     t0 = time.perf_counter()
     r = pr.run_loadcase(sub_loadcase(lc, args.cpu_increments))
     dt = time.perf_counter() - t0
+    assert r["rc"] == 0, "cpu baseline: timed increments did not converge"
+    residuals = int(r["stats"][:, 3].sum())
+    # the oracle's transforms of the nine components, timed alone
+    L = pyoracle.lib()
+    N = grid.n_cells
+    Nk = (args.cpu_cells // 2 + 1) * args.cpu_cells ** 2
+    x = np.random.default_rng(0).random(9 * N)
+    X = np.zeros(2 * 9 * Nk)
+    t1 = time.perf_counter()
+    for _ in range(3):
+        L.orc_fft_r2c(pr.grid, 9, pyoracle.ptr(x), pyoracle.ptr(X))
+    t_r2c = (time.perf_counter() - t1) / 3
+    t1 = time.perf_counter()
+    for _ in range(3):
+        L.orc_fft_c2r(pr.grid, 9, pyoracle.ptr(X), pyoracle.ptr(x))
+    t_c2r = (time.perf_counter() - t1) / 3
+    pr.close()
+    fft_share = residuals * (2 * t_r2c + t_c2r) / dt
+    first = args.warmup + 1
     return {"value": r["n_updates"] / dt, "unit": "material-point updates/s", "cores": threads, "kind": "port",
-            "sample": f"oracle (C/OpenMP restatement), {args.cpu_cells}^3 cells, {args.grains} grains, increments "
-                      f"1-{args.cpu_increments} of the same tension ({r['n_updates']} updates in {dt:.1f} s)"}
+            "sample": f"oracle (C/OpenMP restatement, dPdF and stiffness averages once per solve as on the GPU), "
+                      f"{args.cpu_cells}^3 cells, {args.grains} grains, increments {first}-"
+                      f"{args.warmup + args.cpu_increments} of the same tension after {args.warmup} warm-up "
+                      f"increments ({r['n_updates']} updates in {dt:.1f} s, {residuals} residual evaluations)",
+            "host": cpu, "openmp_threads": threads, "fft_share": fft_share,
+            "residual_evaluations_per_increment": residuals / args.cpu_increments}
 
 
 def main():
@@ -385,8 +446,8 @@ def main():
                    "stress_kernel_ms": round(kt["stress"][0], 3),
                    "kernel_ms_per_increment": breakdown,
                    "cells_x_increments_per_s": args.cells**3 * args.steps / elapsed},
-        "roofline": {"bound": "mfma", "compute": "fp64 vector ALU (MI355X fp64 vector and matrix peaks "
-                                                 "are both 78.6 TFLOP/s)",
+        "roofline": {"bound": "fp64-valu", "compute": "fp64 vector ALU, no MFMA on the path (MI355X fp64 vector "
+                                                      "peak 78.6 TFLOP/s)",
                      "kernel": "stress_kernel<12, true>", "achieved": achieved,
                      "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS,
                      "traffic": args.traffic_bytes or None,

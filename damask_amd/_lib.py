@@ -54,7 +54,7 @@ EXPORTS = ["dmk_mech_init", "dmk_mech_forward", "dmk_mech_materialpoint_forward"
            "dmk_mech_profile", "dmk_mech_kernel_time", "dmk_slip_system", "dmk_adjoint_tape",
            "dmk_adjoint_record", "dmk_adjoint_info", "dmk_adjoint_backward", "dmk_adjoint_release",
            "dmk_adjoint_point_host", "dmk_mech_restart_response", "dmk_grid_set_restart", "dmk_batch_run", "dmk_batch_run_lockstep",
-           "dmk_batch_adjoint", "dmk_curl_rms", "dmk_inv_sym3333"]
+           "dmk_batch_adjoint", "dmk_curl_rms", "dmk_inv_sym3333", "dmk_mech_cell_counters", "dmk_adjoint_tape_segmented"]
 
 _lib = None
 
@@ -97,6 +97,8 @@ def load(path: str = LIB_PATH):
     L.dmk_grid_run_loadcase.argtypes = [vp, C.POINTER(LoadCase), dp, dp, dp, dp, C.POINTER(C.c_int64)]
     L.dmk_mech_counters.restype = C.c_int
     L.dmk_mech_counters.argtypes = [vp, C.POINTER(C.c_int64)]
+    L.dmk_mech_cell_counters.restype = C.c_int
+    L.dmk_mech_cell_counters.argtypes = [vp, ip, ip, ip]
     L.dmk_mech_destroy.restype = None
     L.dmk_mech_destroy.argtypes = [vp]
     L.dmk_last_error.restype = C.c_char_p
@@ -122,10 +124,11 @@ def load(path: str = LIB_PATH):
     L.dmk_mech_kernel_time.argtypes = [vp, C.c_int32, dp, C.POINTER(C.c_int64)]
     L.dmk_slip_system.restype = C.c_int
     L.dmk_slip_system.argtypes = [C.c_int32, C.c_int32, ip, C.c_int32, dp]
-    for f in ("dmk_adjoint_tape", "dmk_adjoint_record", "dmk_adjoint_info", "dmk_adjoint_backward",
+    for f in ("dmk_adjoint_tape", "dmk_adjoint_tape_segmented", "dmk_adjoint_record", "dmk_adjoint_info", "dmk_adjoint_backward",
               "dmk_adjoint_release", "dmk_adjoint_point_host"):
         getattr(L, f).restype = C.c_int
     L.dmk_adjoint_tape.argtypes = [vp, C.c_int32]
+    L.dmk_adjoint_tape_segmented.argtypes = [vp, C.c_int32, C.c_int32]
     L.dmk_adjoint_record.argtypes = [vp]
     L.dmk_adjoint_info.argtypes = [vp, ip, dp, dp]
     L.dmk_adjoint_backward.argtypes = [vp, dp, C.c_double, dp, dp]

@@ -74,12 +74,13 @@ def _P9(P33: np.ndarray) -> np.ndarray:
 
 
 def run_curve(grid, material: Material, loadcases, numerics: Numerics | None = None,
-              max_increments: int | None = None) -> tuple[GridMechSpectralBasic, np.ndarray]:
+              max_increments: int | None = None, segment: int = 0) -> tuple[GridMechSpectralBasic, np.ndarray]:
     """Forward run with a tape; returns the solver (holding the tape) and the
-    volume-averaged P of every recorded increment, (n, 3, 3)."""
+    volume-averaged P of every recorded increment, (n, 3, 3).  segment > 0:
+    checkpoints every segment increments instead of the full tape."""
     s = GridMechSpectralBasic(grid, material, numerics)
     cap = max_increments or 2 * sum(lc.N for lc in loadcases) + 16
-    s.adjoint_tape(cap)
+    s.adjoint_tape(cap, segment)
     for lc in loadcases:
         r = s.run_loadcase(lc)
         if r["rc"] != 0:

@@ -54,9 +54,11 @@ class Batch:
         self._h = (C.c_void_p * len(self.solvers))(*[s.h.value for s in self.solvers])
         self.threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
 
-    def tape(self):
+    def tape(self, segment: int = 0):
+        """Start the adjoint tapes; segment > 0: checkpoints every segment
+        increments and recomputation in the reverse sweep (see adjoint_tape)."""
         for s, (_, _, lc) in zip(self.solvers, self.items):
-            s.adjoint_tape(2 * lc.N + 16)
+            s.adjoint_tape(lc.N if segment else 2 * lc.N + 16, segment)
 
     def run(self, threads: int | None = None, lockstep: bool = False) -> dict:
         """All load cases; with lockstep the RVEs in flight (one per thread)

@@ -7,7 +7,8 @@ spectral-basic phenopowerlaw path consumes:
   ``src/material.f90:95-218``;
 * phenopowerlaw + Hooke parameters, ``src/phase_mechanical_plastic_phenopowerlaw.f90:146-285``
   and ``src/phase_mechanical_elastic.f90:58-79``;
-* load-case parsing incl. masks ('x'), ``src/grid/DAMASK_grid.f90:472-718``;
+* load-case parsing incl. masks ('x') and the load-case checks,
+  ``src/grid/DAMASK_grid.f90:472-718`` (checks ``:661-673``);
 * numerics defaults, ``src/grid/grid_mech_spectral_basic.f90:130-136``,
   ``src/phase_mechanical.f90:281-295``, ``src/grid/DAMASK_grid.f90:147-150``.
 
@@ -293,6 +294,9 @@ def parse_load(text_or_dict) -> tuple[str, list[LoadCase]]:
                 stress_type = STRESS_TYPES[key]
                 sv, sm = _masked(val)
         q_R = ax2qu_degrees(mech.get("R", [0.0, 0.0, 1.0, 0.0]))
+        if solver == "spectral_Galerkin" and "R" in mech:
+            raise ConfigError(830, f"load case {l} contains rotation R, which is not supported when using "
+                                   "the Galerkin solver")
         if def_type is None:
             raise ConfigError(830, f"load case {l} is incomplete: L, F_dot/dot_F, or F missing")
         if np.any(sm == dm):
@@ -313,6 +317,8 @@ def parse_load(text_or_dict) -> tuple[str, list[LoadCase]]:
             raise ConfigError(830, f"non-positive number of increments in load case {l}")
         if lc.f_out < 1:
             raise ConfigError(830, f"non-positive result frequency in load case {l}")
+        if lc.f_restart < 1:
+            raise ConfigError(830, f"non-positive restart frequency in load case {l}")
         cases.append(lc)
     return solver, cases
 

@@ -165,21 +165,24 @@ DI void copy9(double* dst, const double* src, const Lane& l) {
   for (int c = 0; c < 9; c++) st(dst, l, c, ld(src, l, c));
 }
 
-// sub-step scratch of a cell ([36][N]): sub-step F_0, sub-step F, F_p0, L_p0
 // per-cell scratch of the stress integration ([DMK_SUB_ROWS][N]): sub-step
-// F_0, F, F_p0, L_p0, and the Newton line search's last accepted L_p and
-// last direction (kept out of the registers the 8x8 Newton system needs)
-enum { SUB_F0 = 0, SUB_F = 9, SUB_FP0 = 18, SUB_LP0 = 27, SUB_LSOLD = 36, SUB_LSDIR = 45 };
-static_assert(SUB_LSDIR + 9 == DMK_SUB_ROWS, "sub-step scratch rows");
+// F_0, sub-step F, F_p0 and L_p0 (the last two once a sub-step has completed),
+// and the Newton line search's last accepted L_p
+enum { SUB_F0 = 0, SUB_F = 9, SUB_FP0 = 18, SUB_LP0 = 27, SUB_LSOLD = 36 };
+static_assert(SUB_LSOLD + 9 <= DMK_SUB_ROWS, "sub-step scratch rows");
 
-// integrateStress (src/phase_mechanical.f90:379-584) with F_i = I, L_i = 0.
-// F and F_p0 of the sub-step are read from the sub-step scratch.  On success
-// writes P, S, L_p, F_p (det-regularised) and F_e of the cell and returns S.
+// integrateStress (src/phase_mechanical.f90:379-584) with F_i = I, L_i = 0, for
+// the sub-step F (scratch row SUB_F) and sub-step F_p0 (Fp0_p), starting from
+// the L_p in mp.Lp (the previous call's result: phase_mechanical_Lp is not
+// restored between calls).  On success the converged L_p goes back to mp.Lp
+// and S is returned in registers.  P, F_p and F_e follow from (F, F_p0, L_p, S)
+// alone, so they are formed once, by finish_point, after the state integration
+// has converged instead of after every call.
 template <int NSL, bool QP>
 DI int integrate_stress(const DmkPhaseDev* __restrict__ ph, const DmkNumMech& num, const DmkMPDev& mp,
-                        const Lane& ln, double dt, SlipLds<NSL>& sh, int lane, int& newton, double* S) {
+                        const Lane& ln, const double* Fp0_p, double dt, SlipLds<NSL>& sh, int lane, int& newton,
+                        double* S) {
   const double* Fsub_p = mp.sub + SUB_F * ln.sN;
-  const double* Fp0_p = mp.sub + SUB_FP0 * ln.sN;
   const double* xs = &sh.xi[0][lane];
   double* gs = &sh.g[0][lane];
   double* ws = &sh.w[0][lane];
@@ -193,9 +196,13 @@ DI int integrate_stress(const DmkPhaseDev* __restrict__ ph, const DmkNumMech& nu
   }
   double Lpg[9];
   m_load(mp.Lp, ln, Lpg);
-  // the first iteration is always accepted, so both are written before they are read
+  // the line search's last accepted iterate lives in the scratch (read back
+  // only when a step is rejected, and to form the next iterate), its last
+  // direction in registers: the first iteration is always accepted
   double* ls_old = mp.sub + SUB_LSOLD * ln.sN;
-  double* ls_dir = mp.sub + SUB_LSDIR * ln.sN;
+  double dir[9];
+#pragma unroll
+  for (int k = 0; k < 9; k++) dir[k] = 0.0;
   double step = 1.0, res_old = DBL_MAX;
   int it = 0;
   for (;;) {
@@ -228,7 +235,7 @@ DI int integrate_stress(const DmkPhaseDev* __restrict__ ph, const DmkNumMech& nu
     } else {
       step = num.r_linesearch_Lp * step;
 #pragma unroll
-      for (int k = 0; k < 9; k++) Lpg[k] = fma(ld(ls_dir, ln, k), step, ld(ls_old, ln, k));
+      for (int k = 0; k < 9; k++) Lpg[k] = fma(dir[k], step, ld(ls_old, ln, k));
       continue;
     }
     // Jacobian, rebuilt every iteration (f_update_jacobi_Lp = 1):
@@ -284,58 +291,83 @@ DI int integrate_stress(const DmkPhaseDev* __restrict__ ph, const DmkNumMech& nu
     }
     if (dmk_qr_solve<8>(J, res)) return ST_FAIL_LP_DGESV;
     res[8] = tr - res[0] - res[4];
+    // the accepted iterate was just stored: L_p is read back instead of being
+    // kept live across the solve
 #pragma unroll
     for (int k = 0; k < 9; k++) {
-      const double d = -res[k];
-      st(ls_dir, ln, k, d);
-      Lpg[k] = fma(d, step, Lpg[k]);
+      dir[k] = -res[k];
+      Lpg[k] = fma(dir[k], step, ld(ls_old, ln, k));
     }
   }
-
-  double invFp_new[9], Fp_new[9];
+  // the reference's error exit when the new plastic deformation is singular
+  // (math_invert33 of F_p0^-1 (I - dt L_p))
   {
-    double Fp0[9], invFp0[9], B[9];
+    double Fp0[9], invFp0[9], B[9], invFp_new[9], Fp_new[9];
     m_load(Fp0_p, ln, Fp0);
     m_invert(Fp0, invFp0);
 #pragma unroll
     for (int k = 0; k < 9; k++) B[k] = ((k % 4) == 0 ? 1.0 : 0.0) - dt * Lpg[k];
     m_mul(invFp0, B, invFp_new);
+    if (m_invert(invFp_new, Fp_new)) return ST_FAIL_STRESS;
   }
-  if (m_invert(invFp_new, Fp_new)) return ST_FAIL_STRESS;
-  double T1[9], P[9];
-  {
-    double F[9], T2[9];
-    m_load(Fsub_p, ln, F);
-    m_mul(F, invFp_new, T1);
-    m_mul_nt(S, invFp_new, T2);
-    m_mul(T1, T2, P);
-  }
-  m_store(mp.Pph, ln, P);
-  m_store(mp.S, ln, S);
   m_store(mp.Lp, ln, Lpg);
+  return ST_OK;
+}
+
+// The end of integrateStress (src/phase_mechanical.f90:571-581) for the
+// converged L_p (mp.Lp) and S (mp.S) of a sub-step with F (scratch row SUB_F),
+// F_p0 (Fp0_p) and time step dt: the regularised F_p = F_p,new / det(F_p,new)^(1/3)
+// goes to Fp_dst; with OUT also P = F F_p,new^-1 S F_p,new^-T and F_e = F F_p,new^-1.
+template <bool OUT>
+DI void finish_point(const DmkMPDev& mp, const Lane& ln, const double* Fp0_p, double dt, double* Fp_dst) {
+  double invFp_new[9], Fp_new[9];
+  {
+    double Fp0[9], invFp0[9], B[9], Lp[9];
+    m_load(Fp0_p, ln, Fp0);
+    m_invert(Fp0, invFp0);
+    m_load(mp.Lp, ln, Lp);
+#pragma unroll
+    for (int k = 0; k < 9; k++) B[k] = ((k % 4) == 0 ? 1.0 : 0.0) - dt * Lp[k];
+    m_mul(invFp0, B, invFp_new);
+  }
+  m_invert(invFp_new, Fp_new);
+  if (OUT) {
+    double T1[9], P[9];
+    {
+      double F[9], T2[9], S[9];
+      m_load(mp.sub + SUB_F * ln.sN, ln, F);
+      m_load(mp.S, ln, S);
+      m_mul(F, invFp_new, T1);
+      m_mul_nt(S, invFp_new, T2);
+      m_mul(T1, T2, P);
+    }
+    m_store(mp.Pph, ln, P);
+    m_store(mp.Fe, ln, T1);
+  }
   double d = pow_ool(m_det(Fp_new), 1.0 / 3.0);
 #pragma unroll
   for (int k = 0; k < 9; k++) Fp_new[k] = Fp_new[k] / d;
-  m_store(mp.Fp, ln, Fp_new);
-  m_store(mp.Fe, ln, T1);
-  return ST_OK;
+  m_store(Fp_dst, ln, Fp_new);
 }
 
 // integrateStateFPI (src/phase_mechanical.f90:591-676).  State = (xi, gamma):
 // xi in LDS, gamma in mp.gam; its start values xi0_p / gam0_p (the converged
 // increment, or st0s after a completed sub-step); the two previous state
-// rates (dotState_last) in the ping-pong buffers mp.last1 / mp.last2.
+// rates (dotState_last) in the ping-pong buffers mp.last1 / mp.last2.  The
+// initial rate is taken at the stress S_p (S_0 after a restore, else the
+// previous sub-step's S); on success the S of the converged iterate is stored
+// in mp.S.
 template <int NSL, bool QP>
 DI int integrate_state_fpi(const DmkPhaseDev* __restrict__ ph, const DmkNumMech& num, const DmkMPDev& mp,
-                           const Lane& ln, const double* xi0_p, const double* gam0_p, double dt,
-                           SlipLds<NSL>& sh, int lane, int& newton, int& fpi) {
+                           const Lane& ln, const double* xi0_p, const double* gam0_p, const double* Fp0_p,
+                           const double* S_p, double dt, SlipLds<NSL>& sh, int lane, int& newton, int& fpi) {
   double* xs = &sh.xi[0][lane];
   double* gs = &sh.g[0][lane];
   double* ws = &sh.w[0][lane];
   {
-    double S[9];
-    m_load(mp.S, ln, S);
-    kinetics<NSL, QP, false, false>(ph, S, xs, gs, nullptr, nullptr);
+    double S0[9];
+    m_load(S_p, ln, S0);
+    kinetics<NSL, QP, false, false>(ph, S0, xs, gs, nullptr, nullptr);
   }
   if (dot_state<NSL, QP>(ph, xs, gs, ws)) return ST_FAIL_STATE;
 #pragma unroll 1
@@ -351,9 +383,12 @@ DI int integrate_state_fpi(const DmkPhaseDev* __restrict__ ph, const DmkNumMech&
   for (int it = 1; it <= num.N_iter_state_max; it++) {
     fpi++;
     double S[9];
-    status = integrate_stress<NSL, QP>(ph, num, mp, ln, dt, sh, lane, newton, S);
+    status = integrate_stress<NSL, QP>(ph, num, mp, ln, Fp0_p, dt, sh, lane, newton, S);
     if (status != ST_OK) break;
-    // the converged Newton iterate evaluated kinetics_sl at this (S, xi): rates are in gs
+    m_store(mp.S, ln, S);
+    // the converged Newton iterate evaluated kinetics_sl at this (S, xi): rates
+    // are in gs.  A NaN rate ends the iteration with the status of the stress
+    // integration, as the reference's exit does (phase_mechanical.f90:638)
     if (dot_state<NSL, QP>(ph, xs, gs, ws)) break;
     const double* w1p = opaque(cur ? mp.last2 : mp.last1);
     double* w2p = opaque(cur ? mp.last1 : mp.last2);
@@ -408,16 +443,17 @@ DI void stress_body(const DmkMPDev& mp, const double* __restrict__ Ftarget, cons
 
   // isostrain partition (src/homogenization_mechanical_isostrain.f90:60-75)
   copy9(mp.Fphase, Ftarget, ln);
-  // F_p0 of the sub-step always comes from the sub-step scratch; the other
-  // start values are read from the converged increment until a sub-step
-  // completes, then from copies in the sub-step scratch / st0s
-  copy9(sub + SUB_FP0 * ln.sN, mp.Fp0, ln);
+  // start values of the sub-step: those of the converged increment until a
+  // sub-step completes, then copies in the sub-step scratch / st0s
   const double* pF0 = mp.F0;
+  const double* pFp0 = mp.Fp0;
   const double* pLp0 = mp.Lp0;
   const double* pxi0 = mp.xi0;
   const double* pgam0 = mp.gam0;
+  const double* pS = mp.S0;          // stress of the initial state rate
 
   int newton = 0, fpi = 0, nsub = 0;
+  double dt_done = 0.0;              // time step of the last completed sub-step
   // phase_mechanical_constitutive substepping; the first pass takes the
   // cut-back branch, which is phase_restore (src/phase_mechanical.f90:1096-1115)
   double stepFrac = 0.0, step = 1.0 / num.r_cutback;
@@ -429,38 +465,54 @@ DI void stress_body(const DmkMPDev& mp, const double* __restrict__ Ftarget, cons
       step = fmin(1.0 - stepFrac, num.r_increase * step);
       todo = step > 0.0;
       if (todo) {
+        // the completed sub-step's F_p becomes the next one's F_p0
+        finish_point<false>(mp, ln, pFp0, dt_done, sub + SUB_FP0 * ln.sN);
         copy9(sub + SUB_F0 * ln.sN, sub + SUB_F * ln.sN, ln);
         copy9(sub + SUB_LP0 * ln.sN, mp.Lp, ln);
-        copy9(sub + SUB_FP0 * ln.sN, mp.Fp, ln);
 #pragma unroll 1
         for (int r = 0; r < NSL; r++) {
           st(mp.st0s, ln, r, xs[r * SWG]);
           st(mp.st0s, ln, NSL + r, ld(mp.gam, ln, r));
         }
         pF0 = sub + SUB_F0 * ln.sN;
+        pFp0 = sub + SUB_FP0 * ln.sN;
         pLp0 = sub + SUB_LP0 * ln.sN;
         pxi0 = mp.st0s;
         pgam0 = mp.st0s + NSL * ln.sN;
       }
     } else {
+      // cut back: restore the sub-step start (F_p, S = S_0, L_p for an actual
+      // cut-back, state).  F_p, S and gamma are outputs only until they are
+      // read again, so they are restored where they are read: F_p0 and S come
+      // from pFp0 / pS, and gamma is overwritten by the state integration
       step = num.r_cutback * step;
-      copy9(mp.Fp, sub + SUB_FP0 * ln.sN, ln);
-      copy9(mp.S, mp.S0, ln);
+      pS = mp.S0;
       if (step < 1.0) copy9(mp.Lp, pLp0, ln);
 #pragma unroll 1
-      for (int r = 0; r < NSL; r++) {
-        xs[r * SWG] = ld(pxi0, ln, r);
-        st(mp.gam, ln, r, ld(pgam0, ln, r));
-      }
+      for (int r = 0; r < NSL; r++) xs[r * SWG] = ld(pxi0, ln, r);
       todo = step > num.r_cutback_min;
     }
     if (todo) {
 #pragma unroll
       for (int c = 0; c < 9; c++)
         st(sub + SUB_F * ln.sN, ln, c, fma(step, ld(Ftarget, ln, c) - ld(mp.F0, ln, c), ld(pF0, ln, c)));
-      status = integrate_state_fpi<NSL, QP>(ph, num, mp, ln, pxi0, pgam0, step * dt, sh, lane, newton, fpi);
+      status = integrate_state_fpi<NSL, QP>(ph, num, mp, ln, pxi0, pgam0, pFp0, pS, step * dt, sh, lane, newton,
+                                            fpi);
       nsub++;
+      if (status == ST_OK) {
+        pS = mp.S;
+        dt_done = step * dt;
+      }
     }
+  }
+  if (status == ST_OK) {
+    finish_point<true>(mp, ln, pFp0, dt_done, mp.Fp);
+  } else {
+    // the restored sub-step start, as the last cut-back left it
+    copy9(mp.Fp, pFp0, ln);
+    copy9(mp.S, mp.S0, ln);
+#pragma unroll 1
+    for (int r = 0; r < NSL; r++) st(mp.gam, ln, r, ld(pgam0, ln, r));
   }
 #pragma unroll 1
   for (int r = 0; r < NSL; r++) st(mp.xi, ln, r, xs[r * SWG]);

@@ -175,6 +175,15 @@ static void lsq_sym(int m, const double* H, const double* b, double* x) {
 }
 
 // ------------------------------------------------------------------ handle
+// the driver's inputs and state at the start of one increment
+// (DAMASK_grid.f90:318-330): enough to run the increment again
+struct IncStart {
+  dmk_loadcase lc;
+  double t_0 = 0, t = 0, Delta_t = 0, Delta_t_prev = 0;
+  int cutBack = 0, cutBackLevel = 0, guess = 0;
+  bool valid = false;
+};
+
 struct dmk_handle {
   int cells[3];
   double size[3];
@@ -238,6 +247,9 @@ struct dmk_handle {
   // residual evaluation joins one launch with the other RVEs of the batch
   DmkStressBatch* batch = nullptr;
   int batch_slot = -1;
+  // driver state at the start of the increment in progress (run_increment),
+  // kept per recorded increment by the adjoint tape
+  IncStart inc_start;
 };
 
 void dmk_set_stress_batch(dmk_handle* h, DmkStressBatch* b, int slot) {
@@ -1035,12 +1047,38 @@ static int solve_newtonls(dmk_handle* h, int* reason_out) {
 // rule).  It is solved by restarted GMRES.  The local reverse steps
 // (dmk_adjoint.h) give g, then carry the cotangents of F_p and xi to
 // increment n-1 and into the parameters.
+// Solver and material-point state at the end of an increment, enough to run
+// the following increments again (device part in AdjState::ckpt)
+struct HostState {
+  double F_aimDot[9], F_aim[9], F_aim_lastinc[9], P_av[9], P_aim[9];
+  double C_volAvg[81], C_volAvgLastInc[81], C_minMaxAvg[81], C_minMaxAvgLastInc[81], S[81];
+  double C_minMaxAvgRestart[81];
+  double err_div, err_BC, dt, q_R[4];
+  int stress_mask[9];
+  GammaArgs ga;
+  double t, Delta_t, Delta_t_prev;
+  int cutBack, cutBackLevel, totalIncs;
+};
+
 struct AdjState {
   int cap = 0, n = 0;
   size_t slot = 0;                 // doubles per tape entry
-  double* tape = nullptr;          // entries 0..cap: F, F_p, L_p, xi (component-major)
+  double* tape = nullptr;          // entries F, F_p, L_p, xi (component-major): 0..cap, or one segment
   std::vector<double> dt, pav, qR;
   std::vector<int> mask, maxsub;
+  // segmented tape (seg > 0): the tape holds the seg + 1 entries of one
+  // segment of increments (entry k at k - base); a checkpoint of the full
+  // state every seg increments lets the reverse sweep run each earlier
+  // segment again before it is differentiated
+  int seg = 0, base = 0;
+  double* ckpt = nullptr;          // device part of the checkpoints (cap / seg + 1) and of the final state
+  size_t ckpt_slot = 0;
+  std::vector<HostState> ckpt_host;
+  std::vector<IncStart> incs;      // driver state at the start of each recorded increment
+  bool replaying = false;
+  int replay_k = 0;                // the increment being run again
+  int on_tape = 0;                 // segment whose entries the tape holds
+  bool swept = false;              // a segmented reverse sweep ran: the tape no longer ends the run
   int M = 30;                      // GMRES restart length
   GammaArgs ga;                    // Gamma_0 of the adjoint solves (reference medium sym <A>)
   bool work_ready = false;
@@ -1056,7 +1094,94 @@ struct AdjState {
 
 enum { TP_F = 0, TP_FP = 9, TP_LP = 18, TP_XI = 27 };   // tape entry offsets, in fields of N
 
-static double* tape_entry(const dmk_handle* h, int k) { return h->adj->tape + (size_t)k * h->adj->slot; }
+static int run_increment(dmk_handle* h, const dmk_loadcase* lc, double t_0, int* guess, int* its_out, int* nf_out);
+
+static double* tape_entry(const dmk_handle* h, int k) {
+  return h->adj->tape + (size_t)(k - h->adj->base) * h->adj->slot;
+}
+
+// device part of a checkpoint: F, F_lastinc (nx each), the phase F, F_p,
+// L_p, S and the accepted P (9 N each), xi and gamma (nsl_max N each) and the
+// small reduction buffer
+static size_t ckpt_doubles(const dmk_handle* h) {
+  return 2 * (size_t)h->nx + 5 * (size_t)h->n9 + 2 * (size_t)h->nsl_max * h->N + SM_SIZE;
+}
+
+static int ckpt_copy(dmk_handle* h, double* c, bool save) {
+  const size_t N = h->N, nx = h->nx, n9 = h->n9, ns = (size_t)h->nsl_max * N;
+  double* dev[] = {h->F, h->F_lastinc, h->mp.Fphase, h->mp.Fp, h->mp.Lp, h->mp.S, h->P_hom, h->mp.xi, h->mp.gam,
+                   h->small};
+  const size_t n[] = {nx, nx, n9, n9, n9, n9, n9, ns, ns, (size_t)SM_SIZE};
+  size_t off = 0;
+  for (int q = 0; q < 10; q++) {
+    if (save) CK(hipMemcpyAsync(c + off, dev[q], sizeof(double) * n[q], hipMemcpyDeviceToDevice, h->st));
+    else CK(hipMemcpyAsync(dev[q], c + off, sizeof(double) * n[q], hipMemcpyDeviceToDevice, h->st));
+    off += n[q];
+  }
+  return 0;
+}
+
+static int ckpt_save(dmk_handle* h, int i) {
+  AdjState& a = *h->adj;
+  int rc = ckpt_copy(h, a.ckpt + (size_t)i * a.ckpt_slot, true);
+  if (rc) return rc;
+  HostState& q = a.ckpt_host[i];
+  std::memcpy(q.F_aimDot, h->F_aimDot, sizeof q.F_aimDot);
+  std::memcpy(q.F_aim, h->F_aim, sizeof q.F_aim);
+  std::memcpy(q.F_aim_lastinc, h->F_aim_lastinc, sizeof q.F_aim_lastinc);
+  std::memcpy(q.P_av, h->P_av, sizeof q.P_av);
+  std::memcpy(q.P_aim, h->P_aim, sizeof q.P_aim);
+  std::memcpy(q.C_volAvg, h->C_volAvg, sizeof q.C_volAvg);
+  std::memcpy(q.C_volAvgLastInc, h->C_volAvgLastInc, sizeof q.C_volAvgLastInc);
+  std::memcpy(q.C_minMaxAvg, h->C_minMaxAvg, sizeof q.C_minMaxAvg);
+  std::memcpy(q.C_minMaxAvgLastInc, h->C_minMaxAvgLastInc, sizeof q.C_minMaxAvgLastInc);
+  std::memcpy(q.S, h->S, sizeof q.S);
+  std::memcpy(q.C_minMaxAvgRestart, h->C_minMaxAvgRestart, sizeof q.C_minMaxAvgRestart);
+  q.err_div = h->err_div;
+  q.err_BC = h->err_BC;
+  q.dt = h->dt;
+  std::memcpy(q.q_R, h->q_R, sizeof q.q_R);
+  std::memcpy(q.stress_mask, h->stress_mask, sizeof q.stress_mask);
+  q.ga = h->ga;
+  q.t = h->t;
+  q.Delta_t = h->Delta_t;
+  q.Delta_t_prev = h->Delta_t_prev;
+  q.cutBack = h->cutBack;
+  q.cutBackLevel = h->cutBackLevel;
+  q.totalIncs = h->totalIncs;
+  return 0;
+}
+
+static int ckpt_restore(dmk_handle* h, int i) {
+  AdjState& a = *h->adj;
+  int rc = ckpt_copy(h, a.ckpt + (size_t)i * a.ckpt_slot, false);
+  if (rc) return rc;
+  const HostState& q = a.ckpt_host[i];
+  std::memcpy(h->F_aimDot, q.F_aimDot, sizeof q.F_aimDot);
+  std::memcpy(h->F_aim, q.F_aim, sizeof q.F_aim);
+  std::memcpy(h->F_aim_lastinc, q.F_aim_lastinc, sizeof q.F_aim_lastinc);
+  std::memcpy(h->P_av, q.P_av, sizeof q.P_av);
+  std::memcpy(h->P_aim, q.P_aim, sizeof q.P_aim);
+  std::memcpy(h->C_volAvg, q.C_volAvg, sizeof q.C_volAvg);
+  std::memcpy(h->C_volAvgLastInc, q.C_volAvgLastInc, sizeof q.C_volAvgLastInc);
+  std::memcpy(h->C_minMaxAvg, q.C_minMaxAvg, sizeof q.C_minMaxAvg);
+  std::memcpy(h->C_minMaxAvgLastInc, q.C_minMaxAvgLastInc, sizeof q.C_minMaxAvgLastInc);
+  std::memcpy(h->S, q.S, sizeof q.S);
+  std::memcpy(h->C_minMaxAvgRestart, q.C_minMaxAvgRestart, sizeof q.C_minMaxAvgRestart);
+  h->err_div = q.err_div;
+  h->err_BC = q.err_BC;
+  h->dt = q.dt;
+  std::memcpy(h->q_R, q.q_R, sizeof q.q_R);
+  std::memcpy(h->stress_mask, q.stress_mask, sizeof q.stress_mask);
+  h->ga = q.ga;
+  h->t = q.t;
+  h->Delta_t = q.Delta_t;
+  h->Delta_t_prev = q.Delta_t_prev;
+  h->cutBack = q.cutBack;
+  h->cutBackLevel = q.cutBackLevel;
+  h->totalIncs = q.totalIncs;
+  return 0;
+}
 
 static void adj_free(dmk_handle* h) {
   if (!h->adj) return;
@@ -1068,6 +1193,7 @@ static void adj_free(dmk_handle* h) {
   for (hipEvent_t e : h->adj->fac)
     if (e) hipEventDestroy(e);
   if (h->adj->tape) hipFree(h->adj->tape);
+  if (h->adj->ckpt) hipFree(h->adj->ckpt);
   for (void* p : h->adj->work) hipFree(p);
   delete h->adj;
   h->adj = nullptr;
@@ -1091,31 +1217,74 @@ static int tape_store(dmk_handle* h, int k) {
   return 0;
 }
 
-extern "C" int dmk_adjoint_tape(dmk_handle* h, int32_t max_increments) {
-  if (!h || max_increments < 1) { dmk_set_error("dmk_adjoint_tape: invalid argument"); return DMK_ERR_ARG; }
+extern "C" int dmk_adjoint_tape_segmented(dmk_handle* h, int32_t max_increments, int32_t segment) {
+  if (!h || max_increments < 1 || segment < 0) { dmk_set_error("dmk_adjoint_tape: invalid argument"); return DMK_ERR_ARG; }
   if (h->scheme != 0) { dmk_set_error("dmk_adjoint_tape: the adjoint differentiates the spectral-basic scheme"); return DMK_ERR_CONFIG; }
   adj_free(h);
   h->adj = new AdjState;
   AdjState& a = *h->adj;
   a.cap = max_increments;
+  a.seg = segment >= max_increments ? 0 : segment;
   a.slot = (size_t)(27 + h->nsl_max) * h->N;
-  if (hipMalloc((void**)&a.tape, sizeof(double) * a.slot * (a.cap + 1)) != hipSuccess) {
+  const size_t entries = a.seg ? (size_t)a.seg + 1 : (size_t)a.cap + 1;
+  const int nck = a.seg ? a.cap / a.seg + 2 : 0;     // checkpoints 0..cap/seg and the final state
+  a.ckpt_slot = a.seg ? ckpt_doubles(h) : 0;
+  if (hipMalloc((void**)&a.tape, sizeof(double) * a.slot * entries) != hipSuccess ||
+      (nck && hipMalloc((void**)&a.ckpt, sizeof(double) * a.ckpt_slot * nck) != hipSuccess)) {
     hipGetLastError();
-    delete h->adj;
-    h->adj = nullptr;
-    dmk_set_error("dmk_adjoint_tape: %zu bytes of device memory not available",
-                  sizeof(double) * a.slot * (size_t)(max_increments + 1));
+    const size_t bytes = sizeof(double) * (a.slot * entries + a.ckpt_slot * nck);
+    adj_free(h);
+    dmk_set_error("dmk_adjoint_tape: %zu bytes of device memory not available", bytes);
     return DMK_ERR_HIP;
   }
-  return tape_store(h, 0);
+  a.ckpt_host.resize(nck);
+  int rc = tape_store(h, 0);
+  if (rc) return rc;
+  return a.seg ? ckpt_save(h, 0) : 0;
+}
+
+extern "C" int dmk_adjoint_tape(dmk_handle* h, int32_t max_increments) {
+  return dmk_adjoint_tape_segmented(h, max_increments, 0);
 }
 
 extern "C" int dmk_adjoint_record(dmk_handle* h) {
   if (!h || !h->adj) { dmk_set_error("dmk_adjoint_record: no tape"); return DMK_ERR_ARG; }
   AdjState& a = *h->adj;
+  int rc;
+  if (a.replaying) {
+    // an increment of a segment run again from its checkpoint: the same state
+    // as in the forward run, bit for bit
+    const int k = ++a.replay_k;
+    if ((rc = tape_store(h, k))) return rc;
+    CK(hipStreamSynchronize(h->st));
+    if (std::memcmp(h->P_av, &a.pav[9 * (size_t)(k - 1)], sizeof h->P_av) != 0) {
+      dmk_set_error("dmk_adjoint_backward: increment %d run again from its checkpoint differs from the forward run", k);
+      return DMK_ERR_NOCONV;
+    }
+    return 0;
+  }
   if (a.n >= a.cap) { dmk_set_error("dmk_adjoint_record: tape full (%d increments)", a.cap); return DMK_ERR_ARG; }
-  int rc = tape_store(h, a.n + 1);
-  if (rc) return rc;
+  if (a.swept) {
+    dmk_set_error("dmk_adjoint_record: the segmented tape was swept; start a new tape to record further increments");
+    return DMK_ERR_ARG;
+  }
+  const int k = a.n + 1;
+  if (a.seg) {
+    if (!h->inc_start.valid) {
+      dmk_set_error("dmk_adjoint_record: a segmented tape records increments of dmk_grid_run_loadcase only");
+      return DMK_ERR_ARG;
+    }
+    if (k > 1 && (k - 1) % a.seg == 0) {
+      // a new segment starts at k - 1: its entry moves to the segment's first slot
+      const double* last = tape_entry(h, k - 1);
+      a.base = k - 1;
+      a.on_tape = (k - 1) / a.seg;
+      CK(hipMemcpyAsync(tape_entry(h, k - 1), last, sizeof(double) * a.slot, hipMemcpyDeviceToDevice, h->st));
+    }
+    a.incs.push_back(h->inc_start);
+  }
+  if ((rc = tape_store(h, k))) return rc;
+  if (a.seg && k % a.seg == 0 && (rc = ckpt_save(h, k / a.seg))) return rc;
   CKL(dmk_adj_imax(h->mp.n_sub, h->N, h->small + SM_AMAX, h->st));
   CK(hipMemcpyAsync(h->hsmall + SM_AMAX, h->small + SM_AMAX, sizeof(double), hipMemcpyDeviceToHost, h->st));
   CK(hipStreamSynchronize(h->st));
@@ -1332,80 +1501,124 @@ extern "C" int dmk_adjoint_backward(dmk_handle* h, const double* P_av_bar, doubl
     CK(hipEventRecord(a.fac[k % 2], a.st2));
     return 0;
   };
-  CK(hipStreamSynchronize(h->st));
-  if ((rc = factor(a.n))) return rc;
   int cur = 0;
-  for (int k = a.n; k >= 1; k--) {
-    const double* qR = &a.qR[4 * (k - 1)];
-    AdjKArgs ka = local_args(k);
-    // P_av = R <P> R^T (passive rotation of the raw average): <P>_bar = R^T P_av_bar R
-    double pr[9];
-    rotate33(qR, P_av_bar + 9 * (k - 1), true, pr);
-    for (int c = 0; c < 9; c++) ka.Pu[c] = h->wgt * pr[c];
-    ka.Fpb_in = a.Fpb[cur];
-    ka.xib_in = a.xib[cur];
-    ka.A = a.A;
-    ka.g = a.g;
-    ka.mode = 0;
-    CK(hipStreamWaitEvent(h->st, a.fac[k % 2], 0));
-    for (int p = 0; p < h->n_phases; p++) {
-      ka.phase_id = p;
-      ka.nsl = h->ph[p].nsl;
-      CKL(dmk_adj_vjp(&ka, h->ph_dev + p, h->st));
-    }
-    CKL(dmk_adj_max(ka.res, N, h->small + SM_AMAX, h->st));
-    CKL(dmk_colsum(a.A, N, N, 81, h->partial, h->small + SM_ACAV, h->wgt, h->st));
-    CK(hipMemcpyAsync(h->hsmall + SM_AMAX, h->small + SM_AMAX, sizeof(double) * (SM_ACAV + 81 - SM_AMAX),
-                      hipMemcpyDeviceToHost, h->st));
+  // the reverse sweep over increments k1 .. k0 + 1 of the tape in memory
+  auto sweep = [&](int k1, int k0) -> int {
     CK(hipStreamSynchronize(h->st));
-    const double resmax = h->hsmall[SM_AMAX];
-    if (!(resmax < HUGE_VAL)) {
-      dmk_set_error("dmk_adjoint_backward: increment %d: singular increment Jacobian or non-finite state", k);
-      return DMK_ERR_NOCONV;
+    if ((rc = factor(k1))) return rc;
+    for (int k = k1; k > k0; k--) {
+      const double* qR = &a.qR[4 * (k - 1)];
+      AdjKArgs ka = local_args(k);
+      // P_av = R <P> R^T (passive rotation of the raw average): <P>_bar = R^T P_av_bar R
+      double pr[9];
+      rotate33(qR, P_av_bar + 9 * (k - 1), true, pr);
+      for (int c = 0; c < 9; c++) ka.Pu[c] = h->wgt * pr[c];
+      ka.Fpb_in = a.Fpb[cur];
+      ka.xib_in = a.xib[cur];
+      ka.A = a.A;
+      ka.g = a.g;
+      ka.mode = 0;
+      CK(hipStreamWaitEvent(h->st, a.fac[k % 2], 0));
+      for (int p = 0; p < h->n_phases; p++) {
+        ka.phase_id = p;
+        ka.nsl = h->ph[p].nsl;
+        CKL(dmk_adj_vjp(&ka, h->ph_dev + p, h->st));
+      }
+      CKL(dmk_adj_max(ka.res, N, h->small + SM_AMAX, h->st));
+      CKL(dmk_colsum(a.A, N, N, 81, h->partial, h->small + SM_ACAV, h->wgt, h->st));
+      CK(hipMemcpyAsync(h->hsmall + SM_AMAX, h->small + SM_AMAX, sizeof(double) * (SM_ACAV + 81 - SM_AMAX),
+                        hipMemcpyDeviceToHost, h->st));
+      CK(hipStreamSynchronize(h->st));
+      const double resmax = h->hsmall[SM_AMAX];
+      if (!(resmax < HUGE_VAL)) {
+        dmk_set_error("dmk_adjoint_backward: increment %d: singular increment Jacobian or non-finite state", k);
+        return DMK_ERR_NOCONV;
+      }
+      if (k - 1 > k0 && (rc = factor(k - 1))) return rc;
+      // mean rule: masked compliance of <A^T>
+      double Ct[81], Sadj[81];
+      const double* Abar = h->hsmall + SM_ACAV;
+      for (int p = 0; p < 9; p++)
+        for (int q = 0; q < 9; q++) Ct[p + 9 * q] = Abar[q + 9 * p];
+      if ((rc = masked_compliance_of(Ct, &a.mask[9 * (k - 1)], qR, Sadj))) return rc;
+      // preconditioner: Gamma_0 of the reference medium sym <A> of this increment
+      // (any positive-definite medium gives the same solution)
+      double Cs[81];
+      for (int p = 0; p < 81; p++) Cs[p] = 0.5 * (Abar[p] + Ct[p]);
+      a.ga = h->ga;
+      gamma_coef(h->wgt, Cs, &a.ga);
+      AimArgs aa;
+      std::memcpy(aa.S, Sadj, sizeof aa.S);
+      for (int c = 0; c < 9; c++) { aa.P_aim[c] = 0.0; aa.stress_mask[c] = a.mask[9 * (k - 1) + c]; }
+      std::memcpy(aa.q_R, qR, sizeof aa.q_R);
+      aa.rotate = 1;
+      aa.update = 1;
+      CK(hipMemsetAsync(h->small + SM_AFAIM, 0, sizeof(double) * 9, h->st));
+      if ((rc = adj_phi(h, aa, a.g, a.b))) return rc;
+      int its = 0;
+      double rel = 0.0;
+      if ((rc = adj_gmres(h, aa, rtol, 600, &its, &rel))) return rc;
+      ka.mode = 1;
+      ka.mu = a.mu;
+      ka.Fpb_out = a.Fpb[1 - cur];
+      ka.xib_out = a.xib[1 - cur];
+      ka.thb = a.thb;
+      for (int p = 0; p < h->n_phases; p++) {
+        ka.phase_id = p;
+        ka.nsl = h->ph[p].nsl;
+        CKL(dmk_adj_vjp(&ka, h->ph_dev + p, h->st));
+      }
+      // parameter cotangents of this increment, summed per phase on the device
+      CKL(dmk_adj_phase_sum(a.thb, N, h->mp.phase, h->n_phases, nslot, 1, a.red + (size_t)h->n_phases * mmax, h->st));
+      cur = 1 - cur;
+      if (stats) {
+        stats[4 * (k - 1) + 0] = its;
+        stats[4 * (k - 1) + 1] = rel;
+        stats[4 * (k - 1) + 2] = resmax;
+        stats[4 * (k - 1) + 3] = a.maxsub[k - 1];
+      }
     }
-    if (k > 1 && (rc = factor(k - 1))) return rc;
-    // mean rule: masked compliance of <A^T>
-    double Ct[81], Sadj[81];
-    const double* Abar = h->hsmall + SM_ACAV;
-    for (int p = 0; p < 9; p++)
-      for (int q = 0; q < 9; q++) Ct[p + 9 * q] = Abar[q + 9 * p];
-    if ((rc = masked_compliance_of(Ct, &a.mask[9 * (k - 1)], qR, Sadj))) return rc;
-    // preconditioner: Gamma_0 of the reference medium sym <A> of this increment
-    // (any positive-definite medium gives the same solution)
-    double Cs[81];
-    for (int p = 0; p < 81; p++) Cs[p] = 0.5 * (Abar[p] + Ct[p]);
-    a.ga = h->ga;
-    gamma_coef(h->wgt, Cs, &a.ga);
-    AimArgs aa;
-    std::memcpy(aa.S, Sadj, sizeof aa.S);
-    for (int c = 0; c < 9; c++) { aa.P_aim[c] = 0.0; aa.stress_mask[c] = a.mask[9 * (k - 1) + c]; }
-    std::memcpy(aa.q_R, qR, sizeof aa.q_R);
-    aa.rotate = 1;
-    aa.update = 1;
-    CK(hipMemsetAsync(h->small + SM_AFAIM, 0, sizeof(double) * 9, h->st));
-    if ((rc = adj_phi(h, aa, a.g, a.b))) return rc;
-    int its = 0;
-    double rel = 0.0;
-    if ((rc = adj_gmres(h, aa, rtol, 600, &its, &rel))) return rc;
-    ka.mode = 1;
-    ka.mu = a.mu;
-    ka.Fpb_out = a.Fpb[1 - cur];
-    ka.xib_out = a.xib[1 - cur];
-    ka.thb = a.thb;
-    for (int p = 0; p < h->n_phases; p++) {
-      ka.phase_id = p;
-      ka.nsl = h->ph[p].nsl;
-      CKL(dmk_adj_vjp(&ka, h->ph_dev + p, h->st));
+    return 0;
+  };
+  if (!a.seg) {
+    if ((rc = sweep(a.n, 0))) return rc;
+  } else {
+    // segments from the last: the last one is still on the tape; every earlier
+    // one is run again from its checkpoint, recording its entries, before it is
+    // differentiated; the end-of-run state is put back afterwards
+    const int nseg = (a.n + a.seg - 1) / a.seg, fin = a.cap / a.seg + 1;
+    if ((rc = ckpt_save(h, fin))) return rc;
+    for (int sgi = nseg - 1; sgi >= 0; sgi--) {
+      const int k0 = sgi * a.seg, k1 = std::min(a.n, k0 + a.seg);
+      if (sgi != a.on_tape) {
+        if ((rc = ckpt_restore(h, sgi))) return rc;
+        a.base = k0;
+        if ((rc = tape_store(h, k0))) return rc;
+        a.replaying = true;
+        a.replay_k = k0;
+        for (int k = k0 + 1; k <= k1 && !rc; k++) {
+          const IncStart& is = a.incs[k - 1];
+          h->t = is.t;
+          h->Delta_t = is.Delta_t;
+          h->Delta_t_prev = is.Delta_t_prev;
+          h->cutBack = is.cutBack;
+          h->cutBackLevel = is.cutBackLevel;
+          int guess = is.guess, its = 0, nf = 0;
+          const dmk_loadcase lc = is.lc;
+          rc = run_increment(h, &lc, is.t_0, &guess, &its, &nf);
+          if (!rc && a.replay_k != k) {
+            dmk_set_error("dmk_adjoint_backward: increment %d did not converge when run again", k);
+            rc = DMK_ERR_NOCONV;
+          }
+        }
+        a.replaying = false;
+        a.on_tape = sgi;
+        if (rc) return rc;
+      }
+      if ((rc = sweep(k1, k0))) return rc;
     }
-    // parameter cotangents of this increment, summed per phase on the device
-    CKL(dmk_adj_phase_sum(a.thb, N, h->mp.phase, h->n_phases, nslot, 1, a.red + (size_t)h->n_phases * mmax, h->st));
-    cur = 1 - cur;
-    if (stats) {
-      stats[4 * (k - 1) + 0] = its;
-      stats[4 * (k - 1) + 1] = rel;
-      stats[4 * (k - 1) + 2] = resmax;
-      stats[4 * (k - 1) + 3] = a.maxsub[k - 1];
-    }
+    if ((rc = ckpt_restore(h, fin))) return rc;
+    a.swept = true;
   }
   // xi_0 enters as the initial state (phenopowerlaw.f90:259-262)
   CKL(dmk_adj_phase_sum(a.xib[cur], N, h->mp.phase, h->n_phases, mmax, 0, a.red, h->st));
@@ -1956,10 +2169,76 @@ extern "C" int dmk_mech_counters(dmk_handle* h, int64_t out[4]) {
   return 0;
 }
 
+extern "C" int dmk_mech_cell_counters(dmk_handle* h, int32_t* newton, int32_t* fpi, int32_t* substeps) {
+  if (!h) return DMK_ERR_ARG;
+  const size_t b = sizeof(int32_t) * (size_t)h->N;
+  if (newton) CK(hipMemcpyAsync(newton, h->mp.n_newton, b, hipMemcpyDeviceToHost, h->st));
+  if (fpi) CK(hipMemcpyAsync(fpi, h->mp.n_fpi, b, hipMemcpyDeviceToHost, h->st));
+  if (substeps) CK(hipMemcpyAsync(substeps, h->mp.n_sub, b, hipMemcpyDeviceToHost, h->st));
+  CK(hipStreamSynchronize(h->st));
+  return 0;
+}
+
 // math.f90:1222-1234
 static double geometric_fraction(double r, int i, int N) {
   if (std::fabs(r - 1.0) <= 1.0e-9) return 1.0 / (double)N;
   return (1.0 - r) * std::pow(r, i) / (r - std::pow(r, N + 1));
+}
+
+// One increment of the DAMASK_grid loop (DAMASK_grid.f90:330-415): forward,
+// materialpoint_forward, solution, and on non-convergence the cutback that
+// halves the time step, until the increment's full time has converged.  The
+// driver state at entry (t, Delta_t, Delta_t_prev, cutBack, cutBackLevel,
+// guess) is recorded for the adjoint tape so that a segment of increments can
+// be run again from a checkpoint with the same decisions.
+static int run_increment(dmk_handle* h, const dmk_loadcase* lc, double t_0, int* guess, int* its_out,
+                         int* nf_out) {
+  if (h->adj) {
+    IncStart& s = h->inc_start;
+    s.lc = *lc;
+    s.t_0 = t_0;
+    s.t = h->t;
+    s.Delta_t = h->Delta_t;
+    s.Delta_t_prev = h->Delta_t_prev;
+    s.cutBack = h->cutBack;
+    s.cutBackLevel = h->cutBackLevel;
+    s.guess = *guess;
+    s.valid = true;
+  }
+  int stepFraction = 0, its = 0, nf_total = 0;
+  while (stepFraction < (1 << h->cutBackLevel)) {
+    double t_remaining = lc->t + t_0 - h->t;
+    h->t = h->t + h->Delta_t;
+    stepFraction++;
+    int rc = dmk_mech_forward(h, h->cutBack, *guess, h->Delta_t, h->Delta_t_prev, t_remaining, lc);
+    if (rc) return rc;
+    if (!h->cutBack && (rc = dmk_mech_materialpoint_forward(h))) return rc;
+    int32_t it, conv, nf;
+    if ((rc = dmk_mech_solution(h, &it, &conv, &nf))) return rc;
+    its = it;
+    nf_total += nf;
+    if (conv) {
+      if ((rc = dmk_mech_update_coords(h, nullptr, nullptr))) return rc;
+      if (h->adj && (rc = dmk_adjoint_record(h))) return rc;
+      h->Delta_t_prev = h->Delta_t;
+      h->cutBack = 0;
+      *guess = 1;
+    } else if (h->cutBackLevel < h->num.N_cutback_max) {
+      h->cutBack = 1;
+      stepFraction = (stepFraction - 1) * 2;
+      h->cutBackLevel++;
+      h->t = h->t - h->Delta_t;
+      h->Delta_t = h->Delta_t / 2.0;
+    } else {
+      *its_out = its;
+      *nf_out = nf_total;
+      dmk_set_error("IO_error 950: no convergence after %d cutbacks", h->cutBackLevel);
+      return 950;
+    }
+  }
+  *its_out = its;
+  *nf_out = nf_total;
+  return 0;
 }
 
 extern "C" int dmk_grid_run_loadcase(dmk_handle* h, const dmk_loadcase* lc, double* t_io, double* stats,
@@ -1986,36 +2265,12 @@ extern "C" int dmk_grid_run_loadcase(dmk_handle* h, const dmk_loadcase* lc, doub
       if (F_aim_hist) std::memcpy(F_aim_hist + 9 * (inc - 1), h->F_aim, 9 * sizeof(double));
       continue;
     }
-    int stepFraction = 0, its = 0, nf_total = 0;
-    while (stepFraction < (1 << h->cutBackLevel)) {
-      double t_remaining = lc->t + t_0 - h->t;
-      h->t = h->t + h->Delta_t;
-      stepFraction++;
-      int rc = dmk_mech_forward(h, h->cutBack, guess, h->Delta_t, h->Delta_t_prev, t_remaining, lc);
-      if (rc) return rc;
-      if (!h->cutBack && (rc = dmk_mech_materialpoint_forward(h))) return rc;
-      int32_t it, conv, nf;
-      if ((rc = dmk_mech_solution(h, &it, &conv, &nf))) return rc;
-      its = it;
-      nf_total += nf;
-      if (conv) {
-        if ((rc = dmk_mech_update_coords(h, nullptr, nullptr))) return rc;
-        if (h->adj && (rc = dmk_adjoint_record(h))) return rc;
-        h->Delta_t_prev = h->Delta_t;
-        h->cutBack = 0;
-        guess = 1;
-      } else if (h->cutBackLevel < h->num.N_cutback_max) {
-        h->cutBack = 1;
-        stepFraction = (stepFraction - 1) * 2;
-        h->cutBackLevel++;
-        h->t = h->t - h->Delta_t;
-        h->Delta_t = h->Delta_t / 2.0;
-      } else {
-        if (t_io) *t_io = h->t;
-        if (n_updates) *n_updates = h->n_updates - u0;
-        dmk_set_error("IO_error 950: no convergence after %d cutbacks", h->cutBackLevel);
-        return 950;
-      }
+    int its = 0, nf_total = 0;
+    int rc = run_increment(h, lc, t_0, &guess, &its, &nf_total);
+    if (rc) {
+      if (t_io) *t_io = h->t;
+      if (n_updates) *n_updates = h->n_updates - u0;
+      return rc;
     }
     if (stats) {
       stats[4 * (inc - 1) + 0] = h->t;
@@ -2032,12 +2287,6 @@ extern "C" int dmk_grid_run_loadcase(dmk_handle* h, const dmk_loadcase* lc, doub
   return 0;
 }
 
-// Restart (grid_mech_spectral_basic.f90:198-232, DAMASK_grid.f90:300-303).
-// After the fields and the solver scalars of the restart increment are set,
-// the solver evaluates the material at the restored F with dt = 0 (the
-// reference's init does so after reading the restart data); the stiffness
-// averages are then set from the restart data.  Increments up to restart_inc
-// of the following dmk_grid_run_loadcase calls only advance the time.
 extern "C" int dmk_mech_restart_response(dmk_handle* h) {
   if (!h) { dmk_set_error("null handle"); return DMK_ERR_ARG; }
   int rc = constitutive(h, h->F, 0.0, false, true);

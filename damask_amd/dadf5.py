@@ -8,6 +8,8 @@ Layout, as DAMASK_grid writes it:
     /geometry              attributes cells, size, origin; datasets of the same name
     /cell_to/phase         (N, 1) compound {label: fixed string, entry: int64}
     /cell_to/homogenization (N,) compound {label, entry}
+    /setup                 attribute description; one string dataset per input file
+                           (material.yaml, numerics.yaml, load case, geometry)
     /increment_<i>         attribute t/s
         geometry/u_p, u_n  displacements of the cell centres / nodes
         homogenization/<name>/mechanical
@@ -53,6 +55,7 @@ U_INFO = {"u_p": ("displacements of the materialpoints (cell centers)", "m"),
 _hid = C.c_int64
 _herr = C.c_int
 _hsize = C.c_ulonglong
+H5F_ACC_RDONLY = 0x0000
 H5F_ACC_TRUNC = 0x0002
 H5P_DEFAULT = 0
 H5S_ALL = 0
@@ -99,6 +102,15 @@ class _H5:
             "H5Tcreate": (_hid, [C.c_int, C.c_size_t]),
             "H5Tinsert": (_herr, [_hid, C.c_char_p, C.c_size_t, _hid]),
             "H5Tclose": (_herr, [_hid]),
+            "H5Fopen": (_hid, [C.c_char_p, C.c_uint, _hid]),
+            "H5Gopen2": (_hid, [_hid, C.c_char_p, _hid]),
+            "H5Dopen2": (_hid, [_hid, C.c_char_p, _hid]),
+            "H5Dread": (_herr, [_hid, _hid, _hid, _hid, _hid, C.c_void_p]),
+            "H5Dget_space": (_hid, [_hid]),
+            "H5Sget_simple_extent_ndims": (C.c_int, [_hid]),
+            "H5Sget_simple_extent_dims": (C.c_int, [_hid, C.POINTER(_hsize), C.POINTER(_hsize)]),
+            "H5Aopen": (_hid, [_hid, C.c_char_p, _hid]),
+            "H5Aread": (_herr, [_hid, _hid, C.c_void_p]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -178,6 +190,36 @@ class DADF5Writer:
         self._mapping(g, "homogenization", [self.ho_names[h] for h in self.ho_of], self.entry_ho, (self.N,),
                       "cell ID to homogenization results")
         H.L.H5Gclose(g)
+        # input files (src/result.f90:147-148,287-307)
+        self.setup = self._group(self.f, "setup")
+        self._attr_str(self.setup, "description", "input data used to run the simulation")
+        self.setup_names = set()
+
+    def add_setup_file(self, content: str, fname: str, description: str) -> str:
+        """result_addSetupFile (src/result.f90:287-307): the file's text as a
+        fixed-length, NUL-terminated string dataset of shape (1,) named after the
+        file (directories stripped, '.1', '.2', ... appended to repeated names)."""
+        H = self.h5
+        name = os.path.basename(fname)
+        suffix, i = "", 0
+        while name + suffix in self.setup_names:
+            i += 1
+            suffix = f".{i}"
+        name += suffix
+        self.setup_names.add(name)
+        data = content.rstrip(" ").encode()
+        t = H.L.H5Tcopy(H.C_S1)
+        H.L.H5Tset_size(t, len(data) + 1)
+        s = self._space((1,))
+        buf = C.create_string_buffer(data, len(data) + 1)
+        d = H.ok(H.L.H5Dcreate2(self.setup, name.encode(), t, s, H5P_DEFAULT, H5P_DEFAULT, H5P_DEFAULT),
+                 "setup " + name)
+        H.ok(H.L.H5Dwrite(d, t, H5S_ALL, H5S_ALL, H5P_DEFAULT, C.cast(buf, C.c_void_p)), "write setup " + name)
+        self._stamp(d, description)
+        H.L.H5Dclose(d)
+        H.L.H5Sclose(s)
+        H.L.H5Tclose(t)
+        return name
 
     # ------------------------------------------------------------ primitives
     def _group(self, loc, name):
@@ -316,6 +358,9 @@ class DADF5Writer:
 
     def close(self):
         if self.f is not None:
+            if self.setup is not None:
+                self.h5.L.H5Gclose(self.setup)
+                self.setup = None
             self.h5.L.H5Fclose(self.f)
             self.f = None
 
@@ -348,7 +393,7 @@ class ResultOutput:
     Outputs are those listed in material.yaml (phase mechanical and plastic
     ``output`` keys)."""
 
-    def __init__(self, path: str, solver, driver=None, call: str = ""):
+    def __init__(self, path: str, solver, driver=None, call: str = "", setup_files=()):
         self.s = solver
         self.driver = driver
         g = solver.grid
@@ -356,6 +401,12 @@ class ResultOutput:
         self.w = DADF5Writer(path, g.cells, g.size, g.origin, solver.maps["phase"], [p.name for p in mat.phases],
                              [p.lattice for p in mat.phases], solver.maps["homogenization"],
                              [h[0] for h in mat.homogenizations], call=call)
+        # the input files in the order the reference reads them: material and
+        # numerics configuration (src/config.f90:39-52), load case
+        # (src/grid/DAMASK_grid.f90:159), geometry (src/grid/discretization_grid.f90:162)
+        for fname, description in setup_files:
+            with open(fname) as fh:
+                self.w.add_setup_file(fh.read(), fname, description)
         self.systems = [slip_system_labels(solver.lib, p) for p in mat.phases]
         outs = []
         for p in mat.phases:

@@ -206,6 +206,13 @@ class GridMechSpectralBasic:
         _lib.check(self.lib.dmk_mech_counters(self.h, out.ctypes.data_as(C.POINTER(C.c_int64))), "dmk_mech_counters")
         return {"updates": int(out[0]), "newton": int(out[1]), "state": int(out[2]), "residuals": int(out[3])}
 
+    def cell_counters(self) -> dict:
+        """Per-cell Newton iterations, state iterations and sub-steps tried in the last response."""
+        out = {k: np.zeros(self.N, np.int32) for k in ("newton", "fpi", "substeps")}
+        _lib.check(self.lib.dmk_mech_cell_counters(self.h, _lib.iptr(out["newton"]), _lib.iptr(out["fpi"]),
+                                                   _lib.iptr(out["substeps"])), "dmk_mech_cell_counters")
+        return out
+
     KERNEL_GROUPS = ("stress", "tangent", "reduce", "fft_fwd", "gamma_div", "fft_inv", "c_avg")
 
     def profile(self, level: int = 2) -> None:
@@ -260,9 +267,13 @@ class GridMechSpectralBasic:
         _lib.check(self.lib.dmk_grid_set_restart(self.h, int(restart_inc)), "dmk_grid_set_restart")
 
     # --- adjoint ------------------------------------------------------------
-    def adjoint_tape(self, max_increments: int) -> None:
-        """Record the current state and, from now on, every converged increment."""
-        _lib.check(self.lib.dmk_adjoint_tape(self.h, int(max_increments)), "dmk_adjoint_tape")
+    def adjoint_tape(self, max_increments: int, segment: int = 0) -> None:
+        """Record the current state and, from now on, every converged increment.
+        segment > 0: keep only one segment of that many increments on the tape
+        plus a checkpoint every segment increments; the reverse sweep runs the
+        earlier segments again (increments must come from run_loadcase)."""
+        _lib.check(self.lib.dmk_adjoint_tape_segmented(self.h, int(max_increments), int(segment)),
+                   "dmk_adjoint_tape_segmented")
         self.taping = True
 
     def adjoint_record(self) -> None:

@@ -3,6 +3,8 @@ phenopowerlaw material of ``examples/grid/material.yaml:7-24`` and uniaxial
 tension along x (``examples/grid/tensionX.yaml:7-14``)."""
 from __future__ import annotations
 
+import copy
+
 import numpy as np
 import yaml
 
@@ -24,6 +26,7 @@ FCC_PHASE = {
     },
 }
 
+# pure Schmid law on both families ({110}<111> and {112}<111>)
 BCC_PHASE = {
     "lattice": "cI",
     "mechanical": {
@@ -40,6 +43,14 @@ BCC_PHASE = {
         },
     },
 }
+
+
+# the same 24 systems with the non-Schmid coefficients of the reference's
+# BCC regression material (tests/integration/resources/plastic_detect_changes/
+# material.yaml:286-301, six coefficients of 0.1); in a cI lattice they act on
+# the {110}<111> family only (src/crystal.f90:1428-1441)
+BCC_NS_PHASE = copy.deepcopy(BCC_PHASE)
+BCC_NS_PHASE["mechanical"]["plastic"]["a_non-Schmid"] = [[0.1] * 6]
 
 
 def material_dict(quaternions: np.ndarray, phase: dict | None = None, phase_name: str = "pheno") -> dict:

@@ -203,6 +203,13 @@ int dmk_mech_kernel_time(dmk_handle* h, int32_t kernel, double* total_ms, int64_
  * residual evaluations */
 int dmk_mech_counters(dmk_handle* h, int64_t out[4]);
 
+/* per-cell counts of the last material-point response (N each, any may be
+ * NULL): Lp Newton iterations, state fixed-point iterations, and state
+ * integrations, i.e. sub-steps tried, counting the first full step and every
+ * cut-back retry of phase_mechanical_constitutive (phase_mechanical.f90:1008-1090).
+ * No counterpart in the reference (diagnostics of the material-point loop). */
+int dmk_mech_cell_counters(dmk_handle* h, int32_t* newton, int32_t* fpi, int32_t* substeps);
+
 /* Miller indices (direction d1 d2 d3, normal n1 n2 n3) of active slip system a
  * of a phase description (lattice, n_fam, N_sl): the system labels of the
  * xi_sl / gamma_sl results (crystal.f90:1642-1676, 2133-2181). */
@@ -220,6 +227,19 @@ int dmk_slip_system(int32_t lattice, int32_t n_fam, const int32_t* N_sl, int32_t
  * converged increment).  Device memory: (max_increments + 1) (27 + nsl) N
  * doubles.  A new tape replaces the old one. */
 int dmk_adjoint_tape(dmk_handle* h, int32_t max_increments);
+/* The same with per-increment checkpointing and recomputation: the tape holds
+ * the segment + 1 entries of one segment of increments, and every segment
+ * increments a checkpoint of the full solver and material-point state
+ * ((2 nx + 45 N + 2 nsl N) doubles) is kept.  The reverse sweep runs each
+ * earlier segment again from its checkpoint, recording its entries, before it
+ * differentiates it; the gradients are those of the full tape, bit for bit.
+ * Device memory: (segment + 1) (27 + nsl) N doubles for the tape plus
+ * (max_increments / segment + 2) checkpoints.  Increments must be recorded by
+ * dmk_grid_run_loadcase (the driver state of every increment is kept for the
+ * recomputation); after a reverse sweep the handle holds the end-of-run state
+ * again and the tape records no further increments.  segment = 0 or
+ * >= max_increments: a full tape (dmk_adjoint_tape). */
+int dmk_adjoint_tape_segmented(dmk_handle* h, int32_t max_increments, int32_t segment);
 int dmk_adjoint_record(dmk_handle* h);
 /* recorded increments n, their P_av (9 n, load frame) and time steps (n) */
 int dmk_adjoint_info(dmk_handle* h, int32_t* n, double* P_av, double* dt);

@@ -136,6 +136,7 @@ int  orc_mp_nsl(const orc_mp *m);
 /* per-cell statistics of the last response (Newton and FPI iteration counts) */
 void orc_mp_counters(const orc_mp *m, int64_t *newton, int64_t *fpi, int64_t *cutbacks);
 void orc_mp_cell_counters(const orc_mp *m, int64_t *newton, int64_t *fpi);
+void orc_mp_cell_substeps(const orc_mp *m, int64_t *substeps);
 /* single-cell kernels used by tests */
 int  orc_integrate_stress(orc_mp *m, int ce, const double F[9], const double Fp0[9],
                           const double Fi0[9], double dt);
@@ -185,13 +186,32 @@ void orc_solver_forward(orc_solver *s, int cutBack, int guess, double dt, double
 int orc_solver_solution(orc_solver *s, int *its, int *nfuncs);
 void orc_solver_update_coords(orc_solver *s, double *x_p, double *x_n);
 void orc_solver_get(const orc_solver *s, int what, double *dst); /* 0 F field, 1 P field,
-                        2 F_aim, 3 P_av, 4 C_volAvg, 5 C_minMaxAvg, 6 P_aim, 7 err (div, BC) */
+                        2 F_aim, 3 P_av, 4 C_volAvg, 5 C_minMaxAvg, 6 P_aim, 7 err (div, BC),
+                        8 S (masked compliance), 9 F_tau field, 10 err_curl, 11 C_scale (polarization) */
+/* switch a fresh solver to spectral_polarization (grid_mech_spectral_polarization.f90);
+ * nonzero for beta <= 0 */
+/* spectral_basic only: dPdF, orientation and the stiffness averages once per
+ * solve from the last response instead of after every response (the scheme
+ * reads them only between solves); nonzero for other schemes */
+int orc_solver_set_tangent_once(orc_solver *s, int once);
+int orc_solver_set_polarization(orc_solver *s, double alpha, double beta, double eps_abs_curl, double eps_rel_curl);
 /* one load case of DAMASK_grid (DAMASK_grid.f90:288-460).  stats (per increment,
  * 8 doubles): t, cutBackLevel, iterations, residual evaluations, P_av_11..  P_av
  * written separately.  Returns 0 on success, 950 when no convergence. */
 int orc_run_loadcase(orc_solver *s, const orc_loadcase *lc, double *t_io,
                      double *stats /* N*4 */, double *P_av_hist /* N*9 */,
                      double *F_aim_hist /* N*9 */, int64_t *n_updates);
+
+/* The NGMRES iteration of the solver (PETSc SNES NGMRES restated, see
+ * oracle_solver.c) on an affine residual r(x) = A x - b (A n x n column-major),
+ * from x (in/out), history M <= 30, at most max_it iterations, converged when
+ * |r| <= atol.  trace (optional, trace_cap records) receives per iteration
+ * l, selectA, restarted, |r| before, |r(x^M)|, |r(x^A)|, alpha[M], then x^M's
+ * residual, x^M, x^A, x^A's residual, the accepted x, and the history of
+ * residuals and iterates (M n each) the least squares used; *n_trace records.
+ * Returns the convergence reason (> 0 converged). */
+int orc_ngmres_affine(int n, const double *A, const double *b, double *x, int M, int max_it, double atol,
+                      double *trace, int trace_cap, int *n_trace);
 
 #if defined(__cplusplus) && !defined(ORC_NO_EXTERN_C)
 }

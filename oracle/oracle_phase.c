@@ -221,7 +221,8 @@ struct orc_mp {
   double *state, *state0;
   double *hom_P, *hom_dPdF;
   int32_t *status;
-  int64_t *newton, *fpi, *cut;
+  int64_t *newton, *fpi, *cut, *nsub;
+  int lazy_tangent;          /* 1: orientation and dPdF only on request (orc_mp_tangent) */
 };
 
 static double *dalloc(size_t n) { return (double *)calloc(n ? n : 1, sizeof(double)); }
@@ -252,6 +253,7 @@ orc_mp *orc_mp_create(int n_cells, int n_phases, const orc_phase *phases,
   m->newton = (int64_t *)calloc(n_cells, sizeof(int64_t));
   m->fpi = (int64_t *)calloc(n_cells, sizeof(int64_t));
   m->cut = (int64_t *)calloc(n_cells, sizeof(int64_t));
+  m->nsub = (int64_t *)calloc(n_cells, sizeof(int64_t));
   /* phase_mechanical.f90:249-266 with V_e = I */
   for (int ce = 0; ce < n_cells; ce++) {
     double *F = m->F + 9 * ce, *Fp = m->Fp + 9 * ce, I[9];
@@ -281,6 +283,7 @@ void orc_mp_destroy(orc_mp *m) {
   free(m->Lp); free(m->Lp0); free(m->Li); free(m->Li0); free(m->S); free(m->S0);
   free(m->P); free(m->Fe); free(m->O); free(m->state); free(m->state0);
   free(m->hom_P); free(m->hom_dPdF); free(m->status); free(m->newton); free(m->fpi); free(m->cut);
+  free(m->nsub);
   free(m);
 }
 
@@ -511,6 +514,7 @@ static int constitutive(orc_mp *m, int ce, double dt) {
     if (todo) {
       for (int k = 0; k < 9; k++) F[k] = F0[k] + step * (m->F[9 * ce + k] - m->F0[9 * ce + k]);
       status = integrate_state_fpi(m, ce, F, Fp0, Fi0, state0, step * dt);
+      m->nsub[ce]++;
     }
   }
   return status;
@@ -630,7 +634,7 @@ int orc_mp_response(orc_mp *m, double dt, const double *F_target, double *P_out,
   int ns = m->stride;
 #pragma omp parallel for schedule(dynamic, 64)
   for (int ce = 0; ce < N; ce++) {
-    m->newton[ce] = m->fpi[ce] = m->cut[ce] = 0;
+    m->newton[ce] = m->fpi[ce] = m->cut[ce] = m->nsub[ce] = 0;
     /* phase_restore -> mechanical_restore (phase_mechanical.f90:1096-1115) */
     memcpy(m->Fp + 9 * ce, m->Fp0 + 9 * ce, 9 * sizeof(double));
     memcpy(m->Fi + 9 * ce, m->Fi0 + 9 * ce, 9 * sizeof(double));
@@ -646,10 +650,11 @@ int orc_mp_response(orc_mp *m, double dt, const double *F_target, double *P_out,
   if (status == ORC_STATUS_OK) {
 #pragma omp parallel for schedule(dynamic, 64)
     for (int ce = 0; ce < N; ce++) {
+      for (int k = 0; k < 9; k++) m->hom_P[9 * ce + k] = m->P[9 * ce + k] * 1.0;
+      if (m->lazy_tangent) continue;
       orientation(m, ce);
       double d[81];
       orc_dPdF(m, ce, dt, d);
-      for (int k = 0; k < 9; k++) m->hom_P[9 * ce + k] = m->P[9 * ce + k] * 1.0;
       for (int k = 0; k < 81; k++) m->hom_dPdF[(size_t)81 * ce + k] = d[k] * 1.0;
     }
   }
@@ -657,6 +662,20 @@ int orc_mp_response(orc_mp *m, double dt, const double *F_target, double *P_out,
   if (dPdF_out) memcpy(dPdF_out, m->hom_dPdF, sizeof(double) * 81 * N);
   return status;
 }
+
+/* the second loop of homogenization_mechanical_response (homogenization.f90:352-360)
+ * for the state of the last (converged) response: orientation and dPdF of every cell */
+void orc_mp_tangent(orc_mp *m, double dt) {
+#pragma omp parallel for schedule(dynamic, 64)
+  for (int ce = 0; ce < m->N; ce++) {
+    orientation(m, ce);
+    double d[81];
+    orc_dPdF(m, ce, dt, d);
+    for (int k = 0; k < 81; k++) m->hom_dPdF[(size_t)81 * ce + k] = d[k] * 1.0;
+  }
+}
+
+void orc_mp_set_lazy_tangent(orc_mp *m, int lazy) { m->lazy_tangent = lazy; }
 
 const double *orc_mp_dPdF_ptr(const orc_mp *m) { return m->hom_dPdF; }
 int orc_mp_ncells(const orc_mp *m) { return m->N; }
@@ -729,4 +748,10 @@ void orc_mp_set(orc_mp *m, int what, const double *src) {
 /* per-cell Newton and state-iteration counts of the last response */
 void orc_mp_cell_counters(const orc_mp *m, int64_t *newton, int64_t *fpi) {
   for (int i = 0; i < m->N; i++) { newton[i] = m->newton[i]; fpi[i] = m->fpi[i]; }
+}
+
+/* per-cell state integrations (sub-steps tried, the first full step and every
+ * cut-back retry included) of the last response */
+void orc_mp_cell_substeps(const orc_mp *m, int64_t *substeps) {
+  for (int i = 0; i < m->N; i++) substeps[i] = m->nsub[i];
 }

@@ -90,6 +90,7 @@ def lib():
         L.orc_mp_nsl.restype = C.c_int
         L.orc_mp_nsl.argtypes = [vp]
         L.orc_mp_counters.argtypes = [vp, C.POINTER(C.c_int64)] * 1 + [C.POINTER(C.c_int64)] * 2
+        L.orc_mp_cell_substeps.argtypes = [vp, C.POINTER(C.c_int64)]
         L.orc_dPdF.argtypes = [vp, C.c_int, C.c_double, dp]
         L.orc_integrate_stress.restype = C.c_int
         L.orc_integrate_stress.argtypes = [vp, C.c_int, dp, dp, dp, C.c_double]
@@ -137,6 +138,13 @@ def lib():
         L.orc_run_loadcase.restype = C.c_int
         L.orc_run_loadcase.argtypes = [vp, C.POINTER(LoadCase), dp, dp, dp, dp, C.POINTER(C.c_int64)]
         L.orc_set_threads.argtypes = [C.c_int]
+        L.orc_solver_set_tangent_once.restype = C.c_int
+        L.orc_solver_set_tangent_once.argtypes = [vp, C.c_int]
+        L.orc_solver_set_polarization.restype = C.c_int
+        L.orc_solver_set_polarization.argtypes = [vp, C.c_double, C.c_double, C.c_double, C.c_double]
+        L.orc_ngmres_affine.restype = C.c_int
+        L.orc_ngmres_affine.argtypes = [C.c_int, dp, dp, dp, C.c_int, C.c_int, C.c_double, dp, C.c_int,
+                                        C.POINTER(C.c_int)]
         _lib = L
     return _lib
 
@@ -225,7 +233,8 @@ def loadcase(lc) -> LoadCase:
 class Problem:
     """Material point + spectral grid + solver built from parsed inputs."""
 
-    def __init__(self, grid, material, numerics=None, cref_rule: int = 1, threads: int = 0):
+    def __init__(self, grid, material, numerics=None, cref_rule: int = 1, threads: int = 0,
+                 scheme: str = "spectral_basic", tangent_once: bool = False):
         L = lib()
         if threads:
             L.orc_set_threads(threads)
@@ -244,6 +253,17 @@ class Problem:
                                       C.byref(self.ng))
         self.solver = L.orc_solver_create(self.grid, self.mp, C.byref(self.ng))
         L.orc_solver_set_cref_rule(self.solver, cref_rule)
+        if scheme == "spectral_polarization":
+            n = numerics if numerics is not None else None
+            a, b = (n.alpha, n.beta) if n is not None else (1.0, 1.0)
+            ea, er = (n.eps_abs_curl, n.eps_rel_curl) if n is not None else (1.0e-10, 5.0e-4)
+            if L.orc_solver_set_polarization(self.solver, a, b, ea, er):
+                raise ValueError("spectral_polarization needs beta > 0")
+        elif scheme != "spectral_basic":
+            raise ValueError(f"the oracle restates spectral_basic and spectral_polarization, not {scheme}")
+        self.scheme = scheme
+        if tangent_once and L.orc_solver_set_tangent_once(self.solver, 1):
+            raise ValueError("tangent_once is for spectral_basic")
         self.t = 0.0
 
     def close(self):
@@ -266,7 +286,8 @@ class Problem:
         return out
 
     def solver_get(self, what: int) -> np.ndarray:
-        size = {0: 9 * self.N, 1: 9 * self.N, 2: 9, 3: 9, 4: 81, 5: 81, 6: 9, 7: 2, 8: 81}[what]
+        size = {0: 9 * self.N, 1: 9 * self.N, 2: 9, 3: 9, 4: 81, 5: 81, 6: 9, 7: 2, 8: 81, 9: 9 * self.N,
+                10: 1, 11: 81}[what]
         out = np.zeros(size)
         lib().orc_solver_get(self.solver, what, ptr(out))
         return out
@@ -343,3 +364,32 @@ def tangent_loadcase(grid, material, numerics, lc, seeds, cref_rule: int = 1, th
                                         C.byref(c), tangent_rtol, ptr(P), ptr(dP), ptr(stats), C.byref(nu))
     return {"rc": rc, "P_av": P.reshape(-1, 9), "dP_av": dP.reshape(-1, 9), "stats": stats.reshape(-1, 4),
             "n_updates": nu.value}
+
+
+def ngmres_affine(A, b, x0, M=30, max_it=50, atol=1e-12):
+    """The oracle's NGMRES on r(x) = A x - b.  Returns (reason, x, records):
+    one dict per iteration with l, selectA, restarted, the residual norms, alpha
+    (l entries), FM, XM, XA, FA, X (accepted) and the residual / iterate
+    history Fh, Xh (l rows each) the least squares combined."""
+    A = np.asfortranarray(np.asarray(A, float))
+    n = A.shape[0]
+    b = np.ascontiguousarray(b, float)
+    x = np.ascontiguousarray(x0, float).copy()
+    rec = 6 + M + 5 * n + 2 * M * n
+    tr = np.zeros(rec * max_it)
+    nt = C.c_int(0)
+    Af = np.ascontiguousarray(A.reshape(-1, order="F"))
+    reason = lib().orc_ngmres_affine(n, ptr(Af), ptr(b), ptr(x), M, max_it, atol, ptr(tr), max_it, C.byref(nt))
+    out = []
+    for k in range(nt.value):
+        r = tr[k * rec:(k + 1) * rec]
+        l = int(r[0])
+        v = r[6 + M:]
+        d = {"l": l, "selectA": bool(r[1]), "restarted": bool(r[2]), "fnorm": r[3], "fMnorm": r[4],
+             "fAnorm": r[5], "alpha": r[6:6 + l].copy()}
+        for j, key in enumerate(("FM", "XM", "XA", "FA", "X")):
+            d[key] = v[j * n:(j + 1) * n].copy()
+        d["Fh"] = v[5 * n:5 * n + M * n].reshape(M, n)[:l].copy()
+        d["Xh"] = v[5 * n + M * n:5 * n + 2 * M * n].reshape(M, n)[:l].copy()
+        out.append(d)
+    return reason, x, out

@@ -96,9 +96,15 @@ def test_gpu_run_writes_reference_layout(gpu_lib, tmp_path):
     path = str(tmp_path / "golden.hdf5")
     solver = GridMechSpectralBasic(grid, mat)
     drv = DamaskGrid(solver)
-    out = dadf5.ResultOutput(path, solver, drv)
+    setup = [(os.path.join(GOLDEN, "fcc_singlePhase.yaml"), "material configuration"),
+             (os.path.join(GOLDEN, "fcc_tensionY.yaml"), "load case definition (grid solver)"),
+             (os.path.join(GOLDEN, "fcc_6grains6x7x8.vti"), "geometry definition (grid solver)")]
+    out = dadf5.ResultOutput(path, solver, drv, setup_files=setup)
     drv.run(lcs, on_output=out)
     out.close()
+    setup_listing = subprocess.run([H5DUMP, "-H", "-g", "/setup", path], capture_output=True, text=True,
+                                   check=True).stdout
+    assert all(f'DATASET "{os.path.basename(f)}"' in setup_listing for f, _ in setup)
     N = grid.n_cells
     name = mat.phases[0].name
     listing = subprocess.run([H5DUMP, "-H", path], capture_output=True, text=True, check=True).stdout
@@ -109,3 +115,39 @@ def test_gpu_run_writes_reference_layout(gpu_lib, tmp_path):
     assert np.abs(xi - ref["xi_sl_40"]).max() < 2e-5 * ref["xi_sl_40"].max()
     up = _read(path, "/increment_40/geometry/u_p", "<f8").reshape(N, 3)
     assert np.abs(up - ref["u_p_40"]).max() < 1e-4 * np.abs(ref["u_n_40"]).max()
+
+
+def test_setup_group_matches_reference_result(tmp_path):
+    """/setup (src/result.f90:147-148,287-307): the golden case's input files
+    (byte copies of the /setup datasets of the reference's result file),
+    stored under the names the reference used, give datasets of the reference's
+    type (NUL-terminated fixed-length string of the text's length + 1), shape
+    (1,), bytes and descriptions; a repeated name gets the '.1' suffix"""
+    from damask_amd import dadf5
+    grid, mat, _ = fcc_golden_inputs()
+    maps = mat.cell_maps(grid.material)
+    path = str(tmp_path / "setup.hdf5")
+    w = dadf5.DADF5Writer(path, grid.cells, grid.size, grid.origin, maps["phase"], [p.name for p in mat.phases],
+                          [p.lattice for p in mat.phases], maps["homogenization"],
+                          [h[0] for h in mat.homogenizations])
+    files = (("fcc_singlePhase.yaml", "singlePhase.yaml", "material configuration"),
+             ("fcc_tensionY.yaml", "tensionY.yaml", "load case definition (grid solver)"),
+             ("fcc_6grains6x7x8.vti", "6grains6x7x8.vti", "geometry definition (grid solver)"))
+    for src, name, desc in files:
+        with open(os.path.join(GOLDEN, src)) as fh:
+            assert w.add_setup_file(fh.read(), "some/dir/" + name, desc) == name
+    assert w.add_setup_file("x", "tensionY.yaml", "load case definition (grid solver)") == "tensionY.yaml.1"
+    w.close()
+    ours = subprocess.run([H5DUMP, "-H", "-g", "/setup", path], capture_output=True, text=True, check=True).stdout
+    assert '"input data used to run the simulation"' in _attr(path, "/setup/description")
+    for _, name, desc in files:
+        block = ours[ours.index(f'DATASET "{name}"'):]
+        n = len(open(os.path.join(GOLDEN, dict((b, a) for a, b, _ in files)[name])).read().encode())
+        assert f"STRSIZE {n + 1};" in block.split("ATTRIBUTE")[0]
+        assert "DATASPACE  SIMPLE { ( 1 ) / ( 1 ) }" in block.split("ATTRIBUTE")[0]
+        assert f'"{desc}"' in _attr(path, f"/setup/{name}/description")
+        out = path + ".bin"
+        subprocess.run([H5DUMP, "-d", f"/setup/{name}", "-b", "MEMORY", "-o", out, path], check=True,
+                       stdout=subprocess.DEVNULL)
+        with open(out, "rb") as fh, open(os.path.join(GOLDEN, dict((b, a) for a, b, _ in files)[name]), "rb") as src:
+            assert fh.read() == src.read(), name

@@ -11,7 +11,7 @@ pytestmark = pytest.mark.gpu
 
 def _case(cells=8, grains=4, seed=3, t=8.0, N=5, lattice="cF"):
     from damask_amd import config, workloads
-    phase = workloads.BCC_PHASE if lattice == "cI" else None
+    phase = workloads.BCC_NS_PHASE if lattice == "cI" else None
     grid, mat = workloads.polycrystal([cells] * 3, grains, seed, phase=phase)
     mat = config.parse_material(mat)
     mat.phases[0].atol_xi, mat.phases[0].atol_gamma = 1e-8, 1e-14
@@ -124,7 +124,7 @@ def test_adjoint_gradient_matches_forward_mode(gpu_lib, lattice):
     """The GPU adjoint against the FP64 forward-mode reference gradient: the
     CPU oracle's tangent-linear (dual-number) build, one parameter direction
     per run, exact up to the solvers' tolerances (no difference quotient).
-    cF: 12 systems; cI: two families of 12 with non-Schmid terms."""
+    cF: 12 systems; cI: two families of 12, non-Schmid terms on the {110}<111> family."""
     from concurrent.futures import ThreadPoolExecutor
 
     from damask_amd import adjoint
@@ -155,3 +155,30 @@ def test_adjoint_gradient_matches_forward_mode(gpu_lib, lattice):
               f"rel {abs(g[k] - fm[k]) / abs(fm[k]):.2e}")
     scale = np.abs(fm * theta0).max()
     assert np.all(np.abs(g - fm) * np.abs(theta0) <= 1e-5 * np.abs(fm * theta0) + 1e-8 * scale)
+
+
+@pytest.mark.parametrize("segment", [3, 4])
+def test_checkpointed_adjoint_equals_full_tape(gpu_lib, segment):
+    """Checkpointing with recomputation (dmk_adjoint_tape_segmented): one
+    segment on the tape, a checkpoint every `segment` increments, the earlier
+    segments run again in the reverse sweep.  11 increments (segments of 3 and
+    4 leave a short last one): gradients and adjoint statistics equal the full
+    tape's bit for bit, a second reverse sweep gives them again, and the handle
+    holds the end-of-run state afterwards."""
+    from damask_amd import adjoint
+    grid, mat, lcs, num = _case(N=11, t=16.0)
+    s, P = adjoint.run_curve(grid, mat, lcs, num)
+    n = P.shape[0]
+    w = np.linspace(1.0, 2.0, n) / 1e6
+    g_full, st_full = adjoint.curve_gradient(s, 0, _Pbar(n, w), rtol=1e-10)
+    F_end = s.field("F")
+    s.close()
+    s2, P2 = adjoint.run_curve(grid, mat, lcs, num, max_increments=n, segment=segment)
+    assert np.array_equal(P2, P)
+    g_seg, st_seg = adjoint.curve_gradient(s2, 0, _Pbar(n, w), rtol=1e-10)
+    assert np.array_equal(g_seg, g_full)
+    assert np.array_equal(st_seg, st_full)
+    assert np.array_equal(s2.field("F"), F_end)
+    g_again, _ = adjoint.curve_gradient(s2, 0, _Pbar(n, w), rtol=1e-10)
+    assert np.array_equal(g_again, g_full)
+    s2.close()

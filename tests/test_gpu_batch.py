@@ -32,12 +32,13 @@ def test_batch_matches_single_runs(gpu_lib):
 def test_lockstep_batch_matches_single_runs(gpu_lib):
     """dmk_batch_run_lockstep: the stress integrations of all RVEs in flight
     share one launch per slip-system variant (12-system FCC RVEs of two sizes
-    and a 24-system BCC RVE with non-Schmid terms); RVEs finish at different
-    times and leave the lockstep.  Curves equal the single runs bit for bit."""
+    and a 24-system BCC RVE with non-Schmid terms on its {110}<111> family);
+    RVEs finish at different times and leave the lockstep.  Curves equal the
+    single runs bit for bit."""
     from damask_amd import adjoint, batch, config, workloads
     from damask_amd.grid import GridMechSpectralBasic
     items = batch.random_batch(7, master_seed=5, edges=(8, 12), grains=(5, 12), increments=4, strain=0.004)
-    grid, mat = workloads.polycrystal([7, 6, 5], 6, 21, size=(1.0, 0.9, 0.8), phase=workloads.BCC_PHASE)
+    grid, mat = workloads.polycrystal([7, 6, 5], 6, 21, size=(1.0, 0.9, 0.8), phase=workloads.BCC_NS_PHASE)
     _, lcs = config.parse_load(workloads.tension_x(3.0, 3))
     items.append((grid, config.parse_material(mat), lcs[0]))
     b = batch.Batch(items)

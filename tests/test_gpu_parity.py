@@ -28,14 +28,20 @@ def _pair(grid, material, numerics=None, oracle=None):
     return g, o
 
 
+def _phase(lattice):
+    """cF: 12 systems; cI: N_sl = [12, 12], Schmid law; cI_nS: the same with
+    the non-Schmid terms on the {110}<111> family"""
+    from damask_amd import workloads
+    return {"cF": workloads.FCC_PHASE, "cI": workloads.BCC_PHASE, "cI_nS": workloads.BCC_NS_PHASE}[lattice]
+
+
 def _relmax(a, b):
     return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
 
 
-@pytest.mark.parametrize("lattice", ["cF", "cI"])
+@pytest.mark.parametrize("lattice", ["cF", "cI", "cI_nS"])
 def test_init_state(gpu_lib, oracle, lattice):
-    from damask_amd import workloads
-    phase = workloads.FCC_PHASE if lattice == "cF" else workloads.BCC_PHASE
+    phase = _phase(lattice)
     grid, mat = _poly([6, 5, 4], 7, 11, phase)
     g, o = _pair(grid, mat, oracle=oracle)
     # F_p = O.asMatrix at t = 0 (phase_mechanical.f90:255), after the dt = 0
@@ -49,12 +55,11 @@ def test_init_state(gpu_lib, oracle, lattice):
     assert _relmax(g.scalar("C_minMaxAvg"), Cm) < 1e-12
 
 
-@pytest.mark.parametrize("lattice,eps", [("cF", 5e-4), ("cF", 1.5e-3), ("cI", 8e-4)])
+@pytest.mark.parametrize("lattice,eps", [("cF", 5e-4), ("cF", 1.5e-3), ("cI", 8e-4), ("cI_nS", 8e-4)])
 def test_constitutive_response(gpu_lib, oracle, lattice, eps):
     """homogenization_mechanical_response on a perturbed tension field, three
     increments of 0.5 s at 1e-3/s-like rates, with materialpoint_forward between"""
-    from damask_amd import workloads
-    phase = workloads.FCC_PHASE if lattice == "cF" else workloads.BCC_PHASE
+    phase = _phase(lattice)
     grid, mat = _poly([8, 6, 5], 9, 3, phase)
     g, o = _pair(grid, mat, oracle=oracle)
     rng = np.random.default_rng(5)
@@ -281,3 +286,47 @@ def test_restart_continues_the_run(gpu_lib, scheme):
         assert np.abs(a - b).max() < 1e-8 * np.abs(a).max(), k
     straight.close()
     again.close()
+
+
+def test_restart_file_continues_like_the_oracle(gpu_lib, oracle, tmp_path):
+    """<job>_restart.hdf5 in the reference's layout (DAMASK_grid.f90:429-451,
+    grid_mech_spectral_basic.f90:378-416, phase_mechanical.f90:1248-1261):
+    written by a GPU run after increment 4, read by a fresh solver (the
+    reference's restart read: phase data as the increment-start state, S not
+    stored, the response at dt = 0), which continues with increments 5-8.
+    Against the oracle's uninterrupted 8-increment run at tight tolerances:
+    P_av within 1e-8, F, F_p and xi per point within 1e-8."""
+    import copy as _copy
+
+    from damask_amd import config, restart, workloads
+    from damask_amd.grid import GridMechSpectralBasic
+    grid, mat = workloads.polycrystal([8, 8, 8], 5, 7)
+    mat = config.parse_material(mat)
+    _, lcs = config.parse_load(workloads.tension_x(8.0, 8))
+    num = config.Numerics(eps_abs_div=1e-8, eps_rel_div=1e-11, eps_abs_P=1e-3, eps_rel_P=1e-11, N_iter_max=400,
+                          snes_max_it=400, eps_rel_state=1e-12, eps_rel_Lp=1e-11, eps_abs_Lp=1e-16)
+    lc = lcs[0]
+    half = _copy.copy(lc)
+    half.t, half.N = lc.t / 2, lc.N // 2
+    first = GridMechSpectralBasic(grid, mat, num)
+    assert first.run_loadcase(half)["rc"] == 0
+    path = str(tmp_path / "job_restart.hdf5")
+    restart.write_restart(path, first, half.N)
+    first.close()
+    again = GridMechSpectralBasic(grid, mat, num)
+    assert restart.read_restart(path, again) == half.N
+    r_cont = again.run_loadcase(lc)
+    assert r_cont["rc"] == 0
+    o = oracle.Problem(grid, mat, num)
+    ro = o.run_loadcase(lc)
+    assert ro["rc"] == 0
+    P1, P2 = ro["P_av"][half.N:], r_cont["P_av"][half.N:]
+    assert np.abs(P2 - P1).max() < 1e-8 * np.abs(P1).max()
+    F_o = o.solver_get(0).reshape(-1, 3, 3).transpose(0, 2, 1)
+    assert np.abs(again.field("F") - F_o).max() < 1e-8
+    Fp_o = o.get(3).reshape(-1, 3, 3).transpose(0, 2, 1)
+    assert np.abs(again.field("F_p") - Fp_o).max() < 1e-8
+    xo = o.get(8, again.nsl).reshape(grid.n_cells, -1)
+    assert np.abs(again.field("xi_sl") - xo).max() < 1e-8 * xo.max()
+    again.close()
+    o.close()

@@ -11,6 +11,8 @@ C-ABI, against the spectral-basic scheme.
   averaged P within 1e-7 of max |P| and per-cell F within 1e-8.
 * The curl RMS of the device against the oracle's restatement of
   utilities_curlRMS (spectral_utilities.f90:697-757) on random fields.
+* spectral_polarization against the oracle's restatement of the scheme,
+  with and without update_gamma, iteration by iteration.
 """
 import os
 
@@ -101,7 +103,7 @@ def test_scheme_bcc_odd_grid(gpu_lib, scheme):
     (odd N1: no Nyquist plane in x; odd N3), uniaxial tension with lateral
     stress BCs: same equilibrium as the basic scheme at tight tolerances"""
     from damask_amd import config, workloads
-    g, m = workloads.polycrystal([7, 6, 5], 6, 21, size=(1.0, 0.9, 0.8), phase=workloads.BCC_PHASE)
+    g, m = workloads.polycrystal([7, 6, 5], 6, 21, size=(1.0, 0.9, 0.8), phase=workloads.BCC_NS_PHASE)
     mat = config.parse_material(m)
     _, lcs = config.parse_load(workloads.tension_x(6.0, 12, 4))
     num = config.Numerics(eps_abs_div=1e-8, eps_rel_div=1e-11, eps_abs_P=1e-3, eps_rel_P=1e-11,
@@ -116,28 +118,81 @@ def test_scheme_bcc_odd_grid(gpu_lib, scheme):
         assert np.abs(b["F_p"] - a["F_p"]).max() < 1e-8
 
 
-def test_polarization_update_gamma(gpu_lib):
-    """update_gamma: Gamma, C_scale and S_scale follow C_minMaxAvg at every
-    solution (grid_mech_spectral_polarization.f90:301-305).  Default numerics
-    (at eps_rel_div 1e-11 the polarization iteration with update_gamma stalls
-    near err_div ~ 6 Pa in the second increment of this case); F, F_p and P
-    against the basic scheme with update_gamma at the reference
-    consistency test's bound, P_av within twice the stress-BC tolerance
-    max(eps_rel_P |P_av|, eps_abs_P) of both runs."""
+def test_polarization_against_oracle(gpu_lib, oracle):
+    """spectral_polarization on the GPU against the oracle's restatement of
+    grid_mech_spectral_polarization.f90 (form_residual :528-630, converged
+    :482-522, forward :318-400), with and without update_gamma (:301-305):
+    same iteration counts, P_av within 1e-8, F and F_tau within 1e-8"""
     import copy
     from damask_amd import config
+    from damask_amd.grid import GridMechSpectralPolarization
     grid, mat, lcs = _inputs()
     lc = copy.copy(lcs[0])
-    lc.N, lc.t, lc.f_out = 20, 3.0, 5
-    num = config.Numerics(update_gamma=True)
-    rb, _ = _run("spectral_basic", grid, mat, [lc], num)
-    rp, itp = _run("spectral_polarization", grid, mat, [lc], num)
-    print("iterations", itp)
-    for a, b in zip(rb, rp):
-        bc_tol = max(num.eps_rel_P * np.abs(a["P_av"]).max(), num.eps_abs_P)
-        assert np.abs(b["P_av"] - a["P_av"]).max() < 2.0 * bc_tol
-        for k in ("F", "F_p"):
-            np.testing.assert_allclose(b[k], a[k], rtol=1e-5, atol=1e-2, err_msg=k)
+    lc.N, lc.t = 20, 3.0
+    for ug in (False, True):
+        num = config.Numerics(update_gamma=ug)
+        g = GridMechSpectralPolarization(grid, mat, num)
+        o = oracle.Problem(grid, mat, num, scheme="spectral_polarization")
+        rg = g.run_loadcase(lc)
+        ro = o.run_loadcase(lc)
+        assert rg["rc"] == ro["rc"] == 0
+        print("update_gamma", ug, "iterations gpu", rg["stats"][:, 2].tolist(), "oracle", ro["stats"][:, 2].tolist())
+        assert np.array_equal(rg["stats"][:, 1:], ro["stats"][:, 1:])
+        assert np.abs(rg["P_av"] - ro["P_av"]).max() < 1e-8 * np.abs(ro["P_av"]).max()
+        F_o = o.solver_get(0).reshape(-1, 3, 3).transpose(0, 2, 1)
+        Ft_o = o.solver_get(9).reshape(-1, 3, 3).transpose(0, 2, 1)
+        assert np.abs(g.field("F") - F_o).max() < 1e-8
+        assert np.abs(g.field("F_tau") - Ft_o).max() < 1e-8
+        assert abs(g.scalar("err_curl")[0] - o.solver_get(10)[0]) <= 1e-6 * o.solver_get(10)[0] + 1e-20
+        g.close()
+        o.close()
+
+
+def test_polarization_update_gamma_stall_is_the_algorithms(gpu_lib, oracle):
+    """At tight tolerances with update_gamma, the polarization iteration of
+    this case stalls in the second increment with err_div at 6.27 Pa (tolerance
+    7.5e-4), while err_curl and err_BC are at rounding level.  The oracle's
+    restatement of the reference's residual stalls at the same value, so the
+    stall belongs to the restated algorithm, not to the device path.  Its
+    cause, measured on the oracle at the stall: the F residual applies
+    math_invSym3333(dPdF + C_scale), a Mandel (symmetric 6x6) inverse whose
+    range is symmetric tensors, to V = P - F (C_scale:(F_tau - F - I)); the
+    symmetric part of V is driven to 2e-4 Pa but its skew part (47 Pa) is left
+    free, and div P = div V then stays at 6.27 Pa.  Both sides run increments
+    1 and 2 with the same iteration limit; their err_div and P_av agree."""
+    import copy
+    import ctypes as C
+    from damask_amd import config
+    from damask_amd.grid import GridMechSpectralPolarization
+    grid, mat, lcs = _inputs()
+    lc = copy.copy(lcs[0])
+    lc.N, lc.t = 20, 3.0
+    num = config.Numerics(update_gamma=True, eps_abs_div=1e-8, eps_rel_div=1e-11, eps_abs_P=1e-3,
+                          eps_rel_P=1e-11, eps_abs_curl=1e-13, eps_rel_curl=1e-11, N_iter_max=60, snes_max_it=60)
+    g = GridMechSpectralPolarization(grid, mat, num)
+    o = oracle.Problem(grid, mat, num, scheme="spectral_polarization")
+    L = oracle.lib()
+    c = oracle.loadcase(lc)
+    dt = lc.t / lc.N
+    for inc in range(2):
+        g.forward(False, inc > 0, dt, dt, lc.t - inc * dt, lc)
+        g.materialpoint_forward()
+        sg = g.solution()
+        L.orc_solver_forward(o.solver, 0, inc > 0, dt, dt, lc.t - inc * dt, C.byref(c))
+        L.orc_mp_forward(o.mp)
+        its, nf = C.c_int(), C.c_int()
+        conv_o = L.orc_solver_solution(o.solver, C.byref(its), C.byref(nf))
+        e_g, e_o = g.scalar("err"), o.solver_get(7)
+        print(f"increment {inc + 1}: gpu converged {sg.converged} err_div {e_g[0]:.4e}; "
+              f"oracle converged {bool(conv_o)} err_div {e_o[0]:.4e}")
+        assert sg.converged == bool(conv_o)
+    assert not sg.converged
+    assert e_o[0] > 1.0 and abs(e_g[0] - e_o[0]) < 1e-3 * e_o[0]
+    assert g.scalar("err_curl")[0] < 1e-12 and o.solver_get(10)[0] < 1e-12
+    P_o = o.solver_get(3).reshape(3, 3).T
+    assert np.abs(g.scalar("P_av") - P_o).max() < 1e-6 * np.abs(P_o).max()
+    g.close()
+    o.close()
 
 
 @pytest.mark.parametrize("cells", [[6, 7, 8], [8, 8, 8], [5, 6, 4]])

@@ -53,3 +53,28 @@ def test_solver_selection():
         assert e.value.code == 601
     s = grid.numerics_struct(config.Numerics(alpha=1.25), scheme=1)
     assert s.scheme == 1 and s.alpha == 1.25 and s.beta == 1.0 and s.eps_rel_curl == 5.0e-4
+
+
+def test_load_case_errors():
+    """the load-case checks of DAMASK_grid.f90:661-673 raise IO_error 830:
+    a rotation with the Galerkin solver, non-positive r, t, N, f_out and f_restart"""
+    import copy as _copy
+
+    from damask_amd import config, workloads
+    base = workloads.tension_x(10.0, 5)
+    config.parse_load(base)
+    rot = _copy.deepcopy(base)
+    rot["loadstep"][0]["boundary_conditions"]["mechanical"]["R"] = [1.0, 0.0, 0.0, 30.0]
+    config.parse_load(rot)                                   # fine for the basic scheme
+    rot["solver"]["mechanical"] = "spectral_Galerkin"
+    with pytest.raises(config.ConfigError) as e:
+        config.parse_load(rot)
+    assert e.value.code == 830
+    for where, key, v in (("discretization", "r", 0.0), ("discretization", "t", -1.0),
+                          ("discretization", "N", 0), (None, "f_out", 0), (None, "f_restart", 0)):
+        bad = _copy.deepcopy(base)
+        step = bad["loadstep"][0]
+        (step[where] if where else step)[key] = v
+        with pytest.raises(config.ConfigError) as e:
+            config.parse_load(bad)
+        assert e.value.code == 830, (key, v)
