@@ -60,6 +60,10 @@ def parse():
                     help="only run the batched measurement and print it as one JSON line")
     ap.add_argument("--only-adjoint", type=int, default=0,
                     help="only run the adjoint / forward measurement and print it as one JSON line")
+    ap.add_argument("--mixed-batch", type=int, default=64,
+                    help="RVEs of the 48^3 BCC mixed-load batched forward + checkpointed adjoint (0: skip)")
+    ap.add_argument("--mixed-increments", type=int, default=50)
+    ap.add_argument("--only-mixed", type=int, default=0, help="only the 48^3 BCC mixed-load batch, as one JSON line")
     ap.add_argument("--bcc", type=int, default=1,
                     help="also time the same workload with the 24-system BCC phase")
     ap.add_argument("--schemes", type=int, default=1,
@@ -196,6 +200,46 @@ def batched_cost(args):
             "backward_s": rb["wall_s"], "adjoint_over_forward": (rt["wall_s"] + rb["wall_s"]) / r["wall_s"],
             "gmres_rtol": args.adjoint_rtol, "gradient_norm": float(np.sqrt(sum(float(x @ x) for x in g))),
             "lockstep": lock}
+
+
+def mixed_batch_cost(args):
+    """The large batched configuration: 64 RVEs of 48^3 cells, 300 grains each,
+    the 24-system BCC phase (Schmid law), each under its own seeded random
+    symmetric deformation rate for 50 increments.  Batched forward, then a
+    batched forward on a checkpointed tape (a checkpoint every `segment`
+    increments, segment ~ sqrt(increments)) and the batched gradient of the
+    curve loss, whose reverse sweep runs the earlier segments again."""
+    import math
+
+    import numpy as np
+    from damask_amd import batch
+    items = batch.mixed_load_batch(args.mixed_batch, args.seed, edge=48, grains=300, increments=args.mixed_increments)
+    t0 = time.perf_counter()
+    b = batch.Batch(items)
+    t_init = time.perf_counter() - t0
+    r = b.run(threads=args.batch_threads)
+    updates = sum(s.counters()["updates"] for s in b.solvers)
+    cells = int(b.cells.sum())
+    b.close()
+    seg = int(math.ceil(math.sqrt(args.mixed_increments)))
+    b = batch.Batch(items)
+    b.tape(segment=seg)
+    rt = b.run(threads=args.batch_threads)
+    curves = b.curves()
+    Pb = [2.0 * (c - 1.01 * c) / c.shape[0] for c in curves]
+    g, rb = b.gradients(Pb, rtol=args.adjoint_rtol, threads=args.batch_threads)
+    b.close()
+    nsl = items[0][1].phases[0].nsl
+    n = args.mixed_increments
+    full_tape = (n + 1) * (27 + nsl) * 8 * cells
+    seg_tape = ((seg + 1) * (27 + nsl) + (-(-n // seg) + 1) * (2 * 9 + 45 + 2 * nsl)) * 8 * cells
+    return {"workload": f"{args.mixed_batch} RVEs of 48^3, 300 grains, BCC (cI, N_sl = [12, 12], Schmid law), "
+                        f"seeded random symmetric dot_F (equivalent rate 1e-3/s, |trace| <= 1e-4/s), {n} increments",
+            "cells": cells, "threads": args.batch_threads, "init_s": t_init, "forward_s": r["wall_s"],
+            "updates_per_s": updates / r["wall_s"], "segment": seg, "forward_with_tape_s": rt["wall_s"],
+            "backward_s": rb["wall_s"], "adjoint_over_forward": (rt["wall_s"] + rb["wall_s"]) / r["wall_s"],
+            "tape_bytes_full": full_tape, "tape_bytes_checkpointed": seg_tape, "gmres_rtol": args.adjoint_rtol,
+            "gradient_norm": float(np.sqrt(sum(float(x @ x) for x in g)))}
 
 
 def bcc_cost(args, n=20):
@@ -381,6 +425,9 @@ def main():
     if args.only_schemes:
         print(json.dumps({"schemes": schemes_cost(args)}))
         return
+    if args.only_mixed:
+        print(json.dumps({"mixed_batch": mixed_batch_cost(args)}))
+        return
     import numpy as np
     import torch
     from damask_amd import config, flops
@@ -471,6 +518,10 @@ def main():
     if args.schemes:
         line["schemes"] = schemes_cost(args)
         progress("other schemes timed")
+    if args.mixed_batch:
+        line["mixed_batch"] = mixed_batch_cost(args)
+        progress(f"mixed batch: {line['mixed_batch']['updates_per_s'] / 1e6:.1f} M updates/s, adjoint / forward "
+                 f"{line['mixed_batch']['adjoint_over_forward']:.2f}")
     if args.bcc:
         line["bcc"] = bcc_cost(args)
         progress(f"bcc: {line['bcc']['value'] / 1e6:.1f} M updates/s")

@@ -43,6 +43,24 @@ def random_batch(n_rve: int, master_seed: int = 0, edges=(8, 12, 16, 20, 24, 32)
     return out
 
 
+def mixed_load_batch(n_rve: int = 64, master_seed: int = 0, edge: int = 48, grains: int = 300,
+                     increments: int = 50, t: float = 20.0, phase: dict | None = None):
+    """The large batched configuration: n_rve RVEs of edge^3 cells with
+    `grains` seeded Voronoi grains and uniform random orientations each, the
+    24-system BCC phenopowerlaw phase (Schmid law), each loaded by its own
+    seeded random symmetric deformation rate (equivalent rate 1e-3/s, trace
+    within +-1e-4/s) over `increments` increments of t / increments."""
+    rng = np.random.default_rng(master_seed)
+    out = []
+    for _ in range(n_rve):
+        grid, mat = workloads.polycrystal([edge] * 3, grains, int(rng.integers(2**31 - 1)),
+                                          phase=phase or workloads.BCC_PHASE)
+        rate = workloads.random_symmetric_rate(rng)
+        _, lcs = config.parse_load(workloads.deformation_rate(rate, t, increments))
+        out.append((grid, config.parse_material(mat), lcs[0]))
+    return out
+
+
 class Batch:
     """Solvers of a batch of RVEs and the batched forward / adjoint drivers."""
 
@@ -58,7 +76,7 @@ class Batch:
         """Start the adjoint tapes; segment > 0: checkpoints every segment
         increments and recomputation in the reverse sweep (see adjoint_tape)."""
         for s, (_, _, lc) in zip(self.solvers, self.items):
-            s.adjoint_tape(lc.N if segment else 2 * lc.N + 16, segment)
+            s.adjoint_tape(2 * lc.N + 16, segment)
 
     def run(self, threads: int | None = None, lockstep: bool = False) -> dict:
         """All load cases; with lockstep the RVEs in flight (one per thread)

@@ -1071,7 +1071,8 @@ struct AdjState {
   // state every seg increments lets the reverse sweep run each earlier
   // segment again before it is differentiated
   int seg = 0, base = 0;
-  double* ckpt = nullptr;          // device part of the checkpoints (cap / seg + 1) and of the final state
+  std::vector<double*> ckpt;       // device part of the checkpoints (allocated when first saved); the last
+                                   // one (index cap / seg + 1) holds the end-of-run state during a sweep
   size_t ckpt_slot = 0;
   std::vector<HostState> ckpt_host;
   std::vector<IncStart> incs;      // driver state at the start of each recorded increment
@@ -1123,7 +1124,13 @@ static int ckpt_copy(dmk_handle* h, double* c, bool save) {
 
 static int ckpt_save(dmk_handle* h, int i) {
   AdjState& a = *h->adj;
-  int rc = ckpt_copy(h, a.ckpt + (size_t)i * a.ckpt_slot, true);
+  if (!a.ckpt[i] && hipMalloc((void**)&a.ckpt[i], sizeof(double) * a.ckpt_slot) != hipSuccess) {
+    hipGetLastError();
+    a.ckpt[i] = nullptr;
+    dmk_set_error("adjoint checkpoint: %zu bytes of device memory not available", sizeof(double) * a.ckpt_slot);
+    return DMK_ERR_HIP;
+  }
+  int rc = ckpt_copy(h, a.ckpt[i], true);
   if (rc) return rc;
   HostState& q = a.ckpt_host[i];
   std::memcpy(q.F_aimDot, h->F_aimDot, sizeof q.F_aimDot);
@@ -1154,7 +1161,7 @@ static int ckpt_save(dmk_handle* h, int i) {
 
 static int ckpt_restore(dmk_handle* h, int i) {
   AdjState& a = *h->adj;
-  int rc = ckpt_copy(h, a.ckpt + (size_t)i * a.ckpt_slot, false);
+  int rc = ckpt_copy(h, a.ckpt[i], false);
   if (rc) return rc;
   const HostState& q = a.ckpt_host[i];
   std::memcpy(h->F_aimDot, q.F_aimDot, sizeof q.F_aimDot);
@@ -1193,7 +1200,8 @@ static void adj_free(dmk_handle* h) {
   for (hipEvent_t e : h->adj->fac)
     if (e) hipEventDestroy(e);
   if (h->adj->tape) hipFree(h->adj->tape);
-  if (h->adj->ckpt) hipFree(h->adj->ckpt);
+  for (double* c : h->adj->ckpt)
+    if (c) hipFree(c);
   for (void* p : h->adj->work) hipFree(p);
   delete h->adj;
   h->adj = nullptr;
@@ -1229,14 +1237,14 @@ extern "C" int dmk_adjoint_tape_segmented(dmk_handle* h, int32_t max_increments,
   const size_t entries = a.seg ? (size_t)a.seg + 1 : (size_t)a.cap + 1;
   const int nck = a.seg ? a.cap / a.seg + 2 : 0;     // checkpoints 0..cap/seg and the final state
   a.ckpt_slot = a.seg ? ckpt_doubles(h) : 0;
-  if (hipMalloc((void**)&a.tape, sizeof(double) * a.slot * entries) != hipSuccess ||
-      (nck && hipMalloc((void**)&a.ckpt, sizeof(double) * a.ckpt_slot * nck) != hipSuccess)) {
+  if (hipMalloc((void**)&a.tape, sizeof(double) * a.slot * entries) != hipSuccess) {
     hipGetLastError();
-    const size_t bytes = sizeof(double) * (a.slot * entries + a.ckpt_slot * nck);
+    a.tape = nullptr;
     adj_free(h);
-    dmk_set_error("dmk_adjoint_tape: %zu bytes of device memory not available", bytes);
+    dmk_set_error("dmk_adjoint_tape: %zu bytes of device memory not available", sizeof(double) * a.slot * entries);
     return DMK_ERR_HIP;
   }
+  a.ckpt.assign(nck, nullptr);
   a.ckpt_host.resize(nck);
   int rc = tape_store(h, 0);
   if (rc) return rc;

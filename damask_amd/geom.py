@@ -54,18 +54,32 @@ def coordinates0_point(cells, size, origin=(0.0, 0.0, 0.0)) -> np.ndarray:
 
 
 def voronoi_tessellation(cells, size, seeds, periodic: bool = True) -> Grid:
-    """Periodic Voronoi map; material IDs are the seed indices (0-based)."""
+    """Periodic Voronoi map; material IDs are the seed indices (0-based): the
+    nearest seed of every cell centre (minimum image when periodic), the
+    lowest index among equidistant seeds.  A k-d tree answers the queries
+    (periodic through its box size); without scipy every distance is formed."""
     cells = np.asarray(cells, np.int64)
     size = np.asarray(size, float)
     coords = coordinates0_point(cells, size).reshape(-1, 3)
     seeds = np.asarray(seeds, float)
-    best = np.zeros(len(coords), np.int64)
-    chunk = 1 << 15
-    for s in range(0, len(coords), chunk):
-        c = coords[s:s + chunk, None, :] - seeds[None, :, :]
-        if periodic:
-            c = c - size * np.round(c / size)
-        best[s:s + chunk] = np.argmin(np.einsum("ijk,ijk->ij", c, c), axis=1)
+    try:
+        from scipy.spatial import cKDTree
+    except ImportError:
+        cKDTree = None
+    if cKDTree is not None and len(seeds) > 1:
+        tree = cKDTree(np.mod(seeds, size) if periodic else seeds, boxsize=size if periodic else None)
+        d, i = tree.query(coords, k=2)
+        best = i[:, 0].copy()
+        tie = d[:, 0] == d[:, 1]
+        best[tie] = np.minimum(i[tie, 0], i[tie, 1])
+    else:
+        best = np.zeros(len(coords), np.int64)
+        chunk = 1 << 15
+        for s in range(0, len(coords), chunk):
+            c = coords[s:s + chunk, None, :] - seeds[None, :, :]
+            if periodic:
+                c = c - size * np.round(c / size)
+            best[s:s + chunk] = np.argmin(np.einsum("ijk,ijk->ij", c, c), axis=1)
     mat = best.reshape(tuple(cells))          # [x, y, z]
     return Grid(cells, size, np.zeros(3), mat.flatten(order="F"))
 

@@ -71,6 +71,24 @@ def tension_x(t: float, N: int, f_out: int | None = None) -> dict:
                 "discretization": {"t": t, "N": N}, "f_out": f_out or N}]}
 
 
+def deformation_rate(dot_F, t: float, N: int, f_out: int | None = None) -> dict:
+    """Load case with every component of dot_F prescribed (no stress BC)."""
+    return {"solver": {"mechanical": "spectral_basic"},
+            "loadstep": [{"boundary_conditions": {"mechanical": {"dot_F": [[float(x) for x in row] for row in dot_F]}},
+                          "discretization": {"t": t, "N": N}, "f_out": f_out or N}]}
+
+
+def random_symmetric_rate(rng, rate: float = 1.0e-3, max_trace: float = 1.0e-4) -> np.ndarray:
+    """Seeded mixed deformation rate: a symmetric dot_F whose deviatoric part is
+    drawn from a normal distribution and scaled to the equivalent rate
+    sqrt(2/3 dev:dev) = rate, plus a trace drawn uniformly in [-max_trace, max_trace]."""
+    A = rng.standard_normal((3, 3))
+    D = 0.5 * (A + A.T)
+    dev = D - np.trace(D) / 3.0 * np.eye(3)
+    dev *= rate / np.sqrt(2.0 / 3.0 * np.sum(dev * dev))
+    return dev + rng.uniform(-max_trace, max_trace) / 3.0 * np.eye(3)
+
+
 def polycrystal(cells, n_grains: int, seed: int, size=(1.0, 1.0, 1.0), phase: dict | None = None):
     """Seeded periodic Voronoi RVE: returns (Grid, material dict)."""
     rng = np.random.default_rng(seed)

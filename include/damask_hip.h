@@ -233,8 +233,9 @@ int dmk_adjoint_tape(dmk_handle* h, int32_t max_increments);
  * ((2 nx + 45 N + 2 nsl N) doubles) is kept.  The reverse sweep runs each
  * earlier segment again from its checkpoint, recording its entries, before it
  * differentiates it; the gradients are those of the full tape, bit for bit.
- * Device memory: (segment + 1) (27 + nsl) N doubles for the tape plus
- * (max_increments / segment + 2) checkpoints.  Increments must be recorded by
+ * Device memory: (segment + 1) (27 + nsl) N doubles for the tape plus one
+ * checkpoint per segment recorded (allocated as the run reaches it) and one
+ * for the end-of-run state during a sweep.  Increments must be recorded by
  * dmk_grid_run_loadcase (the driver state of every increment is kept for the
  * recomputation); after a reverse sweep the handle holds the end-of-run state
  * again and the tape records no further increments.  segment = 0 or
