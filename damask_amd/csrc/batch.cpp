@@ -12,6 +12,8 @@
 // The many small per-RVE launches, which the few hardware queues serialise,
 // become one launch that fills the GPU.
 #include <algorithm>
+#include <cstdio>
+#include <cstdlib>
 #include <atomic>
 #include <chrono>
 #include <condition_variable>
@@ -144,9 +146,18 @@ void batch_leave(DmkStressBatch* b) {
   if (b->waiting > 0 && b->waiting == b->active) batch_launch_locked(b);
 }
 
+// DMK_BATCH_PROGRESS set (and not "0") in the environment: one line on stderr
+// per finished RVE
+bool progress_lines() {
+  const char* v = getenv("DMK_BATCH_PROGRESS");
+  return v && v[0] && !(v[0] == '0' && v[1] == 0);
+}
+
 template <class F>
 int run_pool(int n, const std::vector<int>& order, int n_threads, F&& job, int32_t* rc_out, double* wall_s) {
-  std::atomic<int> next{0};
+  std::atomic<int> next{0}, done{0};
+  const bool report = progress_lines();
+  const auto start = std::chrono::steady_clock::now();
   auto worker = [&](int slot) {
     for (;;) {
       int k = next.fetch_add(1);
@@ -157,6 +168,12 @@ int run_pool(int n, const std::vector<int>& order, int n_threads, F&& job, int32
       auto t1 = std::chrono::steady_clock::now();
       if (rc_out) rc_out[i] = rc;
       if (wall_s) wall_s[i] = std::chrono::duration<double>(t1 - t0).count();
+      if (report) {
+        int d = done.fetch_add(1) + 1;
+        fprintf(stderr, "batch: RVE %d finished (rc %d), %d of %d after %.1f s\n", i, rc, d, n,
+                std::chrono::duration<double>(t1 - start).count());
+        fflush(stderr);
+      }
     }
   };
   int nt = std::max(1, std::min(n_threads, n));

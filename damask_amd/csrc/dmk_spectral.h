@@ -55,11 +55,17 @@ struct CoordsArgs {
   double wgt;
 };
 
+// geomSize / scaledGeomSize for divergence_correction dc (0 none, 1 grid+size, 2 size)
+void dmk_div_scale(const int cells[3], const double size[3], int dc, double rg[3]);
 int dmk_fft_setup(DmkFFT* fft, const int cells[3]);
 // time the candidate tile shapes on the device (scratch: 9 Nk complex and 9 N
 // real doubles, overwritten) and keep the fastest per pass
 int dmk_fft_tune(DmkFFT* fft, double2* cscratch, double* rscratch, hipStream_t s);
 void dmk_fft_free(DmkFFT* fft);
+// plans with the default (untuned) tile shapes and the twiddles of the three
+// axes written by a kernel into tw (N1 + N2 + N3 complex values, caller memory,
+// never freed by dmk_fft_free)
+int dmk_fft_setup_into(DmkFFT* fft, const int cells[3], double2* tw, hipStream_t s);
 int dmk_fft_r2c(const DmkFFT* fft, int howmany, const double* in, double2* out, hipStream_t s);
 // the x and y passes of dmk_fft_r2c / dmk_fft_c2r only
 int dmk_fft_r2c_xy(const DmkFFT* fft, int howmany, const double* in, double2* out, hipStream_t s);

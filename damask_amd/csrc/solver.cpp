@@ -1854,21 +1854,7 @@ static int init_impl(dmk_handle* h, const int32_t cells[3], const double size[3]
   // spectral set-up (spectral_utilities.f90:173-197)
   for (int d = 0; d < 3; d++) { h->ga.n[d] = cells[d]; h->ga.size[d] = size[d]; }
   h->ga.h = h->h;
-  double scaled[3] = {size[0], size[1], size[2]};
-  {
-    double r[3];
-    for (int d = 0; d < 3; d++) r[d] = size[d] / cells[d];
-    int dc = num->divergence_correction;
-    if (dc == 0 || dc == 1) {
-      const double* v = dc == 1 ? r : size;
-      int imin = 0, imax = 0;
-      for (int d = 1; d < 3; d++) { if (v[d] < v[imin]) imin = d; if (v[d] > v[imax]) imax = d; }
-      for (int j = 0; j < 3; j++)
-        if (j != imin && j != imax)
-          for (int d = 0; d < 3; d++) scaled[d] = dc == 1 ? size[d] / size[j] * (double)cells[j] : size[d] / size[j];
-    }
-  }
-  for (int d = 0; d < 3; d++) h->ga.rg[d] = size[d] / scaled[d];
+  dmk_div_scale(cells, size, num->divergence_correction, h->ga.rg);
   // init: F = I, response at dt = 0, C_ref (grid_mech_spectral_basic.f90:213-235)
   if ((rc = constitutive(h, h->F, 0.0, false, true))) return rc;
   set_gamma_coef(h, h->C_minMaxAvg);

@@ -937,6 +937,25 @@ static int gx_even(int h, int gmax) {
 }
 static int gx_for(int L) { int g = 1024 / L; if (g > 32) g = 32; return g < 1 ? 1 : g; }
 
+// geomSize / scaledGeomSize of the divergence measure for the
+// divergence_correction option dc (0 none, 1 grid+size, 2 size),
+// spectral_utilities.f90:181-197: the median dimension (by size, or by size per
+// cell for grid+size) scales the others; equal extremes pick the last candidate
+void dmk_div_scale(const int cells[3], const double size[3], int dc, double rg[3]) {
+  double scaled[3] = {size[0], size[1], size[2]};
+  double r[3];
+  for (int d = 0; d < 3; d++) r[d] = size[d] / cells[d];
+  if (dc == 0 || dc == 1) {
+    const double* v = dc == 1 ? r : size;
+    int imin = 0, imax = 0;
+    for (int d = 1; d < 3; d++) { if (v[d] < v[imin]) imin = d; if (v[d] > v[imax]) imax = d; }
+    for (int j = 0; j < 3; j++)
+      if (j != imin && j != imax)
+        for (int d = 0; d < 3; d++) scaled[d] = dc == 1 ? size[d] / size[j] * (double)cells[j] : size[d] / size[j];
+  }
+  for (int d = 0; d < 3; d++) rg[d] = size[d] / scaled[d];
+}
+
 int dmk_fft_setup(DmkFFT* fft, const int cells[3]) {
   for (int d = 0; d < 3; d++) {
     fft->n[d] = cells[d];
@@ -956,6 +975,40 @@ int dmk_fft_setup(DmkFFT* fft, const int cells[3]) {
   fft->gx[0] = x_pairs_per_block(cells[0]);
   for (int d = 1; d < 3; d++) fft->gx[d] = gx_even(fft->h, gx_for(cells[d]));
   return 0;
+}
+
+// twiddles exp(-2 pi i k / L), k < L, of the three axes into caller memory
+__global__ void twiddle_kernel(double2* __restrict__ tw, int L0, int L1, int L2) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int L, k;
+  if (i < L0) { L = L0; k = i; }
+  else if (i < L0 + L1) { L = L1; k = i - L0; }
+  else if (i < L0 + L1 + L2) { L = L2; k = i - L0 - L1; }
+  else return;
+  double sn, cs;
+  sincospi(-2.0 * (double)k / (double)L, &sn, &cs);
+  tw[i] = make_double2(cs, sn);
+}
+
+int dmk_fft_setup_into(DmkFFT* fft, const int cells[3], double2* tw, hipStream_t s) {
+  for (int d = 0; d < 3; d++) {
+    fft->n[d] = cells[d];
+    fft->plan[d] = make_plan(cells[d]);
+    if (fft->plan[d].nstages > 16) return 1;
+  }
+  fft->tw[0] = tw;
+  fft->tw[1] = tw + cells[0];
+  fft->tw[2] = tw + cells[0] + cells[1];
+  const int tot = cells[0] + cells[1] + cells[2];
+  hipLaunchKernelGGL(twiddle_kernel, dim3((tot + 255) / 256), dim3(256), 0, s, tw, cells[0], cells[1], cells[2]);
+  fft->h = cells[0] / 2 + 1;
+  fft->gx[0] = x_pairs_per_block(cells[0]);
+  for (int d = 1; d < 3; d++) fft->gx[d] = gx_even(fft->h, gx_for(cells[d]));
+  fft->xy_threads = 0;
+  fft->xy_ip = 0;
+  fft->zgz_g = 0;
+  fft->zgz_threads = 0;
+  return hipGetLastError() == hipSuccess ? 0 : 2;
 }
 
 void dmk_fft_free(DmkFFT* fft) {

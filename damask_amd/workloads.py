@@ -52,6 +52,16 @@ BCC_PHASE = {
 BCC_NS_PHASE = copy.deepcopy(BCC_PHASE)
 BCC_NS_PHASE["mechanical"]["plastic"]["a_non-Schmid"] = [[0.1] * 6]
 
+# BCC_NS_PHASE with the critical resolved shear stresses of that regression
+# material (xi_0 = 405.8 MPa, xi_inf = 872.9 MPa).  At the 95 MPa of
+# BCC_PHASE the non-Schmid terms make the material-point iterations of most
+# cells fail near yield, so full runs cut back over and over (what the
+# sub-stepping tests want, and what makes tangent-linear CPU runs of a whole
+# load case take hours); with these values yield sets in near 0.6 % strain and
+# the iterations converge as for the Schmid phase
+BCC_NS_STRONG_PHASE = copy.deepcopy(BCC_NS_PHASE)
+BCC_NS_STRONG_PHASE["mechanical"]["plastic"].update({"xi_0_sl": [405.8e6, 405.8e6], "xi_inf_sl": [872.9e6, 872.9e6]})
+
 
 def material_dict(quaternions: np.ndarray, phase: dict | None = None, phase_name: str = "pheno") -> dict:
     return {

@@ -11,7 +11,7 @@ pytestmark = pytest.mark.gpu
 
 def _case(cells=8, grains=4, seed=3, t=8.0, N=5, lattice="cF"):
     from damask_amd import config, workloads
-    phase = workloads.BCC_NS_PHASE if lattice == "cI" else None
+    phase = workloads.BCC_NS_STRONG_PHASE if lattice == "cI" else None
     grid, mat = workloads.polycrystal([cells] * 3, grains, seed, phase=phase)
     mat = config.parse_material(mat)
     mat.phases[0].atol_xi, mat.phases[0].atol_gamma = 1e-8, 1e-14
@@ -124,12 +124,14 @@ def test_adjoint_gradient_matches_forward_mode(gpu_lib, lattice):
     """The GPU adjoint against the FP64 forward-mode reference gradient: the
     CPU oracle's tangent-linear (dual-number) build, one parameter direction
     per run, exact up to the solvers' tolerances (no difference quotient).
-    cF: 12 systems; cI: two families of 12, non-Schmid terms on the {110}<111> family."""
+    cF: 12 systems; cI: two families of 12, non-Schmid terms on the {110}<111>
+    family (the regression material's coefficients and critical stresses)."""
     from concurrent.futures import ThreadPoolExecutor
 
     from damask_amd import adjoint
     from oracle import pyoracle
-    grid, mat, lcs, num = _case(cells=4, grains=3, N=4, lattice=lattice)
+    # the cI phase yields near 0.6 %: four increments to 1.6 % reach well into plastic flow
+    grid, mat, lcs, num = _case(cells=4, grains=3, N=4, lattice=lattice, t=8.0 if lattice == "cF" else 16.0)
     s, P = adjoint.run_curve(grid, mat, lcs, num)
     n = P.shape[0]
     w = np.linspace(1.0, 2.0, n) / 1e6

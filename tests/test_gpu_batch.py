@@ -38,8 +38,8 @@ def test_lockstep_batch_matches_single_runs(gpu_lib):
     from damask_amd import adjoint, batch, config, workloads
     from damask_amd.grid import GridMechSpectralBasic
     items = batch.random_batch(7, master_seed=5, edges=(8, 12), grains=(5, 12), increments=4, strain=0.004)
-    grid, mat = workloads.polycrystal([7, 6, 5], 6, 21, size=(1.0, 0.9, 0.8), phase=workloads.BCC_NS_PHASE)
-    _, lcs = config.parse_load(workloads.tension_x(3.0, 3))
+    grid, mat = workloads.polycrystal([7, 6, 5], 6, 21, size=(1.0, 0.9, 0.8), phase=workloads.BCC_NS_STRONG_PHASE)
+    _, lcs = config.parse_load(workloads.tension_x(12.0, 3))
     items.append((grid, config.parse_material(mat), lcs[0]))
     b = batch.Batch(items)
     b.tape()

@@ -126,11 +126,12 @@ def test_64cube_200_grains(gpu_lib, oracle):
     assert np.all(np.diff(rec[:, 3]) > 0.0)                  # hardening: P_xx rises
 
 
-def test_batch_256_rves(gpu_lib):
+def test_batch_256_rves(gpu_lib, monkeypatch):
     """the 256-RVE batched forward + adjoint; 16 RVEs drawn with a seeded
     generator rerun alone give the same curves and gradients bit for bit"""
     from damask_amd import adjoint, batch
     from damask_amd.grid import GridMechSpectralBasic
+    monkeypatch.setenv("DMK_BATCH_PROGRESS", "1")
     items = batch.random_batch(256, 0)
     edges = sorted({int(g.cells[0]) for g, _, _ in items})
     assert edges == [8, 12, 16, 20, 24, 32]
@@ -212,6 +213,7 @@ def test_32cube_adjoint_against_central_differences(gpu_lib):
         lp = np.mean((curve(tp) - target) ** 2)
         lm = np.mean((curve(tm) - target) ** 2)
         fd[k] = (lp - lm) / (2.0 * h * theta0[k])
+        print(f"central difference {k + 1} of {theta0.size} done", flush=True)
     names = adjoint.param_names(mat.phases[0])
     rel = np.abs(g - fd) / np.abs(fd)
     for k in range(theta0.size):
@@ -219,7 +221,7 @@ def test_32cube_adjoint_against_central_differences(gpu_lib):
     assert np.all(rel < 1e-5), dict(zip(names, rel))
 
 
-def test_mixed_load_batch_64x48_checkpointed(gpu_lib):
+def test_mixed_load_batch_64x48_checkpointed(gpu_lib, monkeypatch):
     """64 RVEs of 48^3, 300 grains, the 24-system BCC phase, seeded random
     symmetric deformation rates, 50 increments: batched forward on tapes with a
     checkpoint every 8 increments, batched gradient of the curve loss (the
@@ -228,6 +230,7 @@ def test_mixed_load_batch_64x48_checkpointed(gpu_lib):
     gradients bit for bit."""
     from damask_amd import adjoint, batch
     from damask_amd.grid import GridMechSpectralBasic
+    monkeypatch.setenv("DMK_BATCH_PROGRESS", "1")
     items = batch.mixed_load_batch(64, 0, edge=48, grains=300, increments=50)
     b = batch.Batch(items)
     b.tape(segment=8)

@@ -99,13 +99,14 @@ def test_scheme_converges_to_basic_solution(gpu_lib, scheme):
 
 @pytest.mark.parametrize("scheme", SCHEMES)
 def test_scheme_bcc_odd_grid(gpu_lib, scheme):
-    """cI phase with N_sl = [12, 12] and non-Schmid terms on a 7 x 6 x 5 grid
+    """cI phase with N_sl = [12, 12] and non-Schmid terms (the regression
+    material's coefficients and critical stresses) on a 7 x 6 x 5 grid
     (odd N1: no Nyquist plane in x; odd N3), uniaxial tension with lateral
     stress BCs: same equilibrium as the basic scheme at tight tolerances"""
     from damask_amd import config, workloads
-    g, m = workloads.polycrystal([7, 6, 5], 6, 21, size=(1.0, 0.9, 0.8), phase=workloads.BCC_NS_PHASE)
+    g, m = workloads.polycrystal([7, 6, 5], 6, 21, size=(1.0, 0.9, 0.8), phase=workloads.BCC_NS_STRONG_PHASE)
     mat = config.parse_material(m)
-    _, lcs = config.parse_load(workloads.tension_x(6.0, 12, 4))
+    _, lcs = config.parse_load(workloads.tension_x(12.0, 12, 4))
     num = config.Numerics(eps_abs_div=1e-8, eps_rel_div=1e-11, eps_abs_P=1e-3, eps_rel_P=1e-11,
                           eps_abs_curl=1e-13, eps_rel_curl=1e-11, N_iter_max=400, snes_max_it=400)
     rb, _ = _run("spectral_basic", g, mat, lcs, num)
