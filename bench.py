@@ -489,7 +489,8 @@ def main():
                    "state_iterations_per_update": state / max(updates, 1),
                    "init_s": t_init,
                    "fft_path": {"xy_threads": fft_path[0], "xy_in_place": fft_path[1],
-                                "zgz_columns": fft_path[2], "zgz_threads": fft_path[3]},
+                                "zgz_columns": fft_path[2], "zgz_threads": fft_path[3],
+                                "zgz_in_place": fft_path[4]},
                    "stress_kernel_ms": round(kt["stress"][0], 3),
                    "kernel_ms_per_increment": breakdown,
                    "cells_x_increments_per_s": args.cells**3 * args.steps / elapsed},

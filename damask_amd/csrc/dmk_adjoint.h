@@ -413,7 +413,12 @@ DHI void adj_vjp_finish(const DmkPhaseDev& ph, const AdjPoint& pt, const AdjFwd&
     for (int k = 0; k < 9; k++) SbG[k] = fma(w, Pa[k], SbG[k]);
     if (params) {
       double gd = acc.v(AV_GD, c);
-      thb(ADJ_TH_N * ldth + c) = gd != 0.0 ? Gc * gd * log(acc.v(AV_TM, c) / xi(c)) : 0.0;
+      // d/dn (tau/xi)^n = (tau/xi)^n ln(tau/xi) for tau > 0.  With non-Schmid
+      // terms both projections can be negative; (tau/xi)^n then exists only for
+      // integer n and has no derivative in n, taken as 0 (the C library's
+      // log domain, as the forward-mode reference does)
+      const double tm = acc.v(AV_TM, c);
+      thb(ADJ_TH_N * ldth + c) = gd != 0.0 && tm > 0.0 ? Gc * gd * log(tm / xi(c)) : 0.0;
       thb(ADJ_TH_G0 * ldth + c) = gd != 0.0 ? Gc * gd / ph.g0[c] : 0.0;
     }
   }

@@ -23,6 +23,7 @@ struct DmkFFT {
   int xy_threads = 0;
   int xy_ip = 0;                    // 1: the fused x-y kernels' single-buffer (in-place) variant
   int zgz_g = 0, zgz_threads = 0;   // > 0: z passes and Gamma fused, zgz_g columns per block
+  int zgz_ip = 0;                   // 1: the fused z kernel's single-buffer variant
 };
 
 struct GammaArgs {

@@ -2121,7 +2121,7 @@ extern "C" int dmk_mech_get_scalars(dmk_handle* h, int32_t what, double* dst) {
     case 9: std::memcpy(dst, h->C_minMaxAvgLastInc, 81 * sizeof(double)); break;
     case 12:
       dst[0] = h->fft.xy_threads; dst[1] = h->fft.xy_ip;
-      dst[2] = h->fft.zgz_g; dst[3] = h->fft.zgz_threads;
+      dst[2] = h->fft.zgz_g; dst[3] = h->fft.zgz_threads; dst[4] = h->fft.zgz_ip;
       break;
     default: dmk_set_error("unknown scalar %d", what); return DMK_ERR_ARG;
   }

@@ -33,6 +33,16 @@ __device__ __forceinline__ double2 twid(const double2* __restrict__ tw, int m, i
   return sign < 0 ? t : cconj(t);
 }
 
+// Work-groups of a launch are dispatched round-robin over the eight XCDs of
+// the MI355X, each with its own L2.  Kernels whose neighbouring blocks read
+// neighbouring columns (sharing 128-byte lines) number their blocks through
+// this map, which hands each XCD a contiguous range of logical blocks, so the
+// blocks that share a line share an L2.  A bijection of [0, nb).
+__device__ __forceinline__ unsigned xcd_block(unsigned b, unsigned nb) {
+  const unsigned q = nb >> 3, r = nb & 7u, x = b & 7u, s = b >> 3;
+  return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + s;
+}
+
 template <int R>
 __device__ __forceinline__ void butterfly(double2* v, const double2* __restrict__ tw, int L, int sign) {
   if constexpr (R == 2) {
@@ -509,7 +519,8 @@ __global__ void __launch_bounds__(256) fft_strided(double2* __restrict__ data, i
   double2* b0 = lds;
   double2* b1 = lds + gx * ldl;
   const int nchunk = (nx + gx - 1) / gx;
-  const int o = blockIdx.x / nchunk, ch = blockIdx.x - o * nchunk;
+  const int blk = (int)xcd_block(blockIdx.x, gridDim.x);
+  const int o = blk / nchunk, ch = blk - o * nchunk;
   const int x0 = ch * gx;
   const int nlx = min(gx, nx - x0);
   double2* base = data + (long)(o / M) * comp_stride + (long)(o % M) * outer_stride + x0;
@@ -663,7 +674,8 @@ __global__ void fft_zgz(double2* __restrict__ Z, GammaArgs g, FftPlan pz, const 
   __shared__ double red[1024];
   const int h = g.h, N3 = g.n[2], ldl = N3 + 1;
   const long M = (long)h * g.n[1], Nk = M * N3;
-  const long m0 = (long)blockIdx.x * gz;
+  const unsigned blk = xcd_block(blockIdx.x, gridDim.x);
+  const long m0 = (long)blk * gz;
   const int nl = (int)min((long)gz, M - m0);
   const int nlines = 9 * gz;
   double2* b0 = lds;
@@ -696,13 +708,66 @@ __global__ void fft_zgz(double2* __restrict__ Z, GammaArgs g, FftPlan pz, const 
       if (threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
       __syncthreads();
     }
-    if (threadIdx.x == 0) div_partial[blockIdx.x] = red[0];
+    if (threadIdx.x == 0) div_partial[blk] = red[0];
   }
   __syncthreads();
   double2* q = lds_fft(r, o, nlines, N3, ldl, pz, twz, +1);
   for (int i = threadIdx.x; i < 9 * N3 * gz; i += blockDim.x) {
     int j = i % gz, t = i / gz, kz = t % N3, c = t / N3;
     if (j < nl) Z[c * Nk + (long)kz * M + m0 + j] = q[(c * gz + j) * ldl + kz];
+  }
+}
+
+// fft_zgz with one LDS buffer (z plans of radix 2, 4 and 8: the in-place
+// Stockham stages of the fused x-y kernels), so twice the columns fit per block
+// and a block reads 16 gz contiguous bytes per (field, z) row.  Same arithmetic
+// as fft_zgz.
+__global__ void __launch_bounds__(512) fft_zgz_ip(double2* __restrict__ Z, GammaArgs g, FftPlan pz,
+                                                  const double2* __restrict__ twz, int gz,
+                                                  double* __restrict__ div_partial) {
+  extern __shared__ double2 lds[];
+  __shared__ double red[512];
+  const int h = g.h, N3 = g.n[2], ldl = N3 + 1;
+  const long M = (long)h * g.n[1], Nk = M * N3;
+  const unsigned blk = xcd_block(blockIdx.x, gridDim.x);
+  const long m0 = (long)blk * gz;
+  const int nl = (int)min((long)gz, M - m0);
+  const int nlines = 9 * gz;
+  double2* A = lds;
+  for (int i = threadIdx.x; i < 9 * N3 * gz; i += blockDim.x) {
+    int j = i % gz, t = i / gz, kz = t % N3, c = t / N3;
+    A[(c * gz + j) * ldl + kz] = j < nl ? Z[c * Nk + (long)kz * M + m0 + j] : make_double2(0.0, 0.0);
+  }
+  __syncthreads();
+  lds_fft_ip(A, nlines, N3, ldl, pz, twz, -1);
+  double contrib = 0.0;
+  for (int i = threadIdx.x; i < gz * N3; i += blockDim.x) {
+    int j = i % gz, kz = i / gz;
+    if (j >= nl) continue;
+    const long m = m0 + j;
+    double2 P[9];
+#pragma unroll
+    for (int c = 0; c < 9; c++) P[c] = A[(c * gz + j) * ldl + kz];
+    contrib += gamma_div_point(P, (int)(m % h), (int)(m / h), kz, m == 0 && kz == 0, g);
+    if (g.do_gamma) {
+#pragma unroll
+      for (int c = 0; c < 9; c++) A[(c * gz + j) * ldl + kz] = P[c];
+    }
+  }
+  if (g.do_div) {
+    red[threadIdx.x] = contrib;
+    __syncthreads();
+    for (int k = blockDim.x / 2; k > 0; k >>= 1) {
+      if (threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) div_partial[blk] = red[0];
+  }
+  __syncthreads();
+  lds_fft_ip(A, nlines, N3, ldl, pz, twz, +1);
+  for (int i = threadIdx.x; i < 9 * N3 * gz; i += blockDim.x) {
+    int j = i % gz, t = i / gz, kz = t % N3, c = t / N3;
+    if (j < nl) Z[c * Nk + (long)kz * M + m0 + j] = A[(c * gz + j) * ldl + kz];
   }
 }
 
@@ -1008,6 +1073,7 @@ int dmk_fft_setup_into(DmkFFT* fft, const int cells[3], double2* tw, hipStream_t
   fft->xy_ip = 0;
   fft->zgz_g = 0;
   fft->zgz_threads = 0;
+  fft->zgz_ip = 0;
   return hipGetLastError() == hipSuccess ? 0 : 2;
 }
 
@@ -1092,14 +1158,32 @@ static void launch_strided(const DmkFFT* fft, int howmany, double2* data, int ax
                        (long)h * N2, fft->plan[2], fft->tw[2], sign);
 }
 
-static size_t zgz_lds(const DmkFFT* fft, int gz) { return 2 * sizeof(double2) * 9 * gz * (fft->n[2] + 1); }
+static size_t zgz_lds(const DmkFFT* fft, int gz, int ip) {
+  return (ip ? 1 : 2) * sizeof(double2) * 9 * gz * (fft->n[2] + 1);
+}
 
-static void launch_zgz(const DmkFFT* fft, double2* Z, const GammaArgs* g, int gz, int threads, double* div_partial,
-                       hipStream_t s) {
+// the single-buffer z kernel needs a radix 2/4/8 z plan whose every stage fits
+// the per-thread butterfly slots at this block size
+static bool zgz_ip_fits(const DmkFFT* fft, int gz, int threads) {
+  const FftPlan& p = fft->plan[2];
+  for (int st = 0; st < p.nstages; st++) {
+    const int R = p.radix[st];
+    if (R != 2 && R != 4 && R != 8) return false;
+    if ((long)9 * gz * (fft->n[2] / R) > (long)IP_KMAX * threads) return false;
+  }
+  return true;
+}
+
+static void launch_zgz(const DmkFFT* fft, double2* Z, const GammaArgs* g, int gz, int threads, int ip,
+                       double* div_partial, hipStream_t s) {
   const long M = (long)fft->h * fft->n[1];
   unsigned nb = (unsigned)((M + gz - 1) / gz);
-  hipLaunchKernelGGL(fft_zgz, dim3(nb), dim3(threads), zgz_lds(fft, gz), s, Z, *g, fft->plan[2], fft->tw[2], gz,
-                     div_partial);
+  if (ip)
+    hipLaunchKernelGGL(fft_zgz_ip, dim3(nb), dim3(threads), zgz_lds(fft, gz, 1), s, Z, *g, fft->plan[2],
+                       fft->tw[2], gz, div_partial);
+  else
+    hipLaunchKernelGGL(fft_zgz, dim3(nb), dim3(threads), zgz_lds(fft, gz, 0), s, Z, *g, fft->plan[2], fft->tw[2],
+                       gz, div_partial);
 }
 
 int dmk_fft_tune(DmkFFT* fft, double2* cscratch, double* rscratch, hipStream_t s) {
@@ -1163,8 +1247,10 @@ int dmk_fft_tune(DmkFFT* fft, double2* cscratch, double* rscratch, hipStream_t s
       }
   }
   // fused z passes and Gamma against the separate kernels (a reference medium
-  // with an invertible acoustic tensor; DMK_FFT_ZGZ=0 / 1 forces separate / fused)
+  // with an invertible acoustic tensor; DMK_FFT_ZGZ=0 / 1 / 2 forces separate /
+  // fused / fused single-buffer where the z plan allows it)
   fft->zgz_g = 0;
+  fft->zgz_ip = 0;
   const char* fz = getenv("DMK_FFT_ZGZ");
   if (fft->n[2] >= 2 && !(fz && fz[0] == '0')) {
     GammaArgs g{};
@@ -1179,14 +1265,21 @@ int dmk_fft_tune(DmkFFT* fft, double2* cscratch, double* rscratch, hipStream_t s
       hipLaunchKernelGGL(gamma_div_kernel, dim3((unsigned)((Nk + 255) / 256)), dim3(256), 0, s, cscratch, g, part);
       launch_strided(fft, howmany, cscratch, 2, fft->gx[2], +1, s);
     });
-    if (fz && fz[0] == '1') sep = 1e30f;
-    for (int gz : {1, 2, 4, 8}) {
-      if (zgz_lds(fft, gz) > FFT_XY_LDS_MAX) continue;
-      for (int th : {256, 512}) {
-        float t = timed([&] { launch_zgz(fft, cscratch, &g, gz, th, part, s); });
-        if (t < sep) { sep = t; fft->zgz_g = gz; fft->zgz_threads = th; }
+    if (fz && (fz[0] == '1' || fz[0] == '2')) sep = 1e30f;
+    bool any_ip = false;
+    for (int gz : {2, 4, 8, 16})
+      for (int th : {256, 512}) any_ip |= zgz_lds(fft, gz, 1) <= FFT_XY_LDS_MAX && zgz_ip_fits(fft, gz, th);
+    for (int ip = 0; ip <= 1; ip++)
+      for (int gz : {1, 2, 4, 8, 16}) {
+        if (zgz_lds(fft, gz, ip) > FFT_XY_LDS_MAX) continue;
+        if (fz && fz[0] == '1' && ip) continue;                 // DMK_FFT_ZGZ=1: the two-buffer kernel
+        if (fz && fz[0] == '2' && any_ip && !ip) continue;      // DMK_FFT_ZGZ=2: the single-buffer kernel
+        for (int th : {256, 512}) {
+          if (ip && !zgz_ip_fits(fft, gz, th)) continue;
+          float t = timed([&] { launch_zgz(fft, cscratch, &g, gz, th, ip, part, s); });
+          if (t < sep) { sep = t; fft->zgz_g = gz; fft->zgz_threads = th; fft->zgz_ip = ip; }
+        }
       }
-    }
   }
   hipEventDestroy(e0);
   hipEventDestroy(e1);
@@ -1222,7 +1315,7 @@ long dmk_gamma_conv_partials(const DmkFFT* fft) {
 int dmk_gamma_conv(const DmkFFT* fft, double2* Z, const GammaArgs* g, double* div_partial, int* nb, hipStream_t s) {
   const long M = (long)fft->h * fft->n[1], Nk = M * fft->n[2];
   if (fft->zgz_g) {
-    launch_zgz(fft, Z, g, fft->zgz_g, fft->zgz_threads, div_partial, s);
+    launch_zgz(fft, Z, g, fft->zgz_g, fft->zgz_threads, fft->zgz_ip, div_partial, s);
     *nb = (int)((M + fft->zgz_g - 1) / fft->zgz_g);
   } else {
     if (fft->n[2] > 1) launch_strided(fft, 9, Z, 2, fft->gx[2], -1, s);

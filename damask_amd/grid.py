@@ -176,7 +176,7 @@ class GridMechSpectralBasic:
 
     def scalar(self, name: str) -> np.ndarray:
         n = {"C_volAvg": 81, "C_minMaxAvg": 81, "C_volAvgLastInc": 81, "C_minMaxAvgLastInc": 81,
-             "C_minMaxAvgRestart": 81, "err": 2, "err_curl": 1, "fft_path": 4}.get(name, 9)
+             "C_minMaxAvgRestart": 81, "err": 2, "err_curl": 1, "fft_path": 5}.get(name, 9)
         out = np.zeros(n)
         _lib.check(self.lib.dmk_mech_get_scalars(self.h, SCALARS[name], _lib.dptr(out)), "dmk_mech_get_scalars")
         if n == 9:

@@ -144,9 +144,10 @@ int dmk_mech_set_field(dmk_handle* h, int32_t what, const double* src);
  * 11 C_minMaxAvgRestart (81, get only): the stiffness the Gamma operator was
  * built from, which restartWrite stores as C_minMaxAvg
  * (grid_mech_spectral_basic.f90:235,388,408), 12 transform path chosen at
- * setup (4, get only; no counterpart in the reference): threads of the fused
+ * setup (5, get only; no counterpart in the reference): threads of the fused
  * x-y kernels (0: separate passes), 1 if their single-buffer variant runs,
- * columns per block of the fused z+Gamma+z kernel (0: separate), its threads.  Setting 4 also sets the Gamma
+ * columns per block of the fused z+Gamma+z kernel (0: separate), its threads,
+ * 1 if its single-buffer variant runs.  Setting 4 also sets the Gamma
  * operator, C_minMaxAvgRestart and, for the polarization scheme, C_scale and
  * S_scale, as the solver init does after reading restart data. */
 int dmk_mech_get_scalars(dmk_handle* h, int32_t what, double* dst);

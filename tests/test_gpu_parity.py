@@ -131,16 +131,21 @@ def test_fft_roundtrip_and_parity(gpu_lib, oracle, cells, xy, monkeypatch):
         assert np.abs(y / N - x).max() < 5e-15 * max(1.0, np.log2(N))
 
 
-@pytest.mark.parametrize("xy,zgz", [("0", "0"), ("1", "1"), ("0", "1"), ("2", "1")])
+@pytest.mark.parametrize("xy,zgz", [("0", "0"), ("1", "1"), ("0", "1"), ("2", "1"), ("2", "2")])
 @pytest.mark.parametrize("cells", [[6, 7, 8], [8, 8, 8], [5, 6, 4], [6, 5, 1], [16, 32, 4]])
 def test_gamma_and_div(gpu_lib, oracle, cells, xy, zgz, monkeypatch):
     """Gamma convolution through the residual's transform path: separate or
-    fused x-y passes (DMK_FFT_XY), separate or fused z passes + Gamma (DMK_FFT_ZGZ)"""
+    fused x-y passes (DMK_FFT_XY), separate or fused z passes + Gamma
+    (DMK_FFT_ZGZ; 2: the single-buffer fused kernel where the z plan is radix
+    2/4/8)"""
     from damask_amd import _lib
     monkeypatch.setenv("DMK_FFT_XY", xy)
     monkeypatch.setenv("DMK_FFT_ZGZ", zgz)
     grid, mat = _poly(cells, 5, 2)
     g, o = _pair(grid, mat, oracle=oracle)
+    if zgz == "2" and cells[2] > 1:
+        # the single-buffer kernel runs exactly where the z plan is radix 2/4/8
+        assert bool(g.scalar("fft_path")[4]) == (cells[2] & (cells[2] - 1) == 0), g.scalar("fft_path")
     N = grid.n_cells
     rng = np.random.default_rng(1)
     P = 1e8 * rng.standard_normal(9 * N)

@@ -160,7 +160,10 @@ def test_polarization_update_gamma_stall_is_the_algorithms(gpu_lib, oracle):
     range is symmetric tensors, to V = P - F (C_scale:(F_tau - F - I)); the
     symmetric part of V is driven to 2e-4 Pa but its skew part (47 Pa) is left
     free, and div P = div V then stays at 6.27 Pa.  Both sides run increments
-    1 and 2 with the same iteration limit; their err_div and P_av agree."""
+    1 and 2 with the same iteration limit: both converge in increment 1 (where
+    the iteration count itself follows rounding: 46-59 iterations on either
+    side, depending on the host's reduction order), both stall in increment 2,
+    and their err_div and P_av agree."""
     import copy
     import ctypes as C
     from damask_amd import config
@@ -169,7 +172,7 @@ def test_polarization_update_gamma_stall_is_the_algorithms(gpu_lib, oracle):
     lc = copy.copy(lcs[0])
     lc.N, lc.t = 20, 3.0
     num = config.Numerics(update_gamma=True, eps_abs_div=1e-8, eps_rel_div=1e-11, eps_abs_P=1e-3,
-                          eps_rel_P=1e-11, eps_abs_curl=1e-13, eps_rel_curl=1e-11, N_iter_max=60, snes_max_it=60)
+                          eps_rel_P=1e-11, eps_abs_curl=1e-13, eps_rel_curl=1e-11, N_iter_max=100, snes_max_it=100)
     g = GridMechSpectralPolarization(grid, mat, num)
     o = oracle.Problem(grid, mat, num, scheme="spectral_polarization")
     L = oracle.lib()
@@ -186,8 +189,7 @@ def test_polarization_update_gamma_stall_is_the_algorithms(gpu_lib, oracle):
         e_g, e_o = g.scalar("err"), o.solver_get(7)
         print(f"increment {inc + 1}: gpu converged {sg.converged} err_div {e_g[0]:.4e}; "
               f"oracle converged {bool(conv_o)} err_div {e_o[0]:.4e}")
-        assert sg.converged == bool(conv_o)
-    assert not sg.converged
+        assert sg.converged == bool(conv_o) == (inc == 0)
     assert e_o[0] > 1.0 and abs(e_g[0] - e_o[0]) < 1e-3 * e_o[0]
     assert g.scalar("err_curl")[0] < 1e-12 and o.solver_get(10)[0] < 1e-12
     P_o = o.solver_get(3).reshape(3, 3).T
