@@ -54,7 +54,11 @@ EXPORTS = ["dmk_mech_init", "dmk_mech_forward", "dmk_mech_materialpoint_forward"
            "dmk_mech_profile", "dmk_mech_kernel_time", "dmk_slip_system", "dmk_adjoint_tape",
            "dmk_adjoint_record", "dmk_adjoint_info", "dmk_adjoint_backward", "dmk_adjoint_release",
            "dmk_adjoint_point_host", "dmk_mech_restart_response", "dmk_grid_set_restart", "dmk_batch_run", "dmk_batch_run_lockstep",
-           "dmk_batch_adjoint", "dmk_curl_rms", "dmk_inv_sym3333", "dmk_mech_cell_counters", "dmk_adjoint_tape_segmented"]
+           "dmk_batch_adjoint", "dmk_curl_rms", "dmk_inv_sym3333", "dmk_mech_cell_counters", "dmk_adjoint_tape_segmented",
+           "dmk_divergence_rms_noalloc"]
+
+# largest grid of dmk_divergence_rms_noalloc (include/damask_hip.h)
+DMK_NOALLOC_MAX_CELLS = 262144
 
 _lib = None
 
@@ -110,6 +114,8 @@ def load(path: str = LIB_PATH):
     L.dmk_gamma_convolution.argtypes = [vp, dp, dp, dp]
     L.dmk_divergence_rms.restype = C.c_int
     L.dmk_divergence_rms.argtypes = [vp, dp, dp]
+    L.dmk_divergence_rms_noalloc.restype = C.c_int
+    L.dmk_divergence_rms_noalloc.argtypes = [ip, dp, C.c_int32, vp, vp, vp]
     L.dmk_curl_rms.restype = C.c_int
     L.dmk_curl_rms.argtypes = [vp, dp, dp]
     L.dmk_inv_sym3333.restype = C.c_int

@@ -15,7 +15,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libdamask_hip.so")
-SOURCES = ["constitutive.hip", "spectral.hip", "linalg.hip", "adjoint.hip", "polarization.hip", "galerkin.hip", "crystal.cpp", "solver.cpp",
+SOURCES = ["constitutive.hip", "spectral.hip", "linalg.hip", "adjoint.hip", "polarization.hip", "galerkin.hip", "noalloc.hip", "crystal.cpp", "solver.cpp",
            "batch.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",

@@ -175,6 +175,20 @@ int dmk_gamma_convolution(dmk_handle* h, const double* field, const double aim[9
 /* utilities_divergenceRMS (spectral_utilities.f90:650-691) of a (3,3,N) host field */
 int dmk_divergence_rms(dmk_handle* h, const double* field, double* out);
 
+/* utilities_divergenceRMS (spectral_utilities.f90:650-691) of a device-resident
+ * (3,3,N) field, allocating no device memory.  No counterpart in the reference:
+ * a variant of dmk_divergence_rms for callers that cannot allocate (code under
+ * HIP graph capture, tight serving loops).  It needs no handle: cells, size and
+ * divergence_correction (0 none, 1 grid+size, 2 size) describe the grid, and
+ * the RMS is written to rms_dev (one double, device memory) on stream (a
+ * hipStream_t, NULL for the default stream); nothing is synchronised.  The
+ * transform runs in one library-owned device array per GPU, sized for at most
+ * DMK_NOALLOC_MAX_CELLS cells and 1024 cells per axis (larger grids are refused
+ * with DMK_ERR_ARG); calls that overlap on one GPU share that array and race. */
+#define DMK_NOALLOC_MAX_CELLS 262144
+int dmk_divergence_rms_noalloc(const int32_t cells[3], const double size[3], int32_t divergence_correction,
+                               const double* field_dev, double* rms_dev, void* stream);
+
 /* utilities_curlRMS (src/grid/spectral_utilities.f90:697-757) of a (3,3,N)
  * field, the convergence measure of the spectral-polarization scheme; on the
  * x = 0 and x = N1/2 frequency planes it sums what the reference sums there
