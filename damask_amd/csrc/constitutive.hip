@@ -16,12 +16,15 @@
 //   Q_a = C : P_nS,a (tabulated per phase), vec row-major (math_33to9),
 // which is assembled in registers and solved by Householder QR (dmk_dense.h).
 //
-// Register budget.  The 9x9 system (81 doubles) is the only large register
-// object.  Slip-system loops are rolled so the phase tables are read through
-// scalar loads inside the loop instead of being hoisted into VGPRs; the
-// per-system vectors of a lane (xi, rates, rate derivatives) live in LDS,
-// [system][lane], one wavefront per work-group.  Sub-step start values and
-// the state-rate history of the fixed-point iteration live in global scratch.
+// Register budget.  The 8x8 Newton system (64 doubles) is the only large
+// register object.  Slip-system loops are rolled (the kinetics loop by two) so
+// the phase tables are read through scalar loads inside the loop instead of
+// being hoisted into VGPRs; the per-system vectors of a lane (xi, rates, rate
+// derivatives) and its sub-step bookkeeping live in LDS, [system][lane], one
+// wavefront per work-group.  Sub-step start values, the state-rate history of
+// the fixed-point iteration and the line search's fallback live in global
+// scratch; which start values apply is a per-lane flag, not a per-lane
+// pointer, so no 64-bit addresses stay live across the Newton loop.
 #include "dmk_common.h"
 #include "dmk_math.h"
 #include "dmk_dense.h"
@@ -81,6 +84,9 @@ struct SlipLds {
   double xi[NSL][SWG];    // critical resolved shear stress of the current state
   double g[NSL][SWG];     // slip rates (signed), |rate| once the state rate is formed
   double w[NSL][SWG];     // d rate / d tau in the stress Newton loop, xi rate in the state loop
+  // sub-step bookkeeping of a lane, kept out of the registers the Newton needs
+  double frac[SWG], step[SWG], dt_done[SWG];
+  int nsub[SWG], fpi[SWG];
 };
 
 // kinetics_sl (src/phase_mechanical_plastic_phenopowerlaw.f90:449-488) for all
@@ -94,7 +100,7 @@ DI void kinetics(const DmkPhaseDev* __restrict__ ph, const double* Mp, const dou
     for (int k = 0; k < 9; k++) Lp[k] = 0.0;
   }
   const bool nonschmid = ph->nonschmid;
-#pragma unroll 1
+#pragma unroll 2
   for (int a = 0; a < NSL; a++) {
     double tp = 0.0;
 #pragma unroll
@@ -160,6 +166,14 @@ DI void hooke(const DmkPhaseDev* __restrict__ ph, const double* Fe, double* S) {
   hooke_sym(ph->C11, ph->C12, ph->C44, E, S);
 }
 
+// a per-lane flag re-read in every loop iteration, so that pointers selected
+// by it are formed where they are used instead of being kept live across loops
+DI bool live_flag(bool f) {
+  int v = f ? 1 : 0;
+  asm volatile("" : "+v"(v));
+  return v != 0;
+}
+
 DI void copy9(double* dst, const double* src, const Lane& l) {
 #pragma unroll
   for (int c = 0; c < 9; c++) st(dst, l, c, ld(src, l, c));
@@ -167,9 +181,9 @@ DI void copy9(double* dst, const double* src, const Lane& l) {
 
 // per-cell scratch of the stress integration ([DMK_SUB_ROWS][N]): sub-step
 // F_0, sub-step F, F_p0 and L_p0 (the last two once a sub-step has completed),
-// and the Newton line search's last accepted L_p
-enum { SUB_F0 = 0, SUB_F = 9, SUB_FP0 = 18, SUB_LP0 = 27, SUB_LSOLD = 36 };
-static_assert(SUB_LSOLD + 9 <= DMK_SUB_ROWS, "sub-step scratch rows");
+// and the Newton line search's last accepted L_p and direction
+enum { SUB_F0 = 0, SUB_F = 9, SUB_FP0 = 18, SUB_LP0 = 27, SUB_LSOLD = 36, SUB_LSDIR = 45 };
+static_assert(SUB_LSDIR + 9 <= DMK_SUB_ROWS, "sub-step scratch rows");
 
 // integrateStress (src/phase_mechanical.f90:379-584) with F_i = I, L_i = 0, for
 // the sub-step F (scratch row SUB_F) and sub-step F_p0 (Fp0_p), starting from
@@ -180,7 +194,7 @@ static_assert(SUB_LSOLD + 9 <= DMK_SUB_ROWS, "sub-step scratch rows");
 // has converged instead of after every call.
 template <int NSL, bool QP>
 DI int integrate_stress(const DmkPhaseDev* __restrict__ ph, const DmkNumMech& num, const DmkMPDev& mp,
-                        const Lane& ln, const double* Fp0_p, double dt, SlipLds<NSL>& sh, int lane, int& newton,
+                        const Lane& ln, bool started, double dt, SlipLds<NSL>& sh, int lane, int& newton,
                         double* S) {
   const double* Fsub_p = mp.sub + SUB_F * ln.sN;
   const double* xs = &sh.xi[0][lane];
@@ -188,6 +202,7 @@ DI int integrate_stress(const DmkPhaseDev* __restrict__ ph, const DmkNumMech& nu
   double* ws = &sh.w[0][lane];
   double A[9];
   {
+    const double* Fp0_p = live_flag(started) ? mp.sub + SUB_FP0 * ln.sN : mp.Fp0;
     double Fp0[9], invFp0[9], F[9];
     m_load(Fp0_p, ln, Fp0);
     if (m_invert(Fp0, invFp0)) return ST_FAIL_STRESS;
@@ -196,13 +211,12 @@ DI int integrate_stress(const DmkPhaseDev* __restrict__ ph, const DmkNumMech& nu
   }
   double Lpg[9];
   m_load(mp.Lp, ln, Lpg);
-  // the line search's last accepted iterate lives in the scratch (read back
-  // only when a step is rejected, and to form the next iterate), its last
-  // direction in registers: the first iteration is always accepted
+  // the line search's last accepted iterate and direction live in the
+  // scratch (read back only when a step is rejected, and to form the next
+  // iterate), out of the registers the Newton system needs; the first
+  // iteration is always accepted, so both are written before they are read
   double* ls_old = mp.sub + SUB_LSOLD * ln.sN;
-  double dir[9];
-#pragma unroll
-  for (int k = 0; k < 9; k++) dir[k] = 0.0;
+  double* ls_dir = mp.sub + SUB_LSDIR * ln.sN;
   double step = 1.0, res_old = DBL_MAX;
   int it = 0;
   for (;;) {
@@ -235,7 +249,7 @@ DI int integrate_stress(const DmkPhaseDev* __restrict__ ph, const DmkNumMech& nu
     } else {
       step = num.r_linesearch_Lp * step;
 #pragma unroll
-      for (int k = 0; k < 9; k++) Lpg[k] = fma(dir[k], step, ld(ls_old, ln, k));
+      for (int k = 0; k < 9; k++) Lpg[k] = fma(ld(ls_dir, ln, k), step, ld(ls_old, ln, k));
       continue;
     }
     // Jacobian, rebuilt every iteration (f_update_jacobi_Lp = 1):
@@ -253,11 +267,14 @@ DI int integrate_stress(const DmkPhaseDev* __restrict__ ph, const DmkNumMech& nu
       m_mul(A, B, Fe);
       m_mul_tn(Fe, A, G);
     }
-    double J[64], j8[8];
+    // The trace substitution x_22 = tr(res) - x_00 - x_11 is applied per slip
+    // system: column 8 of each rank-1 term is subtracted from columns 0 and 4
+    // and its product with tr(res) from the right-hand side, so no ninth column
+    // is kept.
+    double J[64];
 #pragma unroll
     for (int r = 0; r < 64; r++) J[r] = (r % 9 == 0) ? 1.0 : 0.0;
-#pragma unroll
-    for (int r = 0; r < 8; r++) j8[r] = 0.0;
+    const double tr = res[0] + res[4] + res[8];
     const bool nonschmid = ph->nonschmid;
 #pragma unroll 1
     for (int a = 0; a < NSL; a++) {
@@ -274,20 +291,16 @@ DI int integrate_stress(const DmkPhaseDev* __restrict__ ph, const DmkNumMech& nu
         m_mul_tn(G, ph->Qp[a], Y);
         u = pos ? u : -u;
       }
+      Y[0] -= Y[8];
+      Y[4] -= Y[8];
+      const double y8tr = Y[8] * tr;
 #pragma unroll
       for (int r = 0; r < 8; r++) {
         double pr = u * ph->P[a][r];
 #pragma unroll
         for (int c = 0; c < 8; c++) J[8 * r + c] = fma(pr, Y[c], J[8 * r + c]);
-        j8[r] = fma(pr, Y[8], j8[r]);
+        res[r] = fma(-pr, y8tr, res[r]);
       }
-    }
-    const double tr = res[0] + res[4] + res[8];
-#pragma unroll
-    for (int r = 0; r < 8; r++) {
-      J[8 * r] -= j8[r];
-      J[8 * r + 4] -= j8[r];
-      res[r] = fma(-j8[r], tr, res[r]);
     }
     if (dmk_qr_solve<8>(J, res)) return ST_FAIL_LP_DGESV;
     res[8] = tr - res[0] - res[4];
@@ -295,13 +308,15 @@ DI int integrate_stress(const DmkPhaseDev* __restrict__ ph, const DmkNumMech& nu
     // kept live across the solve
 #pragma unroll
     for (int k = 0; k < 9; k++) {
-      dir[k] = -res[k];
-      Lpg[k] = fma(dir[k], step, ld(ls_old, ln, k));
+      const double d = -res[k];
+      st(ls_dir, ln, k, d);
+      Lpg[k] = fma(d, step, ld(ls_old, ln, k));
     }
   }
   // the reference's error exit when the new plastic deformation is singular
   // (math_invert33 of F_p0^-1 (I - dt L_p))
   {
+    const double* Fp0_p = live_flag(started) ? mp.sub + SUB_FP0 * ln.sN : mp.Fp0;
     double Fp0[9], invFp0[9], B[9], invFp_new[9], Fp_new[9];
     m_load(Fp0_p, ln, Fp0);
     m_invert(Fp0, invFp0);
@@ -359,31 +374,32 @@ DI void finish_point(const DmkMPDev& mp, const Lane& ln, const double* Fp0_p, do
 // in mp.S.
 template <int NSL, bool QP>
 DI int integrate_state_fpi(const DmkPhaseDev* __restrict__ ph, const DmkNumMech& num, const DmkMPDev& mp,
-                           const Lane& ln, const double* xi0_p, const double* gam0_p, const double* Fp0_p,
-                           const double* S_p, double dt, SlipLds<NSL>& sh, int lane, int& newton, int& fpi) {
+                           const Lane& ln, bool started, bool s_cur, double dt, SlipLds<NSL>& sh, int lane,
+                           int& newton) {
   double* xs = &sh.xi[0][lane];
   double* gs = &sh.g[0][lane];
   double* ws = &sh.w[0][lane];
   {
     double S0[9];
-    m_load(S_p, ln, S0);
+    m_load(s_cur ? mp.S : mp.S0, ln, S0);
     kinetics<NSL, QP, false, false>(ph, S0, xs, gs, nullptr, nullptr);
   }
   if (dot_state<NSL, QP>(ph, xs, gs, ws)) return ST_FAIL_STATE;
 #pragma unroll 1
   for (int r = 0; r < NSL; r++) {
     double dx = ws[r * SWG], dgm = gs[r * SWG];
-    xs[r * SWG] = fma(dx, dt, ld(xi0_p, ln, r));
-    st(mp.gam, ln, r, fma(dgm, dt, ld(gam0_p, ln, r)));
+    const bool stf = live_flag(started);
+    xs[r * SWG] = fma(dx, dt, ld(stf ? mp.st0s : mp.xi0, ln, r));
+    st(mp.gam, ln, r, fma(dgm, dt, ld(stf ? mp.st0s + NSL * ln.sN : mp.gam0, ln, r)));
     st(mp.last1, ln, r, dx);
     st(mp.last1, ln, NSL + r, dgm);
   }
   int status = ST_FAIL_STATE;
   bool cur = false;                   // last1 holds dotState_last(:,1) when !cur
   for (int it = 1; it <= num.N_iter_state_max; it++) {
-    fpi++;
+    sh.fpi[lane]++;
     double S[9];
-    status = integrate_stress<NSL, QP>(ph, num, mp, ln, Fp0_p, dt, sh, lane, newton, S);
+    status = integrate_stress<NSL, QP>(ph, num, mp, ln, started, dt, sh, lane, newton, S);
     if (status != ST_OK) break;
     m_store(mp.S, ln, S);
     // the converged Newton iterate evaluated kinetics_sl at this (S, xi): rates
@@ -392,8 +408,9 @@ DI int integrate_state_fpi(const DmkPhaseDev* __restrict__ ph, const DmkNumMech&
     if (dot_state<NSL, QP>(ph, xs, gs, ws)) break;
     const double* w1p = opaque(cur ? mp.last2 : mp.last1);
     double* w2p = opaque(cur ? mp.last1 : mp.last2);
-    const double* x0 = opaque(xi0_p);
-    const double* g0 = opaque(gam0_p);
+    const bool stf = live_flag(started);
+    const double* x0 = opaque(stf ? mp.st0s : mp.xi0);
+    const double* g0 = opaque(stf ? mp.st0s + NSL * ln.sN : mp.gam0);
     // damper (src/phase_mechanical.f90:653-674)
     double w1v[2 * NSL];
     double d12 = 0.0, d22 = 0.0, d01 = 0.0;
@@ -444,29 +461,31 @@ DI void stress_body(const DmkMPDev& mp, const double* __restrict__ Ftarget, cons
   // isostrain partition (src/homogenization_mechanical_isostrain.f90:60-75)
   copy9(mp.Fphase, Ftarget, ln);
   // start values of the sub-step: those of the converged increment until a
-  // sub-step completes, then copies in the sub-step scratch / st0s
-  const double* pF0 = mp.F0;
-  const double* pFp0 = mp.Fp0;
-  const double* pLp0 = mp.Lp0;
-  const double* pxi0 = mp.xi0;
-  const double* pgam0 = mp.gam0;
-  const double* pS = mp.S0;          // stress of the initial state rate
+  // sub-step completes (started), then copies in the sub-step scratch / st0s;
+  // the initial state rate is taken at S_0 until a sub-step completes, then at
+  // the last completed sub-step's S (s_cur)
+  bool started = false, s_cur = false;
 
-  int newton = 0, fpi = 0, nsub = 0;
-  double dt_done = 0.0;              // time step of the last completed sub-step
+  int newton = 0;
+  sh.fpi[lane] = 0;
+  sh.nsub[lane] = 0;
+  sh.dt_done[lane] = 0.0;            // time step of the last completed sub-step
   // phase_mechanical_constitutive substepping; the first pass takes the
   // cut-back branch, which is phase_restore (src/phase_mechanical.f90:1096-1115)
-  double stepFrac = 0.0, step = 1.0 / num.r_cutback;
+  sh.frac[lane] = 0.0;
+  sh.step[lane] = 1.0 / num.r_cutback;
   int status = ST_ITERATING;
   bool todo = true;
   while (todo) {
     if (status == ST_OK) {
-      stepFrac = stepFrac + step;
-      step = fmin(1.0 - stepFrac, num.r_increase * step);
+      const double stepFrac = sh.frac[lane] + sh.step[lane];
+      const double step = fmin(1.0 - stepFrac, num.r_increase * sh.step[lane]);
+      sh.frac[lane] = stepFrac;
+      sh.step[lane] = step;
       todo = step > 0.0;
       if (todo) {
         // the completed sub-step's F_p becomes the next one's F_p0
-        finish_point<false>(mp, ln, pFp0, dt_done, sub + SUB_FP0 * ln.sN);
+        finish_point<false>(mp, ln, started ? sub + SUB_FP0 * ln.sN : mp.Fp0, sh.dt_done[lane], sub + SUB_FP0 * ln.sN);
         copy9(sub + SUB_F0 * ln.sN, sub + SUB_F * ln.sN, ln);
         copy9(sub + SUB_LP0 * ln.sN, mp.Lp, ln);
 #pragma unroll 1
@@ -474,52 +493,50 @@ DI void stress_body(const DmkMPDev& mp, const double* __restrict__ Ftarget, cons
           st(mp.st0s, ln, r, xs[r * SWG]);
           st(mp.st0s, ln, NSL + r, ld(mp.gam, ln, r));
         }
-        pF0 = sub + SUB_F0 * ln.sN;
-        pFp0 = sub + SUB_FP0 * ln.sN;
-        pLp0 = sub + SUB_LP0 * ln.sN;
-        pxi0 = mp.st0s;
-        pgam0 = mp.st0s + NSL * ln.sN;
+        started = true;
       }
     } else {
       // cut back: restore the sub-step start (F_p, S = S_0, L_p for an actual
       // cut-back, state).  F_p, S and gamma are outputs only until they are
       // read again, so they are restored where they are read: F_p0 and S come
-      // from pFp0 / pS, and gamma is overwritten by the state integration
-      step = num.r_cutback * step;
-      pS = mp.S0;
-      if (step < 1.0) copy9(mp.Lp, pLp0, ln);
+      // from the sub-step start / S_0, and gamma is overwritten by the state
+      // integration
+      const double step = num.r_cutback * sh.step[lane];
+      sh.step[lane] = step;
+      s_cur = false;
+      if (step < 1.0) copy9(mp.Lp, started ? sub + SUB_LP0 * ln.sN : mp.Lp0, ln);
 #pragma unroll 1
-      for (int r = 0; r < NSL; r++) xs[r * SWG] = ld(pxi0, ln, r);
+      for (int r = 0; r < NSL; r++) xs[r * SWG] = ld(started ? mp.st0s : mp.xi0, ln, r);
       todo = step > num.r_cutback_min;
     }
     if (todo) {
+      const double* pF0 = started ? sub + SUB_F0 * ln.sN : mp.F0;
 #pragma unroll
       for (int c = 0; c < 9; c++)
-        st(sub + SUB_F * ln.sN, ln, c, fma(step, ld(Ftarget, ln, c) - ld(mp.F0, ln, c), ld(pF0, ln, c)));
-      status = integrate_state_fpi<NSL, QP>(ph, num, mp, ln, pxi0, pgam0, pFp0, pS, step * dt, sh, lane, newton,
-                                            fpi);
-      nsub++;
+        st(sub + SUB_F * ln.sN, ln, c, fma(sh.step[lane], ld(Ftarget, ln, c) - ld(mp.F0, ln, c), ld(pF0, ln, c)));
+      status = integrate_state_fpi<NSL, QP>(ph, num, mp, ln, started, s_cur, sh.step[lane] * dt, sh, lane, newton);
+      sh.nsub[lane]++;
       if (status == ST_OK) {
-        pS = mp.S;
-        dt_done = step * dt;
+        s_cur = true;
+        sh.dt_done[lane] = sh.step[lane] * dt;
       }
     }
   }
   if (status == ST_OK) {
-    finish_point<true>(mp, ln, pFp0, dt_done, mp.Fp);
+    finish_point<true>(mp, ln, started ? sub + SUB_FP0 * ln.sN : mp.Fp0, sh.dt_done[lane], mp.Fp);
   } else {
     // the restored sub-step start, as the last cut-back left it
-    copy9(mp.Fp, pFp0, ln);
+    copy9(mp.Fp, started ? sub + SUB_FP0 * ln.sN : mp.Fp0, ln);
     copy9(mp.S, mp.S0, ln);
 #pragma unroll 1
-    for (int r = 0; r < NSL; r++) st(mp.gam, ln, r, ld(pgam0, ln, r));
+    for (int r = 0; r < NSL; r++) st(mp.gam, ln, r, ld(started ? mp.st0s + NSL * ln.sN : mp.gam0, ln, r));
   }
 #pragma unroll 1
   for (int r = 0; r < NSL; r++) st(mp.xi, ln, r, xs[r * SWG]);
   mp.status[e] = status;
   mp.n_newton[e] = newton;
-  mp.n_fpi[e] = fpi;
-  mp.n_sub[e] = nsub;
+  mp.n_fpi[e] = sh.fpi[lane];
+  mp.n_sub[e] = sh.nsub[lane];
 }
 
 template <int NSL, bool QP>
