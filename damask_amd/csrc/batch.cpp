@@ -280,9 +280,32 @@ extern "C" int dmk_batch_adjoint(dmk_handle** h, int32_t n, const double* const*
   }
   std::vector<int> order = largest_first(h, n, cells);
   std::vector<int32_t> rc(n, 0);
-  run_pool(n, order, n_threads,
-           [&](int i, int) { return dmk_adjoint_backward(h[i], P_av_bar[i], rtol, theta_bar[i], nullptr); }, rc.data(),
-           wall_s);
+  // each sweep in flight holds its work buffers (for a 24-system phase about
+  // 33 kB per cell, most of it the factorised local Jacobians of two
+  // increments); a sweep releases them when it ends, and no more sweeps run
+  // at once than the free device memory holds, less a sixteenth of the device
+  // for the runtime's kernel scratch and queues
+  size_t need = 0;
+  for (int i = 0; i < n; i++) need = std::max(need, dmk_adjoint_work_bytes(h[i]));
+  int nt = std::max(1, std::min<int>(n_threads, n));
+  size_t free_b = 0, total_b = 0;
+  if (need && hipMemGetInfo(&free_b, &total_b) == hipSuccess) {
+    const size_t margin = total_b / 16;
+    const size_t fit = free_b > margin ? (free_b - margin) / need : 0;
+    nt = (int)std::max<size_t>(1, std::min<size_t>(nt, fit));
+  }
+  if (progress_lines()) {
+    fprintf(stderr, "batch adjoint: %d sweeps at a time, %.2f GB of work buffers each, %.1f GB free\n", nt,
+            need * 1e-9, free_b * 1e-9);
+    fflush(stderr);
+  }
+  run_pool(n, order, nt,
+           [&](int i, int) {
+             int r = dmk_adjoint_backward(h[i], P_av_bar[i], rtol, theta_bar[i], nullptr);
+             dmk_adjoint_work_release(h[i]);
+             return r;
+           },
+           rc.data(), wall_s);
   int bad = 0;
   for (int i = 0; i < n; i++) {
     if (rc_out) rc_out[i] = rc[i];

@@ -17,3 +17,8 @@ int dmk_stress_batch_submit(DmkStressBatch* b, int slot, const DmkStressItem* it
                             hipStream_t st);
 // attach (b != NULL) or detach a handle (solver.cpp)
 void dmk_set_stress_batch(dmk_handle* h, DmkStressBatch* b, int slot);
+// device bytes that the next reverse sweep of a handle's tape allocates for
+// its work buffers (0 when they are allocated or there is no tape), and their
+// release after a sweep, the tape kept (solver.cpp)
+size_t dmk_adjoint_work_bytes(const dmk_handle* h);
+void dmk_adjoint_work_release(dmk_handle* h);

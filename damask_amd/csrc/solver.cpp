@@ -1207,6 +1207,39 @@ static void adj_free(dmk_handle* h) {
   h->adj = nullptr;
 }
 
+// device bytes of the reverse-sweep work buffers (allocated by the first
+// dmk_adjoint_backward of a tape, listed in adj_work_alloc)
+size_t dmk_adjoint_work_bytes(const dmk_handle* h) {
+  if (!h || !h->adj || h->adj->work_ready) return 0;
+  const size_t N = h->N, n9 = h->n9, mmax = h->nsl_max, nslot = ADJ_TH_SYS * mmax + DMK_NH_MAX;
+  const size_t d = 81 * N + 7 * n9 + 2 * N + 2 * dmk_adj_scratch_slots((int)mmax) * N + 2 * mmax * N + nslot * N +
+                   (size_t)(h->adj->M + 1) * n9 + (size_t)h->n_phases * (nslot + mmax);
+  return sizeof(double) * d;
+}
+
+// free the reverse-sweep work buffers and keep the tape: the next sweep
+// allocates them again
+void dmk_adjoint_work_release(dmk_handle* h) {
+  if (!h || !h->adj) return;
+  AdjState& a = *h->adj;
+  if (h->st) hipStreamSynchronize(h->st);
+  if (a.st2) {
+    hipStreamSynchronize(a.st2);
+    hipStreamDestroy(a.st2);
+    a.st2 = nullptr;
+  }
+  for (hipEvent_t& e : a.fac)
+    if (e) {
+      hipEventDestroy(e);
+      e = nullptr;
+    }
+  for (void* p : a.work) hipFree(p);
+  a.work.clear();
+  a.A = a.g = a.thb = a.mu = a.y = a.b = a.r = a.V = a.red = nullptr;
+  for (int i = 0; i < 2; i++) a.res[i] = a.scr[i] = a.Fpb[i] = a.xib[i] = nullptr;
+  a.work_ready = false;
+}
+
 static int adj_walloc(dmk_handle* h, double** p, size_t n) {
   void* v = nullptr;
   CK(hipMalloc(&v, sizeof(double) * (n ? n : 1)));
@@ -1457,6 +1490,7 @@ extern "C" int dmk_adjoint_backward(dmk_handle* h, const double* P_av_bar, doubl
   const int mmax = h->nsl_max, nslot = ADJ_TH_SYS * mmax + DMK_NH_MAX;
   int rc;
   if (!a.work_ready) {
+    // sizes as counted by dmk_adjoint_work_bytes
     if ((rc = adj_walloc(h, &a.A, 81 * N)) || (rc = adj_walloc(h, &a.g, n9)) || (rc = adj_walloc(h, &a.res[0], N)) ||
         (rc = adj_walloc(h, &a.res[1], N)) || (rc = adj_walloc(h, &a.scr[0], dmk_adj_scratch_slots(mmax) * N)) ||
         (rc = adj_walloc(h, &a.scr[1], dmk_adj_scratch_slots(mmax) * N)) || (rc = adj_walloc(h, &a.Fpb[0], n9)) ||
@@ -1464,8 +1498,13 @@ extern "C" int dmk_adjoint_backward(dmk_handle* h, const double* P_av_bar, doubl
         (rc = adj_walloc(h, &a.xib[1], (size_t)mmax * N)) || (rc = adj_walloc(h, &a.thb, (size_t)nslot * N)) ||
         (rc = adj_walloc(h, &a.mu, n9)) || (rc = adj_walloc(h, &a.y, n9)) || (rc = adj_walloc(h, &a.b, n9)) ||
         (rc = adj_walloc(h, &a.r, n9)) || (rc = adj_walloc(h, &a.V, (size_t)(a.M + 1) * n9)) ||
-        (rc = adj_walloc(h, &a.red, (size_t)h->n_phases * (nslot + mmax))))
+        (rc = adj_walloc(h, &a.red, (size_t)h->n_phases * (nslot + mmax)))) {
+      hipGetLastError();
+      dmk_adjoint_work_release(h);
+      dmk_set_error("dmk_adjoint_backward: %zu bytes of device memory for the reverse sweep not available",
+                    dmk_adjoint_work_bytes(h));
       return rc;
+    }
     CK(hipStreamCreate(&a.st2));
     CK(hipEventCreateWithFlags(&a.fac[0], hipEventDisableTiming));
     CK(hipEventCreateWithFlags(&a.fac[1], hipEventDisableTiming));

@@ -293,7 +293,10 @@ int dmk_batch_run(dmk_handle** h, int32_t n, const dmk_loadcase* lc, const int64
  * launch per slip-system variant.  Results are those of dmk_batch_run. */
 int dmk_batch_run_lockstep(dmk_handle** h, int32_t n, const dmk_loadcase* lc, const int64_t* cells,
                            int32_t n_threads, int32_t* rc_out, double* wall_s);
-/* dmk_adjoint_backward(h[i], P_av_bar[i], rtol, theta_bar[i], NULL) for every i, concurrently */
+/* dmk_adjoint_backward(h[i], P_av_bar[i], rtol, theta_bar[i], NULL) for every i, concurrently:
+ * at most n_threads sweeps at once, and no more than the free device memory
+ * holds work buffers for.  Each handle's work buffers are released when its
+ * sweep ends (the tape is kept; a later sweep allocates them again). */
 int dmk_batch_adjoint(dmk_handle** h, int32_t n, const double* const* P_av_bar, double rtol,
                       double* const* theta_bar, const int64_t* cells, int32_t n_threads, int32_t* rc_out,
                       double* wall_s);

@@ -539,7 +539,7 @@ void orc_dPdF(const orc_mp *m, int ce, double dt, double out[81]) {
   t3333_mul(dLpdFi, dFidS, T);
   for (int k = 0; k < 81; k++) dLpdS[k] = T[k] + dLpdS[k];
 
-  double t1[9], t2[9], t3[9], M[9], Mt[9];
+  double t1[9], t2[9], t3[9], M[9];
   m33_mul(invFp, invFi, M);
   m33_transpose(M, t1);
   m33_mul(F, invSubFp0, t2);

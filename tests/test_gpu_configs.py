@@ -163,10 +163,20 @@ def test_batch_256_rves(gpu_lib, monkeypatch):
 
 
 def _tight():
+    """Solver settings for difference quotients at a relative step of 1e-6.
+    Measured at 32^3: the divergence of the FP64 transforms levels off near
+    2e-12 |P_av| after about 20 NGMRES iterations, so a tolerance near it
+    stops the two runs of a quotient after different iteration counts, and
+    the jump of P_av between two counts, amplified by the 1e-6 step, swamps the
+    derivative.  Every increment therefore runs N_iter_min = 40 iterations
+    (grid_mech_spectral_basic.f90:436-441 requires totalIter >= N_iter_min
+    before the tolerances count), by which both runs sit at the rounding floor,
+    and the tolerances (1e-9 of |P_av|) are then met; the material-point
+    iterations converge to 1e-11 of L_p and 1e-13 of the state."""
     from damask_amd.config import Numerics
-    return Numerics(eps_abs_div=1e-10, eps_rel_div=1e-13, eps_abs_P=1e-4, eps_rel_P=1e-13,
-                    N_iter_max=600, snes_max_it=600, eps_rel_state=1e-14, N_iter_state_max=200,
-                    eps_rel_Lp=1e-13, eps_abs_Lp=1e-17, N_iter_Lp_max=200)
+    return Numerics(N_iter_min=40, eps_abs_div=1e-8, eps_rel_div=1e-9, eps_abs_P=1e-3, eps_rel_P=1e-9,
+                    N_iter_max=400, snes_max_it=400, eps_rel_state=1e-13, N_iter_state_max=200,
+                    eps_rel_Lp=1e-11, eps_abs_Lp=1e-16, N_iter_Lp_max=200)
 
 
 def test_32cube_adjoint_against_central_differences(gpu_lib):
@@ -174,29 +184,36 @@ def test_32cube_adjoint_against_central_differences(gpu_lib):
     deviation of the P_xx curve from the curve at seeded parameters perturbed
     within +-10 %.  The adjoint gradient with respect to the 12 parameters
     (xi_0, xi_inf, h_0, n, a, h_sl-sl[7]) against central differences of the
-    GPU forward run at a relative step of 1e-6, every parameter within 1e-5
-    relative, no absolute floor.  Solver and material-point tolerances are
-    tightened to the rounding level so that the differences resolve the
-    derivative."""
+    GPU forward run, both runs of a quotient with the same spectral iteration
+    counts in every increment (40, see _tight):
+
+    * at a relative step of 1e-4 every parameter within 1e-5 relative;
+    * at a relative step of 1e-6 every parameter within 1e-5 relative, or,
+      where the quotient's own noise is larger, within three times that noise.
+      The noise is measured, not assumed: P_av after 40 and after 41
+      iterations of every increment differs by the FP64 floor of the spectral
+      solve, and that difference, carried through the loss and divided by the
+      step, bounds what a quotient can resolve for the parameter.  The
+      parameters the noise limits are printed."""
     from damask_amd import adjoint
     from damask_amd.grid import GridMechSpectralBasic
     grid, mat, lc = _workload(32, 50, 30.0, 60)
-    mat.phases[0].atol_xi, mat.phases[0].atol_gamma = 1e-9, 1e-15
+    mat.phases[0].atol_xi, mat.phases[0].atol_gamma = 1e-8, 1e-14
     num = _tight()
     theta0 = adjoint.get_params(mat.phases[0])
     rng = np.random.default_rng(1)
     theta_t = theta0 * (1.0 + 0.1 * rng.uniform(-1.0, 1.0, theta0.size))
 
-    def curve(theta):
+    def curve(theta, nm=num):
         m2 = copy.deepcopy(mat)
         m2.phases[0] = adjoint.set_params(mat.phases[0], theta)
-        s = GridMechSpectralBasic(grid, m2, num)
+        s = GridMechSpectralBasic(grid, m2, nm)
         r = s.run_loadcase(lc)
         s.close()
-        assert r["rc"] == 0
-        return adjoint._P33(r["P_av"])[:, 0, 0]
+        assert r["rc"] == 0, f"forward run did not converge (increments done: {int((r['stats'][:, 0] > 0).sum())})"
+        return adjoint._P33(r["P_av"])[:, 0, 0], r["stats"][:, 2]
 
-    target = curve(theta_t)
+    target, _ = curve(theta_t)
     s, P = adjoint.run_curve(grid, mat, [lc], num)
     n = P.shape[0]
     assert n == 60
@@ -204,21 +221,38 @@ def test_32cube_adjoint_against_central_differences(gpu_lib):
     Pb[:, 0, 0] = 2.0 * (P[:, 0, 0] - target) / n
     g, st = adjoint.curve_gradient(s, 0, Pb, rtol=1e-12)
     s.close()
-    h = 1e-6
-    fd = np.zeros_like(theta0)
-    for k in range(theta0.size):
-        tp, tm = theta0.copy(), theta0.copy()
-        tp[k] *= 1.0 + h
-        tm[k] *= 1.0 - h
-        lp = np.mean((curve(tp) - target) ** 2)
-        lm = np.mean((curve(tm) - target) ** 2)
-        fd[k] = (lp - lm) / (2.0 * h * theta0[k])
-        print(f"central difference {k + 1} of {theta0.size} done", flush=True)
+    # the floor: the same run with one more iteration per increment
+    num41 = _tight()
+    num41.N_iter_min = 41
+    c40, _ = curve(theta0)
+    c41, i41 = curve(theta0, num41)
+    assert np.all(i41 == 41)
+    dP = np.abs(c41 - c40)
+    dL = 2.0 * np.mean(np.abs(c40 - target) * dP)            # loss change the floor can cause
+    print(f"target and adjoint done; spectral floor max |dP_xx|/|P_xx| = {(dP / np.abs(c40)).max():.2e}", flush=True)
     names = adjoint.param_names(mat.phases[0])
-    rel = np.abs(g - fd) / np.abs(fd)
-    for k in range(theta0.size):
-        print(f"{names[k]:12s} adjoint {g[k]: .12e}  fd {fd[k]: .12e}  rel {rel[k]:.2e}")
-    assert np.all(rel < 1e-5), dict(zip(names, rel))
+    for h in (1e-4, 1e-6):
+        fd = np.zeros_like(theta0)
+        for k in range(theta0.size):
+            tp, tm = theta0.copy(), theta0.copy()
+            tp[k] *= 1.0 + h
+            tm[k] *= 1.0 - h
+            cp, ip = curve(tp)
+            cm, im = curve(tm)
+            assert np.array_equal(ip, im) and np.all(ip == 40)
+            fd[k] = (np.mean((cp - target) ** 2) - np.mean((cm - target) ** 2)) / (2.0 * h * theta0[k])
+        rel = np.abs(g - fd) / np.abs(fd)
+        noise = dL / (h * np.abs(theta0) * np.abs(fd))
+        print(f"step {h:g}:", flush=True)
+        for k in range(theta0.size):
+            print(f"  {names[k]:12s} adjoint {g[k]: .12e}  fd {fd[k]: .12e}  rel {rel[k]:.2e}  "
+                  f"quotient noise {noise[k]:.1e}")
+        if h == 1e-4:
+            assert np.all(rel < 1e-5), dict(zip(names, rel))
+        else:
+            limited = [names[k] for k in range(theta0.size) if rel[k] >= 1e-5]
+            print("  limited by the quotient's noise:", limited or "none")
+            assert np.all((rel < 1e-5) | (rel < 3.0 * noise)), dict(zip(names, rel))
 
 
 def test_mixed_load_batch_64x48_checkpointed(gpu_lib, monkeypatch):
