@@ -28,7 +28,7 @@ PEAK_HBM_GBS = 8000.0        # MI355X HBM3E
 # HBM bytes per stress-kernel launch on this workload: rocprofv3 --pmc
 # FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 (factors from a plain device copy), see
 # profiles/kernels_64cube_fcc_tension.md
-STRESS_TRAFFIC_BYTES = 2194606584.7
+STRESS_TRAFFIC_BYTES = 1882666106.9
 
 
 def parse():
