@@ -54,7 +54,7 @@ def parse():
     ap.add_argument("--batch", type=int, default=256,
                     help="RVEs of the batched forward + adjoint measurement (0: skip)")
     ap.add_argument("--batch-threads", type=int, default=16)
-    ap.add_argument("--lockstep-threads", type=int, default=32,
+    ap.add_argument("--lockstep-threads", type=int, default=256,
                     help="RVEs in flight of the lockstep batched forward (one host thread each; 0: skip)")
     ap.add_argument("--only-batch", type=int, default=0,
                     help="only run the batched measurement and print it as one JSON line")
@@ -180,8 +180,9 @@ def batched_cost(args):
         ul = sum(s.counters()["updates"] for s in b.solvers)
         b.close()
         lock = {"threads": args.lockstep_threads, "forward_s": rl["wall_s"], "updates_per_s": ul / rl["wall_s"],
-                "note": "dmk_batch_run_lockstep: the stress integrations of the RVEs in flight share one launch "
-                        "per residual evaluation round"}
+                "note": "dmk_batch_run_lockstep: the residual evaluations of the RVEs in flight run as rounds, "
+                        "one launch per kernel over all RVEs of a round (the transforms per grid size) and one "
+                        "host synchronisation per round"}
     b = batch.Batch(items)
     b.tape()
     rt = b.run(threads=args.batch_threads)

@@ -11,6 +11,17 @@
 // over all of them on the batch's stream, and every RVE's stream waits for it.
 // The many small per-RVE launches, which the few hardware queues serialise,
 // become one launch that fills the GPU.
+//
+// For the basic scheme the whole residual evaluation joins the round: each
+// RVE submits, with its stress items, the arguments of its response
+// reductions, transform with Gamma, dot products and report (DmkResidDev).
+// The round then runs every kernel once over all its RVEs (the transform
+// kernels once per FFT configuration), gathers the reports into one buffer
+// and copies it to the host: one host synchronisation per round instead of
+// one per RVE.  The kernels run the per-block code of the single-RVE
+// kernels, so each RVE's results are those of its own run, bit for bit.
+// DMK_LOCKSTEP=stress in the environment keeps the residual evaluations per
+// RVE (the stress integrations alone are batched).
 #include <algorithm>
 #include <cstdio>
 #include <cstdlib>
@@ -24,6 +35,7 @@
 
 #include "dmk_common.h"
 #include "dmk_batch.h"
+#include "dmk_resid.h"
 #include "dmk_linalg.h"
 #include "../../include/damask_hip.h"
 
@@ -32,6 +44,9 @@ struct DmkStressBatch {
     int slot, n;
     DmkStressItem items[DMK_MAX_PHASES];
     int key[DMK_MAX_PHASES];
+    bool resid;                      // the rest of a residual evaluation follows the stress
+    const DmkFFT* fft;
+    DmkResidDev rq;
   };
   static constexpr int NSTAGE = 4;   // staging buffers of the item tables in flight
   std::mutex mu;
@@ -48,13 +63,27 @@ struct DmkStressBatch {
   hipEvent_t staged[NSTAGE] = {};
   int cap = 0, stage = 0;
   int64_t launches = 0;
+  // residual evaluations of the round: argument tables, reports (device and
+  // pinned host), and per slot where its last report went
+  bool resid = false;
+  int slots = 0;
+  DmkResidDev* rhost[NSTAGE] = {};
+  DmkResidDev* rdev[NSTAGE] = {};
+  double* rep_dev[NSTAGE] = {};
+  double* rep_host[NSTAGE] = {};
+  std::vector<int> rep_stage, rep_idx, order;
 };
 
 namespace {
 
-int batch_init(DmkStressBatch* b, int slots) {
+int batch_init(DmkStressBatch* b, int slots, bool resid) {
   b->cap = slots * DMK_MAX_PHASES;
+  b->slots = slots;
+  b->resid = resid;
   b->reqs.reserve(slots);
+  b->rep_stage.assign(slots, 0);
+  b->rep_idx.assign(slots, 0);
+  b->order.reserve(slots);
   if (hipStreamCreateWithFlags(&b->st, hipStreamNonBlocking) != hipSuccess) return DMK_ERR_HIP;
   for (auto& e : b->done)
     if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) return DMK_ERR_HIP;
@@ -66,6 +95,12 @@ int batch_init(DmkStressBatch* b, int slots) {
         hipMalloc((void**)&b->dev[s], sizeof(DmkStressItem) * b->cap) != hipSuccess ||
         hipEventCreateWithFlags(&b->staged[s], hipEventDisableTiming) != hipSuccess)
       return DMK_ERR_HIP;
+    if (resid && (hipHostMalloc((void**)&b->rhost[s], sizeof(DmkResidDev) * slots, hipHostMallocDefault) != hipSuccess ||
+                  hipMalloc((void**)&b->rdev[s], sizeof(DmkResidDev) * slots) != hipSuccess ||
+                  hipMalloc((void**)&b->rep_dev[s], sizeof(double) * REP_SIZE * slots) != hipSuccess ||
+                  hipHostMalloc((void**)&b->rep_host[s], sizeof(double) * REP_SIZE * slots, hipHostMallocDefault) !=
+                      hipSuccess))
+      return DMK_ERR_HIP;
   }
   return 0;
 }
@@ -76,14 +111,72 @@ void batch_free(DmkStressBatch* b) {
     if (b->host[s]) hipHostFree(b->host[s]);
     if (b->dev[s]) hipFree(b->dev[s]);
     if (b->staged[s]) hipEventDestroy(b->staged[s]);
+    if (b->rhost[s]) hipHostFree(b->rhost[s]);
+    if (b->rdev[s]) hipFree(b->rdev[s]);
+    if (b->rep_dev[s]) hipFree(b->rep_dev[s]);
+    if (b->rep_host[s]) hipHostFree(b->rep_host[s]);
   }
   for (auto& e : b->ev_in) if (e) hipEventDestroy(e);
   for (auto& e : b->done) if (e) hipEventDestroy(e);
   if (b->st) hipStreamDestroy(b->st);
 }
 
+// FFT configurations that may share one launch of the transform kernels
+bool same_fft(const DmkFFT* a, const DmkFFT* c) {
+  for (int d = 0; d < 3; d++)
+    if (a->n[d] != c->n[d]) return false;
+  return a->xy_threads == c->xy_threads && a->xy_ip == c->xy_ip && a->zgz_g == c->zgz_g &&
+         a->zgz_threads == c->zgz_threads && a->zgz_ip == c->zgz_ip;
+}
+bool fft_less(const DmkFFT* a, const DmkFFT* c) {
+  const int ka[8] = {a->n[0], a->n[1], a->n[2], a->xy_threads, a->xy_ip, a->zgz_g, a->zgz_threads, a->zgz_ip};
+  const int kc[8] = {c->n[0], c->n[1], c->n[2], c->xy_threads, c->xy_ip, c->zgz_g, c->zgz_threads, c->zgz_ip};
+  return std::lexicographical_compare(ka, ka + 8, kc, kc + 8);
+}
+
+// with b->mu held, after the stress launches of stage s: the rest of the
+// round's residual evaluations (their table uploaded to rdev[s] already)
+int resid_launch_locked(DmkStressBatch* b, int s, int nr) {
+  const DmkResidDev* R = b->rdev[s];
+  int max_nc_resp = 0, max_nc_dots = 0, max_jobs = 0, max_sums = 0;
+  for (int k = 0; k < nr; k++) {
+    const DmkResidDev& q = b->rhost[s][k];
+    max_nc_resp = std::max(max_nc_resp, q.nc_resp);
+    max_nc_dots = std::max(max_nc_dots, q.nc_dots);
+    max_jobs = std::max(max_jobs, q.J.n);
+    max_sums = std::max(max_sums, q.S.k);
+  }
+  if (dmk_resid_reduce_batch(R, nr, max_nc_resp, b->st)) return DMK_ERR_HIP;
+  for (int k0 = 0; k0 < nr;) {
+    const DmkFFT* f = b->reqs[b->order[k0]].fft;
+    int k1 = k0 + 1;
+    while (k1 < nr && same_fft(f, b->reqs[b->order[k1]].fft)) k1++;
+    const int rc = dmk_transform_batch(f, R + k0, k1 - k0, b->st);
+    if (rc == 1) {
+      // no fused kernels for this configuration: each RVE's transform on its own
+      for (int k = k0; k < k1; k++) {
+        const DmkResidDev& q = b->rhost[s][k];
+        const DmkFFT* fk = b->reqs[b->order[k]].fft;
+        int nb = 0;
+        if (dmk_fft_r2c_xy(fk, 9, q.Pnew, q.Z, b->st) || dmk_gamma_conv(fk, q.Z, &q.g, q.divp, &nb, b->st) ||
+            dmk_fft_c2r_xy(fk, 9, q.Z, q.r, b->st))
+          return DMK_ERR_HIP;
+      }
+    } else if (rc) {
+      return DMK_ERR_HIP;
+    }
+    k0 = k1;
+  }
+  if (dmk_resid_dots_batch(R, nr, max_nc_dots, max_jobs, max_sums, b->rep_dev[s], b->st)) return DMK_ERR_HIP;
+  if (hipMemcpyAsync(b->rep_host[s], b->rep_dev[s], sizeof(double) * REP_SIZE * nr, hipMemcpyDeviceToHost, b->st) !=
+      hipSuccess)
+    return DMK_ERR_HIP;
+  return 0;
+}
+
 // with b->mu held: one launch per slip-system variant over every waiting
-// request, then release the waiters
+// request (and the rest of the residual evaluations among them), then
+// release the waiters
 void batch_launch_locked(DmkStressBatch* b) {
   const int s = b->stage;
   b->stage = (s + 1) % DmkStressBatch::NSTAGE;
@@ -116,17 +209,34 @@ void batch_launch_locked(DmkStressBatch* b) {
       }
     off += g[k].n;
   }
+  // residual evaluations, ordered by FFT configuration
+  b->order.clear();
+  for (int i = 0; i < (int)b->reqs.size(); i++)
+    if (b->reqs[i].resid) b->order.push_back(i);
+  std::stable_sort(b->order.begin(), b->order.end(),
+                   [&](int x, int y) { return fft_less(b->reqs[x].fft, b->reqs[y].fft); });
+  const int nr = (int)b->order.size();
+  if (nr > b->slots) rc = DMK_ERR_ARG;
+  for (int k = 0; k < nr && !rc; k++) {
+    const DmkStressBatch::Req& r = b->reqs[b->order[k]];
+    b->rhost[s][k] = r.rq;
+    b->rep_stage[r.slot] = s;
+    b->rep_idx[r.slot] = k;
+  }
   if (!rc && off > 0) {
     if (hipMemcpyAsync(b->dev[s], b->host[s], sizeof(DmkStressItem) * off, hipMemcpyHostToDevice, b->st) !=
             hipSuccess ||
+        (nr > 0 && hipMemcpyAsync(b->rdev[s], b->rhost[s], sizeof(DmkResidDev) * nr, hipMemcpyHostToDevice, b->st) !=
+                       hipSuccess) ||
         hipEventRecord(b->staged[s], b->st) != hipSuccess)
       rc = DMK_ERR_HIP;
     for (int k = 0; k < ng && !rc; k++)
       if (dmk_launch_stress_batch(b->dev[s] + g[k].off, g[k].n, g[k].blocks, g[k].key / 2, g[k].key & 1, b->st))
         rc = DMK_ERR_HIP;
+    if (!rc && nr > 0) rc = resid_launch_locked(b, s, nr);
   }
   if (!rc && hipEventRecord(b->done[b->gen & 1], b->st) != hipSuccess) rc = DMK_ERR_HIP;
-  if (rc) dmk_set_error("lockstep batch: stress launch failed");
+  if (rc) dmk_set_error("lockstep batch: launch of the round failed");
   b->rc[b->gen & 1] = rc;
   b->gen++;
   b->launches++;
@@ -193,9 +303,9 @@ std::vector<int> largest_first(dmk_handle** h, int n, const int64_t* cells) {
 
 }  // namespace
 
-int dmk_stress_batch_submit(DmkStressBatch* b, int slot, const DmkStressItem* items, const int* key, int n,
-                            hipStream_t st) {
-  if (n < 1 || n > DMK_MAX_PHASES || slot < 0 || slot >= (int)b->ev_in.size()) {
+static int submit(DmkStressBatch* b, int slot, const DmkStressItem* items, const int* key, int n,
+                  const DmkResidDev* rq, const DmkFFT* fft, hipStream_t st, const double** report) {
+  if (n < 1 || n > DMK_MAX_PHASES || slot < 0 || slot >= (int)b->ev_in.size() || (rq && !b->resid)) {
     dmk_set_error("lockstep batch: bad submission");
     return DMK_ERR_ARG;
   }
@@ -206,6 +316,9 @@ int dmk_stress_batch_submit(DmkStressBatch* b, int slot, const DmkStressItem* it
   r.n = n;
   std::memcpy(r.items, items, sizeof(DmkStressItem) * n);
   std::memcpy(r.key, key, sizeof(int) * n);
+  r.resid = rq != nullptr;
+  r.fft = fft;
+  if (rq) r.rq = *rq;
   b->reqs.push_back(r);
   b->waiting++;
   const uint64_t my = b->gen;
@@ -215,14 +328,34 @@ int dmk_stress_batch_submit(DmkStressBatch* b, int slot, const DmkStressItem* it
   // again only two batches later, which needs this thread's next submission
   const hipEvent_t ev = b->done[my & 1];
   const int rc = b->rc[my & 1];
+  const int rs = b->rep_stage[slot], ri = b->rep_idx[slot];
   lk.unlock();
   if (rc) {
-    dmk_set_error("lockstep batch: stress launch failed");
+    dmk_set_error("lockstep batch: launch of the round failed");
     return rc;
   }
   if (hipStreamWaitEvent(st, ev, 0) != hipSuccess) { dmk_set_error("lockstep batch: stream wait"); return DMK_ERR_HIP; }
+  if (rq) {
+    // the report reaches the host at the end of the round; the next round
+    // holding this slot needs this thread's next submission, so it stays
+    if (hipEventSynchronize(ev) != hipSuccess) { dmk_set_error("lockstep batch: round failed"); return DMK_ERR_HIP; }
+    *report = b->rep_host[rs] + (size_t)ri * REP_SIZE;
+  }
   return 0;
 }
+
+int dmk_stress_batch_submit(DmkStressBatch* b, int slot, const DmkStressItem* items, const int* key, int n,
+                            hipStream_t st) {
+  return submit(b, slot, items, key, n, nullptr, nullptr, st, nullptr);
+}
+
+int dmk_resid_batch_submit(DmkStressBatch* b, int slot, const DmkStressItem* items, const int* key, int n,
+                           const DmkResidDev* rq, const DmkFFT* fft, hipStream_t st, const double** report) {
+  if (!rq || !fft || !report) { dmk_set_error("lockstep batch: bad submission"); return DMK_ERR_ARG; }
+  return submit(b, slot, items, key, n, rq, fft, st, report);
+}
+
+bool dmk_stress_batch_resid(const DmkStressBatch* b) { return b && b->resid; }
 
 static int batch_run(dmk_handle** h, int32_t n, const dmk_loadcase* lc, const int64_t* cells, int32_t n_threads,
                      int32_t* rc_out, double* wall_s, bool lockstep, const char* name) {
@@ -232,7 +365,8 @@ static int batch_run(dmk_handle** h, int32_t n, const dmk_loadcase* lc, const in
   const int nt = std::max(1, std::min<int>(n_threads, n));
   DmkStressBatch b;
   if (lockstep) {
-    int e = batch_init(&b, nt);
+    const char* mode = getenv("DMK_LOCKSTEP");
+    int e = batch_init(&b, nt, !(mode && std::strcmp(mode, "stress") == 0));
     if (e) {
       batch_free(&b);
       dmk_set_error("%s: HIP resources of the lockstep batch", name);

@@ -17,6 +17,18 @@ int dmk_stress_batch_submit(DmkStressBatch* b, int slot, const DmkStressItem* it
                             hipStream_t st);
 // attach (b != NULL) or detach a handle (solver.cpp)
 void dmk_set_stress_batch(dmk_handle* h, DmkStressBatch* b, int slot);
+// whether the batch takes whole residual evaluations of the basic scheme
+bool dmk_stress_batch_resid(const DmkStressBatch* b);
+struct DmkResidDev;
+struct DmkFFT;
+// Hand a whole residual evaluation to the batch: the stress items and the
+// arguments of the rest (rq, the RVE's FFT configuration fft, which must
+// stay valid until the call returns).  Returns once the round holding it has
+// finished on the device, with `st` made to wait for it and *report pointing
+// at the rq->rep_n report doubles on the host (valid until this thread's
+// next submission).
+int dmk_resid_batch_submit(DmkStressBatch* b, int slot, const DmkStressItem* items, const int* key, int n,
+                           const DmkResidDev* rq, const DmkFFT* fft, hipStream_t st, const double** report);
 // device bytes that the next reverse sweep of a handle's tape allocates for
 // its work buffers (0 when they are allocated or there is no tape), and their
 // release after a sweep, the tape kept (solver.cpp)

@@ -5,6 +5,7 @@
 // run to run.
 #include "dmk_common.h"
 #include "dmk_linalg.h"
+#include "dmk_resid.h"
 #include "dmk_rot.h"
 
 #include <float.h>
@@ -65,11 +66,11 @@ __global__ void __launch_bounds__(RB) dot_partial(const double* __restrict__ x, 
   if (threadIdx.x == 0) partial[(long)blockIdx.y * nchunk + blockIdx.x] = r;
 }
 
-__global__ void __launch_bounds__(RB) dot_jobs_partial(DotJobs J, long n, double* __restrict__ partial, int nchunk) {
-  const int j = blockIdx.y;
+__device__ __forceinline__ void dot_jobs_partial_body(const DotJobs& J, long n, double* __restrict__ partial,
+                                                      int nchunk, int bx, int j) {
   const double* __restrict__ x = J.x[j];
   const double* __restrict__ y = J.y[j];
-  long s = (long)blockIdx.x * PER_BLOCK;
+  long s = (long)bx * PER_BLOCK;
   long e = min(s + PER_BLOCK, n);
   double acc = 0.0;
   if (J.mode[j] == 0) {
@@ -78,14 +79,23 @@ __global__ void __launch_bounds__(RB) dot_jobs_partial(DotJobs J, long n, double
     for (long i = s + threadIdx.x; i < e; i += RB) { double d = y[i] - x[i]; acc = fma(d, d, acc); }
   }
   double r = block_sum(acc);
-  if (threadIdx.x == 0) partial[(long)j * nchunk + blockIdx.x] = r;
+  if (threadIdx.x == 0) partial[(long)j * nchunk + bx] = r;
+}
+
+__global__ void __launch_bounds__(RB) dot_jobs_partial(DotJobs J, long n, double* __restrict__ partial, int nchunk) {
+  dot_jobs_partial_body(J, n, partial, nchunk, blockIdx.x, blockIdx.y);
+}
+
+__device__ __forceinline__ void accept_P_body(const int* __restrict__ flag, const double* __restrict__ P_old,
+                                              double* __restrict__ P_new, long n) {
+  if (flag[0] == ST_OK) return;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+    P_new[i] = P_old[i];
 }
 
 __global__ void accept_P_kernel(const int* __restrict__ flag, const double* __restrict__ P_old,
                                 double* __restrict__ P_new, long n) {
-  if (flag[0] == ST_OK) return;
-  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
-    P_new[i] = P_old[i];
+  accept_P_body(flag, P_old, P_new, n);
 }
 
 // one thread: the order of operations is that of formResidual (:478-496)
@@ -131,19 +141,19 @@ __global__ void residual_aim_kernel(AimArgs a, const double* __restrict__ pav_ra
 // quantity): the convergence flag and counters; the raw volume average of P when
 // every cell converged (otherwise P stays the previous response's, and so does
 // its average); then, when asked, the stress-BC bookkeeping of aim_update.
-__global__ void __launch_bounds__(RB) response_partial(const int32_t* __restrict__ st, const int32_t* __restrict__ nn,
-                                                       const int32_t* __restrict__ nf, const double* __restrict__ P,
-                                                       long N, double* __restrict__ part, int nchunk) {
+__device__ __forceinline__ void response_partial_body(const int32_t* __restrict__ st, const int32_t* __restrict__ nn,
+                                                      const int32_t* __restrict__ nf, const double* __restrict__ P,
+                                                      long N, double* __restrict__ part, int nchunk, int bx, int by) {
   __shared__ double sb[RB], s1[RB], s2[RB];
-  long s = (long)blockIdx.x * PER_BLOCK, e = min(s + PER_BLOCK, N);
-  if (blockIdx.y > 0) {
-    // column c = blockIdx.y - 1 of P
-    const int c = blockIdx.y - 1;
+  long s = (long)bx * PER_BLOCK, e = min(s + PER_BLOCK, N);
+  if (by > 0) {
+    // column c = by - 1 of P
+    const int c = by - 1;
     const double* x = P + (long)c * N;
     double acc = 0.0;
     for (long i = s + threadIdx.x; i < e; i += RB) acc += x[i];
     double r = block_sum(acc);
-    if (threadIdx.x == 0) part[(long)(3 + c) * nchunk + blockIdx.x] = r;
+    if (threadIdx.x == 0) part[(long)(3 + c) * nchunk + bx] = r;
     return;
   }
   double bad = 0.0, a = 0.0, b = 0.0;
@@ -163,10 +173,16 @@ __global__ void __launch_bounds__(RB) response_partial(const int32_t* __restrict
     __syncthreads();
   }
   if (threadIdx.x == 0) {
-    part[blockIdx.x] = sb[0];
-    part[nchunk + blockIdx.x] = s1[0];
-    part[2 * nchunk + blockIdx.x] = s2[0];
+    part[bx] = sb[0];
+    part[nchunk + bx] = s1[0];
+    part[2 * nchunk + bx] = s2[0];
   }
+}
+
+__global__ void __launch_bounds__(RB) response_partial(const int32_t* __restrict__ st, const int32_t* __restrict__ nn,
+                                                       const int32_t* __restrict__ nf, const double* __restrict__ P,
+                                                       long N, double* __restrict__ part, int nchunk) {
+  response_partial_body(st, nn, nf, P, N, part, nchunk, blockIdx.x, blockIdx.y);
 }
 
 // sum over the 64 lanes of a wavefront (fixed butterfly order)
@@ -183,11 +199,11 @@ __device__ __forceinline__ double wave_max(double v) {
 
 // 12 wavefronts, one per reduced quantity: 0 the failure flag, 1-2 the
 // iteration counts, 3-11 the columns of P; then one lane does the rest
-__global__ void __launch_bounds__(768) response_finalize(const double* __restrict__ part, int nchunk, double wgt,
-                                                         int* __restrict__ flag, unsigned long long* __restrict__ cnt,
-                                                         double* __restrict__ pav, AimArgs a, int do_aim,
-                                                         double* __restrict__ faim, double* __restrict__ rep,
-                                                         double* __restrict__ aim) {
+__device__ __forceinline__ void response_finalize_body(const double* __restrict__ part, int nchunk, double wgt,
+                                                       int* __restrict__ flag, unsigned long long* __restrict__ cnt,
+                                                       double* __restrict__ pav, const AimArgs& a, int do_aim,
+                                                       double* __restrict__ faim, double* __restrict__ rep,
+                                                       double* __restrict__ aim) {
   __shared__ double res[12];
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const double* p = part + (long)w * nchunk;
@@ -212,14 +228,59 @@ __global__ void __launch_bounds__(768) response_finalize(const double* __restric
   }
 }
 
+__global__ void __launch_bounds__(768) response_finalize(const double* __restrict__ part, int nchunk, double wgt,
+                                                         int* __restrict__ flag, unsigned long long* __restrict__ cnt,
+                                                         double* __restrict__ pav, AimArgs a, int do_aim,
+                                                         double* __restrict__ faim, double* __restrict__ rep,
+                                                         double* __restrict__ aim) {
+  response_finalize_body(part, nchunk, wgt, flag, cnt, pav, a, do_aim, faim, rep, aim);
+}
+
 // several sums of partial results in one launch (blockIdx.x = job)
-__global__ void __launch_bounds__(RB) finalize_jobs(SumJobs J) {
-  const int j = blockIdx.x;
+__device__ __forceinline__ void finalize_jobs_body(const SumJobs& J, int j) {
   const double* p = J.p[j];
   double acc = 0.0;
   for (int i = threadIdx.x; i < J.n[j]; i += RB) acc += p[i];
   double r = block_sum(acc);
   if (threadIdx.x == 0) J.out[j][0] = r * J.scale[j];
+}
+
+__global__ void __launch_bounds__(RB) finalize_jobs(SumJobs J) { finalize_jobs_body(J, blockIdx.x); }
+
+// The reductions of a lockstep round: the kernels above over every RVE of
+// the round (blockIdx.z or .y = RVE; blocks past an RVE's own count return)
+__global__ void __launch_bounds__(RB) response_partial_batch(const DmkResidDev* __restrict__ R) {
+  const DmkResidDev& q = R[blockIdx.z];
+  if ((int)blockIdx.x >= q.nc_resp) return;
+  response_partial_body(q.st, q.nn, q.nf, q.Pnew, q.N, q.partial, q.nc_resp, blockIdx.x, blockIdx.y);
+}
+
+__global__ void __launch_bounds__(768) response_finalize_batch(const DmkResidDev* __restrict__ R) {
+  const DmkResidDev& q = R[blockIdx.x];
+  response_finalize_body(q.partial, q.nc_resp, q.wgt, q.flag, q.cnt, q.pav, q.a, 1, q.faim, q.rep, q.aim);
+}
+
+__global__ void accept_P_batch(const DmkResidDev* __restrict__ R) {
+  const DmkResidDev& q = R[blockIdx.y];
+  accept_P_body(q.flag, q.Pold, q.Pnew, q.n9);
+}
+
+__global__ void __launch_bounds__(RB) dot_jobs_partial_batch(const DmkResidDev* __restrict__ R) {
+  const DmkResidDev& q = R[blockIdx.z];
+  if ((int)blockIdx.x >= q.nc_dots || (int)blockIdx.y >= q.J.n) return;
+  dot_jobs_partial_body(q.J, q.nx, q.partial, q.nc_dots, blockIdx.x, blockIdx.y);
+}
+
+__global__ void __launch_bounds__(RB) finalize_jobs_batch(const DmkResidDev* __restrict__ R) {
+  const DmkResidDev& q = R[blockIdx.y];
+  if ((int)blockIdx.x >= q.S.k) return;
+  finalize_jobs_body(q.S, blockIdx.x);
+}
+
+// the report blocks of the round's RVEs, one after the other (REP_SIZE apart)
+__global__ void rep_gather_batch(const DmkResidDev* __restrict__ R, double* __restrict__ out) {
+  const DmkResidDev& q = R[blockIdx.x];
+  for (int i = threadIdx.x; i < q.rep_n; i += blockDim.x) out[(long)blockIdx.x * REP_SIZE + i] = q.rep[i];
 }
 
 __global__ void axpby_kernel(long n, double a, const double* __restrict__ x, double b, const double* __restrict__ y,
@@ -491,6 +552,23 @@ int dmk_dots_sums(const DotJobs* jobs, long n, double* partial, double* out, con
   }
   if (J.k == 0) return 0;
   hipLaunchKernelGGL(finalize_jobs, dim3(J.k), dim3(RB), 0, s, J);
+  return hipGetLastError() == hipSuccess ? 0 : 2;
+}
+
+int dmk_resid_reduce_batch(const DmkResidDev* R, int nr, int max_nc_resp, hipStream_t s) {
+  if (nr < 1) return 0;
+  hipLaunchKernelGGL(response_partial_batch, dim3(max_nc_resp, 10, nr), dim3(RB), 0, s, R);
+  hipLaunchKernelGGL(response_finalize_batch, dim3(nr), dim3(768), 0, s, R);
+  hipLaunchKernelGGL(accept_P_batch, dim3(512, nr), dim3(256), 0, s, R);
+  return hipGetLastError() == hipSuccess ? 0 : 2;
+}
+
+int dmk_resid_dots_batch(const DmkResidDev* R, int nr, int max_nc_dots, int max_jobs, int max_sums, double* rep_out,
+                         hipStream_t s) {
+  if (nr < 1) return 0;
+  if (max_jobs > 0) hipLaunchKernelGGL(dot_jobs_partial_batch, dim3(max_nc_dots, max_jobs, nr), dim3(RB), 0, s, R);
+  if (max_sums > 0) hipLaunchKernelGGL(finalize_jobs_batch, dim3(max_sums, nr), dim3(RB), 0, s, R);
+  hipLaunchKernelGGL(rep_gather_batch, dim3(nr), dim3(128), 0, s, R, rep_out);
   return hipGetLastError() == hipSuccess ? 0 : 2;
 }
 

@@ -29,6 +29,7 @@
 #include "dmk_polar.h"
 #include "dmk_galerkin.h"
 #include "dmk_batch.h"
+#include "dmk_resid.h"
 #include "../../include/damask_hip.h"
 
 int dmk_build_phase(const dmk_phase_desc* d, DmkPhaseDev* p);
@@ -208,6 +209,11 @@ struct dmk_handle {
   double* divp = nullptr;            // per-block divergence sums of the basic residual
   int div_nb = 0;                    // how many
   bool div_pending = false;          // divp holds sums the next residual_finish adds up
+  // a basic residual evaluation left to residual_finish, which hands it to
+  // the lockstep batch whole (input X, output r)
+  bool lk_pending = false;
+  const double* lk_X = nullptr;
+  double* lk_r = nullptr;
   long *sel, *csel;
   int* flag;
   unsigned long long* cnt;
@@ -453,6 +459,12 @@ static int residual_enqueue(dmk_handle* h, const double* X, double* r, int iter)
   h->n_residuals++;
   if (h->scheme == 1) return residual_enqueue_pol(h, X, r);
   if (h->scheme == 2) return residual_enqueue_gk(h, X, r);
+  if (h->batch && dmk_stress_batch_resid(h->batch)) {
+    h->lk_pending = true;
+    h->lk_X = X;
+    h->lk_r = r;
+    return 0;
+  }
   AimArgs a;
   aim_args(h, true, true, &a);
   int rc = response_enqueue(h, X, h->dt, &a);
@@ -565,9 +577,13 @@ static int residual_enqueue_gk(dmk_handle* h, const double* X, double* r) {
   return 0;
 }
 
+static int residual_report(dmk_handle* h, int nd, double* dots_out);
+static int residual_lockstep(dmk_handle* h, const DotJobs* jobs, double* dots_out);
+
 // The one synchronisation of a residual evaluation: optional dot products
 // (written after the report), report to the host, solver scalars updated.
 static int residual_finish(dmk_handle* h, const DotJobs* jobs, double* dots_out) {
+  if (h->lk_pending) return residual_lockstep(h, jobs, dots_out);
   int nd = jobs ? jobs->n : 0;
   if (nd > DMK_MAX_DOTS) { dmk_set_error("too many dot products"); return DMK_ERR_ARG; }
   SumJobs extra;
@@ -590,6 +606,11 @@ static int residual_finish(dmk_handle* h, const DotJobs* jobs, double* dots_out)
   kt_collect(h, KT_GAMMA);
   kt_collect(h, KT_FFT_INV);
   if (h->scheme != 0) kt_collect(h, KT_TANGENT);
+  return residual_report(h, nd, dots_out);
+}
+
+// the report of a residual evaluation (in hsmall) into the solver scalars
+static int residual_report(dmk_handle* h, int nd, double* dots_out) {
   const double* rep = h->hsmall + SM_REP;
   std::memcpy(h->P_av, rep + REP_PAV, sizeof h->P_av);
   if (h->scheme != 2) std::memcpy(h->F_aim, rep + REP_FAIM, sizeof h->F_aim);
@@ -603,6 +624,71 @@ static int residual_finish(dmk_handle* h, const DotJobs* jobs, double* dots_out)
   }
   if (nd) std::memcpy(dots_out, rep + REP_DOTS, sizeof(double) * nd);
   return 0;
+}
+
+// The basic residual evaluation left by residual_enqueue, as one RVE of a
+// lockstep round: the same launches as response_enqueue, residual_enqueue
+// and residual_finish, with the same arguments, run by the round over all its
+// RVEs; the round's report block of this RVE replaces the copy to hsmall.
+static int residual_lockstep(dmk_handle* h, const DotJobs* jobs, double* dots_out) {
+  h->lk_pending = false;
+  const int nd = jobs ? jobs->n : 0;
+  if (nd > DMK_MAX_DOTS) { dmk_set_error("too many dot products"); return DMK_ERR_ARG; }
+  DmkStressItem items[DMK_MAX_PHASES];
+  int key[DMK_MAX_PHASES];
+  for (int p = 0; p < h->n_phases; p++) {
+    items[p] = DmkStressItem{h->mp, h->lk_X, h->ph_dev + p, h->nm, h->dt, p, 0};
+    key[p] = 2 * h->ph[p].nsl + (h->ph[p].quarter_pow ? 1 : 0);
+  }
+  DmkResidDev q{};
+  q.st = h->mp.status;
+  q.nn = h->mp.n_newton;
+  q.nf = h->mp.n_fpi;
+  q.Pnew = h->mp.Pph;
+  q.N = h->N;
+  q.wgt = h->wgt;
+  q.partial = h->partial;
+  q.nc_resp = dmk_nchunk(h->N);
+  q.flag = h->flag;
+  q.cnt = h->cnt;
+  q.pav = h->small + SM_PAV;
+  q.faim = h->small + SM_FAIM;
+  q.rep = h->small + SM_REP;
+  q.aim = h->small + SM_AIM;
+  aim_args(h, true, true, &q.a);
+  q.Pold = h->P_hom;
+  q.n9 = h->n9;
+  std::swap(h->P_hom, h->mp.Pph);
+  h->n_updates += h->N;
+  h->ga.do_div = 1;
+  h->ga.do_gamma = 1;
+  h->ga.aim_dev = h->small + SM_AIM;
+  q.Z = h->Z;
+  q.r = h->lk_r;
+  q.divp = h->divp;
+  q.g = h->ga;
+  if (jobs) q.J = *jobs;
+  q.J.n = nd;
+  q.nx = h->nx;
+  q.nc_dots = dmk_nchunk(h->nx);
+  // the divergence sum first, then the dot products (as dmk_dots_sums orders them)
+  q.S.p[0] = h->divp;
+  q.S.n[0] = dmk_gamma_conv_nb(&h->fft);
+  q.S.scale[0] = 1.0;
+  q.S.out[0] = h->small + SM_REP + REP_DIV2;
+  q.S.k = 1;
+  for (int j = 0; j < nd; j++, q.S.k++) {
+    q.S.p[q.S.k] = h->partial + (long)j * q.nc_dots;
+    q.S.n[q.S.k] = q.nc_dots;
+    q.S.scale[q.S.k] = 1.0;
+    q.S.out[q.S.k] = h->small + SM_REP + REP_DOTS + j;
+  }
+  q.rep_n = REP_DOTS + nd;
+  const double* rep = nullptr;
+  int rc = dmk_resid_batch_submit(h->batch, h->batch_slot, items, key, h->n_phases, &q, &h->fft, h->st, &rep);
+  if (rc) return rc;
+  std::memcpy(h->hsmall + SM_REP, rep, sizeof(double) * q.rep_n);
+  return residual_report(h, nd, dots_out);
 }
 
 static int form_residual(dmk_handle* h, const double* X, double* r, int iter) {

@@ -17,6 +17,7 @@
 //   (:857-968).
 #include "dmk_common.h"
 #include "dmk_spectral.h"
+#include "dmk_resid.h"
 
 namespace {
 
@@ -305,13 +306,13 @@ __device__ void lds_fft_ip(double2* buf, int nlines, int L, int ldl, const FftPl
   }
 }
 
-__global__ __launch_bounds__(512) void fft_xy_r2c_ip(const double* __restrict__ in, double2* __restrict__ out, int N1, int N2, int N3,
-                              FftPlan px, const double2* __restrict__ twx, FftPlan py,
-                              const double2* __restrict__ twy) {
+__device__ __forceinline__ void xy_r2c_ip_body(const double* __restrict__ in, double2* __restrict__ out, int N1,
+                                               int N2, int N3, const FftPlan& px, const double2* __restrict__ twx,
+                                               const FftPlan& py, const double2* __restrict__ twy, unsigned plane) {
   extern __shared__ double2 lds[];
   const int h = N1 / 2 + 1, np = (N2 + 1) / 2, lx = N1 + 1, ly = N2 + 1;
   double2* A = lds;
-  const double* src = in + (long)blockIdx.x * N1 * N2;
+  const double* src = in + (long)plane * N1 * N2;
   for (int i = threadIdx.x; i < 2 * np * N1; i += blockDim.x) {
     int line = i / N1, pos = i - line * N1;
     double v = line < N2 ? src[(long)line * N1 + pos] : 0.0;
@@ -349,20 +350,26 @@ __global__ __launch_bounds__(512) void fft_xy_r2c_ip(const double* __restrict__ 
   }
   __syncthreads();
   lds_fft_ip(A, h, N2, ly, py, twy, -1);
-  double2* dst = out + (blockIdx.x / N3) * ((long)h * N2 * N3) + (long)(blockIdx.x % N3) * h * N2;
+  double2* dst = out + (plane / N3) * ((long)h * N2 * N3) + (long)(plane % N3) * h * N2;
   for (int i = threadIdx.x; i < N2 * h; i += blockDim.x) {
     int ky = i / h, kx = i - ky * h;
     dst[i] = A[kx * ly + ky];
   }
 }
 
-__global__ __launch_bounds__(512) void fft_yx_c2r_ip(const double2* __restrict__ in, double* __restrict__ out, int N1, int N2, int N3,
+__global__ __launch_bounds__(512) void fft_xy_r2c_ip(const double* __restrict__ in, double2* __restrict__ out, int N1, int N2, int N3,
                               FftPlan px, const double2* __restrict__ twx, FftPlan py,
                               const double2* __restrict__ twy) {
+  xy_r2c_ip_body(in, out, N1, N2, N3, px, twx, py, twy, blockIdx.x);
+}
+
+__device__ __forceinline__ void yx_c2r_ip_body(const double2* __restrict__ in, double* __restrict__ out, int N1,
+                                               int N2, int N3, const FftPlan& px, const double2* __restrict__ twx,
+                                               const FftPlan& py, const double2* __restrict__ twy, unsigned plane) {
   extern __shared__ double2 lds[];
   const int h = N1 / 2 + 1, np = (N2 + 1) / 2, lx = N1 + 1, ly = N2 + 1;
   double2* A = lds;
-  const double2* src = in + (blockIdx.x / N3) * ((long)h * N2 * N3) + (long)(blockIdx.x % N3) * h * N2;
+  const double2* src = in + (plane / N3) * ((long)h * N2 * N3) + (long)(plane % N3) * h * N2;
   for (int i = threadIdx.x; i < N2 * h; i += blockDim.x) {
     int ky = i / h, kx = i - ky * h;
     A[kx * ly + ky] = src[i];
@@ -401,12 +408,18 @@ __global__ __launch_bounds__(512) void fft_yx_c2r_ip(const double2* __restrict__
   }
   __syncthreads();
   lds_fft_ip(A, np, N1, lx, px, twx, +1);
-  double* dst = out + (long)blockIdx.x * N1 * N2;
+  double* dst = out + (long)plane * N1 * N2;
   for (int i = threadIdx.x; i < N1 * N2; i += blockDim.x) {
     int line = i / N1, pos = i - line * N1;
     double2 z = A[(line >> 1) * lx + pos];
     dst[i] = (line & 1) ? z.y : z.x;
   }
+}
+
+__global__ __launch_bounds__(512) void fft_yx_c2r_ip(const double2* __restrict__ in, double* __restrict__ out, int N1, int N2, int N3,
+                              FftPlan px, const double2* __restrict__ twx, FftPlan py,
+                              const double2* __restrict__ twy) {
+  yx_c2r_ip_body(in, out, N1, N2, N3, px, twx, py, twy, blockIdx.x);
 }
 
 // Fused x and y passes, one block per (field, z) plane: the real plane and its
@@ -424,15 +437,14 @@ __device__ __forceinline__ int xy_buf(int N1, int N2) {
   return a > b ? a : b;
 }
 
-__global__ void fft_xy_r2c(const double* __restrict__ in, double2* __restrict__ out, int N1, int N2, int N3,
-                           FftPlan px, const double2* __restrict__ twx, FftPlan py,
-                           const double2* __restrict__ twy) {
+__device__ __forceinline__ void xy_r2c_body(const double* __restrict__ in, double2* __restrict__ out, int N1, int N2,
+                                            int N3, const FftPlan& px, const double2* __restrict__ twx,
+                                            const FftPlan& py, const double2* __restrict__ twy, unsigned plane) {
   extern __shared__ double2 lds[];
   const int h = N1 / 2 + 1, np = (N2 + 1) / 2, lx = N1 + 1, ly = N2 + 1;
   double2* A = lds;
   double2* B = lds + xy_buf(N1, N2);
-  const long plane = blockIdx.x;                 // field * N3 + z
-  const double* src = in + plane * N1 * N2;
+  const double* src = in + (long)plane * N1 * N2;   // plane = field * N3 + z
   for (int i = threadIdx.x; i < 2 * np * N1; i += blockDim.x) {
     int line = i / N1, pos = i - line * N1;
     double v = line < N2 ? src[(long)line * N1 + pos] : 0.0;
@@ -458,21 +470,27 @@ __global__ void fft_xy_r2c(const double* __restrict__ in, double2* __restrict__ 
   }
   __syncthreads();
   double2* q = lds_fft(t, r, h, N2, ly, py, twy, -1);
-  double2* dst = out + (blockIdx.x / N3) * ((long)h * N2 * N3) + (long)(blockIdx.x % N3) * h * N2;
+  double2* dst = out + (plane / N3) * ((long)h * N2 * N3) + (long)(plane % N3) * h * N2;
   for (int i = threadIdx.x; i < N2 * h; i += blockDim.x) {
     int ky = i / h, kx = i - ky * h;
     dst[i] = q[kx * ly + ky];
   }
 }
 
-__global__ void fft_yx_c2r(const double2* __restrict__ in, double* __restrict__ out, int N1, int N2, int N3,
+__global__ void fft_xy_r2c(const double* __restrict__ in, double2* __restrict__ out, int N1, int N2, int N3,
                            FftPlan px, const double2* __restrict__ twx, FftPlan py,
                            const double2* __restrict__ twy) {
+  xy_r2c_body(in, out, N1, N2, N3, px, twx, py, twy, blockIdx.x);
+}
+
+__device__ __forceinline__ void yx_c2r_body(const double2* __restrict__ in, double* __restrict__ out, int N1, int N2,
+                                            int N3, const FftPlan& px, const double2* __restrict__ twx,
+                                            const FftPlan& py, const double2* __restrict__ twy, unsigned plane) {
   extern __shared__ double2 lds[];
   const int h = N1 / 2 + 1, np = (N2 + 1) / 2, lx = N1 + 1, ly = N2 + 1;
   double2* A = lds;
   double2* B = lds + xy_buf(N1, N2);
-  const double2* src = in + (blockIdx.x / N3) * ((long)h * N2 * N3) + (long)(blockIdx.x % N3) * h * N2;
+  const double2* src = in + (plane / N3) * ((long)h * N2 * N3) + (long)(plane % N3) * h * N2;
   for (int i = threadIdx.x; i < N2 * h; i += blockDim.x) {
     int ky = i / h, kx = i - ky * h;
     A[kx * ly + ky] = src[i];
@@ -499,12 +517,18 @@ __global__ void fft_yx_c2r(const double2* __restrict__ in, double* __restrict__ 
   }
   __syncthreads();
   double2* q = lds_fft(t, r, np, N1, lx, px, twx, +1);
-  double* dst = out + (long)blockIdx.x * N1 * N2;
+  double* dst = out + (long)plane * N1 * N2;
   for (int i = threadIdx.x; i < N1 * N2; i += blockDim.x) {
     int line = i / N1, pos = i - line * N1;
     double2 z = q[(line >> 1) * lx + pos];
     dst[i] = (line & 1) ? z.y : z.x;
   }
+}
+
+__global__ void fft_yx_c2r(const double2* __restrict__ in, double* __restrict__ out, int N1, int N2, int N3,
+                           FftPlan px, const double2* __restrict__ twx, FftPlan py,
+                           const double2* __restrict__ twy) {
+  yx_c2r_body(in, out, N1, N2, N3, px, twx, py, twy, blockIdx.x);
 }
 
 // c2c pass along a strided axis.  A line group is `gx` lines adjacent in x
@@ -668,13 +692,13 @@ __global__ void __launch_bounds__(256) gamma_div_kernel(double2* __restrict__ Z,
 // m = kx + h ky (all z) of the nine fields in LDS, as lines (field, column)
 // of length N3: each frequency then has its nine components at hand, and the
 // spectrum makes one round trip through HBM instead of three.
-__global__ void fft_zgz(double2* __restrict__ Z, GammaArgs g, FftPlan pz, const double2* __restrict__ twz, int gz,
-                        double* __restrict__ div_partial) {
+__device__ __forceinline__ void zgz_body(double2* __restrict__ Z, const GammaArgs& g, const FftPlan& pz,
+                                         const double2* __restrict__ twz, int gz, double* __restrict__ div_partial,
+                                         unsigned blk) {
   extern __shared__ double2 lds[];
   __shared__ double red[1024];
   const int h = g.h, N3 = g.n[2], ldl = N3 + 1;
   const long M = (long)h * g.n[1], Nk = M * N3;
-  const unsigned blk = xcd_block(blockIdx.x, gridDim.x);
   const long m0 = (long)blk * gz;
   const int nl = (int)min((long)gz, M - m0);
   const int nlines = 9 * gz;
@@ -718,18 +742,22 @@ __global__ void fft_zgz(double2* __restrict__ Z, GammaArgs g, FftPlan pz, const 
   }
 }
 
+__global__ void fft_zgz(double2* __restrict__ Z, GammaArgs g, FftPlan pz, const double2* __restrict__ twz, int gz,
+                        double* __restrict__ div_partial) {
+  zgz_body(Z, g, pz, twz, gz, div_partial, xcd_block(blockIdx.x, gridDim.x));
+}
+
 // fft_zgz with one LDS buffer (z plans of radix 2, 4 and 8: the in-place
 // Stockham stages of the fused x-y kernels), so twice the columns fit per block
 // and a block reads 16 gz contiguous bytes per (field, z) row.  Same arithmetic
 // as fft_zgz.
-__global__ void __launch_bounds__(512) fft_zgz_ip(double2* __restrict__ Z, GammaArgs g, FftPlan pz,
-                                                  const double2* __restrict__ twz, int gz,
-                                                  double* __restrict__ div_partial) {
+__device__ __forceinline__ void zgz_ip_body(double2* __restrict__ Z, const GammaArgs& g, const FftPlan& pz,
+                                            const double2* __restrict__ twz, int gz, double* __restrict__ div_partial,
+                                            unsigned blk) {
   extern __shared__ double2 lds[];
   __shared__ double red[512];
   const int h = g.h, N3 = g.n[2], ldl = N3 + 1;
   const long M = (long)h * g.n[1], Nk = M * N3;
-  const unsigned blk = xcd_block(blockIdx.x, gridDim.x);
   const long m0 = (long)blk * gz;
   const int nl = (int)min((long)gz, M - m0);
   const int nlines = 9 * gz;
@@ -769,6 +797,53 @@ __global__ void __launch_bounds__(512) fft_zgz_ip(double2* __restrict__ Z, Gamma
     int j = i % gz, t = i / gz, kz = t % N3, c = t / N3;
     if (j < nl) Z[c * Nk + (long)kz * M + m0 + j] = A[(c * gz + j) * ldl + kz];
   }
+}
+
+__global__ void __launch_bounds__(512) fft_zgz_ip(double2* __restrict__ Z, GammaArgs g, FftPlan pz,
+                                                  const double2* __restrict__ twz, int gz,
+                                                  double* __restrict__ div_partial) {
+  zgz_ip_body(Z, g, pz, twz, gz, div_partial, xcd_block(blockIdx.x, gridDim.x));
+}
+
+// The transforms of a lockstep round: the fused kernels above over the RVEs
+// of one FFT configuration (blockIdx.y = RVE), each with its own fields and
+// Gamma arguments
+__global__ __launch_bounds__(512) void fft_xy_r2c_ip_batch(const DmkResidDev* __restrict__ R, int N1, int N2, int N3,
+                                                            FftPlan px, const double2* __restrict__ twx, FftPlan py,
+                                                            const double2* __restrict__ twy) {
+  const DmkResidDev& q = R[blockIdx.y];
+  xy_r2c_ip_body(q.Pnew, q.Z, N1, N2, N3, px, twx, py, twy, blockIdx.x);
+}
+
+__global__ void fft_xy_r2c_batch(const DmkResidDev* __restrict__ R, int N1, int N2, int N3, FftPlan px,
+                                 const double2* __restrict__ twx, FftPlan py, const double2* __restrict__ twy) {
+  const DmkResidDev& q = R[blockIdx.y];
+  xy_r2c_body(q.Pnew, q.Z, N1, N2, N3, px, twx, py, twy, blockIdx.x);
+}
+
+__global__ __launch_bounds__(512) void fft_yx_c2r_ip_batch(const DmkResidDev* __restrict__ R, int N1, int N2, int N3,
+                                                            FftPlan px, const double2* __restrict__ twx, FftPlan py,
+                                                            const double2* __restrict__ twy) {
+  const DmkResidDev& q = R[blockIdx.y];
+  yx_c2r_ip_body(q.Z, q.r, N1, N2, N3, px, twx, py, twy, blockIdx.x);
+}
+
+__global__ void fft_yx_c2r_batch(const DmkResidDev* __restrict__ R, int N1, int N2, int N3, FftPlan px,
+                                 const double2* __restrict__ twx, FftPlan py, const double2* __restrict__ twy) {
+  const DmkResidDev& q = R[blockIdx.y];
+  yx_c2r_body(q.Z, q.r, N1, N2, N3, px, twx, py, twy, blockIdx.x);
+}
+
+__global__ void __launch_bounds__(512) fft_zgz_ip_batch(const DmkResidDev* __restrict__ R, FftPlan pz,
+                                                        const double2* __restrict__ twz, int gz) {
+  const DmkResidDev& q = R[blockIdx.y];
+  zgz_ip_body(q.Z, q.g, pz, twz, gz, q.divp, xcd_block(blockIdx.x, gridDim.x));
+}
+
+__global__ void fft_zgz_batch(const DmkResidDev* __restrict__ R, FftPlan pz, const double2* __restrict__ twz,
+                              int gz) {
+  const DmkResidDev& q = R[blockIdx.y];
+  zgz_body(q.Z, q.g, pz, twz, gz, q.divp, xcd_block(blockIdx.x, gridDim.x));
 }
 
 // utilities_curlRMS (spectral_utilities.f90:697-757): per frequency the
@@ -1304,6 +1379,39 @@ int dmk_fft_c2r_xy(const DmkFFT* fft, int howmany, double2* in, double* out, hip
     launch_x_c2r(fft, howmany, in, out, fft->gx[0], s);
   }
   return hipGetLastError() == hipSuccess ? 0 : 2;
+}
+
+int dmk_transform_batch(const DmkFFT* fft, const DmkResidDev* R, int nr, hipStream_t s) {
+  if (nr < 1) return 0;
+  if (!fft->xy_threads || !fft->zgz_g) return 1;
+  const int N1 = fft->n[0], N2 = fft->n[1], N3 = fft->n[2];
+  const dim3 gxy(9 * N3, nr);
+  const long M = (long)fft->h * N2;
+  const dim3 gzz((unsigned)((M + fft->zgz_g - 1) / fft->zgz_g), nr);
+  if (fft->xy_ip)
+    hipLaunchKernelGGL(fft_xy_r2c_ip_batch, gxy, dim3(fft->xy_threads), xy_lds(fft) / 2, s, R, N1, N2, N3,
+                       fft->plan[0], fft->tw[0], fft->plan[1], fft->tw[1]);
+  else
+    hipLaunchKernelGGL(fft_xy_r2c_batch, gxy, dim3(fft->xy_threads), xy_lds(fft), s, R, N1, N2, N3, fft->plan[0],
+                       fft->tw[0], fft->plan[1], fft->tw[1]);
+  if (fft->zgz_ip)
+    hipLaunchKernelGGL(fft_zgz_ip_batch, gzz, dim3(fft->zgz_threads), zgz_lds(fft, fft->zgz_g, 1), s, R,
+                       fft->plan[2], fft->tw[2], fft->zgz_g);
+  else
+    hipLaunchKernelGGL(fft_zgz_batch, gzz, dim3(fft->zgz_threads), zgz_lds(fft, fft->zgz_g, 0), s, R, fft->plan[2],
+                       fft->tw[2], fft->zgz_g);
+  if (fft->xy_ip)
+    hipLaunchKernelGGL(fft_yx_c2r_ip_batch, gxy, dim3(fft->xy_threads), xy_lds(fft) / 2, s, R, N1, N2, N3,
+                       fft->plan[0], fft->tw[0], fft->plan[1], fft->tw[1]);
+  else
+    hipLaunchKernelGGL(fft_yx_c2r_batch, gxy, dim3(fft->xy_threads), xy_lds(fft), s, R, N1, N2, N3, fft->plan[0],
+                       fft->tw[0], fft->plan[1], fft->tw[1]);
+  return hipGetLastError() == hipSuccess ? 0 : 2;
+}
+
+int dmk_gamma_conv_nb(const DmkFFT* fft) {
+  const long M = (long)fft->h * fft->n[1], Nk = M * fft->n[2];
+  return (int)(fft->zgz_g ? (M + fft->zgz_g - 1) / fft->zgz_g : (Nk + 255) / 256);
 }
 
 long dmk_gamma_conv_partials(const DmkFFT* fft) {
