@@ -29,6 +29,7 @@
 #include <chrono>
 #include <condition_variable>
 #include <cstring>
+#include <memory>
 #include <mutex>
 #include <thread>
 #include <vector>
@@ -72,6 +73,16 @@ struct DmkStressBatch {
   double* rep_dev[NSTAGE] = {};
   double* rep_host[NSTAGE] = {};
   std::vector<int> rep_stage, rep_idx, order;
+  // DMK_BATCH_TIMING: host time of the submitting threads spent waiting for
+  // the round to start (the other RVEs' submissions and the launch), waiting
+  // for the round on the device, and the launches themselves
+  bool timing = false;
+  std::atomic<int64_t> t_wait_ns{0}, t_gpu_ns{0}, t_launch_ns{0}, n_sub{0};
+  // and on the device, per round: the batch stream waiting for the RVEs'
+  // streams (and the table uploads), the stress launches, the rest
+  hipEvent_t tev[NSTAGE][4] = {};
+  double d_ms[3] = {0.0, 0.0, 0.0};
+  int64_t d_rounds = 0;
 };
 
 namespace {
@@ -91,6 +102,9 @@ int batch_init(DmkStressBatch* b, int slots, bool resid) {
   for (auto& e : b->ev_in)
     if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) return DMK_ERR_HIP;
   for (int s = 0; s < DmkStressBatch::NSTAGE; s++) {
+    if (b->timing)
+      for (auto& e : b->tev[s])
+        if (hipEventCreate(&e) != hipSuccess) return DMK_ERR_HIP;
     if (hipHostMalloc((void**)&b->host[s], sizeof(DmkStressItem) * b->cap, hipHostMallocDefault) != hipSuccess ||
         hipMalloc((void**)&b->dev[s], sizeof(DmkStressItem) * b->cap) != hipSuccess ||
         hipEventCreateWithFlags(&b->staged[s], hipEventDisableTiming) != hipSuccess)
@@ -111,6 +125,8 @@ void batch_free(DmkStressBatch* b) {
     if (b->host[s]) hipHostFree(b->host[s]);
     if (b->dev[s]) hipFree(b->dev[s]);
     if (b->staged[s]) hipEventDestroy(b->staged[s]);
+    for (auto& e : b->tev[s])
+      if (e) hipEventDestroy(e);
     if (b->rhost[s]) hipHostFree(b->rhost[s]);
     if (b->rdev[s]) hipFree(b->rdev[s]);
     if (b->rep_dev[s]) hipFree(b->rep_dev[s]);
@@ -178,10 +194,13 @@ int resid_launch_locked(DmkStressBatch* b, int s, int nr) {
 // request (and the rest of the residual evaluations among them), then
 // release the waiters
 void batch_launch_locked(DmkStressBatch* b) {
+  const auto t_start = std::chrono::steady_clock::now();
   const int s = b->stage;
   b->stage = (s + 1) % DmkStressBatch::NSTAGE;
   int rc = 0;
   if (hipEventSynchronize(b->staged[s]) != hipSuccess) rc = DMK_ERR_HIP;   // slot's previous upload done
+  const bool tm = b->timing && !rc;
+  if (tm) hipEventRecord(b->tev[s][0], b->st);
   for (auto& r : b->reqs)
     if (!rc && hipStreamWaitEvent(b->st, b->ev_in[r.slot], 0) != hipSuccess) rc = DMK_ERR_HIP;
   struct Group { int key, off, n, blocks; };
@@ -230,14 +249,20 @@ void batch_launch_locked(DmkStressBatch* b) {
                        hipSuccess) ||
         hipEventRecord(b->staged[s], b->st) != hipSuccess)
       rc = DMK_ERR_HIP;
+    if (tm) hipEventRecord(b->tev[s][1], b->st);
     for (int k = 0; k < ng && !rc; k++)
       if (dmk_launch_stress_batch(b->dev[s] + g[k].off, g[k].n, g[k].blocks, g[k].key / 2, g[k].key & 1, b->st))
         rc = DMK_ERR_HIP;
+    if (tm) hipEventRecord(b->tev[s][2], b->st);
     if (!rc && nr > 0) rc = resid_launch_locked(b, s, nr);
+    if (tm) hipEventRecord(b->tev[s][3], b->st);
   }
   if (!rc && hipEventRecord(b->done[b->gen & 1], b->st) != hipSuccess) rc = DMK_ERR_HIP;
   if (rc) dmk_set_error("lockstep batch: launch of the round failed");
   b->rc[b->gen & 1] = rc;
+  if (b->timing)
+    b->t_launch_ns += std::chrono::duration_cast<std::chrono::nanoseconds>(std::chrono::steady_clock::now() - t_start)
+                          .count();
   b->gen++;
   b->launches++;
   b->reqs.clear();
@@ -310,6 +335,7 @@ static int submit(DmkStressBatch* b, int slot, const DmkStressItem* items, const
     return DMK_ERR_ARG;
   }
   if (hipEventRecord(b->ev_in[slot], st) != hipSuccess) { dmk_set_error("lockstep batch: event record"); return DMK_ERR_HIP; }
+  const auto t0 = std::chrono::steady_clock::now();
   std::unique_lock<std::mutex> lk(b->mu);
   DmkStressBatch::Req r;
   r.slot = slot;
@@ -330,6 +356,7 @@ static int submit(DmkStressBatch* b, int slot, const DmkStressItem* items, const
   const int rc = b->rc[my & 1];
   const int rs = b->rep_stage[slot], ri = b->rep_idx[slot];
   lk.unlock();
+  const auto t1 = std::chrono::steady_clock::now();
   if (rc) {
     dmk_set_error("lockstep batch: launch of the round failed");
     return rc;
@@ -340,6 +367,20 @@ static int submit(DmkStressBatch* b, int slot, const DmkStressItem* items, const
     // holding this slot needs this thread's next submission, so it stays
     if (hipEventSynchronize(ev) != hipSuccess) { dmk_set_error("lockstep batch: round failed"); return DMK_ERR_HIP; }
     *report = b->rep_host[rs] + (size_t)ri * REP_SIZE;
+    if (b->timing && ri == 0) {
+      // one thread per round: the round's device intervals
+      float ms[3] = {0.0f, 0.0f, 0.0f};
+      for (int k = 0; k < 3; k++) hipEventElapsedTime(&ms[k], b->tev[rs][k], b->tev[rs][k + 1]);
+      std::lock_guard<std::mutex> g(b->mu);
+      for (int k = 0; k < 3; k++) b->d_ms[k] += ms[k];
+      b->d_rounds++;
+    }
+  }
+  if (b->timing) {
+    const auto t2 = std::chrono::steady_clock::now();
+    b->t_wait_ns += std::chrono::duration_cast<std::chrono::nanoseconds>(t1 - t0).count();
+    b->t_gpu_ns += std::chrono::duration_cast<std::chrono::nanoseconds>(t2 - t1).count();
+    b->n_sub++;
   }
   return 0;
 }
@@ -363,12 +404,30 @@ static int batch_run(dmk_handle** h, int32_t n, const dmk_loadcase* lc, const in
   std::vector<int> order = largest_first(h, n, cells);
   std::vector<int32_t> rc(n, 0);
   const int nt = std::max(1, std::min<int>(n_threads, n));
-  DmkStressBatch b;
+  // lockstep groups (DMK_LOCKSTEP_GROUPS, default 1): thread t belongs to
+  // group t % ng.  The groups' rounds are independent, so one group's host
+  // work between rounds overlaps the other groups' rounds on the device.  On
+  // the 256-RVE batch the rounds are bound by the device (the stress
+  // integrations), and 2 or 4 groups ran no faster than one.
+  int ng = 0;
+  if (lockstep) {
+    const char* gs = getenv("DMK_LOCKSTEP_GROUPS");
+    ng = gs && gs[0] ? std::atoi(gs) : 1;
+    ng = std::max(1, std::min(ng, nt));
+  }
+  std::vector<std::unique_ptr<DmkStressBatch>> grp;
   if (lockstep) {
     const char* mode = getenv("DMK_LOCKSTEP");
-    int e = batch_init(&b, nt, !(mode && std::strcmp(mode, "stress") == 0));
+    const char* tm = getenv("DMK_BATCH_TIMING");
+    const bool timing = tm && tm[0] && !(tm[0] == '0' && tm[1] == 0);
+    int e = 0;
+    for (int k = 0; k < ng && !e; k++) {
+      grp.emplace_back(new DmkStressBatch);
+      grp.back()->timing = timing;
+      e = batch_init(grp.back().get(), (nt - k + ng - 1) / ng, !(mode && std::strcmp(mode, "stress") == 0));
+    }
     if (e) {
-      batch_free(&b);
+      for (auto& b : grp) batch_free(b.get());
       dmk_set_error("%s: HIP resources of the lockstep batch", name);
       return e;
     }
@@ -377,15 +436,32 @@ static int batch_run(dmk_handle** h, int32_t n, const dmk_loadcase* lc, const in
            [&](int i, int slot) {
              if (!lockstep)
                return dmk_grid_run_loadcase(h[i], &lc[i], nullptr, nullptr, nullptr, nullptr, nullptr);
-             batch_enter(&b);
-             dmk_set_stress_batch(h[i], &b, slot);
+             DmkStressBatch* b = grp[slot % ng].get();
+             batch_enter(b);
+             dmk_set_stress_batch(h[i], b, slot / ng);
              int r = dmk_grid_run_loadcase(h[i], &lc[i], nullptr, nullptr, nullptr, nullptr, nullptr);
              dmk_set_stress_batch(h[i], nullptr, -1);
-             batch_leave(&b);
+             batch_leave(b);
              return r;
            },
            rc.data(), wall_s);
-  if (lockstep) batch_free(&b);
+  for (int k = 0; k < ng; k++) {
+    DmkStressBatch& b = *grp[k];
+    if (!b.timing || b.n_sub == 0) continue;
+    const double ns = (double)b.n_sub, nr = (double)std::max<uint64_t>(b.gen, 1);
+    fprintf(stderr,
+            "lockstep group %d timing: %llu rounds, %lld submissions; per submission %.3f ms waiting for the round "
+            "to start, %.3f ms for the device; launches %.3f ms per round\n",
+            k, (unsigned long long)b.gen, (long long)b.n_sub.load(), b.t_wait_ns * 1e-6 / ns, b.t_gpu_ns * 1e-6 / ns,
+            b.t_launch_ns * 1e-6 / nr);
+    if (b.d_rounds)
+      fprintf(stderr,
+              "lockstep group %d timing on the device, per round: %.3f ms waiting for the RVE streams and uploads, "
+              "%.3f ms stress, %.3f ms the rest of the residual evaluations\n",
+              k, b.d_ms[0] / b.d_rounds, b.d_ms[1] / b.d_rounds, b.d_ms[2] / b.d_rounds);
+    fflush(stderr);
+  }
+  for (auto& b : grp) batch_free(b.get());
   int bad = 0;
   for (int i = 0; i < n; i++) {
     if (rc_out) rc_out[i] = rc[i];
