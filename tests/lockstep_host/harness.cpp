@@ -1,4 +1,4 @@
-// Host harness of the lockstep stress batch (damask_amd/csrc/batch.cpp).
+// Host harness of the lockstep batch (damask_amd/csrc/batch.cpp).
 //
 // batch.cpp is compiled unchanged against the host stand-in of the HIP runtime
 // in hip/hip_runtime.h: every stream is an in-order queue run by its own host
@@ -10,9 +10,14 @@
 // check on their stream afterwards that the batched launch wrote the same tag
 // into their "P".  The launch checks that it reads the tag of the round it was
 // submitted for, i.e. that it ran after the submitter's input and before the
-// submitter's next input.  Built with ThreadSanitizer, every host-side access
-// of the batch (requests, staging tables, generation counter, events) is also
-// checked for data races.  Exit status 0: every check passed.
+// submitter's next input.  Half of the RVEs submit whole residual
+// evaluations instead: the round's reductions, transforms (batched, or per
+// RVE for configurations without fused kernels) and report gather read their
+// argument tables as uploaded for that round, and the thread checks that the
+// report it is handed is its own, of that round.  Built with ThreadSanitizer,
+// every host-side access of the batch (requests, staging tables, reports,
+// generation counter, events) is also checked for data races.  Exit status 0:
+// every check passed.
 #include <hip/hip_runtime.h>
 
 #include <atomic>
@@ -32,6 +37,7 @@
 #include "dmk_batch.h"
 #include "dmk_common.h"
 #include "dmk_linalg.h"
+#include "dmk_resid.h"
 #include "../../include/damask_hip.h"
 
 // ------------------------------------------------------------- runtime stand-in
@@ -147,6 +153,12 @@ hipError_t hipMalloc(void** p, size_t bytes) { *p = std::malloc(bytes); return *
 hipError_t hipFree(void* p) { std::free(p); return hipSuccess; }
 hipError_t hipGetLastError(void) { return hipSuccess; }
 const char* hipGetErrorString(hipError_t) { return "mock"; }
+hipError_t hipEventCreate(hipEvent_t* e) { *e = new MockEvent; return hipSuccess; }
+hipError_t hipEventElapsedTime(float* ms, hipEvent_t, hipEvent_t) { *ms = 0.0f; return hipSuccess; }
+hipError_t hipMemGetInfo(size_t* free_bytes, size_t* total_bytes) {
+  *free_bytes = *total_bytes = (size_t)1 << 40;
+  return hipSuccess;
+}
 
 // ------------------------------------------------------------- solver stand-in
 struct dmk_handle {
@@ -155,8 +167,11 @@ struct dmk_handle {
   DmkStressBatch* batch = nullptr;
   int slot = -1;
   int64_t Ftag = 0, Ptag = 0;            // "device" input and output of the stress integration
+  int64_t Ttag = 0, Rtag = 0;            // output of the transform and of the report gather
+  bool resid = false;                    // submits whole residual evaluations
+  DmkFFT fft;
   std::atomic<int> errors{0};
-  std::atomic<int64_t> launched{0};
+  std::atomic<int64_t> launched{0}, transformed{0};
 };
 
 static thread_local char g_err[256];
@@ -196,6 +211,75 @@ extern "C" int dmk_launch_stress_batch(const DmkStressItem* items_dev, int n_ite
   return 0;
 }
 
+// the round's residual kernels: a request is identified by its handle (rep)
+// and round (wgt); each kernel checks that it runs after the submitter's input
+// of that round and after the round's earlier kernels
+static dmk_handle* req_handle(const DmkResidDev& q) { return reinterpret_cast<dmk_handle*>(q.rep); }
+
+int dmk_resid_reduce_batch(const DmkResidDev* R, int nr, int, hipStream_t s) {
+  s->push([=] {
+    for (int k = 0; k < nr; k++) {
+      dmk_handle* h = req_handle(R[k]);
+      const int64_t round = (int64_t)R[k].wgt;
+      if (h->Ftag != round || h->Ptag != round) g_bad++;
+    }
+  });
+  return 0;
+}
+
+static void transform_one(dmk_handle* h, int64_t round) {
+  if (h->Ptag != round) g_bad++;
+  h->Ttag = round;
+  h->transformed++;
+}
+
+int dmk_transform_batch(const DmkFFT* fft, const DmkResidDev* R, int nr, hipStream_t s) {
+  if (!fft->xy_threads) return 1;
+  const int xy = fft->xy_threads;
+  s->push([=] {
+    for (int k = 0; k < nr; k++) {
+      dmk_handle* h = req_handle(R[k]);
+      if (h->fft.xy_threads != xy) g_bad++;      // one configuration per batched launch
+      transform_one(h, (int64_t)R[k].wgt);
+    }
+  });
+  return 0;
+}
+
+// per-RVE transforms (configuration without fused kernels): the request's
+// handle rides in Pnew and its round in the Gamma arguments
+int dmk_fft_r2c_xy(const DmkFFT* fft, int, const double* in, double2*, hipStream_t s) {
+  if (fft->xy_threads) g_bad++;
+  dmk_handle* h = reinterpret_cast<dmk_handle*>(const_cast<double*>(in));
+  s->push([=] { if (h->Ptag != h->Ftag) g_bad++; });
+  return 0;
+}
+int dmk_gamma_conv(const DmkFFT*, double2*, const GammaArgs* g, double* div_partial, int* nb, hipStream_t s) {
+  dmk_handle* h = reinterpret_cast<dmk_handle*>(div_partial);
+  const int64_t round = (int64_t)g->aim[0];
+  *nb = 1;
+  s->push([=] { transform_one(h, round); });
+  return 0;
+}
+int dmk_fft_c2r_xy(const DmkFFT*, int, double2*, double*, hipStream_t) { return 0; }
+
+int dmk_resid_dots_batch(const DmkResidDev* R, int nr, int, int, int, double* rep_out, hipStream_t s) {
+  s->push([=] {
+    for (int k = 0; k < nr; k++) {
+      dmk_handle* h = req_handle(R[k]);
+      const int64_t round = (int64_t)R[k].wgt;
+      if (h->Ttag != round) g_bad++;
+      rep_out[(size_t)k * REP_SIZE] = (double)round;
+      rep_out[(size_t)k * REP_SIZE + 1] = (double)h->id;
+      h->Rtag = round;
+    }
+  });
+  return 0;
+}
+
+size_t dmk_adjoint_work_bytes(const dmk_handle*) { return 0; }
+void dmk_adjoint_work_release(dmk_handle*) {}
+
 extern "C" int dmk_grid_run_loadcase(dmk_handle* h, const dmk_loadcase*, double*, double*, double*, double*,
                                      int64_t*) {
   std::mt19937 rng(1000u + (unsigned)h->id);
@@ -212,11 +296,29 @@ extern "C" int dmk_grid_run_loadcase(dmk_handle* h, const dmk_loadcase*, double*
       key[p] = 2 * h->nsl[p] + 1;
     }
     if (!h->batch) return DMK_ERR_ARG;
-    int rc = dmk_stress_batch_submit(h->batch, h->slot, items, key, h->n_phases, h->st);
-    if (rc) return rc;
-    h->st->push([h, r] {
-      if (h->Ptag != r) h->errors++;                     // the stream waited for the batched launch
-    });
+    if (h->resid && dmk_stress_batch_resid(h->batch)) {
+      DmkResidDev q{};
+      q.rep = reinterpret_cast<double*>(h);
+      q.wgt = (double)r;
+      q.Pnew = reinterpret_cast<double*>(h);
+      q.divp = reinterpret_cast<double*>(h);
+      q.g.aim[0] = (double)r;
+      q.rep_n = 2;
+      const double* rep = nullptr;
+      int rc = dmk_resid_batch_submit(h->batch, h->slot, items, key, h->n_phases, &q, &h->fft, h->st, &rep);
+      if (rc) return rc;
+      // the report of this RVE and round, read on the host after the round
+      if (!rep || rep[0] != (double)r || rep[1] != (double)h->id) h->errors++;
+      h->st->push([h, r] {
+        if (h->Rtag != r) h->errors++;
+      });
+    } else {
+      int rc = dmk_stress_batch_submit(h->batch, h->slot, items, key, h->n_phases, h->st);
+      if (rc) return rc;
+      h->st->push([h, r] {
+        if (h->Ptag != r) h->errors++;                   // the stream waited for the batched launch
+      });
+    }
     if (rng() % 4 == 0) hipStreamSynchronize(h->st);     // a host round trip, as the solver makes
     if (rng() % 8 == 0) std::this_thread::sleep_for(std::chrono::microseconds(rng() % 200));
   }
@@ -237,7 +339,7 @@ int main(int argc, char** argv) {
     std::vector<dmk_handle*> hs(n);
     std::vector<int64_t> cells(n);
     std::vector<dmk_loadcase> lcs(n);
-    int64_t expected = 0;
+    int64_t expected = 0, expected_tr = 0;
     for (int i = 0; i < n; i++) {
       dmk_handle* h = new dmk_handle;
       h->id = i;
@@ -245,6 +347,12 @@ int main(int argc, char** argv) {
       h->n_phases = 1 + (int)(rng() % 3);
       for (int p = 0; p < h->n_phases; p++) h->nsl[p] = (rng() % 2) ? 12 : 24;
       h->cells = 512 * (1 + (int)(rng() % 64));
+      h->resid = rng() % 2;
+      const int edge = 8 * (1 + (int)(rng() % 3));
+      for (int d = 0; d < 3; d++) h->fft.n[d] = edge;
+      h->fft.xy_threads = (rng() % 3) ? 256 : 0;
+      h->fft.zgz_g = h->fft.xy_threads ? 2 : 0;
+      if (h->resid) expected_tr += h->rounds;
       hipStreamCreateWithFlags(&h->st, 0);
       hs[i] = h;
       cells[i] = h->cells;
@@ -253,18 +361,20 @@ int main(int argc, char** argv) {
     std::vector<int32_t> rc(n, -1);
     std::vector<double> wall(n, 0.0);
     int r = dmk_batch_run_lockstep(hs.data(), n, lcs.data(), cells.data(), threads, rc.data(), wall.data());
-    int64_t launched = 0, errors = 0;
+    int64_t launched = 0, errors = 0, transformed = 0;
     for (int i = 0; i < n; i++) {
       launched += hs[i]->launched;
+      transformed += hs[i]->transformed;
       errors += hs[i]->errors;
       if (rc[i] != 0) errors++;
       hipStreamDestroy(hs[i]->st);
       delete hs[i];
     }
-    std::printf("rep %d: rc %d, RVEs %d, threads %d, items launched %lld of %lld, stream-order errors %lld, "
-                "launch-check errors %lld\n", rep, r, n, threads, (long long)launched, (long long)expected,
-                (long long)errors, (long long)g_bad.load());
-    if (r != 0 || launched != expected || errors) bad_total++;
+    std::printf("rep %d: rc %d, RVEs %d, threads %d, items launched %lld of %lld, residual transforms %lld of %lld, "
+                "stream-order errors %lld, launch-check errors %lld\n", rep, r, n, threads, (long long)launched,
+                (long long)expected, (long long)transformed, (long long)expected_tr, (long long)errors,
+                (long long)g_bad.load());
+    if (r != 0 || launched != expected || transformed != expected_tr || errors) bad_total++;
     expected_total += expected;
   }
   if (g_bad.load()) bad_total++;

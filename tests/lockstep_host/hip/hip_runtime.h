@@ -34,3 +34,9 @@ hipError_t hipMalloc(void** p, size_t bytes);
 hipError_t hipFree(void* p);
 hipError_t hipGetLastError(void);
 const char* hipGetErrorString(hipError_t e);
+hipError_t hipEventCreate(hipEvent_t* e);
+hipError_t hipEventElapsedTime(float* ms, hipEvent_t a, hipEvent_t b);
+hipError_t hipMemGetInfo(size_t* free_bytes, size_t* total_bytes);
+struct double2 {
+  double x, y;
+};

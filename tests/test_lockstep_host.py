@@ -1,4 +1,4 @@
-"""The lockstep stress batch (damask_amd/csrc/batch.cpp) on the host: the file
+"""The lockstep batch (damask_amd/csrc/batch.cpp) on the host: the file
 is compiled unchanged against a host stand-in of the HIP runtime
 (tests/lockstep_host) and driven by 32 threads over RVEs that leave the
 lockstep at different rounds.  Built with ThreadSanitizer it must finish with
@@ -23,14 +23,14 @@ def _build(out, extra):
         pytest.skip("no g++")
     cmd = ["g++", "-std=c++17", "-O1", "-g", *extra, *INC, *SRC, "-o", out, "-lpthread"]
     r = subprocess.run(cmd, capture_output=True, text=True)
-    if r.returncode != 0:
-        pytest.skip("host build failed: " + r.stderr[-400:])
+    assert r.returncode == 0, "host build of batch.cpp failed: " + r.stderr[-2000:]
     return out
 
 
-def test_lockstep_batch_host_logic_under_thread_sanitizer(tmp_path):
+@pytest.mark.parametrize("groups", ["1", "2"])
+def test_lockstep_batch_host_logic_under_thread_sanitizer(tmp_path, groups):
     exe = _build(str(tmp_path / "lockstep_tsan"), ["-fsanitize=thread"])
-    env = dict(os.environ, TSAN_OPTIONS="halt_on_error=1")
+    env = dict(os.environ, TSAN_OPTIONS="halt_on_error=1", DMK_LOCKSTEP_GROUPS=groups)
     r = subprocess.run([exe, "120", "32", "2"], capture_output=True, text=True, timeout=600, env=env)
     print(r.stdout[-2000:], r.stderr[-4000:])
     assert "ThreadSanitizer" not in r.stderr
