@@ -55,7 +55,7 @@ EXPORTS = ["dmk_mech_init", "dmk_mech_forward", "dmk_mech_materialpoint_forward"
            "dmk_adjoint_record", "dmk_adjoint_info", "dmk_adjoint_backward", "dmk_adjoint_release",
            "dmk_adjoint_point_host", "dmk_mech_restart_response", "dmk_grid_set_restart", "dmk_batch_run", "dmk_batch_run_lockstep",
            "dmk_batch_adjoint", "dmk_curl_rms", "dmk_inv_sym3333", "dmk_mech_cell_counters", "dmk_adjoint_tape_segmented",
-           "dmk_divergence_rms_noalloc"]
+           "dmk_divergence_rms_noalloc", "dmk_batch_adjoint_lockstep"]
 
 # largest grid of dmk_divergence_rms_noalloc (include/damask_hip.h)
 DMK_NOALLOC_MAX_CELLS = 262144
@@ -151,6 +151,8 @@ def load(path: str = LIB_PATH):
     L.dmk_batch_adjoint.restype = C.c_int
     L.dmk_batch_adjoint.argtypes = [C.POINTER(vp), C.c_int32, C.POINTER(dp), C.c_double, C.POINTER(dp),
                                     C.POINTER(C.c_int64), C.c_int32, ip, dp]
+    L.dmk_batch_adjoint_lockstep.restype = C.c_int
+    L.dmk_batch_adjoint_lockstep.argtypes = L.dmk_batch_adjoint.argtypes
     _lib = L
     return L
 

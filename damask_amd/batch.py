@@ -96,8 +96,11 @@ class Batch:
     def curves(self) -> list[np.ndarray]:
         return [adjoint._P33(s.adjoint_info()["P_av"]) for s in self.solvers]
 
-    def gradients(self, P_bar: list[np.ndarray], rtol: float = 1e-6, threads: int | None = None):
-        """Per RVE: gradient of sum(P_bar * P_av) with respect to its phase's parameters."""
+    def gradients(self, P_bar: list[np.ndarray], rtol: float = 1e-6, threads: int | None = None,
+                  lockstep: bool = False):
+        """Per RVE: gradient of sum(P_bar * P_av) with respect to its phase's parameters.  With
+        lockstep the sweeps in flight (one per thread) run their adjoint operators and dot
+        products as shared rounds (dmk_batch_adjoint_lockstep); the gradients are the same."""
         n = len(self.solvers)
         pb = [adjoint._P9(p) for p in P_bar]
         tb = [np.zeros((1, ADJ_NOUT)) for _ in range(n)]
@@ -106,11 +109,12 @@ class Batch:
         rc = np.zeros(n, np.int32)
         wall = np.zeros(n)
         t0 = time.perf_counter()
-        r = self.lib.dmk_batch_adjoint(self._h, n, PB, float(rtol), TB,
-                                       self.cells.ctypes.data_as(C.POINTER(C.c_int64)), threads or self.threads,
-                                       _lib.iptr(rc), _lib.dptr(wall))
+        name = "dmk_batch_adjoint_lockstep" if lockstep else "dmk_batch_adjoint"
+        r = getattr(self.lib, name)(self._h, n, PB, float(rtol), TB,
+                                    self.cells.ctypes.data_as(C.POINTER(C.c_int64)), threads or self.threads,
+                                    _lib.iptr(rc), _lib.dptr(wall))
         t = time.perf_counter() - t0
-        _lib.check(r, "dmk_batch_adjoint")
+        _lib.check(r, name)
         grads = [adjoint.reduce_cotangent(s.material.phases[0], tb[i][0]) for i, s in enumerate(self.solvers)]
         return grads, {"wall_s": t, "rve_wall_s": wall}
 

@@ -6,6 +6,8 @@
 
 #include "dmk_adjoint.h"
 #include "dmk_adjoint_k.h"
+#include "dmk_resid.h"
+#include "dmk_vec.h"
 #include "../../include/damask_hip.h"
 
 int dmk_build_phase(const dmk_phase_desc* d, DmkPhaseDev* p);
@@ -186,10 +188,35 @@ __global__ void __launch_bounds__(COOP_CELLS * COOP_LANES) adj_lu_kernel(AdjKArg
   for (int j = l; j < n; j += COOP_LANES) Gg[sc.nmax * sc.nmax + j] = (double)pv[j];
 }
 
-// K3 (per cell).  mode 0: tangent A from the 9 solutions, F cotangent g for
-// the output cotangent (Pu, Fpb_in, xib_in).  mode 1: cotangents of F_p0,
-// xi_0 and the parameters for (Pu - mu, Fpb_in, xib_in).  Both reuse the LU
-// and the per-system temporaries of K1/K2 of the same increment.
+// K3a (per cell, with K3 mode 0): the tangent A from the 9 solutions,
+// column by column
+__global__ void __launch_bounds__(ADJ_WG) adj_tangent_kernel(AdjKArgs a, const DmkPhaseDev* __restrict__ php) {
+  const int e = blockIdx.x * ADJ_WG + threadIdx.x;
+  if (e >= a.N || a.phase[e] != a.phase_id) return;
+  if (!(a.res[e] < HUGE_VAL)) return;
+  const DmkPhaseDev& ph = *php;
+  const long N = a.N;
+  const AdjScr sc = adj_scr(a);
+  AdjPoint pt;
+  load_point(a, e, pt);
+  AdjFwd f;
+  adj_fwd33(ph, pt, f);
+  const double* X = sc.x + (size_t)e * 9 * sc.nmax;
+#pragma unroll 1
+  for (int q = 0; q < 9; q++) {
+    double x[9], c[9];
+    for (int k = 0; k < 9; k++) x[k] = X[q * sc.nmax + k];
+    adj_tangent_dP(ph, pt, f, q, x, c);
+    for (int k = 0; k < 9; k++) a.A[(size_t)(k + 9 * q) * N + e] = c[k];
+  }
+}
+
+// K3 (per cell).  MODE 0: F cotangent g for the output cotangent (Pu,
+// Fpb_in, xib_in).
+// MODE 1: cotangents of F_p0, xi_0 and the parameters for (Pu - mu, Fpb_in,
+// xib_in).  Both reuse the LU and the per-system temporaries of K1/K2 of the
+// same increment; a kernel per mode keeps each one's registers to its own.
+template <int MODE>
 __global__ void __launch_bounds__(ADJ_WG) adj_vjp_kernel(AdjKArgs a, const DmkPhaseDev* __restrict__ php) {
   const int e = blockIdx.x * ADJ_WG + threadIdx.x;
   if (e >= a.N || a.phase[e] != a.phase_id) return;
@@ -204,25 +231,15 @@ __global__ void __launch_bounds__(ADJ_WG) adj_vjp_kernel(AdjKArgs a, const DmkPh
   AdjVec xib{(double*)a.xib_in, (size_t)N, (size_t)e};
   AdjFwd f;
   adj_fwd33(ph, pt, f);
-  if (a.mode == 0) {
-    const double* X = sc.x + (size_t)e * 9 * sc.nmax;
-    double A[81];
-    for (int q = 0; q < 9; q++) {
-      double x[9];
-      for (int k = 0; k < 9; k++) x[k] = X[q * sc.nmax + k];
-      adj_tangent_col(ph, pt, f, q, x, A);
-    }
-    for (int k = 0; k < 81; k++) a.A[(size_t)k * N + e] = A[k];
-  }
   double Pb[9], Pr[9], Fpb[9], Fb[9], Fp0ib[9], lam[9 + DMK_NSL_MAX];
   for (int k = 0; k < 9; k++) Pb[k] = a.Pu[k];
-  if (a.mode == 1 && a.mu)
+  if (MODE == 1 && a.mu)
     for (int k = 0; k < 9; k++) Pb[k] -= a.mu[(size_t)k * N + e];
   fld_to_rm(Pb, Pr);
   load_rm(a.Fpb_in, N, e, Fpb);
   adj_vjp_rhs(ph, pt, f, Pr, Fpb, xib, lam, Fb, Fp0ib);
   adj_solve_t(acc, 9 + ph.nsl, lam);
-  if (a.mode == 0) {
+  if (MODE == 0) {
     AdjVec none{nullptr, 0, 0};
     adj_vjp_finish(ph, pt, f, xi, acc, lam, Fb, Fp0ib, false, nullptr, none, none, 0);
     store_rm(a.g, N, e, Fb);
@@ -235,10 +252,8 @@ __global__ void __launch_bounds__(ADJ_WG) adj_vjp_kernel(AdjKArgs a, const DmkPh
 }
 
 // y_q = sum_p A[p + 9 q] mu_p per cell (the transposed tangent)
-__global__ void adj_atmul_kernel(long N, const double* __restrict__ A, const double* __restrict__ mu,
-                                 double* __restrict__ y) {
-  long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= N) return;
+__device__ __forceinline__ void adj_atmul_body(long N, const double* __restrict__ A, const double* __restrict__ mu,
+                                               double* __restrict__ y, long e) {
   double m[9];
   for (int p = 0; p < 9; p++) m[p] = mu[p * N + e];
   for (int q = 0; q < 9; q++) {
@@ -246,6 +261,21 @@ __global__ void adj_atmul_kernel(long N, const double* __restrict__ A, const dou
     for (int p = 0; p < 9; p++) s = fma(A[(p + 9 * q) * N + e], m[p], s);
     y[q * N + e] = s;
   }
+}
+
+__global__ void adj_atmul_kernel(long N, const double* __restrict__ A, const double* __restrict__ mu,
+                                 double* __restrict__ y) {
+  long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= N) return;
+  adj_atmul_body(N, A, mu, y, e);
+}
+
+// the adjoint-operator requests of a lockstep round (blockIdx.y = request)
+__global__ void adj_atmul_batch_kernel(const DmkResidDev* __restrict__ R) {
+  const DmkResidDev& q = R[blockIdx.y];
+  long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q.kind != DMK_RQ_ADJ_OP || !q.A || e >= q.N) return;   // no A: the operator's argument is Pnew itself
+  adj_atmul_body(q.N, q.A, q.v, q.Pnew, e);
 }
 
 // out[p * nslot + s] = sum over the cells of phase p of v[s * N + e]
@@ -317,9 +347,7 @@ __global__ void __launch_bounds__(ADJ_RB) adj_imax_kernel(const int32_t* __restr
 __global__ void adj_gs_kernel(long n, double* __restrict__ w, const double* __restrict__ V, long ldV, GsArgs c) {
   long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  double v = w[i];
-  for (int k = 0; k < c.l; k++) v = fma(-c.c[k], V[k * ldV + i], v);
-  w[i] = v;
+  w[i] = dmk_gs_elem(w[i], V, ldV, i, c.l, c.c);
 }
 
 int dmk_adj_factor(const AdjKArgs* a, const DmkPhaseDev* ph, hipStream_t s) {
@@ -336,7 +364,12 @@ int dmk_adj_factor(const AdjKArgs* a, const DmkPhaseDev* ph, hipStream_t s) {
 int dmk_adj_vjp(const AdjKArgs* a, const DmkPhaseDev* ph, hipStream_t s) {
   if (a->nsl < 1 || a->nsl > a->mmax || a->mmax > DMK_NSL_MAX) return 1;
   unsigned nb = (unsigned)((a->N + ADJ_WG - 1) / ADJ_WG);
-  hipLaunchKernelGGL(adj_vjp_kernel, dim3(nb), dim3(ADJ_WG), 0, s, *a, ph);
+  if (a->mode == 0) {
+    hipLaunchKernelGGL(adj_tangent_kernel, dim3(nb), dim3(ADJ_WG), 0, s, *a, ph);
+    hipLaunchKernelGGL(adj_vjp_kernel<0>, dim3(nb), dim3(ADJ_WG), 0, s, *a, ph);
+  } else {
+    hipLaunchKernelGGL(adj_vjp_kernel<1>, dim3(nb), dim3(ADJ_WG), 0, s, *a, ph);
+  }
   return hipGetLastError() == hipSuccess ? 0 : 2;
 }
 
@@ -350,6 +383,12 @@ int dmk_adj_local(const AdjKArgs* a, const DmkPhaseDev* ph, hipStream_t s) {
 
 int dmk_adj_atmul(long N, const double* A, const double* mu, double* y, hipStream_t s) {
   hipLaunchKernelGGL(adj_atmul_kernel, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, s, N, A, mu, y);
+  return hipGetLastError() == hipSuccess ? 0 : 2;
+}
+
+int dmk_adj_atmul_batch(const DmkResidDev* R, int nr, long max_N, hipStream_t s) {
+  if (nr < 1) return 0;
+  hipLaunchKernelGGL(adj_atmul_batch_kernel, dim3((unsigned)((max_N + 255) / 256), nr), dim3(256), 0, s, R);
   return hipGetLastError() == hipSuccess ? 0 : 2;
 }
 

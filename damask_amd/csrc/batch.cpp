@@ -54,9 +54,16 @@ struct DmkStressBatch {
   std::condition_variable cv;
   int active = 0, waiting = 0;
   uint64_t gen = 0;                  // batches launched
-  int rc[2] = {0, 0};                // launch status of the last two batches
+  long wait_us = -1;                 // >= 0: a round starts after this wait without the RVEs not yet submitted
   hipStream_t st = nullptr;
-  hipEvent_t done[2] = {};           // batch g recorded into done[g & 1]
+  // per stage: the launch status and completion event of the stage's last
+  // round, and how many of its submitters still use them (or its reports);
+  // a new round on the stage waits until none does
+  int rc[NSTAGE] = {};
+  hipEvent_t done[NSTAGE] = {};
+  int users[NSTAGE] = {};
+  std::condition_variable cv_free;   // a stage's users reached zero
+  bool launching = false;            // a launch is waiting for its stage: no other starts
   std::vector<hipEvent_t> ev_in;     // per slot: the submitter's input is ready
   std::vector<Req> reqs;
   DmkStressItem* host[NSTAGE] = {};
@@ -72,7 +79,7 @@ struct DmkStressBatch {
   DmkResidDev* rdev[NSTAGE] = {};
   double* rep_dev[NSTAGE] = {};
   double* rep_host[NSTAGE] = {};
-  std::vector<int> rep_stage, rep_idx, order;
+  std::vector<int> slot_stage, rep_idx, order;
   // DMK_BATCH_TIMING: host time of the submitting threads spent waiting for
   // the round to start (the other RVEs' submissions and the launch), waiting
   // for the round on the device, and the launches themselves
@@ -92,12 +99,15 @@ int batch_init(DmkStressBatch* b, int slots, bool resid) {
   b->slots = slots;
   b->resid = resid;
   b->reqs.reserve(slots);
-  b->rep_stage.assign(slots, 0);
+  b->slot_stage.assign(slots, 0);
+  const char* w = getenv("DMK_LOCKSTEP_WAIT_US");
+  b->wait_us = w && w[0] ? std::atol(w) : -1;
   b->rep_idx.assign(slots, 0);
   b->order.reserve(slots);
   if (hipStreamCreateWithFlags(&b->st, hipStreamNonBlocking) != hipSuccess) return DMK_ERR_HIP;
   for (auto& e : b->done)
     if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) return DMK_ERR_HIP;
+  for (auto& u : b->users) u = 0;
   b->ev_in.assign(slots, nullptr);
   for (auto& e : b->ev_in)
     if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) return DMK_ERR_HIP;
@@ -154,19 +164,37 @@ bool fft_less(const DmkFFT* a, const DmkFFT* c) {
 // round's residual evaluations (their table uploaded to rdev[s] already)
 int resid_launch_locked(DmkStressBatch* b, int s, int nr) {
   const DmkResidDev* R = b->rdev[s];
-  int max_nc_resp = 0, max_nc_dots = 0, max_jobs = 0, max_sums = 0;
+  int max_nc_resp = 0, max_nc_dots = 0, max_jobs = 0, max_sums = 0, max_nc_adj = 0, ntr = 0;
+  long max_N_adj = 0, max_pre = 0;
+  bool any_res = false, any_adj = false;
   for (int k = 0; k < nr; k++) {
     const DmkResidDev& q = b->rhost[s][k];
-    max_nc_resp = std::max(max_nc_resp, q.nc_resp);
+    for (int o = 0; o < q.npre; o++) max_pre = std::max(max_pre, q.pre[o].n);
     max_nc_dots = std::max(max_nc_dots, q.nc_dots);
     max_jobs = std::max(max_jobs, q.J.n);
     max_sums = std::max(max_sums, q.S.k);
+    if (q.kind == DMK_RQ_DOTS) continue;     // sorted last: no transform
+    ntr = k + 1;
+    if (q.kind == DMK_RQ_RESIDUAL) {
+      any_res = true;
+      max_nc_resp = std::max(max_nc_resp, q.nc_resp);
+    } else {
+      any_adj = true;
+      max_nc_adj = std::max(max_nc_adj, q.nc_resp);
+      max_N_adj = std::max(max_N_adj, q.N);
+    }
   }
-  if (dmk_resid_reduce_batch(R, nr, max_nc_resp, b->st)) return DMK_ERR_HIP;
-  for (int k0 = 0; k0 < nr;) {
+  // the requests' vector steps first; then residual evaluations: response
+  // reductions; adjoint operators: A^T v and the mean rule (each kernel
+  // skips the other kind)
+  if (max_pre > 0 && dmk_pre_batch(R, nr, max_pre, b->st)) return DMK_ERR_HIP;
+  if (any_res && dmk_resid_reduce_batch(R, ntr, max_nc_resp, b->st)) return DMK_ERR_HIP;
+  if (any_adj && (dmk_adj_atmul_batch(R, ntr, max_N_adj, b->st) || dmk_adj_mean_batch(R, ntr, max_nc_adj, b->st)))
+    return DMK_ERR_HIP;
+  for (int k0 = 0; k0 < ntr;) {
     const DmkFFT* f = b->reqs[b->order[k0]].fft;
     int k1 = k0 + 1;
-    while (k1 < nr && same_fft(f, b->reqs[b->order[k1]].fft)) k1++;
+    while (k1 < ntr && same_fft(f, b->reqs[b->order[k1]].fft)) k1++;
     const int rc = dmk_transform_batch(f, R + k0, k1 - k0, b->st);
     if (rc == 1) {
       // no fused kernels for this configuration: each RVE's transform on its own
@@ -190,19 +218,27 @@ int resid_launch_locked(DmkStressBatch* b, int s, int nr) {
   return 0;
 }
 
-// with b->mu held: one launch per slip-system variant over every waiting
-// request (and the rest of the residual evaluations among them), then
-// release the waiters
-void batch_launch_locked(DmkStressBatch* b) {
+// with b->mu held (through lk) and no launch in progress: one launch per
+// slip-system variant over every waiting request (and the rest of the
+// residual evaluations among them), then release the waiters
+void batch_launch_locked(DmkStressBatch* b, std::unique_lock<std::mutex>& lk) {
   const auto t_start = std::chrono::steady_clock::now();
   const int s = b->stage;
   b->stage = (s + 1) % DmkStressBatch::NSTAGE;
+  // the stage's previous round: every submitter done with its event and
+  // report.  The wait releases the lock (those submitters may still have to
+  // take it to leave their own wait); requests submitted meanwhile join this
+  // round, and no other launch starts.
+  b->launching = true;
+  b->cv_free.wait(lk, [&] { return b->users[s] == 0; });
+  b->launching = false;
   int rc = 0;
   if (hipEventSynchronize(b->staged[s]) != hipSuccess) rc = DMK_ERR_HIP;   // slot's previous upload done
   const bool tm = b->timing && !rc;
   if (tm) hipEventRecord(b->tev[s][0], b->st);
   for (auto& r : b->reqs)
-    if (!rc && hipStreamWaitEvent(b->st, b->ev_in[r.slot], 0) != hipSuccess) rc = DMK_ERR_HIP;
+    if (!rc && !(r.resid && r.rq.idle) && hipStreamWaitEvent(b->st, b->ev_in[r.slot], 0) != hipSuccess)
+      rc = DMK_ERR_HIP;
   struct Group { int key, off, n, blocks; };
   Group g[4];
   int ng = 0, off = 0;
@@ -232,19 +268,22 @@ void batch_launch_locked(DmkStressBatch* b) {
   b->order.clear();
   for (int i = 0; i < (int)b->reqs.size(); i++)
     if (b->reqs[i].resid) b->order.push_back(i);
-  std::stable_sort(b->order.begin(), b->order.end(),
-                   [&](int x, int y) { return fft_less(b->reqs[x].fft, b->reqs[y].fft); });
+  std::stable_sort(b->order.begin(), b->order.end(), [&](int x, int y) {
+    const bool dx = b->reqs[x].rq.kind == DMK_RQ_DOTS, dy = b->reqs[y].rq.kind == DMK_RQ_DOTS;
+    if (dx || dy) return !dx && dy;                  // dot products alone last
+    return fft_less(b->reqs[x].fft, b->reqs[y].fft);
+  });
   const int nr = (int)b->order.size();
   if (nr > b->slots) rc = DMK_ERR_ARG;
+  for (auto& r : b->reqs) b->slot_stage[r.slot] = s;
   for (int k = 0; k < nr && !rc; k++) {
     const DmkStressBatch::Req& r = b->reqs[b->order[k]];
     b->rhost[s][k] = r.rq;
-    b->rep_stage[r.slot] = s;
     b->rep_idx[r.slot] = k;
   }
-  if (!rc && off > 0) {
-    if (hipMemcpyAsync(b->dev[s], b->host[s], sizeof(DmkStressItem) * off, hipMemcpyHostToDevice, b->st) !=
-            hipSuccess ||
+  if (!rc && (off > 0 || nr > 0)) {
+    if ((off > 0 && hipMemcpyAsync(b->dev[s], b->host[s], sizeof(DmkStressItem) * off, hipMemcpyHostToDevice,
+                                   b->st) != hipSuccess) ||
         (nr > 0 && hipMemcpyAsync(b->rdev[s], b->rhost[s], sizeof(DmkResidDev) * nr, hipMemcpyHostToDevice, b->st) !=
                        hipSuccess) ||
         hipEventRecord(b->staged[s], b->st) != hipSuccess)
@@ -257,9 +296,10 @@ void batch_launch_locked(DmkStressBatch* b) {
     if (!rc && nr > 0) rc = resid_launch_locked(b, s, nr);
     if (tm) hipEventRecord(b->tev[s][3], b->st);
   }
-  if (!rc && hipEventRecord(b->done[b->gen & 1], b->st) != hipSuccess) rc = DMK_ERR_HIP;
+  if (!rc && hipEventRecord(b->done[s], b->st) != hipSuccess) rc = DMK_ERR_HIP;
   if (rc) dmk_set_error("lockstep batch: launch of the round failed");
-  b->rc[b->gen & 1] = rc;
+  b->rc[s] = rc;
+  b->users[s] = (int)b->reqs.size();
   if (b->timing)
     b->t_launch_ns += std::chrono::duration_cast<std::chrono::nanoseconds>(std::chrono::steady_clock::now() - t_start)
                           .count();
@@ -276,9 +316,9 @@ void batch_enter(DmkStressBatch* b) {
 }
 
 void batch_leave(DmkStressBatch* b) {
-  std::lock_guard<std::mutex> lk(b->mu);
+  std::unique_lock<std::mutex> lk(b->mu);
   b->active--;
-  if (b->waiting > 0 && b->waiting == b->active) batch_launch_locked(b);
+  if (!b->launching && b->waiting > 0 && b->waiting == b->active) batch_launch_locked(b, lk);
 }
 
 // DMK_BATCH_PROGRESS set (and not "0") in the environment: one line on stderr
@@ -329,74 +369,148 @@ std::vector<int> largest_first(dmk_handle** h, int n, const int64_t* cells) {
 }  // namespace
 
 static int submit(DmkStressBatch* b, int slot, const DmkStressItem* items, const int* key, int n,
-                  const DmkResidDev* rq, const DmkFFT* fft, hipStream_t st, const double** report) {
-  if (n < 1 || n > DMK_MAX_PHASES || slot < 0 || slot >= (int)b->ev_in.size() || (rq && !b->resid)) {
+                  const DmkResidDev* rq, const DmkFFT* fft, hipStream_t st, double* report, hipEvent_t* done_out) {
+  const bool needs_fft = rq && rq->kind != DMK_RQ_DOTS;
+  if (n < (rq ? 0 : 1) || n > DMK_MAX_PHASES || slot < 0 || slot >= (int)b->ev_in.size() || (rq && !b->resid) ||
+      (needs_fft && !fft) || (rq && rq->kind == DMK_RQ_RESIDUAL && n < 1)) {
     dmk_set_error("lockstep batch: bad submission");
     return DMK_ERR_ARG;
   }
-  if (hipEventRecord(b->ev_in[slot], st) != hipSuccess) { dmk_set_error("lockstep batch: event record"); return DMK_ERR_HIP; }
+  if (!(rq && rq->idle) && hipEventRecord(b->ev_in[slot], st) != hipSuccess) {
+    dmk_set_error("lockstep batch: event record");
+    return DMK_ERR_HIP;
+  }
   const auto t0 = std::chrono::steady_clock::now();
   std::unique_lock<std::mutex> lk(b->mu);
   DmkStressBatch::Req r;
   r.slot = slot;
   r.n = n;
-  std::memcpy(r.items, items, sizeof(DmkStressItem) * n);
+  if (n > 0) std::memcpy(r.items, items, sizeof(DmkStressItem) * n);
   std::memcpy(r.key, key, sizeof(int) * n);
   r.resid = rq != nullptr;
   r.fft = fft;
   if (rq) r.rq = *rq;
   b->reqs.push_back(r);
   b->waiting++;
+  // the first round launched after this submission holds it: launched here
+  // when every RVE still running has submitted, or by a waiter whose wait
+  // ran out (wait_us >= 0; the RVEs not yet submitted join the next round)
   const uint64_t my = b->gen;
-  if (b->waiting == b->active) batch_launch_locked(b);
-  else b->cv.wait(lk, [&] { return b->gen != my; });
-  // batch `my` is the one that held this request; done[my & 1] is recorded
-  // again only two batches later, which needs this thread's next submission
-  const hipEvent_t ev = b->done[my & 1];
-  const int rc = b->rc[my & 1];
-  const int rs = b->rep_stage[slot], ri = b->rep_idx[slot];
+  if (!b->launching && b->waiting == b->active) {
+    batch_launch_locked(b, lk);
+  } else if (b->wait_us < 0) {
+    b->cv.wait(lk, [&] { return b->gen != my; });
+  } else {
+    // system_clock: its timed wait is pthread_cond_timedwait, which race
+    // checkers see release the mutex (the steady clock's wait is not seen so)
+    while (b->gen == my)
+      if (!b->cv.wait_until(lk, std::chrono::system_clock::now() + std::chrono::microseconds(b->wait_us),
+                            [&] { return b->gen != my; }) &&
+          !b->launching)
+        batch_launch_locked(b, lk);
+  }
+  const int s = b->slot_stage[slot], ri = b->rep_idx[slot];
+  const hipEvent_t ev = b->done[s];
+  const int rc = b->rc[s];
   lk.unlock();
   const auto t1 = std::chrono::steady_clock::now();
+  int err = 0;
+  float ms[3] = {-1.0f, 0.0f, 0.0f};
   if (rc) {
     dmk_set_error("lockstep batch: launch of the round failed");
-    return rc;
-  }
-  if (hipStreamWaitEvent(st, ev, 0) != hipSuccess) { dmk_set_error("lockstep batch: stream wait"); return DMK_ERR_HIP; }
-  if (rq) {
-    // the report reaches the host at the end of the round; the next round
-    // holding this slot needs this thread's next submission, so it stays
-    if (hipEventSynchronize(ev) != hipSuccess) { dmk_set_error("lockstep batch: round failed"); return DMK_ERR_HIP; }
-    *report = b->rep_host[rs] + (size_t)ri * REP_SIZE;
-    if (b->timing && ri == 0) {
-      // one thread per round: the round's device intervals
-      float ms[3] = {0.0f, 0.0f, 0.0f};
-      for (int k = 0; k < 3; k++) hipEventElapsedTime(&ms[k], b->tev[rs][k], b->tev[rs][k + 1]);
-      std::lock_guard<std::mutex> g(b->mu);
-      for (int k = 0; k < 3; k++) b->d_ms[k] += ms[k];
-      b->d_rounds++;
+    err = rc;
+  } else if (!done_out && hipStreamWaitEvent(st, ev, 0) != hipSuccess) {
+    dmk_set_error("lockstep batch: stream wait");
+    err = DMK_ERR_HIP;
+  } else if (rq) {
+    // the report reaches the host at the end of the round
+    if (hipEventSynchronize(ev) != hipSuccess) {
+      dmk_set_error("lockstep batch: round failed");
+      err = DMK_ERR_HIP;
+    } else {
+      if (rq->rep_n > 0) std::memcpy(report, b->rep_host[s] + (size_t)ri * REP_SIZE, sizeof(double) * rq->rep_n);
+      if (b->timing && ri == 0) {
+        // one thread per round: the round's device intervals
+        for (int k = 0; k < 3; k++) hipEventElapsedTime(&ms[k], b->tev[s][k], b->tev[s][k + 1]);
+      }
     }
+  }
+  if (done_out) *done_out = ev;
+  {
+    std::lock_guard<std::mutex> g(b->mu);
+    if (--b->users[s] == 0) b->cv_free.notify_all();
   }
   if (b->timing) {
     const auto t2 = std::chrono::steady_clock::now();
     b->t_wait_ns += std::chrono::duration_cast<std::chrono::nanoseconds>(t1 - t0).count();
     b->t_gpu_ns += std::chrono::duration_cast<std::chrono::nanoseconds>(t2 - t1).count();
     b->n_sub++;
+    if (ms[0] >= 0.0f) {
+      std::lock_guard<std::mutex> g(b->mu);
+      for (int k = 0; k < 3; k++) b->d_ms[k] += ms[k];
+      b->d_rounds++;
+    }
   }
-  return 0;
+  return err;
 }
 
 int dmk_stress_batch_submit(DmkStressBatch* b, int slot, const DmkStressItem* items, const int* key, int n,
                             hipStream_t st) {
-  return submit(b, slot, items, key, n, nullptr, nullptr, st, nullptr);
+  return submit(b, slot, items, key, n, nullptr, nullptr, st, nullptr, nullptr);
 }
 
 int dmk_resid_batch_submit(DmkStressBatch* b, int slot, const DmkStressItem* items, const int* key, int n,
-                           const DmkResidDev* rq, const DmkFFT* fft, hipStream_t st, const double** report) {
-  if (!rq || !fft || !report) { dmk_set_error("lockstep batch: bad submission"); return DMK_ERR_ARG; }
-  return submit(b, slot, items, key, n, rq, fft, st, report);
+                           const DmkResidDev* rq, const DmkFFT* fft, hipStream_t st, double* report,
+                           hipEvent_t* done_out) {
+  if (!rq || (rq->rep_n > 0 && !report)) { dmk_set_error("lockstep batch: bad submission"); return DMK_ERR_ARG; }
+  return submit(b, slot, items, key, n, rq, fft, st, report, done_out);
 }
 
 bool dmk_stress_batch_resid(const DmkStressBatch* b) { return b && b->resid; }
+
+// lockstep groups (DMK_LOCKSTEP_GROUPS, default 1): thread t belongs to
+// group t % ng.  The groups' rounds are independent, so one group's host
+// work between rounds overlaps the other groups' rounds on the device.  On
+// the 256-RVE batch the rounds are bound by the device (the stress
+// integrations), and 2 or 4 groups ran no faster than one.
+static int make_groups(std::vector<std::unique_ptr<DmkStressBatch>>& grp, int nt, int& ng) {
+  const char* gs = getenv("DMK_LOCKSTEP_GROUPS");
+  ng = gs && gs[0] ? std::atoi(gs) : 1;
+  ng = std::max(1, std::min(ng, nt));
+  const char* mode = getenv("DMK_LOCKSTEP");
+  const char* tm = getenv("DMK_BATCH_TIMING");
+  const bool timing = tm && tm[0] && !(tm[0] == '0' && tm[1] == 0);
+  int e = 0;
+  for (int k = 0; k < ng && !e; k++) {
+    grp.emplace_back(new DmkStressBatch);
+    grp.back()->timing = timing;
+    e = batch_init(grp.back().get(), (nt - k + ng - 1) / ng, !(mode && std::strcmp(mode, "stress") == 0));
+  }
+  if (e) {
+    for (auto& b : grp) batch_free(b.get());
+    grp.clear();
+  }
+  return e;
+}
+
+static void print_timing(const std::vector<std::unique_ptr<DmkStressBatch>>& grp) {
+  for (size_t k = 0; k < grp.size(); k++) {
+    const DmkStressBatch& b = *grp[k];
+    if (!b.timing || b.n_sub == 0) continue;
+    const double ns = (double)b.n_sub, nr = (double)std::max<uint64_t>(b.gen, 1);
+    fprintf(stderr,
+            "lockstep group %zu timing: %llu rounds, %lld submissions; per submission %.3f ms waiting for the round "
+            "to start, %.3f ms for the device; launches %.3f ms per round\n",
+            k, (unsigned long long)b.gen, (long long)b.n_sub.load(), b.t_wait_ns * 1e-6 / ns, b.t_gpu_ns * 1e-6 / ns,
+            b.t_launch_ns * 1e-6 / nr);
+    if (b.d_rounds)
+      fprintf(stderr,
+              "lockstep group %zu timing on the device, per round: %.3f ms waiting for the RVE streams and uploads, "
+              "%.3f ms stress, %.3f ms the rest of the requests\n",
+              k, b.d_ms[0] / b.d_rounds, b.d_ms[1] / b.d_rounds, b.d_ms[2] / b.d_rounds);
+    fflush(stderr);
+  }
+}
 
 static int batch_run(dmk_handle** h, int32_t n, const dmk_loadcase* lc, const int64_t* cells, int32_t n_threads,
                      int32_t* rc_out, double* wall_s, bool lockstep, const char* name) {
@@ -404,30 +518,11 @@ static int batch_run(dmk_handle** h, int32_t n, const dmk_loadcase* lc, const in
   std::vector<int> order = largest_first(h, n, cells);
   std::vector<int32_t> rc(n, 0);
   const int nt = std::max(1, std::min<int>(n_threads, n));
-  // lockstep groups (DMK_LOCKSTEP_GROUPS, default 1): thread t belongs to
-  // group t % ng.  The groups' rounds are independent, so one group's host
-  // work between rounds overlaps the other groups' rounds on the device.  On
-  // the 256-RVE batch the rounds are bound by the device (the stress
-  // integrations), and 2 or 4 groups ran no faster than one.
+  std::vector<std::unique_ptr<DmkStressBatch>> grp;
   int ng = 0;
   if (lockstep) {
-    const char* gs = getenv("DMK_LOCKSTEP_GROUPS");
-    ng = gs && gs[0] ? std::atoi(gs) : 1;
-    ng = std::max(1, std::min(ng, nt));
-  }
-  std::vector<std::unique_ptr<DmkStressBatch>> grp;
-  if (lockstep) {
-    const char* mode = getenv("DMK_LOCKSTEP");
-    const char* tm = getenv("DMK_BATCH_TIMING");
-    const bool timing = tm && tm[0] && !(tm[0] == '0' && tm[1] == 0);
-    int e = 0;
-    for (int k = 0; k < ng && !e; k++) {
-      grp.emplace_back(new DmkStressBatch);
-      grp.back()->timing = timing;
-      e = batch_init(grp.back().get(), (nt - k + ng - 1) / ng, !(mode && std::strcmp(mode, "stress") == 0));
-    }
+    int e = make_groups(grp, nt, ng);
     if (e) {
-      for (auto& b : grp) batch_free(b.get());
       dmk_set_error("%s: HIP resources of the lockstep batch", name);
       return e;
     }
@@ -445,22 +540,7 @@ static int batch_run(dmk_handle** h, int32_t n, const dmk_loadcase* lc, const in
              return r;
            },
            rc.data(), wall_s);
-  for (int k = 0; k < ng; k++) {
-    DmkStressBatch& b = *grp[k];
-    if (!b.timing || b.n_sub == 0) continue;
-    const double ns = (double)b.n_sub, nr = (double)std::max<uint64_t>(b.gen, 1);
-    fprintf(stderr,
-            "lockstep group %d timing: %llu rounds, %lld submissions; per submission %.3f ms waiting for the round "
-            "to start, %.3f ms for the device; launches %.3f ms per round\n",
-            k, (unsigned long long)b.gen, (long long)b.n_sub.load(), b.t_wait_ns * 1e-6 / ns, b.t_gpu_ns * 1e-6 / ns,
-            b.t_launch_ns * 1e-6 / nr);
-    if (b.d_rounds)
-      fprintf(stderr,
-              "lockstep group %d timing on the device, per round: %.3f ms waiting for the RVE streams and uploads, "
-              "%.3f ms stress, %.3f ms the rest of the residual evaluations\n",
-              k, b.d_ms[0] / b.d_rounds, b.d_ms[1] / b.d_rounds, b.d_ms[2] / b.d_rounds);
-    fflush(stderr);
-  }
+  print_timing(grp);
   for (auto& b : grp) batch_free(b.get());
   int bad = 0;
   for (int i = 0; i < n; i++) {
@@ -481,11 +561,11 @@ extern "C" int dmk_batch_run_lockstep(dmk_handle** h, int32_t n, const dmk_loadc
   return batch_run(h, n, lc, cells, n_threads, rc_out, wall_s, true, "dmk_batch_run_lockstep");
 }
 
-extern "C" int dmk_batch_adjoint(dmk_handle** h, int32_t n, const double* const* P_av_bar, double rtol,
-                                 double* const* theta_bar, const int64_t* cells, int32_t n_threads, int32_t* rc_out,
-                                 double* wall_s) {
+static int batch_adjoint(dmk_handle** h, int32_t n, const double* const* P_av_bar, double rtol,
+                         double* const* theta_bar, const int64_t* cells, int32_t n_threads, int32_t* rc_out,
+                         double* wall_s, bool lockstep, const char* name) {
   if (!h || !P_av_bar || !theta_bar || n < 1) {
-    dmk_set_error("dmk_batch_adjoint: invalid argument");
+    dmk_set_error("%s: invalid argument", name);
     return DMK_ERR_ARG;
   }
   std::vector<int> order = largest_first(h, n, cells);
@@ -505,22 +585,55 @@ extern "C" int dmk_batch_adjoint(dmk_handle** h, int32_t n, const double* const*
     nt = (int)std::max<size_t>(1, std::min<size_t>(nt, fit));
   }
   if (progress_lines()) {
-    fprintf(stderr, "batch adjoint: %d sweeps at a time, %.2f GB of work buffers each, %.1f GB free\n", nt,
+    fprintf(stderr, "%s: %d sweeps at a time, %.2f GB of work buffers each, %.1f GB free\n", name, nt,
             need * 1e-9, free_b * 1e-9);
     fflush(stderr);
   }
+  std::vector<std::unique_ptr<DmkStressBatch>> grp;
+  int ng = 0;
+  if (lockstep) {
+    int e = make_groups(grp, nt, ng);
+    if (e) {
+      dmk_set_error("%s: HIP resources of the lockstep batch", name);
+      return e;
+    }
+  }
   run_pool(n, order, nt,
-           [&](int i, int) {
+           [&](int i, int slot) {
+             DmkStressBatch* b = lockstep ? grp[slot % ng].get() : nullptr;
+             if (b) {
+               batch_enter(b);
+               dmk_set_stress_batch(h[i], b, slot / ng);
+             }
              int r = dmk_adjoint_backward(h[i], P_av_bar[i], rtol, theta_bar[i], nullptr);
+             if (b) {
+               dmk_set_stress_batch(h[i], nullptr, -1);
+               batch_leave(b);
+             }
              dmk_adjoint_work_release(h[i]);
              return r;
            },
            rc.data(), wall_s);
+  print_timing(grp);
+  for (auto& b : grp) batch_free(b.get());
   int bad = 0;
   for (int i = 0; i < n; i++) {
     if (rc_out) rc_out[i] = rc[i];
     if (rc[i]) bad++;
   }
-  if (bad) { dmk_set_error("dmk_batch_adjoint: %d of %d RVEs failed", bad, n); return DMK_ERR_NOCONV; }
+  if (bad) { dmk_set_error("%s: %d of %d RVEs failed", name, bad, n); return DMK_ERR_NOCONV; }
   return 0;
+}
+
+extern "C" int dmk_batch_adjoint(dmk_handle** h, int32_t n, const double* const* P_av_bar, double rtol,
+                                 double* const* theta_bar, const int64_t* cells, int32_t n_threads, int32_t* rc_out,
+                                 double* wall_s) {
+  return batch_adjoint(h, n, P_av_bar, rtol, theta_bar, cells, n_threads, rc_out, wall_s, false, "dmk_batch_adjoint");
+}
+
+extern "C" int dmk_batch_adjoint_lockstep(dmk_handle** h, int32_t n, const double* const* P_av_bar, double rtol,
+                                          double* const* theta_bar, const int64_t* cells, int32_t n_threads,
+                                          int32_t* rc_out, double* wall_s) {
+  return batch_adjoint(h, n, P_av_bar, rtol, theta_bar, cells, n_threads, rc_out, wall_s, true,
+                       "dmk_batch_adjoint_lockstep");
 }

@@ -288,9 +288,10 @@ DHI void adj_tangent_rhs(const DmkPhaseDev& ph, const AdjPoint& pt, const AdjFwd
   }
 }
 
-// ... and column q of A[p + 9 q] = d P_p / d F_q (field order) from dL_p = x[0..8]
-DHI void adj_tangent_col(const DmkPhaseDev& ph, const AdjPoint& pt, const AdjFwd& f, int q, const double* x,
-                         double* A) {
+// ... and column q of A[p + 9 q] = d P_p / d F_q (field order) from dL_p = x[0..8],
+// written to c[p] (p = 0..8)
+DHI void adj_tangent_dP(const DmkPhaseDev& ph, const AdjPoint& pt, const AdjFwd& f, int q, const double* x,
+                        double* c) {
   const int q0 = q % 3, q1 = q / 3;
   const double dt = pt.dt;
   double SMt[9], MSMt[9], X[9];
@@ -322,7 +323,12 @@ DHI void adj_tangent_col(const DmkPhaseDev& ph, const AdjPoint& pt, const AdjFwd
   m_mul_nt(T1, dM, T2);
   for (int k = 0; k < 9; k++) dP[k] += T2[k];
   for (int i = 0; i < 3; i++)
-    for (int j = 0; j < 3; j++) A[(i + 3 * j) + 9 * q] = dP[3 * i + j];
+    for (int j = 0; j < 3; j++) c[i + 3 * j] = dP[3 * i + j];
+}
+
+DHI void adj_tangent_col(const DmkPhaseDev& ph, const AdjPoint& pt, const AdjFwd& f, int q, const double* x,
+                         double* A) {
+  adj_tangent_dP(ph, pt, f, q, x, A + 9 * q);
 }
 
 DHI void adj_tangent(const DmkPhaseDev& ph, const AdjPoint& pt, const AdjFwd& f, const AdjAcc& acc, double* A) {

@@ -21,14 +21,18 @@ void dmk_set_stress_batch(dmk_handle* h, DmkStressBatch* b, int slot);
 bool dmk_stress_batch_resid(const DmkStressBatch* b);
 struct DmkResidDev;
 struct DmkFFT;
-// Hand a whole residual evaluation to the batch: the stress items and the
-// arguments of the rest (rq, the RVE's FFT configuration fft, which must
-// stay valid until the call returns).  Returns once the round holding it has
-// finished on the device, with `st` made to wait for it and *report pointing
-// at the rq->rep_n report doubles on the host (valid until this thread's
-// next submission).
+// Hand a request of a lockstep round to the batch (struct DmkResidDev: a
+// whole residual evaluation with its stress items, an application of the
+// adjoint's operator, or dot products; fft is the RVE's FFT configuration and
+// must stay valid until the call returns).  Returns once the round holding it
+// has finished on the device, with the rq->rep_n report doubles copied into
+// report and `st` made to wait for it; with done_out the wait is left to the
+// caller, who gets the round's completion event (rounds complete in order:
+// a later round's event covers it).  rq->idle: `st` has had no work since
+// the caller's previous round, so the round does not wait for it.
 int dmk_resid_batch_submit(DmkStressBatch* b, int slot, const DmkStressItem* items, const int* key, int n,
-                           const DmkResidDev* rq, const DmkFFT* fft, hipStream_t st, const double** report);
+                           const DmkResidDev* rq, const DmkFFT* fft, hipStream_t st, double* report,
+                           hipEvent_t* done_out);
 // device bytes that the next reverse sweep of a handle's tape allocates for
 // its work buffers (0 when they are allocated or there is no tape), and their
 // release after a sweep, the tape kept (solver.cpp)

@@ -6,6 +6,7 @@
 #include "dmk_common.h"
 #include "dmk_linalg.h"
 #include "dmk_resid.h"
+#include "dmk_vec.h"
 #include "dmk_rot.h"
 
 #include <float.h>
@@ -31,24 +32,34 @@ __device__ double block_sum(double v) {
 }
 
 // column sums: blockIdx.y = column (base + col * ld), blockIdx.x = chunk
-__global__ void __launch_bounds__(RB) colsum_partial(const double* __restrict__ base, long ld, long n,
-                                                     double* __restrict__ partial, int nchunk) {
-  const double* x = base + (long)blockIdx.y * ld;
-  long s = (long)blockIdx.x * PER_BLOCK;
+__device__ __forceinline__ void colsum_partial_body(const double* __restrict__ base, long ld, long n,
+                                                    double* __restrict__ partial, int nchunk, int bx, int col) {
+  const double* x = base + (long)col * ld;
+  long s = (long)bx * PER_BLOCK;
   long e = min(s + PER_BLOCK, n);
   double acc = 0.0;
   for (long i = s + threadIdx.x; i < e; i += RB) acc += x[i];
   double r = block_sum(acc);
-  if (threadIdx.x == 0) partial[(long)blockIdx.y * nchunk + blockIdx.x] = r;
+  if (threadIdx.x == 0) partial[(long)col * nchunk + bx] = r;
+}
+
+__global__ void __launch_bounds__(RB) colsum_partial(const double* __restrict__ base, long ld, long n,
+                                                     double* __restrict__ partial, int nchunk) {
+  colsum_partial_body(base, ld, n, partial, nchunk, blockIdx.x, blockIdx.y);
+}
+
+__device__ __forceinline__ void finalize_sum_body(const double* __restrict__ partial, int nchunk, double scale,
+                                                  double* __restrict__ out, int col) {
+  const double* p = partial + (long)col * nchunk;
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < nchunk; i += RB) acc += p[i];
+  double r = block_sum(acc);
+  if (threadIdx.x == 0) out[col] = r * scale;
 }
 
 __global__ void __launch_bounds__(RB) finalize_sum(const double* __restrict__ partial, int nchunk,
                                                    double scale, double* __restrict__ out) {
-  const double* p = partial + (long)blockIdx.x * nchunk;
-  double acc = 0.0;
-  for (int i = threadIdx.x; i < nchunk; i += RB) acc += p[i];
-  double r = block_sum(acc);
-  if (threadIdx.x == 0) out[blockIdx.x] = r * scale;
+  finalize_sum_body(partial, nchunk, scale, out, blockIdx.x);
 }
 
 __global__ void __launch_bounds__(RB) dot_partial(const double* __restrict__ x, const double* __restrict__ Y, long ldY,
@@ -131,6 +142,26 @@ __global__ void residual_aim_kernel(AimArgs a, const double* __restrict__ pav_ra
                                     const int* __restrict__ flag, double* __restrict__ rep, double* __restrict__ aim) {
   if (threadIdx.x != 0) return;
   aim_update(a, pav_raw, faim, flag[0], rep, aim);
+}
+
+// the mean rule of the adjoint-operator requests of a round: column sums of
+// A^T v (dmk_colsum), then dmk_residual_aim on them
+__global__ void __launch_bounds__(RB) adj_colsum_batch(const DmkResidDev* __restrict__ R) {
+  const DmkResidDev& q = R[blockIdx.z];
+  if (q.kind != DMK_RQ_ADJ_OP || (int)blockIdx.x >= q.nc_resp) return;
+  colsum_partial_body(q.Pnew, q.N, q.N, q.partial, q.nc_resp, blockIdx.x, blockIdx.y);
+}
+
+__global__ void __launch_bounds__(RB) adj_colsum_finalize_batch(const DmkResidDev* __restrict__ R) {
+  const DmkResidDev& q = R[blockIdx.y];
+  if (q.kind != DMK_RQ_ADJ_OP) return;
+  finalize_sum_body(q.partial, q.nc_resp, q.wgt, q.apav, blockIdx.x);
+}
+
+__global__ void adj_aim_batch(const DmkResidDev* __restrict__ R) {
+  const DmkResidDev& q = R[blockIdx.x];
+  if (q.kind != DMK_RQ_ADJ_OP || threadIdx.x != 0) return;
+  aim_update(q.a, q.apav, q.afaim, q.flag[0], q.arep, q.aaim);
 }
 
 // The reductions of one constitutive response in two launches.  Partial
@@ -251,17 +282,19 @@ __global__ void __launch_bounds__(RB) finalize_jobs(SumJobs J) { finalize_jobs_b
 // the round (blockIdx.z or .y = RVE; blocks past an RVE's own count return)
 __global__ void __launch_bounds__(RB) response_partial_batch(const DmkResidDev* __restrict__ R) {
   const DmkResidDev& q = R[blockIdx.z];
-  if ((int)blockIdx.x >= q.nc_resp) return;
+  if (q.kind != DMK_RQ_RESIDUAL || (int)blockIdx.x >= q.nc_resp) return;
   response_partial_body(q.st, q.nn, q.nf, q.Pnew, q.N, q.partial, q.nc_resp, blockIdx.x, blockIdx.y);
 }
 
 __global__ void __launch_bounds__(768) response_finalize_batch(const DmkResidDev* __restrict__ R) {
   const DmkResidDev& q = R[blockIdx.x];
+  if (q.kind != DMK_RQ_RESIDUAL) return;
   response_finalize_body(q.partial, q.nc_resp, q.wgt, q.flag, q.cnt, q.pav, q.a, 1, q.faim, q.rep, q.aim);
 }
 
 __global__ void accept_P_batch(const DmkResidDev* __restrict__ R) {
   const DmkResidDev& q = R[blockIdx.y];
+  if (q.kind != DMK_RQ_RESIDUAL) return;
   accept_P_body(q.flag, q.Pold, q.Pnew, q.n9);
 }
 
@@ -286,7 +319,20 @@ __global__ void rep_gather_batch(const DmkResidDev* __restrict__ R, double* __re
 __global__ void axpby_kernel(long n, double a, const double* __restrict__ x, double b, const double* __restrict__ y,
                              double* __restrict__ z) {
   long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) z[i] = a * x[i] + b * y[i];
+  if (i < n) z[i] = dmk_axpby_elem(a, x[i], b, y[i]);
+}
+
+// the vector steps of a lockstep round's requests (blockIdx.y = request),
+// each element through the steps in order
+__global__ void pre_batch_kernel(const DmkResidDev* __restrict__ R) {
+  const DmkResidDev& q = R[blockIdx.y];
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  for (int k = 0; k < q.npre; k++) {
+    const DmkPreOp& o = q.pre[k];
+    if (i >= o.n) continue;
+    if (o.type == DMK_PRE_GS) o.w[i] = dmk_gs_elem(o.w[i], o.V, o.ldV, i, o.l, o.c);
+    else o.z[i] = dmk_axpby_elem(o.a, o.x[i], o.b, o.y[i]);
+  }
 }
 
 __global__ void combine_kernel(long n, const double* __restrict__ XM, CombineArgs c, double* __restrict__ XA) {
@@ -560,6 +606,20 @@ int dmk_resid_reduce_batch(const DmkResidDev* R, int nr, int max_nc_resp, hipStr
   hipLaunchKernelGGL(response_partial_batch, dim3(max_nc_resp, 10, nr), dim3(RB), 0, s, R);
   hipLaunchKernelGGL(response_finalize_batch, dim3(nr), dim3(768), 0, s, R);
   hipLaunchKernelGGL(accept_P_batch, dim3(512, nr), dim3(256), 0, s, R);
+  return hipGetLastError() == hipSuccess ? 0 : 2;
+}
+
+int dmk_pre_batch(const DmkResidDev* R, int nr, long max_n, hipStream_t s) {
+  if (nr < 1 || max_n < 1) return 0;
+  hipLaunchKernelGGL(pre_batch_kernel, dim3((unsigned)((max_n + 255) / 256), nr), dim3(256), 0, s, R);
+  return hipGetLastError() == hipSuccess ? 0 : 2;
+}
+
+int dmk_adj_mean_batch(const DmkResidDev* R, int nr, int max_nc, hipStream_t s) {
+  if (nr < 1) return 0;
+  hipLaunchKernelGGL(adj_colsum_batch, dim3(max_nc, 9, nr), dim3(RB), 0, s, R);
+  hipLaunchKernelGGL(adj_colsum_finalize_batch, dim3(9, nr), dim3(RB), 0, s, R);
+  hipLaunchKernelGGL(adj_aim_batch, dim3(nr), dim3(64), 0, s, R);
   return hipGetLastError() == hipSuccess ? 0 : 2;
 }
 

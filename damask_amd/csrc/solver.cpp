@@ -684,10 +684,9 @@ static int residual_lockstep(dmk_handle* h, const DotJobs* jobs, double* dots_ou
     q.S.out[q.S.k] = h->small + SM_REP + REP_DOTS + j;
   }
   q.rep_n = REP_DOTS + nd;
-  const double* rep = nullptr;
-  int rc = dmk_resid_batch_submit(h->batch, h->batch_slot, items, key, h->n_phases, &q, &h->fft, h->st, &rep);
+  int rc = dmk_resid_batch_submit(h->batch, h->batch_slot, items, key, h->n_phases, &q, &h->fft, h->st,
+                                  h->hsmall + SM_REP, nullptr);
   if (rc) return rc;
-  std::memcpy(h->hsmall + SM_REP, rep, sizeof(double) * q.rep_n);
   return residual_report(h, nd, dots_out);
 }
 
@@ -1177,6 +1176,13 @@ struct AdjState {
   // GMRES of increment k runs on the solver stream; scr / res alternate
   hipStream_t st2 = nullptr;
   hipEvent_t fac[2] = {nullptr, nullptr};
+  // in a lockstep batch: the GMRES's vector steps waiting for its next round
+  // request, whether the solver stream has had work since the last round, and
+  // the last round's completion event the stream has yet to wait for
+  DmkPreOp pre[DMK_PRE_OPS];
+  int npre = 0;
+  bool lk_busy = true;
+  hipEvent_t lk_done = nullptr;
 };
 
 enum { TP_F = 0, TP_FP = 9, TP_LP = 18, TP_XI = 27 };   // tape entry offsets, in fields of N
@@ -1439,28 +1445,174 @@ extern "C" int dmk_adjoint_release(dmk_handle* h) {
   return 0;
 }
 
-static int adj_dots(dmk_handle* h, const DotJobs& J, double* out) { return sync_dots(h, J, out); }
+static bool in_lockstep(const dmk_handle* h) { return h->batch && dmk_stress_batch_resid(h->batch); }
 
-// out = Phi(y): Gamma_0 convolution with the mean rule of the increment
-static int adj_phi(dmk_handle* h, const AimArgs& aa, const double* y, double* out) {
-  const GammaArgs& ga = h->adj->ga;
-  CKL(dmk_colsum(y, h->N, h->N, 9, h->partial, h->small + SM_APAV, h->wgt, h->st));
-  CKL(dmk_residual_aim(&aa, h->small + SM_APAV, h->small + SM_AFAIM, h->flag, h->small + SM_AREP,
-                       h->small + SM_AAIM, h->st));
-  CKL(dmk_fft_r2c(&h->fft, 9, y, h->Z, h->st));
-  GammaArgs g = ga;
-  g.do_div = 0;
-  g.do_gamma = 1;
-  g.aim_dev = h->small + SM_AAIM;
-  CKL(dmk_gamma_div(h->Z, &g, h->partial, h->st));
-  CKL(dmk_fft_c2r(&h->fft, 9, h->Z, out, h->st));
+// the dot products of J over the nx-long fields into the report region
+// SM_ADOTS: as a request of the batch's round (kind DMK_RQ_ADJ_OP or
+// DMK_RQ_DOTS, q filled by the caller) or enqueued on the handle's stream
+static void adj_dot_request(dmk_handle* h, const DotJobs* J, DmkResidDev& q) {
+  const int nd = J ? J->n : 0;
+  if (J) q.J = *J;
+  q.J.n = nd;
+  q.partial = h->partial;
+  q.nx = h->nx;
+  q.nc_dots = dmk_nchunk(h->nx);
+  q.S.k = 0;
+  for (int j = 0; j < nd; j++, q.S.k++) {
+    q.S.p[q.S.k] = h->partial + (long)j * q.nc_dots;
+    q.S.n[q.S.k] = q.nc_dots;
+    q.S.scale[q.S.k] = 1.0;
+    q.S.out[q.S.k] = h->small + SM_ADOTS + j;
+  }
+  q.rep = h->small + SM_ADOTS;
+  q.rep_n = nd;
+}
+
+// one request of the handle's lockstep round, carrying the pending vector
+// steps; defer: the solver stream's wait for the round is left to
+// adj_lk_sync (within the GMRES nothing else runs on that stream)
+static int adj_submit(dmk_handle* h, DmkResidDev& q, double* d, bool defer) {
+  AdjState& a = *h->adj;
+  q.npre = a.npre;
+  for (int k = 0; k < a.npre; k++) q.pre[k] = a.pre[k];
+  q.idle = a.lk_busy ? 0 : 1;
+  hipEvent_t ev = nullptr;
+  int rc = dmk_resid_batch_submit(h->batch, h->batch_slot, nullptr, nullptr, 0, &q, &h->fft, h->st, d,
+                                  defer ? &ev : nullptr);
+  if (rc) return rc;
+  a.npre = 0;
+  if (defer) {
+    a.lk_done = ev;
+    a.lk_busy = false;
+  } else {
+    a.lk_done = nullptr;
+    a.lk_busy = true;
+  }
   return 0;
 }
 
-// out = Phi(A^T v)
-static int adj_op(dmk_handle* h, const AimArgs& aa, const double* v, double* out) {
-  CKL(dmk_adj_atmul(h->N, h->adj->A, v, h->adj->y, h->st));
-  return adj_phi(h, aa, h->adj->y, out);
+// the pending vector steps as a request of their own
+static int adj_flush_pre(dmk_handle* h) {
+  if (h->adj->npre == 0) return 0;
+  DmkResidDev q{};
+  q.kind = DMK_RQ_DOTS;
+  adj_dot_request(h, nullptr, q);
+  return adj_submit(h, q, nullptr, true);
+}
+
+// before other work on the solver stream: the vector steps still pending,
+// and the stream's wait for the last round
+static int adj_lk_sync(dmk_handle* h) {
+  if (!in_lockstep(h)) return 0;
+  AdjState& a = *h->adj;
+  int rc = adj_flush_pre(h);
+  if (rc) return rc;
+  if (a.lk_done) CK(hipStreamWaitEvent(h->st, a.lk_done, 0));
+  a.lk_done = nullptr;
+  a.lk_busy = true;
+  return 0;
+}
+
+// w -= sum_k c_k V_k (dmk_adj_gs), or the same step queued for the next round
+static int adj_vgs(dmk_handle* h, long n, double* w, const double* V, long ldV, const GsArgs& c) {
+  if (!in_lockstep(h)) {
+    CKL(dmk_adj_gs(n, w, V, ldV, &c, h->st));
+    return 0;
+  }
+  AdjState& a = *h->adj;
+  if (c.l > DMK_PRE_GS_MAX) { dmk_set_error("adjoint GMRES: basis longer than a round's step takes"); return DMK_ERR_CONFIG; }
+  int rc;
+  if (a.npre == DMK_PRE_OPS && (rc = adj_flush_pre(h))) return rc;
+  DmkPreOp& o = a.pre[a.npre++];
+  o = DmkPreOp{};
+  o.type = DMK_PRE_GS;
+  o.n = n;
+  o.w = w;
+  o.V = V;
+  o.ldV = ldV;
+  o.l = c.l;
+  for (int k = 0; k < c.l; k++) o.c[k] = c.c[k];
+  return 0;
+}
+
+// z = a x + b y (dmk_axpby), or the same step queued for the next round
+static int adj_vaxpby(dmk_handle* h, long n, double av, const double* x, double bv, const double* y, double* z) {
+  if (!in_lockstep(h)) {
+    CKL(dmk_axpby(n, av, x, bv, y, z, h->st));
+    return 0;
+  }
+  AdjState& a = *h->adj;
+  int rc;
+  if (a.npre == DMK_PRE_OPS && (rc = adj_flush_pre(h))) return rc;
+  DmkPreOp& o = a.pre[a.npre++];
+  o = DmkPreOp{};
+  o.type = DMK_PRE_AXPBY;
+  o.n = n;
+  o.a = av;
+  o.x = x;
+  o.b = bv;
+  o.y = y;
+  o.z = z;
+  return 0;
+}
+
+static int adj_dots(dmk_handle* h, const DotJobs& J, double* out) {
+  if (J.n > DMK_MAX_DOTS) { dmk_set_error("too many dot products"); return DMK_ERR_ARG; }
+  if (in_lockstep(h)) {
+    DmkResidDev q{};
+    q.kind = DMK_RQ_DOTS;
+    adj_dot_request(h, &J, q);
+    return adj_submit(h, q, out, true);
+  }
+  CKL(dmk_dots_sums(&J, h->nx, h->partial, h->small + SM_ADOTS, nullptr, h->st));
+  CK(hipMemcpyAsync(h->hsmall + SM_ADOTS, h->small + SM_ADOTS, sizeof(double) * J.n, hipMemcpyDeviceToHost, h->st));
+  CK(hipStreamSynchronize(h->st));
+  std::memcpy(out, h->hsmall + SM_ADOTS, sizeof(double) * J.n);
+  return 0;
+}
+
+// out = Phi(A^T v) (v given) or Phi(y): the Gamma_0 convolution with the
+// mean rule of the increment; then the dot products J (optional) into d.  In
+// a lockstep batch all of it is one request of the round; otherwise the same
+// kernels on the handle's stream.
+static int adj_apply(dmk_handle* h, const AimArgs& aa, const double* v, const double* y, double* out,
+                     const DotJobs* J, double* d, bool defer = false) {
+  if (v) y = h->adj->y;
+  GammaArgs g = h->adj->ga;
+  g.do_div = 0;
+  g.do_gamma = 1;
+  g.aim_dev = h->small + SM_AAIM;
+  if (in_lockstep(h)) {
+    DmkResidDev q{};
+    q.kind = DMK_RQ_ADJ_OP;
+    q.A = v ? h->adj->A : nullptr;
+    q.v = v;
+    q.Pnew = const_cast<double*>(y);
+    q.N = h->N;
+    q.wgt = h->wgt;
+    q.nc_resp = dmk_nchunk(h->N);
+    q.flag = h->flag;
+    q.a = aa;
+    q.apav = h->small + SM_APAV;
+    q.afaim = h->small + SM_AFAIM;
+    q.arep = h->small + SM_AREP;
+    q.aaim = h->small + SM_AAIM;
+    q.Z = h->Z;
+    q.r = out;
+    q.divp = h->divp;
+    q.g = g;
+    adj_dot_request(h, J, q);
+    return adj_submit(h, q, d, defer);
+  }
+  if (v) CKL(dmk_adj_atmul(h->N, h->adj->A, v, h->adj->y, h->st));
+  CKL(dmk_colsum(y, h->N, h->N, 9, h->partial, h->small + SM_APAV, h->wgt, h->st));
+  CKL(dmk_residual_aim(&aa, h->small + SM_APAV, h->small + SM_AFAIM, h->flag, h->small + SM_AREP,
+                       h->small + SM_AAIM, h->st));
+  CKL(dmk_fft_r2c_xy(&h->fft, 9, y, h->Z, h->st));
+  int nb = 0;
+  CKL(dmk_gamma_conv(&h->fft, h->Z, &g, h->divp, &nb, h->st));
+  CKL(dmk_fft_c2r_xy(&h->fft, 9, h->Z, out, h->st));
+  return J && J->n ? adj_dots(h, *J, d) : 0;
 }
 
 // Restarted GMRES for Phi(A^T mu) = b, started from the mu in place (the
@@ -1475,9 +1627,11 @@ static int adj_gmres(dmk_handle* h, const AimArgs& aa, double rtol, int maxit, i
   DotJobs J;
   double d[DMK_MAX_DOTS];
   int rc;
-  // r = b - Phi(A^T mu), |b|, |r|
-  if ((rc = adj_op(h, aa, a.mu, a.r))) return rc;
-  CKL(dmk_axpby(n, 1.0, a.b, -1.0, a.r, a.r, h->st));
+  // r = b - Phi(A^T mu), |b|, |r|.  In a lockstep batch every step below is
+  // a round request or a vector step carried by the next one; the solver
+  // stream waits for the rounds at the end (adj_lk_sync).
+  if ((rc = adj_apply(h, aa, a.mu, nullptr, a.r, nullptr, nullptr, true))) return rc;
+  if ((rc = adj_vaxpby(h, n, 1.0, a.b, -1.0, a.r, a.r))) return rc;
   J.n = 0;
   add_dot(&J, a.b, a.b, 0);
   add_dot(&J, a.r, a.r, 0);
@@ -1488,13 +1642,14 @@ static int adj_gmres(dmk_handle* h, const AimArgs& aa, double rtol, int maxit, i
   *rel_out = 0.0;
   if (bnorm != bnorm || rnorm != rnorm) { dmk_set_error("adjoint: non-finite right-hand side"); return DMK_ERR_NOCONV; }
   if (bnorm == 0.0) {
+    if ((rc = adj_lk_sync(h))) return rc;
     CK(hipMemsetAsync(a.mu, 0, sizeof(double) * n, h->st));
     return 0;
   }
   int its = 0;
   std::vector<double> H((size_t)(M + 1) * M), cs(M), sn(M), gv(M + 1), yv(M);
   while (rnorm > rtol * bnorm && its < maxit) {
-    CKL(dmk_axpby(n, 1.0 / rnorm, a.r, 0.0, a.r, a.V, h->st));
+    if ((rc = adj_vaxpby(h, n, 1.0 / rnorm, a.r, 0.0, a.r, a.V))) return rc;
     std::fill(H.begin(), H.end(), 0.0);
     std::fill(gv.begin(), gv.end(), 0.0);
     gv[0] = rnorm;
@@ -1502,25 +1657,27 @@ static int adj_gmres(dmk_handle* h, const AimArgs& aa, double rtol, int maxit, i
     for (; j < M && its < maxit; j++) {
       its++;
       double* w = a.V + (size_t)(j + 1) * n;
-      if ((rc = adj_op(h, aa, a.V + (size_t)j * n, w))) return rc;
       double hc[DMK_GS_MAX] = {0}, hn = 0.0;
       for (int pass = 0; pass < 2; pass++) {
         J.n = 0;
         for (int i = 0; i <= j; i++) add_dot(&J, a.V + (size_t)i * n, w, 0);
         add_dot(&J, w, w, 0);
-        if ((rc = adj_dots(h, J, d))) return rc;
+        // w = Phi(A^T V_j) and its products with the basis (second pass: again after the update)
+        if (pass == 0) rc = adj_apply(h, aa, a.V + (size_t)j * n, nullptr, w, &J, d, true);
+        else rc = adj_dots(h, J, d);
+        if (rc) return rc;
         GsArgs c;
         c.l = j + 1;
         double hh = 0.0;
         for (int i = 0; i <= j; i++) { c.c[i] = d[i]; hc[i] += d[i]; hh += d[i] * d[i]; }
-        CKL(dmk_adj_gs(n, w, a.V, n, &c, h->st));
+        if ((rc = adj_vgs(h, n, w, a.V, n, c))) return rc;
         double w2 = d[j + 1], r2 = w2 - hh;
         hn = std::sqrt(std::fmax(r2, 0.0));
         if (r2 > 1e-4 * w2) break;
       }
       for (int i = 0; i <= j; i++) H[i + (size_t)(M + 1) * j] = hc[i];
       H[j + 1 + (size_t)(M + 1) * j] = hn;
-      if (hn > 0.0) CKL(dmk_axpby(n, 1.0 / hn, w, 0.0, w, w, h->st));
+      if (hn > 0.0 && (rc = adj_vaxpby(h, n, 1.0 / hn, w, 0.0, w, w))) return rc;
       for (int i = 0; i < j; i++) {
         double* c0 = &H[i + (size_t)(M + 1) * j];
         double t = cs[i] * c0[0] + sn[i] * c0[1];
@@ -1547,15 +1704,16 @@ static int adj_gmres(dmk_handle* h, const AimArgs& aa, double rtol, int maxit, i
     GsArgs c;
     c.l = j;
     for (int i = 0; i < j; i++) c.c[i] = -yv[i];
-    CKL(dmk_adj_gs(n, a.mu, a.V, n, &c, h->st));
+    if ((rc = adj_vgs(h, n, a.mu, a.V, n, c))) return rc;
     // true residual (restart vector and the reported value)
-    if ((rc = adj_op(h, aa, a.mu, a.r))) return rc;
-    CKL(dmk_axpby(n, 1.0, a.b, -1.0, a.r, a.r, h->st));
+    if ((rc = adj_apply(h, aa, a.mu, nullptr, a.r, nullptr, nullptr, true))) return rc;
+    if ((rc = adj_vaxpby(h, n, 1.0, a.b, -1.0, a.r, a.r))) return rc;
     J.n = 0;
     add_dot(&J, a.r, a.r, 0);
     if ((rc = adj_dots(h, J, d))) return rc;
     rnorm = std::sqrt(d[0]);
   }
+  if ((rc = adj_lk_sync(h))) return rc;
   *its_out = its;
   *rel_out = rnorm / bnorm;
   return 0;
@@ -1687,7 +1845,7 @@ extern "C" int dmk_adjoint_backward(dmk_handle* h, const double* P_av_bar, doubl
       aa.rotate = 1;
       aa.update = 1;
       CK(hipMemsetAsync(h->small + SM_AFAIM, 0, sizeof(double) * 9, h->st));
-      if ((rc = adj_phi(h, aa, a.g, a.b))) return rc;
+      if ((rc = adj_apply(h, aa, nullptr, a.g, a.b, nullptr, nullptr))) return rc;
       int its = 0;
       double rel = 0.0;
       if ((rc = adj_gmres(h, aa, rtol, 600, &its, &rel))) return rc;

@@ -300,6 +300,16 @@ int dmk_batch_run_lockstep(dmk_handle** h, int32_t n, const dmk_loadcase* lc, co
 int dmk_batch_adjoint(dmk_handle** h, int32_t n, const double* const* P_av_bar, double rtol,
                       double* const* theta_bar, const int64_t* cells, int32_t n_threads, int32_t* rc_out,
                       double* wall_s);
+/* as dmk_batch_adjoint, with the sweeps in flight (one per host thread) in
+ * lockstep: each application of the adjoint's operator (A^T v, the mean
+ * rule, the transform with Gamma_0) with its dot products, and each set of
+ * dot products alone, waits until every sweep still running has reached its
+ * own, and all of them run as one round (one launch per kernel, the
+ * transforms per FFT configuration) with one host synchronisation.
+ * Gradients are those of dmk_batch_adjoint. */
+int dmk_batch_adjoint_lockstep(dmk_handle** h, int32_t n, const double* const* P_av_bar, double rtol,
+                               double* const* theta_bar, const int64_t* cells, int32_t n_threads,
+                               int32_t* rc_out, double* wall_s);
 
 void dmk_mech_destroy(dmk_handle* h);
 const char* dmk_last_error(void);

@@ -14,7 +14,12 @@
 // evaluations instead: the round's reductions, transforms (batched, or per
 // RVE for configurations without fused kernels) and report gather read their
 // argument tables as uploaded for that round, and the thread checks that the
-// report it is handed is its own, of that round.  Built with ThreadSanitizer,
+// report it is handed is its own, of that round.  A third of the RVEs submit
+// dot-product requests as the adjoint GMRES does: in blocks whose requests
+// after the first are idle (their stream had no work, so the round does not
+// wait for it), each carrying a vector step that must see the block's input,
+// with the stream's wait for the rounds deferred to the end of the block.
+// Built with ThreadSanitizer,
 // every host-side access of the batch (requests, staging tables, reports,
 // generation counter, events) is also checked for data races.  Exit status 0:
 // every check passed.
@@ -168,7 +173,9 @@ struct dmk_handle {
   int slot = -1;
   int64_t Ftag = 0, Ptag = 0;            // "device" input and output of the stress integration
   int64_t Ttag = 0, Rtag = 0;            // output of the transform and of the report gather
+  int64_t Vtag = 0;                      // output of a request's vector step
   bool resid = false;                    // submits whole residual evaluations
+  bool dots = false;                     // submits dot-product requests as the adjoint GMRES does
   DmkFFT fft;
   std::atomic<int> errors{0};
   std::atomic<int64_t> launched{0}, transformed{0};
@@ -268,10 +275,34 @@ int dmk_resid_dots_batch(const DmkResidDev* R, int nr, int, int, int, double* re
     for (int k = 0; k < nr; k++) {
       dmk_handle* h = req_handle(R[k]);
       const int64_t round = (int64_t)R[k].wgt;
-      if (h->Ttag != round) g_bad++;
+      if (R[k].kind == DMK_RQ_DOTS ? h->Vtag != round : h->Ttag != round) g_bad++;
       rep_out[(size_t)k * REP_SIZE] = (double)round;
       rep_out[(size_t)k * REP_SIZE + 1] = (double)h->id;
       h->Rtag = round;
+    }
+  });
+  return 0;
+}
+
+// adjoint-operator requests (the harness submits none): must never be reached with one
+int dmk_adj_atmul_batch(const DmkResidDev*, int, long, hipStream_t) {
+  g_bad++;
+  return 0;
+}
+int dmk_adj_mean_batch(const DmkResidDev*, int, int, hipStream_t) {
+  g_bad++;
+  return 0;
+}
+// vector steps: each dot-product request of the harness carries one, which
+// must run after the submitter's input of its round
+int dmk_pre_batch(const DmkResidDev* R, int nr, long, hipStream_t s) {
+  s->push([=] {
+    for (int k = 0; k < nr; k++) {
+      if (R[k].npre == 0) continue;
+      dmk_handle* h = req_handle(R[k]);
+      const int64_t round = (int64_t)R[k].wgt;
+      if (R[k].npre != 1 || h->Ftag != R[k].pre[0].a) g_bad++;
+      h->Vtag = round;
     }
   });
   return 0;
@@ -284,7 +315,8 @@ extern "C" int dmk_grid_run_loadcase(dmk_handle* h, const dmk_loadcase*, double*
                                      int64_t*) {
   std::mt19937 rng(1000u + (unsigned)h->id);
   for (int r = 1; r <= h->rounds; r++) {
-    h->st->push([h, r] { h->Ftag = r; });                // input of this residual evaluation
+    if (!h->dots || (r - 1) % 5 == 0)
+      h->st->push([h, r] { h->Ftag = r; });              // input of this residual evaluation (or block)
     DmkStressItem items[DMK_MAX_PHASES];
     int key[DMK_MAX_PHASES];
     for (int p = 0; p < h->n_phases; p++) {
@@ -296,6 +328,34 @@ extern "C" int dmk_grid_run_loadcase(dmk_handle* h, const dmk_loadcase*, double*
       key[p] = 2 * h->nsl[p] + 1;
     }
     if (!h->batch) return DMK_ERR_ARG;
+    if (h->dots && dmk_stress_batch_resid(h->batch)) {
+      // as the adjoint GMRES: blocks of 5 rounds, the stream busy (input
+      // written) before the first request of a block only, the stream's wait
+      // for the rounds deferred to the end of the block
+      static thread_local hipEvent_t last = nullptr;
+      const bool first = (r - 1) % 5 == 0, lastr = r % 5 == 0 || r == h->rounds;
+      DmkResidDev q{};
+      q.kind = DMK_RQ_DOTS;
+      q.idle = first ? 0 : 1;
+      q.rep = reinterpret_cast<double*>(h);
+      q.wgt = (double)r;
+      q.rep_n = 2;
+      q.npre = 1;
+      q.pre[0].type = DMK_PRE_AXPBY;
+      q.pre[0].n = 1;
+      q.pre[0].a = (double)(r - (r - 1) % 5);             // the block's input tag
+      double rep[2] = {-1.0, -1.0};
+      int rc = dmk_resid_batch_submit(h->batch, h->slot, nullptr, nullptr, 0, &q, nullptr, h->st, rep, &last);
+      if (rc) return rc;
+      if (rep[0] != (double)r || rep[1] != (double)h->id) h->errors++;
+      if (lastr) {
+        hipStreamWaitEvent(h->st, last, 0);
+        h->st->push([h, r] {
+          if (h->Rtag != r) h->errors++;                   // the deferred wait covers the block's last round
+        });
+      }
+      continue;
+    }
     if (h->resid && dmk_stress_batch_resid(h->batch)) {
       DmkResidDev q{};
       q.rep = reinterpret_cast<double*>(h);
@@ -304,11 +364,11 @@ extern "C" int dmk_grid_run_loadcase(dmk_handle* h, const dmk_loadcase*, double*
       q.divp = reinterpret_cast<double*>(h);
       q.g.aim[0] = (double)r;
       q.rep_n = 2;
-      const double* rep = nullptr;
-      int rc = dmk_resid_batch_submit(h->batch, h->slot, items, key, h->n_phases, &q, &h->fft, h->st, &rep);
+      double rep[2] = {-1.0, -1.0};
+      int rc = dmk_resid_batch_submit(h->batch, h->slot, items, key, h->n_phases, &q, &h->fft, h->st, rep, nullptr);
       if (rc) return rc;
       // the report of this RVE and round, read on the host after the round
-      if (!rep || rep[0] != (double)r || rep[1] != (double)h->id) h->errors++;
+      if (rep[0] != (double)r || rep[1] != (double)h->id) h->errors++;
       h->st->push([h, r] {
         if (h->Rtag != r) h->errors++;
       });
@@ -347,7 +407,9 @@ int main(int argc, char** argv) {
       h->n_phases = 1 + (int)(rng() % 3);
       for (int p = 0; p < h->n_phases; p++) h->nsl[p] = (rng() % 2) ? 12 : 24;
       h->cells = 512 * (1 + (int)(rng() % 64));
-      h->resid = rng() % 2;
+      const unsigned kind = rng() % 3;
+      h->resid = kind == 1;
+      h->dots = kind == 2;
       const int edge = 8 * (1 + (int)(rng() % 3));
       for (int d = 0; d < 3; d++) h->fft.n[d] = edge;
       h->fft.xy_threads = (rng() % 3) ? 256 : 0;
@@ -356,7 +418,7 @@ int main(int argc, char** argv) {
       hipStreamCreateWithFlags(&h->st, 0);
       hs[i] = h;
       cells[i] = h->cells;
-      expected += (int64_t)h->rounds * h->n_phases;
+      if (!h->dots) expected += (int64_t)h->rounds * h->n_phases;
     }
     std::vector<int32_t> rc(n, -1);
     std::vector<double> wall(n, 0.0);

@@ -6,16 +6,24 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 
-def test_batch_matches_single_runs(gpu_lib):
+@pytest.mark.parametrize("lockstep", [False, True])
+def test_batch_matches_single_runs(gpu_lib, monkeypatch, lockstep):
+    """Taped forward runs and gradients of six RVEs of two sizes, as
+    concurrent handles or in lockstep (the forward's residual evaluations and
+    the reverse sweeps' adjoint operators and dot products in shared rounds,
+    the sweeps leaving the lockstep at different times), equal the single
+    runs bit for bit."""
     from damask_amd import adjoint, batch, config
     from damask_amd.grid import GridMechSpectralBasic
+    monkeypatch.delenv("DMK_LOCKSTEP", raising=False)
+    monkeypatch.setenv("DMK_LOCKSTEP_GROUPS", "1")
     items = batch.random_batch(6, master_seed=11, edges=(8, 12), grains=(5, 12), increments=4, strain=0.004)
     b = batch.Batch(items)
     b.tape()
-    b.run(threads=4)
+    b.run(threads=4, lockstep=lockstep)
     curves = b.curves()
     Pbar = [np.broadcast_to(np.eye(3), c.shape) * 1e-6 for c in curves]
-    grads, _ = b.gradients(Pbar, rtol=1e-8, threads=4)
+    grads, _ = b.gradients(Pbar, rtol=1e-8, threads=4, lockstep=lockstep)
     b.close()
     for (grid, mat, lc), c, g, pb in zip(items, curves, grads, Pbar):
         s = GridMechSpectralBasic(grid, mat)

@@ -27,10 +27,12 @@ def _build(out, extra):
     return out
 
 
-@pytest.mark.parametrize("groups", ["1", "2"])
-def test_lockstep_batch_host_logic_under_thread_sanitizer(tmp_path, groups):
+@pytest.mark.parametrize("groups,wait_us", [("1", ""), ("2", ""), ("1", "30"), ("2", "0")])
+def test_lockstep_batch_host_logic_under_thread_sanitizer(tmp_path, groups, wait_us):
+    """one or two lockstep groups; rounds that wait for every RVE, or that
+    start after wait_us microseconds without the RVEs not yet submitted"""
     exe = _build(str(tmp_path / "lockstep_tsan"), ["-fsanitize=thread"])
-    env = dict(os.environ, TSAN_OPTIONS="halt_on_error=1", DMK_LOCKSTEP_GROUPS=groups)
+    env = dict(os.environ, TSAN_OPTIONS="halt_on_error=1", DMK_LOCKSTEP_GROUPS=groups, DMK_LOCKSTEP_WAIT_US=wait_us)
     r = subprocess.run([exe, "120", "32", "2"], capture_output=True, text=True, timeout=600, env=env)
     print(r.stdout[-2000:], r.stderr[-4000:])
     assert "ThreadSanitizer" not in r.stderr
