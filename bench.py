@@ -163,8 +163,10 @@ def progress(msg):
 
 def batched_cost(args):
     """256 independent RVEs (edge 8-32, 10-50 grains, xi_0 and n drawn per RVE,
-    tension along a random direction, 40 increments to 2 %): batched forward,
-    and batched forward with tape + adjoint gradient over the batched forward."""
+    tension along a random direction, 40 increments to 2 %): batched forward as
+    concurrent handles and in lockstep rounds, and taped lockstep forward +
+    lockstep reverse sweeps (the gradient), with the concurrent reverse sweeps
+    beside them."""
     import numpy as np
     from damask_amd import batch
     items = batch.random_batch(args.batch, args.seed)
@@ -173,34 +175,37 @@ def batched_cost(args):
     updates = sum(s.counters()["updates"] for s in b.solvers)
     cells = int(b.cells.sum())
     b.close()
-    lock = None
-    if args.lockstep_threads:
-        b = batch.Batch(items)
-        rl = b.run(threads=args.lockstep_threads, lockstep=True)
-        ul = sum(s.counters()["updates"] for s in b.solvers)
-        b.close()
-        lock = {"threads": args.lockstep_threads, "forward_s": rl["wall_s"], "updates_per_s": ul / rl["wall_s"],
-                "note": "dmk_batch_run_lockstep: the residual evaluations of the RVEs in flight run as rounds, "
-                        "one launch per kernel over all RVEs of a round (the transforms per grid size) and one "
-                        "host synchronisation per round"}
+    conc = {"threads": args.batch_threads, "forward_s": r["wall_s"], "updates_per_s": updates / r["wall_s"],
+            "note": "dmk_batch_run: one handle (stream) per RVE driven by a native thread pool"}
+    if not args.lockstep_threads:
+        return {"workload": f"{args.batch} RVEs, edge 8-32, 10-50 grains, xi_0 and n per RVE, tension along a "
+                            "random direction, 40 increments to 2 %", "cells": cells, "concurrent": conc}
+    b = batch.Batch(items)
+    rl = b.run(threads=args.lockstep_threads, lockstep=True)
+    ul = sum(s.counters()["updates"] for s in b.solvers)
+    b.close()
     b = batch.Batch(items)
     b.tape()
-    rt = b.run(threads=args.batch_threads)
+    rt = b.run(threads=args.lockstep_threads, lockstep=True)
     curves = b.curves()
     Pb = []
     for c in curves:
         p = np.zeros_like(c)
         p[:, 0, 0] = 2.0 * (c[:, 0, 0] - 1.01 * c[:, 0, 0]) / c.shape[0]
         Pb.append(p)
-    g, rb = b.gradients(Pb, rtol=args.adjoint_rtol, threads=args.batch_threads)
+    g, rb = b.gradients(Pb, rtol=args.adjoint_rtol, threads=args.lockstep_threads, lockstep=True)
+    gc, rbc = b.gradients(Pb, rtol=args.adjoint_rtol, threads=args.batch_threads)
     b.close()
+    conc["backward_s"] = rbc["wall_s"]
+    conc["gradients_equal_lockstep"] = bool(all(np.array_equal(x, y) for x, y in zip(g, gc)))
     return {"workload": f"{args.batch} RVEs, edge 8-32, 10-50 grains, xi_0 and n per RVE, tension along a random "
-                        "direction, 40 increments to 2 %; one handle (stream) per RVE driven by a native thread pool",
-            "cells": cells, "threads": args.batch_threads, "forward_s": r["wall_s"],
-            "updates_per_s": updates / r["wall_s"], "forward_with_tape_s": rt["wall_s"],
-            "backward_s": rb["wall_s"], "adjoint_over_forward": (rt["wall_s"] + rb["wall_s"]) / r["wall_s"],
-            "gmres_rtol": args.adjoint_rtol, "gradient_norm": float(np.sqrt(sum(float(x @ x) for x in g))),
-            "lockstep": lock}
+                        "direction, 40 increments to 2 %; dmk_batch_run_lockstep / dmk_batch_adjoint_lockstep: "
+                        "the RVEs in flight (one host thread each) run their residual evaluations, and their "
+                        "reverse sweeps' adjoint operators and dot products, as rounds of one launch per kernel",
+            "cells": cells, "threads": args.lockstep_threads, "forward_s": rl["wall_s"],
+            "updates_per_s": ul / rl["wall_s"], "forward_with_tape_s": rt["wall_s"], "backward_s": rb["wall_s"],
+            "adjoint_over_forward": (rt["wall_s"] + rb["wall_s"]) / rl["wall_s"], "gmres_rtol": args.adjoint_rtol,
+            "gradient_norm": float(np.sqrt(sum(float(x @ x) for x in g))), "concurrent": conc}
 
 
 def mixed_batch_cost(args):
@@ -516,7 +521,13 @@ def main():
         progress(f"adjoint / forward: {line['adjoint']['adjoint_over_forward']:.2f}")
     if args.batch:
         line["batched"] = batched_cost(args)
-        progress(f"batched: {line['batched']['updates_per_s'] / 1e6:.1f} M updates/s")
+        bt = line["batched"]
+        if "updates_per_s" in bt:
+            progress(f"batched: {bt['updates_per_s'] / 1e6:.1f} M updates/s in lockstep "
+                     f"({bt['concurrent']['updates_per_s'] / 1e6:.1f} M concurrent), adjoint / forward "
+                     f"{bt['adjoint_over_forward']:.2f}")
+        else:
+            progress(f"batched: {bt['concurrent']['updates_per_s'] / 1e6:.1f} M updates/s concurrent")
     if args.schemes:
         line["schemes"] = schemes_cost(args)
         progress("other schemes timed")

@@ -75,3 +75,29 @@ def test_lockstep_batch_matches_single_runs(gpu_lib, monkeypatch, mode):
         s.close()
         assert c1.shape == c.shape
         assert np.array_equal(c1, c), np.abs(c1 - c).max() / np.abs(c1).max()
+
+
+def test_lockstep_checkpointed_gradients_match(gpu_lib, monkeypatch):
+    """Checkpointed tapes (a checkpoint every 2 increments) recorded in
+    lockstep rounds; the lockstep reverse sweeps run the earlier segments
+    again, so their rounds mix residual evaluations of the recomputed
+    increments with adjoint-operator and dot-product requests.  Curves and
+    gradients equal those of concurrent handles, bit for bit."""
+    from damask_amd import batch
+    monkeypatch.delenv("DMK_LOCKSTEP", raising=False)
+    monkeypatch.setenv("DMK_LOCKSTEP_GROUPS", "1")
+    items = batch.random_batch(5, master_seed=13, edges=(8, 12), grains=(5, 12), increments=5, strain=0.005)
+    out = []
+    for lockstep in (False, True):
+        b = batch.Batch(items)
+        b.tape(segment=2)
+        b.run(threads=5, lockstep=lockstep)
+        curves = b.curves()
+        Pbar = [np.broadcast_to(np.eye(3), c.shape) * 1e-6 for c in curves]
+        grads, _ = b.gradients(Pbar, rtol=1e-8, threads=5, lockstep=lockstep)
+        b.close()
+        out.append((curves, grads))
+    for c0, c1 in zip(out[0][0], out[1][0]):
+        assert np.array_equal(c0, c1)
+    for g0, g1 in zip(out[0][1], out[1][1]):
+        assert np.all(np.isfinite(g0)) and np.array_equal(g0, g1)
