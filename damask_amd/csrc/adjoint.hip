@@ -33,8 +33,9 @@ __device__ __forceinline__ void store_rm(double* f, long N, int e, const double*
     for (int j = 0; j < 3; j++) f[(size_t)(i + 3 * j) * N + e] = T[3 * i + j];
 }
 
-// scratch regions: per-system temporaries [AV_N mmax][N], then per cell the
-// matrix block (nmax^2 + nmax) and the 9 tangent right-hand sides (9 nmax)
+// scratch regions, each entry-major over the cells ([entry][N]): per-system
+// temporaries (AV_N mmax entries), the matrix block (nmax^2 + nmax: G_y row
+// major, then the pivots) and the 9 tangent right-hand sides (9 nmax)
 struct AdjScr {
   double *av, *g, *x;
   int nmax, blk;
@@ -65,7 +66,7 @@ __global__ void __launch_bounds__(ADJ_WG) adj_assemble_kernel(AdjKArgs a, const 
   const AdjScr sc = adj_scr(a);
   AdjPoint pt;
   load_point(a, e, pt);
-  AdjAcc acc{sc.av, (size_t)a.N, (size_t)e, sc.g + (size_t)e * sc.blk, sc.nmax, a.mmax};
+  AdjAcc acc{sc.av, (size_t)a.N, (size_t)e, sc.g, sc.nmax, a.mmax};
   AdjVec xi{(double*)a.xi, (size_t)a.N, (size_t)e}, xi0{(double*)a.xi0, (size_t)a.N, (size_t)e};
   AdjFwd f;
   if (adj_forward(ph, pt, xi, f, acc)) {
@@ -74,11 +75,12 @@ __global__ void __launch_bounds__(ADJ_WG) adj_assemble_kernel(AdjKArgs a, const 
   }
   a.res[e] = adj_residual(ph, pt, xi, xi0, acc, a.eps_rel_Lp, a.eps_abs_Lp, a.eps_rel_state);
   adj_assemble(ph, pt, xi, acc);
-  double* X = sc.x + (size_t)e * 9 * sc.nmax;
+  double* X = sc.x + e;      // entry r at X[r * N]
   for (int q = 0; q < 9; q++) {
-    double xl[9], w[DMK_NSL_MAX];
-    adj_tangent_rhs(ph, pt, f, acc, q, xl, X + q * sc.nmax + 9, w);
-    for (int i = 0; i < 9; i++) X[q * sc.nmax + i] = xl[i];
+    double xl[9], xs[DMK_NSL_MAX], w[DMK_NSL_MAX];
+    adj_tangent_rhs(ph, pt, f, acc, q, xl, xs, w);
+    for (int i = 0; i < 9; i++) X[(size_t)(q * sc.nmax + i) * a.N] = xl[i];
+    for (int i = 0; i < ph.nsl; i++) X[(size_t)(q * sc.nmax + 9 + i) * a.N] = xs[i];
   }
 }
 
@@ -99,12 +101,14 @@ __global__ void __launch_bounds__(COOP_CELLS * COOP_LANES) adj_lu_kernel(AdjKArg
   double* G = lds + (size_t)g * n * LS;
   int* pv = (int*)(lds + (size_t)COOP_CELLS * n * LS) + g * n;
   const AdjScr sc = adj_scr(a);
-  double* Gg = sc.g + (size_t)(act ? e : 0) * sc.blk;
-  double* Xg = sc.x + (size_t)(act ? e : 0) * 9 * sc.nmax;
+  // entry r of this cell at Gg[r * N], Xg[r * N]
+  double* Gg = sc.g + (act ? e : 0);
+  double* Xg = sc.x + (act ? e : 0);
+  const size_t N = a.N;
   if (act) {
     for (int i = 0; i < n; i++) {
-      for (int j = l; j < n; j += COOP_LANES) G[i * LS + j] = Gg[i * sc.nmax + j];
-      if (l < 9) G[i * LS + n + l] = Xg[l * sc.nmax + i];
+      for (int j = l; j < n; j += COOP_LANES) G[i * LS + j] = Gg[(size_t)(i * sc.nmax + j) * N];
+      if (l < 9) G[i * LS + n + l] = Xg[(size_t)(l * sc.nmax + i) * N];
     }
   }
   bool sing = false;
@@ -182,10 +186,10 @@ __global__ void __launch_bounds__(COOP_CELLS * COOP_LANES) adj_lu_kernel(AdjKArg
     return;
   }
   for (int i = 0; i < n; i++) {
-    for (int j = l; j < n; j += COOP_LANES) Gg[i * sc.nmax + j] = G[i * LS + j];
-    if (l < 9) Xg[l * sc.nmax + i] = G[i * LS + n + l];
+    for (int j = l; j < n; j += COOP_LANES) Gg[(size_t)(i * sc.nmax + j) * N] = G[i * LS + j];
+    if (l < 9) Xg[(size_t)(l * sc.nmax + i) * N] = G[i * LS + n + l];
   }
-  for (int j = l; j < n; j += COOP_LANES) Gg[sc.nmax * sc.nmax + j] = (double)pv[j];
+  for (int j = l; j < n; j += COOP_LANES) Gg[(size_t)(sc.nmax * sc.nmax + j) * N] = (double)pv[j];
 }
 
 // K3a (per cell, with K3 mode 0): the tangent A from the 9 solutions,
@@ -201,11 +205,11 @@ __global__ void __launch_bounds__(ADJ_WG) adj_tangent_kernel(AdjKArgs a, const D
   load_point(a, e, pt);
   AdjFwd f;
   adj_fwd33(ph, pt, f);
-  const double* X = sc.x + (size_t)e * 9 * sc.nmax;
+  const double* X = sc.x + e;      // entry r at X[r * N]
 #pragma unroll 1
   for (int q = 0; q < 9; q++) {
     double x[9], c[9];
-    for (int k = 0; k < 9; k++) x[k] = X[q * sc.nmax + k];
+    for (int k = 0; k < 9; k++) x[k] = X[(size_t)(q * sc.nmax + k) * N];
     adj_tangent_dP(ph, pt, f, q, x, c);
     for (int k = 0; k < 9; k++) a.A[(size_t)(k + 9 * q) * N + e] = c[k];
   }
@@ -226,7 +230,7 @@ __global__ void __launch_bounds__(ADJ_WG) adj_vjp_kernel(AdjKArgs a, const DmkPh
   const AdjScr sc = adj_scr(a);
   AdjPoint pt;
   load_point(a, e, pt);
-  AdjAcc acc{sc.av, (size_t)N, (size_t)e, sc.g + (size_t)e * sc.blk, sc.nmax, a.mmax};
+  AdjAcc acc{sc.av, (size_t)N, (size_t)e, sc.g, sc.nmax, a.mmax};
   AdjVec xi{(double*)a.xi, (size_t)N, (size_t)e};
   AdjVec xib{(double*)a.xib_in, (size_t)N, (size_t)e};
   AdjFwd f;

@@ -31,16 +31,17 @@
 #include "dmk_adjoint_k.h"
 
 // Per-cell storage: per-system temporaries av ([slot][avN] at cell e) and the
-// cell's matrix block g (G_y row-major with row stride nmax, then the pivots).
-// Device: av component-major over the cells, g contiguous per cell; host
+// matrix block g (G_y row-major with row stride nmax, then the pivots), both
+// entry-major over the cells (entry r of cell e at r * avN + e), so that a
+// wavefront of consecutive cells reads and writes each entry coalesced; host
 // check: avN = 1, e = 0.
 struct AdjAcc {
   double* av;
   size_t avN, e;
   double* g;
   int nmax, mmax;       // nmax = 9 + mmax
-  DHI double& G(int i, int j) const { return g[(size_t)i * nmax + j]; }
-  DHI double& piv(int i) const { return g[(size_t)nmax * nmax + i]; }
+  DHI double& G(int i, int j) const { return g[((size_t)i * nmax + j) * avN + e]; }
+  DHI double& piv(int i) const { return g[((size_t)nmax * nmax + i) * avN + e]; }
   DHI double& v(int k, int a) const { return av[((size_t)k * mmax + a) * avN + e]; }
 };
 
