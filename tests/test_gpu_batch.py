@@ -37,7 +37,7 @@ def test_batch_matches_single_runs(gpu_lib, monkeypatch, lockstep):
         s.close()
 
 
-@pytest.mark.parametrize("mode", ["residual", "residual, 2 groups", "stress"])
+@pytest.mark.parametrize("mode", ["residual", "residual, 2 groups", "residual, partial rounds", "stress"])
 def test_lockstep_batch_matches_single_runs(gpu_lib, monkeypatch, mode):
     """dmk_batch_run_lockstep: the RVEs in flight run each residual
     evaluation as one round (mode "residual": stress integration, response
@@ -45,7 +45,9 @@ def test_lockstep_batch_matches_single_runs(gpu_lib, monkeypatch, mode):
     per kernel over the round, the transforms per FFT configuration, with the
     odd 7x6x5 grid's configuration among them; with 2 groups,
     DMK_LOCKSTEP_GROUPS=2, the host threads form two lockstep groups whose
-    rounds alternate; mode "stress", DMK_LOCKSTEP=stress: only the stress
+    rounds alternate; with partial rounds, DMK_LOCKSTEP_WAIT_US=0, a round
+    starts as soon as a waiting RVE's wait runs out, without the RVEs not yet
+    submitted; mode "stress", DMK_LOCKSTEP=stress: only the stress
     integrations are shared).  12-system
     FCC RVEs of two sizes and a 24-system BCC RVE with non-Schmid terms on its
     {110}<111> family; RVEs finish at different times and leave the lockstep.
@@ -56,6 +58,7 @@ def test_lockstep_batch_matches_single_runs(gpu_lib, monkeypatch, mode):
     else:
         monkeypatch.delenv("DMK_LOCKSTEP", raising=False)
     monkeypatch.setenv("DMK_LOCKSTEP_GROUPS", "2" if "2 groups" in mode else "1")
+    monkeypatch.setenv("DMK_LOCKSTEP_WAIT_US", "0" if "partial" in mode else "")
     from damask_amd.grid import GridMechSpectralBasic
     items = batch.random_batch(7, master_seed=5, edges=(8, 12), grains=(5, 12), increments=4, strain=0.004)
     grid, mat = workloads.polycrystal([7, 6, 5], 6, 21, size=(1.0, 0.9, 0.8), phase=workloads.BCC_NS_STRONG_PHASE)
