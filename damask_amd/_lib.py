@@ -55,7 +55,7 @@ EXPORTS = ["dmk_mech_init", "dmk_mech_forward", "dmk_mech_materialpoint_forward"
            "dmk_adjoint_record", "dmk_adjoint_info", "dmk_adjoint_backward", "dmk_adjoint_release",
            "dmk_adjoint_point_host", "dmk_mech_restart_response", "dmk_grid_set_restart", "dmk_batch_run", "dmk_batch_run_lockstep",
            "dmk_batch_adjoint", "dmk_curl_rms", "dmk_inv_sym3333", "dmk_mech_cell_counters", "dmk_adjoint_tape_segmented",
-           "dmk_divergence_rms_noalloc", "dmk_batch_adjoint_lockstep"]
+           "dmk_divergence_rms_noalloc", "dmk_batch_adjoint_lockstep", "dmk_pow_law"]
 
 # largest grid of dmk_divergence_rms_noalloc (include/damask_hip.h)
 DMK_NOALLOC_MAX_CELLS = 262144
@@ -153,6 +153,8 @@ def load(path: str = LIB_PATH):
                                     C.POINTER(C.c_int64), C.c_int32, ip, dp]
     L.dmk_batch_adjoint_lockstep.restype = C.c_int
     L.dmk_batch_adjoint_lockstep.argtypes = L.dmk_batch_adjoint.argtypes
+    L.dmk_pow_law.restype = C.c_int
+    L.dmk_pow_law.argtypes = [dp, dp, dp, C.c_int32, C.c_int32]
     _lib = L
     return L
 

@@ -198,6 +198,14 @@ int dmk_curl_rms(dmk_handle* h, const double* field, double* out);
 /* math_invSym3333 (src/math.f90:511-532): inverse through the Mandel 6x6 form
  * (:826-904) of a column-major 3x3x3x3; nonzero on a zero pivot */
 int dmk_inv_sym3333(const double* A, double* out);
+/* x^y of the phenopowerlaw kinetics and hardening ((tau/xi)^n,
+ * phase_mechanical_plastic_phenopowerlaw.f90:474-476; |1 - xi/xi_inf|^a,
+ * :386-387) for n points
+ * (host arrays), evaluated on the device as the stress kernel evaluates it:
+ * quarter != 0 the quarter-power path of phases whose exponents are all
+ * positive multiples of 1/4 up to 64 (other exponents are rejected), else the
+ * library pow.  A check of the kernels' arithmetic; it allocates. */
+int dmk_pow_law(const double* x, const double* y, double* out, int32_t n, int32_t quarter);
 /* unnormalised 3-D r2c / c2r of howmany (<= 9) interleaved fields, FFTW's
  * conventions (spectral_utilities.f90:239-261): real (howmany, N1, N2, N3) <->
  * complex (howmany, N1/2+1, N2, N3) as (re, im) pairs; c2r ignores the
