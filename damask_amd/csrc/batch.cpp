@@ -422,13 +422,18 @@ static int submit(DmkStressBatch* b, int slot, const DmkStressItem* items, const
   } else if (!done_out && hipStreamWaitEvent(st, ev, 0) != hipSuccess) {
     dmk_set_error("lockstep batch: stream wait");
     err = DMK_ERR_HIP;
-  } else if (rq) {
-    // the report reaches the host at the end of the round
+  } else if (rq && rq->rep_n > 0) {
+    // the report reaches the host at the end of the round; a request without
+    // one (an operator application whose result only later requests read,
+    // vector steps) returns at the launch, and its stage may then be taken by
+    // a later round while this one runs: the stage's uploads are ordered
+    // behind it on the batch stream, and the reports of the stage are read
+    // only by submitters that hold it
     if (hipEventSynchronize(ev) != hipSuccess) {
       dmk_set_error("lockstep batch: round failed");
       err = DMK_ERR_HIP;
     } else {
-      if (rq->rep_n > 0) std::memcpy(report, b->rep_host[s] + (size_t)ri * REP_SIZE, sizeof(double) * rq->rep_n);
+      std::memcpy(report, b->rep_host[s] + (size_t)ri * REP_SIZE, sizeof(double) * rq->rep_n);
       if (b->timing && ri == 0) {
         // one thread per round: the round's device intervals
         for (int k = 0; k < 3; k++) hipEventElapsedTime(&ms[k], b->tev[s][k], b->tev[s][k + 1]);

@@ -24,12 +24,15 @@ struct DmkFFT;
 // Hand a request of a lockstep round to the batch (struct DmkResidDev: a
 // whole residual evaluation with its stress items, an application of the
 // adjoint's operator, or dot products; fft is the RVE's FFT configuration and
-// must stay valid until the call returns).  Returns once the round holding it
-// has finished on the device, with the rq->rep_n report doubles copied into
-// report and `st` made to wait for it; with done_out the wait is left to the
-// caller, who gets the round's completion event (rounds complete in order:
-// a later round's event covers it).  rq->idle: `st` has had no work since
-// the caller's previous round, so the round does not wait for it.
+// must stay valid until the call returns).  With rq->rep_n > 0 it returns
+// once the round holding it has finished on the device, with the rq->rep_n
+// report doubles copied into report; without a report, once the round is
+// launched.  Either way `st` is made to wait for the round; with done_out
+// that wait is left to the caller, who gets the round's completion event
+// (rounds complete in order: a later round's event covers it, and the event
+// may be recorded again by a later round before the caller waits on it, which
+// only lengthens the wait).  rq->idle: `st` has had no work since the
+// caller's previous round, so the round does not wait for it.
 int dmk_resid_batch_submit(DmkStressBatch* b, int slot, const DmkStressItem* items, const int* key, int n,
                            const DmkResidDev* rq, const DmkFFT* fft, hipStream_t st, double* report,
                            hipEvent_t* done_out);

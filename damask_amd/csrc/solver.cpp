@@ -1845,7 +1845,7 @@ extern "C" int dmk_adjoint_backward(dmk_handle* h, const double* P_av_bar, doubl
       aa.rotate = 1;
       aa.update = 1;
       CK(hipMemsetAsync(h->small + SM_AFAIM, 0, sizeof(double) * 9, h->st));
-      if ((rc = adj_apply(h, aa, nullptr, a.g, a.b, nullptr, nullptr))) return rc;
+      if ((rc = adj_apply(h, aa, nullptr, a.g, a.b, nullptr, nullptr, true))) return rc;
       int its = 0;
       double rel = 0.0;
       if ((rc = adj_gmres(h, aa, rtol, 600, &its, &rel))) return rc;

@@ -339,7 +339,10 @@ extern "C" int dmk_grid_run_loadcase(dmk_handle* h, const dmk_loadcase*, double*
       q.idle = first ? 0 : 1;
       q.rep = reinterpret_cast<double*>(h);
       q.wgt = (double)r;
-      q.rep_n = 2;
+      // every third request carries no report (as the operator applications
+      // whose results only later requests read): it returns at the launch
+      const bool norep = r % 3 == 0;
+      q.rep_n = norep ? 0 : 2;
       q.npre = 1;
       q.pre[0].type = DMK_PRE_AXPBY;
       q.pre[0].n = 1;
@@ -347,7 +350,8 @@ extern "C" int dmk_grid_run_loadcase(dmk_handle* h, const dmk_loadcase*, double*
       double rep[2] = {-1.0, -1.0};
       int rc = dmk_resid_batch_submit(h->batch, h->slot, nullptr, nullptr, 0, &q, nullptr, h->st, rep, &last);
       if (rc) return rc;
-      if (rep[0] != (double)r || rep[1] != (double)h->id) h->errors++;
+      if (!norep && (rep[0] != (double)r || rep[1] != (double)h->id)) h->errors++;
+      if (norep && (rep[0] != -1.0 || rep[1] != -1.0)) h->errors++;
       if (lastr) {
         hipStreamWaitEvent(h->st, last, 0);
         h->st->push([h, r] {
