@@ -165,11 +165,12 @@ bool fft_less(const DmkFFT* a, const DmkFFT* c) {
 int resid_launch_locked(DmkStressBatch* b, int s, int nr) {
   const DmkResidDev* R = b->rdev[s];
   int max_nc_resp = 0, max_nc_dots = 0, max_jobs = 0, max_sums = 0, max_nc_adj = 0, ntr = 0;
-  long max_N_adj = 0, max_pre = 0;
+  long max_N_adj = 0, max_pre = 0, max_post = 0;
   bool any_res = false, any_adj = false;
   for (int k = 0; k < nr; k++) {
     const DmkResidDev& q = b->rhost[s][k];
     for (int o = 0; o < q.npre; o++) max_pre = std::max(max_pre, q.pre[o].n);
+    if (q.npost) max_post = std::max(max_post, q.post.n);
     max_nc_dots = std::max(max_nc_dots, q.nc_dots);
     max_jobs = std::max(max_jobs, q.J.n);
     max_sums = std::max(max_sums, q.S.k);
@@ -211,6 +212,8 @@ int resid_launch_locked(DmkStressBatch* b, int s, int nr) {
     }
     k0 = k1;
   }
+  // the vector steps that follow a request's transform, before its dot products
+  if (max_post > 0 && dmk_post_batch(R, nr, max_post, b->st)) return DMK_ERR_HIP;
   if (dmk_resid_dots_batch(R, nr, max_nc_dots, max_jobs, max_sums, b->rep_dev[s], b->st)) return DMK_ERR_HIP;
   if (hipMemcpyAsync(b->rep_host[s], b->rep_dev[s], sizeof(double) * REP_SIZE * nr, hipMemcpyDeviceToHost, b->st) !=
       hipSuccess)

@@ -41,6 +41,10 @@ struct DmkResidDev {
   int idle;                    // the RVE's stream has had no work since its last round: no input event
   int npre;
   DmkPreOp pre[DMK_PRE_OPS];
+  // a vector step (DMK_PRE_AXPBY) after the request's transform and before
+  // its dot products, e.g. r = b - Phi(A^T mu) ahead of |r|
+  int npost;
+  DmkPreOp post;
   // response reductions: status, iteration counts and P columns of the new
   // response (Pnew, N cells) -> flag, counters, raw P_av, aim bookkeeping
   const int32_t* st;
@@ -94,8 +98,10 @@ int dmk_gamma_conv_nb(const DmkFFT* fft);
 int dmk_resid_reduce_batch(const DmkResidDev* R, int nr, int max_nc_resp, hipStream_t s);
 // the adjoint-operator requests' A^T v (adjoint.hip) and mean rule (linalg.hip)
 int dmk_adj_atmul_batch(const DmkResidDev* R, int nr, long max_N, hipStream_t s);
-// the requests' vector steps (linalg.hip; max_n bounds their lengths)
+// the requests' vector steps (linalg.hip; max_n bounds their lengths): those
+// before the request's work, and those after its transform
 int dmk_pre_batch(const DmkResidDev* R, int nr, long max_n, hipStream_t s);
+int dmk_post_batch(const DmkResidDev* R, int nr, long max_n, hipStream_t s);
 int dmk_adj_mean_batch(const DmkResidDev* R, int nr, int max_nc, hipStream_t s);
 int dmk_resid_dots_batch(const DmkResidDev* R, int nr, int max_nc_dots, int max_jobs, int max_sums, double* rep_out,
                          hipStream_t s);

@@ -335,6 +335,14 @@ __global__ void pre_batch_kernel(const DmkResidDev* __restrict__ R) {
   }
 }
 
+__global__ void post_batch_kernel(const DmkResidDev* __restrict__ R) {
+  const DmkResidDev& q = R[blockIdx.y];
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q.npost == 0) return;
+  const DmkPreOp& o = q.post;
+  if (i < o.n) o.z[i] = dmk_axpby_elem(o.a, o.x[i], o.b, o.y[i]);
+}
+
 __global__ void combine_kernel(long n, const double* __restrict__ XM, CombineArgs c, double* __restrict__ XA) {
   long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
@@ -612,6 +620,12 @@ int dmk_resid_reduce_batch(const DmkResidDev* R, int nr, int max_nc_resp, hipStr
 int dmk_pre_batch(const DmkResidDev* R, int nr, long max_n, hipStream_t s) {
   if (nr < 1 || max_n < 1) return 0;
   hipLaunchKernelGGL(pre_batch_kernel, dim3((unsigned)((max_n + 255) / 256), nr), dim3(256), 0, s, R);
+  return hipGetLastError() == hipSuccess ? 0 : 2;
+}
+
+int dmk_post_batch(const DmkResidDev* R, int nr, long max_n, hipStream_t s) {
+  if (nr < 1 || max_n < 1) return 0;
+  hipLaunchKernelGGL(post_batch_kernel, dim3((unsigned)((max_n + 255) / 256), nr), dim3(256), 0, s, R);
   return hipGetLastError() == hipSuccess ? 0 : 2;
 }
 

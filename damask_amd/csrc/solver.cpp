@@ -1572,11 +1572,12 @@ static int adj_dots(dmk_handle* h, const DotJobs& J, double* out) {
 }
 
 // out = Phi(A^T v) (v given) or Phi(y): the Gamma_0 convolution with the
-// mean rule of the increment; then the dot products J (optional) into d.  In
+// mean rule of the increment; then the vector step post (optional) and the
+// dot products J (optional) into d.  In
 // a lockstep batch all of it is one request of the round; otherwise the same
 // kernels on the handle's stream.
 static int adj_apply(dmk_handle* h, const AimArgs& aa, const double* v, const double* y, double* out,
-                     const DotJobs* J, double* d, bool defer = false) {
+                     const DotJobs* J, double* d, bool defer = false, const DmkPreOp* post = nullptr) {
   if (v) y = h->adj->y;
   GammaArgs g = h->adj->ga;
   g.do_div = 0;
@@ -1601,6 +1602,10 @@ static int adj_apply(dmk_handle* h, const AimArgs& aa, const double* v, const do
     q.r = out;
     q.divp = h->divp;
     q.g = g;
+    if (post) {
+      q.npost = 1;
+      q.post = *post;
+    }
     adj_dot_request(h, J, q);
     return adj_submit(h, q, d, defer);
   }
@@ -1612,7 +1617,21 @@ static int adj_apply(dmk_handle* h, const AimArgs& aa, const double* v, const do
   int nb = 0;
   CKL(dmk_gamma_conv(&h->fft, h->Z, &g, h->divp, &nb, h->st));
   CKL(dmk_fft_c2r_xy(&h->fft, 9, h->Z, out, h->st));
+  if (post) CKL(dmk_axpby(post->n, post->a, post->x, post->b, post->y, post->z, h->st));
   return J && J->n ? adj_dots(h, *J, d) : 0;
+}
+
+// z = a x + b y as the vector step that follows an operator application
+static DmkPreOp axpby_step(long n, double av, const double* x, double bv, const double* y, double* z) {
+  DmkPreOp o{};
+  o.type = DMK_PRE_AXPBY;
+  o.n = n;
+  o.a = av;
+  o.x = x;
+  o.b = bv;
+  o.y = y;
+  o.z = z;
+  return o;
 }
 
 // Restarted GMRES for Phi(A^T mu) = b, started from the mu in place (the
@@ -1630,12 +1649,11 @@ static int adj_gmres(dmk_handle* h, const AimArgs& aa, double rtol, int maxit, i
   // r = b - Phi(A^T mu), |b|, |r|.  In a lockstep batch every step below is
   // a round request or a vector step carried by the next one; the solver
   // stream waits for the rounds at the end (adj_lk_sync).
-  if ((rc = adj_apply(h, aa, a.mu, nullptr, a.r, nullptr, nullptr, true))) return rc;
-  if ((rc = adj_vaxpby(h, n, 1.0, a.b, -1.0, a.r, a.r))) return rc;
+  const DmkPreOp resid = axpby_step(n, 1.0, a.b, -1.0, a.r, a.r);
   J.n = 0;
   add_dot(&J, a.b, a.b, 0);
   add_dot(&J, a.r, a.r, 0);
-  if ((rc = adj_dots(h, J, d))) return rc;
+  if ((rc = adj_apply(h, aa, a.mu, nullptr, a.r, &J, d, true, &resid))) return rc;
   const double bnorm = std::sqrt(d[0]);
   double rnorm = std::sqrt(d[1]);
   *its_out = 0;
@@ -1706,11 +1724,9 @@ static int adj_gmres(dmk_handle* h, const AimArgs& aa, double rtol, int maxit, i
     for (int i = 0; i < j; i++) c.c[i] = -yv[i];
     if ((rc = adj_vgs(h, n, a.mu, a.V, n, c))) return rc;
     // true residual (restart vector and the reported value)
-    if ((rc = adj_apply(h, aa, a.mu, nullptr, a.r, nullptr, nullptr, true))) return rc;
-    if ((rc = adj_vaxpby(h, n, 1.0, a.b, -1.0, a.r, a.r))) return rc;
     J.n = 0;
     add_dot(&J, a.r, a.r, 0);
-    if ((rc = adj_dots(h, J, d))) return rc;
+    if ((rc = adj_apply(h, aa, a.mu, nullptr, a.r, &J, d, true, &resid))) return rc;
     rnorm = std::sqrt(d[0]);
   }
   if ((rc = adj_lk_sync(h))) return rc;
