@@ -90,6 +90,12 @@ struct DmkStressBatch {
   hipEvent_t tev[NSTAGE][4] = {};
   double d_ms[3] = {0.0, 0.0, 0.0};
   int64_t d_rounds = 0;
+  // and per round (by number, the first MAX_STAMPED rounds): steady-clock
+  // times of the launch's end, of the round's end on the device (a host
+  // function enqueued behind it) and of the first submitter's return with its
+  // report
+  static constexpr uint64_t MAX_STAMPED = 1 << 16;
+  std::unique_ptr<std::atomic<int64_t>[]> stamp;
 };
 
 namespace {
@@ -303,9 +309,19 @@ void batch_launch_locked(DmkStressBatch* b, std::unique_lock<std::mutex>& lk) {
   if (rc) dmk_set_error("lockstep batch: launch of the round failed");
   b->rc[s] = rc;
   b->users[s] = (int)b->reqs.size();
-  if (b->timing)
-    b->t_launch_ns += std::chrono::duration_cast<std::chrono::nanoseconds>(std::chrono::steady_clock::now() - t_start)
-                          .count();
+  if (b->timing) {
+    const auto t_end = std::chrono::steady_clock::now();
+    b->t_launch_ns += std::chrono::duration_cast<std::chrono::nanoseconds>(t_end - t_start).count();
+    if (!rc && b->stamp && b->gen < DmkStressBatch::MAX_STAMPED) {
+      std::atomic<int64_t>* st = &b->stamp[3 * b->gen];
+      st[0].store(t_end.time_since_epoch().count());
+      hipLaunchHostFunc(
+          b->st, [](void* p) {
+            static_cast<std::atomic<int64_t>*>(p)->store(std::chrono::steady_clock::now().time_since_epoch().count());
+          },
+          st + 1);
+    }
+  }
   b->gen++;
   b->launches++;
   b->reqs.clear();
@@ -441,6 +457,13 @@ static int submit(DmkStressBatch* b, int slot, const DmkStressItem* items, const
         // one thread per round: the round's device intervals
         for (int k = 0; k < 3; k++) hipEventElapsedTime(&ms[k], b->tev[s][k], b->tev[s][k + 1]);
       }
+      if (b->timing && b->stamp && my < DmkStressBatch::MAX_STAMPED) {
+        const int64_t now = std::chrono::steady_clock::now().time_since_epoch().count();
+        std::atomic<int64_t>& w = b->stamp[3 * my + 2];
+        int64_t cur = w.load();
+        while ((cur == 0 || now < cur) && !w.compare_exchange_weak(cur, now)) {
+        }
+      }
     }
   }
   if (done_out) *done_out = ev;
@@ -494,6 +517,10 @@ static int make_groups(std::vector<std::unique_ptr<DmkStressBatch>>& grp, int nt
   for (int k = 0; k < ng && !e; k++) {
     grp.emplace_back(new DmkStressBatch);
     grp.back()->timing = timing;
+    if (timing) {
+      grp.back()->stamp.reset(new std::atomic<int64_t>[3 * DmkStressBatch::MAX_STAMPED]);
+      for (uint64_t i = 0; i < 3 * DmkStressBatch::MAX_STAMPED; i++) grp.back()->stamp[i].store(0);
+    }
     e = batch_init(grp.back().get(), (nt - k + ng - 1) / ng, !(mode && std::strcmp(mode, "stress") == 0), wait_us);
   }
   if (e) {
@@ -513,6 +540,23 @@ static void print_timing(const std::vector<std::unique_ptr<DmkStressBatch>>& grp
             "to start, %.3f ms for the device; launches %.3f ms per round\n",
             k, (unsigned long long)b.gen, (long long)b.n_sub.load(), b.t_wait_ns * 1e-6 / ns, b.t_gpu_ns * 1e-6 / ns,
             b.t_launch_ns * 1e-6 / nr);
+    if (b.stamp) {
+      hipStreamSynchronize(b.st);     // the host functions of the last rounds
+      double dev = 0.0, wake = 0.0;
+      int64_t m = 0;
+      for (uint64_t g = 0; g < std::min<uint64_t>(b.gen, DmkStressBatch::MAX_STAMPED); g++) {
+        const int64_t t0 = b.stamp[3 * g].load(), t1 = b.stamp[3 * g + 1].load(), t2 = b.stamp[3 * g + 2].load();
+        if (!t0 || !t1 || !t2) continue;
+        dev += (double)(t1 - t0);
+        wake += (double)(t2 - t1);
+        m++;
+      }
+      if (m)
+        fprintf(stderr,
+                "lockstep group %zu timing per round (%lld rounds): %.3f ms from the launch's end to the round's end on "
+                "the device, %.3f ms from there to the first submitter's return with its report\n",
+                k, (long long)m, dev * 1e-6 / m, wake * 1e-6 / m);
+    }
     if (b.d_rounds)
       fprintf(stderr,
               "lockstep group %zu timing on the device, per round: %.3f ms waiting for the RVE streams and uploads, "

@@ -96,6 +96,10 @@ struct MockEvent {
 };
 
 hipError_t hipStreamCreateWithFlags(hipStream_t* s, unsigned) { *s = new MockStream; return hipSuccess; }
+hipError_t hipLaunchHostFunc(hipStream_t s, hipHostFn_t fn, void* user) {
+  s->push([=] { fn(user); });
+  return hipSuccess;
+}
 hipError_t hipStreamDestroy(hipStream_t s) {
   s->sync();
   {

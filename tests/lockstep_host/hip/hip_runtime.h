@@ -37,6 +37,8 @@ const char* hipGetErrorString(hipError_t e);
 hipError_t hipEventCreate(hipEvent_t* e);
 hipError_t hipEventElapsedTime(float* ms, hipEvent_t a, hipEvent_t b);
 hipError_t hipMemGetInfo(size_t* free_bytes, size_t* total_bytes);
+typedef void (*hipHostFn_t)(void*);
+hipError_t hipLaunchHostFunc(hipStream_t s, hipHostFn_t fn, void* user);
 struct double2 {
   double x, y;
 };
