@@ -80,7 +80,9 @@ class Batch:
 
     def run(self, threads: int | None = None, lockstep: bool = False) -> dict:
         """All load cases; with lockstep the RVEs in flight (one per thread)
-        share one stress-integration launch per residual evaluation."""
+        run their residual evaluations as shared rounds, one launch per kernel
+        over the round (dmk_batch_run_lockstep; DMK_LOCKSTEP=stress shares the
+        stress integration only)."""
         lcs = (_lib.LoadCase * len(self.items))(*[loadcase_struct(lc) for _, _, lc in self.items])
         n = len(self.solvers)
         rc = np.zeros(n, np.int32)

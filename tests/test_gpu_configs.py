@@ -126,18 +126,24 @@ def test_64cube_200_grains(gpu_lib, oracle):
     assert np.all(np.diff(rec[:, 3]) > 0.0)                  # hardening: P_xx rises
 
 
-def test_batch_256_rves(gpu_lib, monkeypatch):
-    """the 256-RVE batched forward + adjoint; 16 RVEs drawn with a seeded
-    generator rerun alone give the same curves and gradients bit for bit"""
+@pytest.mark.parametrize("lockstep", [False, True])
+def test_batch_256_rves(gpu_lib, monkeypatch, lockstep):
+    """the 256-RVE batched forward + adjoint, as 16 concurrent handles or in
+    lockstep rounds with all 256 RVEs in flight (one host thread each, the
+    bench's path); 16 RVEs drawn with a seeded generator rerun alone give the
+    same curves and gradients bit for bit"""
     from damask_amd import adjoint, batch
     from damask_amd.grid import GridMechSpectralBasic
     monkeypatch.setenv("DMK_BATCH_PROGRESS", "1")
+    for v in ("DMK_LOCKSTEP", "DMK_LOCKSTEP_GROUPS", "DMK_LOCKSTEP_WAIT_US"):
+        monkeypatch.delenv(v, raising=False)
+    threads = 256 if lockstep else 16
     items = batch.random_batch(256, 0)
     edges = sorted({int(g.cells[0]) for g, _, _ in items})
     assert edges == [8, 12, 16, 20, 24, 32]
     b = batch.Batch(items)
     b.tape()
-    r = b.run(threads=16)
+    r = b.run(threads=threads, lockstep=lockstep)
     curves = b.curves()
     assert all(c.shape == (40, 3, 3) for c in curves)
     Pbar = []
@@ -145,7 +151,7 @@ def test_batch_256_rves(gpu_lib, monkeypatch):
         p = np.zeros_like(c)
         p[:, 0, 0] = 2.0 * (c[:, 0, 0] - 1.01 * c[:, 0, 0]) / c.shape[0]
         Pbar.append(p)
-    grads, rb = b.gradients(Pbar, rtol=1e-6, threads=16)
+    grads, rb = b.gradients(Pbar, rtol=1e-6, threads=threads, lockstep=lockstep)
     b.close()
     print(f"batch forward with tape {r['wall_s']:.2f} s, adjoint {rb['wall_s']:.2f} s")
     assert all(np.all(np.isfinite(gr)) for gr in grads)
