@@ -214,10 +214,12 @@ def mixed_batch_cost(args):
     symmetric deformation rate for 50 increments.  Batched forward, then a
     batched forward on a checkpointed tape (a checkpoint every `segment`
     increments, segment ~ sqrt(increments)) and the batched gradient of the
-    curve loss, whose reverse sweep runs the earlier segments again."""
+    curve loss, whose reverse sweep runs the earlier segments again; then, if
+    the whole tape fits, the same without checkpoints."""
     import math
 
     import numpy as np
+    import torch
     from damask_amd import batch
     items = batch.mixed_load_batch(args.mixed_batch, args.seed, edge=48, grains=300, increments=args.mixed_increments)
     t0 = time.perf_counter()
@@ -227,6 +229,7 @@ def mixed_batch_cost(args):
     updates = sum(s.counters()["updates"] for s in b.solvers)
     cells = int(b.cells.sum())
     b.close()
+    progress(f"mixed batch forward: {r['wall_s']:.1f} s")
     seg = int(math.ceil(math.sqrt(args.mixed_increments)))
     b = batch.Batch(items)
     b.tape(segment=seg)
@@ -235,17 +238,38 @@ def mixed_batch_cost(args):
     Pb = [2.0 * (c - 1.01 * c) / c.shape[0] for c in curves]
     g, rb = b.gradients(Pb, rtol=args.adjoint_rtol, threads=args.batch_threads)
     b.close()
+    progress(f"mixed batch, checkpointed: forward with tapes {rt['wall_s']:.1f} s, reverse sweeps {rb['wall_s']:.1f} s")
     nsl = items[0][1].phases[0].nsl
     n = args.mixed_increments
     full_tape = (n + 1) * (27 + nsl) * 8 * cells
     seg_tape = ((seg + 1) * (27 + nsl) + (-(-n // seg) + 1) * (2 * 9 + 45 + 2 * nsl)) * 8 * cells
+    # the whole tape (no recomputation) where it fits beside the solver buffers
+    # and a full pool of reverse sweeps: capacity n + 4 increments (this load
+    # has no cutbacks; a run that recorded more would fail)
+    whole = None
+    b = batch.Batch(items)
+    cap = n + 4
+    need = (cap + 1) * (27 + nsl) * 8 * cells
+    free, total = torch.cuda.mem_get_info()
+    # a sweep's work buffers: about 34 kB per cell of the 24-system phase
+    work = 34_000 * int(b.cells.max())
+    if need + args.batch_threads * work + total // 16 < free:
+        b.tape(segment=0, capacity=cap)
+        rw = b.run(threads=args.batch_threads)
+        progress(f"mixed batch, whole tape: forward with tapes {rw['wall_s']:.1f} s")
+        gw, rbw = b.gradients(Pb, rtol=args.adjoint_rtol, threads=args.batch_threads)
+        whole = {"capacity": cap, "tape_bytes": need, "forward_with_tape_s": rw["wall_s"], "backward_s": rbw["wall_s"],
+                 "adjoint_over_forward": (rw["wall_s"] + rbw["wall_s"]) / r["wall_s"],
+                 "gradients_equal_checkpointed": all(np.array_equal(x, y) for x, y in zip(g, gw))}
+        progress(f"mixed batch, whole tape: adjoint / forward {whole['adjoint_over_forward']:.2f}")
+    b.close()
     return {"workload": f"{args.mixed_batch} RVEs of 48^3, 300 grains, BCC (cI, N_sl = [12, 12], Schmid law), "
                         f"seeded random symmetric dot_F (equivalent rate 1e-3/s, |trace| <= 1e-4/s), {n} increments",
             "cells": cells, "threads": args.batch_threads, "init_s": t_init, "forward_s": r["wall_s"],
             "updates_per_s": updates / r["wall_s"], "segment": seg, "forward_with_tape_s": rt["wall_s"],
             "backward_s": rb["wall_s"], "adjoint_over_forward": (rt["wall_s"] + rb["wall_s"]) / r["wall_s"],
             "tape_bytes_full": full_tape, "tape_bytes_checkpointed": seg_tape, "gmres_rtol": args.adjoint_rtol,
-            "gradient_norm": float(np.sqrt(sum(float(x @ x) for x in g)))}
+            "gradient_norm": float(np.sqrt(sum(float(x @ x) for x in g))), "whole_tape": whole}
 
 
 def bcc_cost(args, n=20):

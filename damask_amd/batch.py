@@ -72,11 +72,13 @@ class Batch:
         self._h = (C.c_void_p * len(self.solvers))(*[s.h.value for s in self.solvers])
         self.threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
 
-    def tape(self, segment: int = 0):
+    def tape(self, segment: int = 0, capacity: int | None = None):
         """Start the adjoint tapes; segment > 0: checkpoints every segment
-        increments and recomputation in the reverse sweep (see adjoint_tape)."""
+        increments and recomputation in the reverse sweep (see adjoint_tape).
+        capacity: increments a tape holds, allocated up front (default 2 N + 16
+        per load case, room for cutbacks); a run that records more fails."""
         for s, (_, _, lc) in zip(self.solvers, self.items):
-            s.adjoint_tape(2 * lc.N + 16, segment)
+            s.adjoint_tape(capacity if capacity is not None else 2 * lc.N + 16, segment)
 
     def run(self, threads: int | None = None, lockstep: bool = False) -> dict:
         """All load cases; with lockstep the RVEs in flight (one per thread)
