@@ -1170,7 +1170,7 @@ struct AdjState {
   bool work_ready = false;
   double *A = nullptr, *g = nullptr, *res[2] = {nullptr, nullptr}, *scr[2] = {nullptr, nullptr},
          *Fpb[2] = {nullptr, nullptr}, *xib[2] = {nullptr, nullptr}, *thb = nullptr, *mu = nullptr, *y = nullptr,
-         *b = nullptr, *r = nullptr, *V = nullptr, *red = nullptr;
+         *b = nullptr, *r = nullptr, *V = nullptr, *red = nullptr, *mu_prev = nullptr;
   std::vector<void*> work;
   // the factorisation of increment k - 1 (tape only) runs on st2 while the
   // GMRES of increment k runs on the solver stream; scr / res alternate
@@ -1304,7 +1304,7 @@ static void adj_free(dmk_handle* h) {
 size_t dmk_adjoint_work_bytes(const dmk_handle* h) {
   if (!h || !h->adj || h->adj->work_ready) return 0;
   const size_t N = h->N, n9 = h->n9, mmax = h->nsl_max, nslot = ADJ_TH_SYS * mmax + DMK_NH_MAX;
-  const size_t d = 81 * N + 7 * n9 + 2 * N + 2 * dmk_adj_scratch_slots((int)mmax) * N + 2 * mmax * N + nslot * N +
+  const size_t d = 81 * N + 8 * n9 + 2 * N + 2 * dmk_adj_scratch_slots((int)mmax) * N + 2 * mmax * N + nslot * N +
                    (size_t)(h->adj->M + 1) * n9 + (size_t)h->n_phases * (nslot + mmax);
   return sizeof(double) * d;
 }
@@ -1327,7 +1327,7 @@ void dmk_adjoint_work_release(dmk_handle* h) {
     }
   for (void* p : a.work) hipFree(p);
   a.work.clear();
-  a.A = a.g = a.thb = a.mu = a.y = a.b = a.r = a.V = a.red = nullptr;
+  a.A = a.g = a.thb = a.mu = a.y = a.b = a.r = a.V = a.red = a.mu_prev = nullptr;
   for (int i = 0; i < 2; i++) a.res[i] = a.scr[i] = a.Fpb[i] = a.xib[i] = nullptr;
   a.work_ready = false;
 }
@@ -1635,10 +1635,11 @@ static DmkPreOp axpby_step(long n, double av, const double* x, double bv, const 
 }
 
 // Restarted GMRES for Phi(A^T mu) = b, started from the mu in place (the
-// previous increment's adjoint field: consecutive increments' adjoints differ
-// little).  Arnoldi by classical Gram-Schmidt with the norm from the same
-// reduction (|w - V h|^2 = |w|^2 - |h|^2), so one host synchronisation per
-// step; a second pass when that cancels by more than a factor 100.
+// caller's extrapolation of the previous increments' adjoint fields, which
+// differ little from one increment to the next).  Arnoldi by classical
+// Gram-Schmidt with the norm from the same reduction (|w - V h|^2 = |w|^2 -
+// |h|^2), so one host synchronisation per step; a second pass when that
+// cancels by more than a factor 100.
 static int adj_gmres(dmk_handle* h, const AimArgs& aa, double rtol, int maxit, int* its_out, double* rel_out) {
   AdjState& a = *h->adj;
   const long n = h->n9;
@@ -1756,7 +1757,8 @@ extern "C" int dmk_adjoint_backward(dmk_handle* h, const double* P_av_bar, doubl
         (rc = adj_walloc(h, &a.scr[1], dmk_adj_scratch_slots(mmax) * N)) || (rc = adj_walloc(h, &a.Fpb[0], n9)) ||
         (rc = adj_walloc(h, &a.Fpb[1], n9)) || (rc = adj_walloc(h, &a.xib[0], (size_t)mmax * N)) ||
         (rc = adj_walloc(h, &a.xib[1], (size_t)mmax * N)) || (rc = adj_walloc(h, &a.thb, (size_t)nslot * N)) ||
-        (rc = adj_walloc(h, &a.mu, n9)) || (rc = adj_walloc(h, &a.y, n9)) || (rc = adj_walloc(h, &a.b, n9)) ||
+        (rc = adj_walloc(h, &a.mu, n9)) || (rc = adj_walloc(h, &a.mu_prev, n9)) || (rc = adj_walloc(h, &a.y, n9)) ||
+        (rc = adj_walloc(h, &a.b, n9)) ||
         (rc = adj_walloc(h, &a.r, n9)) || (rc = adj_walloc(h, &a.V, (size_t)(a.M + 1) * n9)) ||
         (rc = adj_walloc(h, &a.red, (size_t)h->n_phases * (nslot + mmax)))) {
       hipGetLastError();
@@ -1809,6 +1811,7 @@ extern "C" int dmk_adjoint_backward(dmk_handle* h, const double* P_av_bar, doubl
     return 0;
   };
   int cur = 0;
+  int nsolved = 0;     // increments solved so far, across the segments
   // the reverse sweep over increments k1 .. k0 + 1 of the tape in memory
   auto sweep = [&](int k1, int k0) -> int {
     CK(hipStreamSynchronize(h->st));
@@ -1860,11 +1863,21 @@ extern "C" int dmk_adjoint_backward(dmk_handle* h, const double* P_av_bar, doubl
       std::memcpy(aa.q_R, qR, sizeof aa.q_R);
       aa.rotate = 1;
       aa.update = 1;
+      // initial guess of the GMRES: the linear extrapolation 2 mu_(k+1) -
+      // mu_(k+2) of the two increments solved before (after one, its mu; the
+      // quadratic extrapolation measured more GMRES steps than the linear one)
+      if (nsolved >= 2) {
+        CKL(dmk_axpby((long)n9, 2.0, a.mu, -1.0, a.mu_prev, a.mu_prev, h->st));
+        std::swap(a.mu, a.mu_prev);
+      } else if (nsolved == 1) {
+        CKL(dmk_axpby((long)n9, 1.0, a.mu, 0.0, a.mu, a.mu_prev, h->st));
+      }
       CK(hipMemsetAsync(h->small + SM_AFAIM, 0, sizeof(double) * 9, h->st));
       if ((rc = adj_apply(h, aa, nullptr, a.g, a.b, nullptr, nullptr, true))) return rc;
       int its = 0;
       double rel = 0.0;
       if ((rc = adj_gmres(h, aa, rtol, 600, &its, &rel))) return rc;
+      nsolved++;
       ka.mode = 1;
       ka.mu = a.mu;
       ka.Fpb_out = a.Fpb[1 - cur];
