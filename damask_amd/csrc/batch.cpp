@@ -176,7 +176,10 @@ int resid_launch_locked(DmkStressBatch* b, int s, int nr) {
   for (int k = 0; k < nr; k++) {
     const DmkResidDev& q = b->rhost[s][k];
     for (int o = 0; o < q.npre; o++) max_pre = std::max(max_pre, q.pre[o].n);
-    if (q.npost) max_post = std::max(max_post, q.post.n);
+    if (q.npost) {
+      if (q.npost != 1 || q.post.type != DMK_PRE_AXPBY) return DMK_ERR_ARG;   // the step after a transform is z = a x + b y
+      max_post = std::max(max_post, q.post.n);
+    }
     max_nc_dots = std::max(max_nc_dots, q.nc_dots);
     max_jobs = std::max(max_jobs, q.J.n);
     max_sums = std::max(max_sums, q.S.k);
