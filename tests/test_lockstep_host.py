@@ -40,6 +40,21 @@ def test_lockstep_batch_host_logic_under_thread_sanitizer(tmp_path, groups, wait
     assert r.stdout.strip().endswith("ok")
 
 
+def test_lockstep_timing_under_thread_sanitizer(tmp_path):
+    """DMK_BATCH_TIMING: the per-round timing (events, host functions behind
+    each round, the first submitter's return) is recorded without data races
+    and printed per group"""
+    exe = _build(str(tmp_path / "lockstep_tsan_timing"), ["-fsanitize=thread"])
+    env = dict(os.environ, TSAN_OPTIONS="halt_on_error=1", DMK_LOCKSTEP_GROUPS="2", DMK_LOCKSTEP_WAIT_US="30",
+               DMK_BATCH_TIMING="1")
+    r = subprocess.run([exe, "120", "32", "2"], capture_output=True, text=True, timeout=600, env=env)
+    print(r.stdout[-2000:], r.stderr[-4000:])
+    assert "ThreadSanitizer" not in r.stderr
+    assert r.returncode == 0
+    assert r.stdout.strip().endswith("ok")
+    assert "lockstep group 0 timing" in r.stderr and "lockstep group 1 timing" in r.stderr
+
+
 def test_lockstep_harness_detects_missing_stream_waits(tmp_path):
     exe = _build(str(tmp_path / "lockstep_plain"), [])
     r = subprocess.run([exe, "120", "32", "2", "drop-waits"], capture_output=True, text=True, timeout=600)
